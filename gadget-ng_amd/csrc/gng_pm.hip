@@ -1,0 +1,534 @@
+// gng_pm.hip -- plan, dispatch and C ABI of the MI355X PM solver.
+//
+// The plan owns the device meshes (allocated once at create, as the
+// reference's handle does: crates/gadget-ng-hip/hip/pm_gravity.hip:278-334)
+// and grows the particle buffers on demand (pm_gravity.hip:376-392).  Every
+// solve is a fixed sequence of launches on one stream; host entry points add
+// the H2D/D2H copies and a stream synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/gng_pm.h"
+#include "pm_kernels.hpp"
+
+using namespace gng;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define GNG_HIP(call)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return fail(e_ == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP,                         \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                            \
+  } while (0)
+
+constexpr int kStages = 8;
+
+// ---------------------------------------------------------------------------
+// per-mesh-size kernel table
+// ---------------------------------------------------------------------------
+
+constexpr int pow2_floor(int v) {
+  int p = 1;
+  while (p * 2 <= v) p *= 2;
+  return p;
+}
+constexpr long long pow2_part(long long v) {
+  long long p = 1;
+  while (v % (p * 2) == 0) p *= 2;
+  return p;
+}
+template <int N>
+constexpr int x_rows() {
+  constexpr int H = N / 2;
+  constexpr int a = pow2_floor(2048 / H > 0 ? 2048 / H : 1);
+  constexpr long long b = pow2_part((long long)N * N);
+  return a < b ? a : (int)b;
+}
+
+struct MeshOps {
+  int nm;
+  int x_rows;
+  int yz_cols;
+  void (*x_fwd)(dim3, dim3, hipStream_t, const long long*, const SolveScalars*, long long, float2*, const float2*);
+  void (*y_fwd)(dim3, dim3, hipStream_t, float2*, const float2*);
+  void (*z_green)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
+  void (*y_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
+  void (*x_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
+};
+
+template <int N>
+MeshOps make_ops() {
+  constexpr int R = x_rows<N>();
+  constexpr int C = YZCfg<N>::C;
+  MeshOps o;
+  o.nm = N;
+  o.x_rows = R;
+  o.yz_cols = C;
+  o.x_fwd = [](dim3 g, dim3 b, hipStream_t s, const long long* m, const SolveScalars* sc, long long n, float2* P,
+               const float2* tw) { hipLaunchKernelGGL((k_fft_x_fwd<N, R>), g, b, 0, s, m, sc, n, P, tw); };
+  o.y_fwd = [](dim3 g, dim3 b, hipStream_t s, float2* P, const float2* tw) {
+    hipLaunchKernelGGL((k_fft_y_fwd<N, C>), g, b, 0, s, P, tw);
+  };
+  o.z_green = [](dim3 g, dim3 b, hipStream_t s, const float2* P, float2* A, float2* B, const float2* tw,
+                 GreenParams gp) { hipLaunchKernelGGL((k_fft_z_green<N, C>), g, b, 0, s, P, A, B, tw, gp); };
+  o.y_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* B, float2* P, const float2* tw) {
+    hipLaunchKernelGGL((k_fft_y_inv<N, C>), g, b, 0, s, A, B, P, tw);
+  };
+  o.x_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* P, float2* B, const float2* tw) {
+    hipLaunchKernelGGL((k_fft_x_inv<N, R>), g, b, 0, s, A, P, B, tw);
+  };
+  return o;
+}
+
+const std::vector<MeshOps>& ops_table() {
+  static const std::vector<MeshOps> t = {
+      make_ops<4>(),   make_ops<8>(),   make_ops<16>(),  make_ops<32>(),  make_ops<64>(),
+      make_ops<128>(), make_ops<256>(), make_ops<512>(), make_ops<1024>(), make_ops<12>(),
+      make_ops<24>(),  make_ops<48>(),  make_ops<96>(),  make_ops<192>(), make_ops<384>(),
+  };
+  return t;
+}
+
+const MeshOps* find_ops(int nm) {
+  for (const MeshOps& o : ops_table())
+    if (o.nm == nm) return &o;
+  return nullptr;
+}
+
+}  // namespace
+
+struct gng_pm_plan {
+  gng_pm_config cfg{};
+  const MeshOps* ops = nullptr;
+  int device = 0;
+  std::mutex mu;
+
+  long long nm3 = 0;
+  unsigned long long* d_fixed = nullptr;  // nm^3 int64 fixed-point mass
+  float2* d_P = nullptr;                  // three nm^3/2 complex meshes
+  float2* d_A = nullptr;
+  float2* d_B = nullptr;
+  float2* d_tw = nullptr;  // nm-th roots of unity
+  float* d_deconv = nullptr;
+  SolveScalars* d_sc = nullptr;
+
+  // particle staging for the host entry points
+  long long cap = 0;
+  float* d_part = nullptr;  // x, y, z, m, ax, ay, az (7 * cap floats)
+  long long idx_cap = 0;
+  long long* d_idx = nullptr;
+  hipStream_t stream = nullptr;
+
+  long long last_n = 0;
+  bool timing = false;
+  hipEvent_t ev[kStages + 1] = {};
+  float stage_ms[kStages] = {};
+};
+
+namespace {
+
+int set_device(gng_pm_plan* p) {
+  GNG_HIP(hipSetDevice(p->device));
+  return GNG_OK;
+}
+
+void free_plan(gng_pm_plan* p) {
+  if (!p) return;
+  hipSetDevice(p->device);
+  hipFree(p->d_fixed);
+  hipFree(p->d_P);
+  hipFree(p->d_A);
+  hipFree(p->d_B);
+  hipFree(p->d_tw);
+  hipFree(p->d_deconv);
+  hipFree(p->d_sc);
+  hipFree(p->d_part);
+  hipFree(p->d_idx);
+  for (auto& e : p->ev)
+    if (e) hipEventDestroy(e);
+  if (p->stream) hipStreamDestroy(p->stream);
+  delete p;
+}
+
+CellMap cell_map(const gng_pm_plan* p) {
+  const double inv = (double)p->cfg.grid_size / p->cfg.box_size;
+  CellMap cm;
+  cm.hi = (float)inv;
+  cm.lo = (float)(inv - (double)cm.hi);
+  cm.nm = p->cfg.grid_size;
+  return cm;
+}
+
+GreenParams green_params(const gng_pm_plan* p, double g) {
+  const double L = p->cfg.box_size;
+  const double kf = 2.0 * M_PI / L;
+  double a = 0.0;
+  if (p->cfg.r_split > 0.0) a += 0.5 * kf * kf * p->cfg.r_split * p->cfg.r_split;
+  if (p->cfg.plummer_eps > 0.0) a += kf * kf * p->cfg.plummer_eps * p->cfg.plummer_eps;
+  GreenParams gp;
+  gp.c64 = 2.0 * g / (L * L);
+  gp.a64 = a;
+  gp.c = (float)gp.c64;
+  gp.a = (float)a;
+  gp.deconv = p->cfg.deconvolve ? p->d_deconv : nullptr;
+  return gp;
+}
+
+inline void mark(gng_pm_plan* p, int i, hipStream_t s) {
+  if (p->timing) hipEventRecord(p->ev[i], s);
+}
+
+// The whole pipeline on device pointers.
+int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z, const float* m, long long n,
+                 const long long* idx, long long k, double g, float* ax, float* ay, float* az, hipStream_t s) {
+  const int nm = p->cfg.grid_size;
+  const int H = nm / 2;
+  const MeshOps& o = *p->ops;
+  const CellMap cm = cell_map(p);
+  mark(p, 0, s);
+  hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, p->d_sc);
+  {
+    long long blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_mass_max, dim3((unsigned)blocks), dim3(256), 0, s, m, n, p->d_sc);
+  }
+  {
+    const long long n16 = p->nm3 * 8 / 16;
+    long long blocks = (n16 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<uint4*>(p->d_fixed), n16);
+  }
+  mark(p, 1, s);
+  if (n > 0)
+    hipLaunchKernelGGL(k_deposit_atomic, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z, m, n, cm,
+                       p->d_sc, p->d_fixed);
+  mark(p, 2, s);
+  const long long rows = (long long)nm * nm;
+  o.x_fwd(dim3((unsigned)(rows / o.x_rows)), dim3(256), s, reinterpret_cast<const long long*>(p->d_fixed), p->d_sc,
+          n, p->d_P, p->d_tw);
+  mark(p, 3, s);
+  const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
+  o.y_fwd(yz_grid, dim3(256), s, p->d_P, p->d_tw);
+  mark(p, 4, s);
+  o.z_green(yz_grid, dim3(256), s, p->d_P, p->d_A, p->d_B, p->d_tw, green_params(p, g));
+  mark(p, 5, s);
+  o.y_inv(yz_grid, dim3(256), s, p->d_A, p->d_B, p->d_P, p->d_tw);
+  mark(p, 6, s);
+  o.x_inv(dim3((unsigned)(rows / o.x_rows), 3), dim3(256), s, p->d_A, p->d_P, p->d_B, p->d_tw);
+  mark(p, 7, s);
+  if (k > 0)
+    hipLaunchKernelGGL(k_interp, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, x, y, z, idx, k, cm,
+                       reinterpret_cast<const float*>(p->d_A), reinterpret_cast<const float*>(p->d_P),
+                       reinterpret_cast<const float*>(p->d_B), ax, ay, az);
+  mark(p, 8, s);
+  GNG_HIP(hipGetLastError());
+  p->last_n = n;
+  return GNG_OK;
+}
+
+int collect_timing(gng_pm_plan* p) {
+  if (!p->timing) return GNG_OK;
+  GNG_HIP(hipEventSynchronize(p->ev[kStages]));
+  for (int i = 0; i < kStages; ++i) GNG_HIP(hipEventElapsedTime(&p->stage_ms[i], p->ev[i], p->ev[i + 1]));
+  return GNG_OK;
+}
+
+int ensure_particles(gng_pm_plan* p, long long n, long long k, bool need_idx) {
+  const long long need = n > k ? n : k;
+  if (need > p->cap) {
+    hipFree(p->d_part);
+    p->d_part = nullptr;
+    p->cap = 0;
+    GNG_HIP(hipMalloc(&p->d_part, sizeof(float) * 7 * (size_t)need));
+    p->cap = need;
+  }
+  if (need_idx && k > p->idx_cap) {
+    hipFree(p->d_idx);
+    p->d_idx = nullptr;
+    p->idx_cap = 0;
+    GNG_HIP(hipMalloc(&p->d_idx, sizeof(long long) * (size_t)k));
+    p->idx_cap = k;
+  }
+  return GNG_OK;
+}
+
+int validate(gng_pm_plan* p, long long n, const void* idx, long long k) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  if (n < 0 || k < 0) return fail(GNG_ERR_INVALID, "negative particle count");
+  if (!idx && k != n) return fail(GNG_ERR_INVALID, "idx is NULL but k != n");
+  if (n > (1LL << 31)) return fail(GNG_ERR_INVALID, "n above 2^31 is not supported");
+  return GNG_OK;
+}
+
+// Host SoA f32 solve (shared by the legacy and the f32/f64 entry points).
+int solve_host_soa(gng_pm_plan* p, const float* x, const float* y, const float* z, const float* m, long long n,
+                   const int64_t* idx, long long k, double g, float* ax, float* ay, float* az) {
+  int rc = ensure_particles(p, n, k, idx != nullptr);
+  if (rc) return rc;
+  const size_t cap = (size_t)p->cap;
+  float* d = p->d_part;
+  float *dx = d, *dy = d + cap, *dz = d + 2 * cap, *dm = d + 3 * cap;
+  float *dax = d + 4 * cap, *day = d + 5 * cap, *daz = d + 6 * cap;
+  hipStream_t s = p->stream;
+  const size_t nb = sizeof(float) * (size_t)n;
+  if (n > 0) {
+    GNG_HIP(hipMemcpyAsync(dx, x, nb, hipMemcpyHostToDevice, s));
+    GNG_HIP(hipMemcpyAsync(dy, y, nb, hipMemcpyHostToDevice, s));
+    GNG_HIP(hipMemcpyAsync(dz, z, nb, hipMemcpyHostToDevice, s));
+    GNG_HIP(hipMemcpyAsync(dm, m, nb, hipMemcpyHostToDevice, s));
+  }
+  if (idx && k > 0) {
+    for (long long j = 0; j < k; ++j)
+      if (idx[j] < 0 || idx[j] >= n) return fail(GNG_ERR_INVALID, "index out of range");
+    GNG_HIP(hipMemcpyAsync(p->d_idx, idx, sizeof(long long) * (size_t)k, hipMemcpyHostToDevice, s));
+  }
+  rc = run_pipeline(p, dx, dy, dz, dm, n, idx ? p->d_idx : nullptr, k, g, dax, day, daz, s);
+  if (rc) return rc;
+  const size_t kb = sizeof(float) * (size_t)k;
+  if (k > 0) {
+    GNG_HIP(hipMemcpyAsync(ax, dax, kb, hipMemcpyDeviceToHost, s));
+    GNG_HIP(hipMemcpyAsync(ay, day, kb, hipMemcpyDeviceToHost, s));
+    GNG_HIP(hipMemcpyAsync(az, daz, kb, hipMemcpyDeviceToHost, s));
+  }
+  GNG_HIP(hipStreamSynchronize(s));
+  return collect_timing(p);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gng_pm_abi_version(void) { return GNG_PM_ABI_VERSION; }
+
+const char* gng_last_error_string(void) { return g_last_error.c_str(); }
+
+int gng_pm_mesh_supported(int grid_size) { return find_ops(grid_size) ? 1 : 0; }
+
+int gng_pm_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
+  if (!out) return fail(GNG_ERR_INVALID, "null output pointer");
+  *out = nullptr;
+  if (!cfg) return fail(GNG_ERR_INVALID, "null config");
+  if (!(cfg->box_size > 0.0) || !std::isfinite(cfg->box_size)) return fail(GNG_ERR_INVALID, "box_size must be > 0");
+  const MeshOps* ops = find_ops(cfg->grid_size);
+  if (!ops)
+    return fail(GNG_ERR_UNSUPPORTED, "grid_size " + std::to_string(cfg->grid_size) +
+                                         " is not compiled in (supported: 2^a 3^b from 4 to 1024)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(GNG_ERR_NO_DEVICE, "no HIP device");
+  gng_pm_plan* p = new (std::nothrow) gng_pm_plan();
+  if (!p) return fail(GNG_ERR_OOM, "host allocation failed");
+  p->cfg = *cfg;
+  p->ops = ops;
+  if (cfg->device >= 0) {
+    if (cfg->device >= ndev) {
+      delete p;
+      return fail(GNG_ERR_INVALID, "device ordinal out of range");
+    }
+    p->device = cfg->device;
+  } else {
+    hipGetDevice(&p->device);
+  }
+  auto bail = [&](int rc) {
+    free_plan(p);
+    return rc;
+  };
+  int rc = set_device(p);
+  if (rc) return bail(rc);
+  const int nm = cfg->grid_size;
+  p->nm3 = (long long)nm * nm * nm;
+  const size_t half = sizeof(float2) * (size_t)(p->nm3 / 2);
+#define GNG_ALLOC(ptr, bytes)                                                                             \
+  do {                                                                                                    \
+    hipError_t e_ = hipMalloc(&(ptr), (bytes));                                                           \
+    if (e_ != hipSuccess)                                                                                 \
+      return bail(fail(e_ == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP,                             \
+                       std::string("hipMalloc ") + #ptr + ": " + hipGetErrorString(e_)));                 \
+  } while (0)
+  GNG_ALLOC(p->d_fixed, sizeof(unsigned long long) * (size_t)p->nm3);
+  GNG_ALLOC(p->d_P, half);
+  GNG_ALLOC(p->d_A, half);
+  GNG_ALLOC(p->d_B, half);
+  GNG_ALLOC(p->d_tw, sizeof(float2) * (size_t)nm);
+  GNG_ALLOC(p->d_deconv, sizeof(float) * (size_t)nm);
+  GNG_ALLOC(p->d_sc, sizeof(SolveScalars));
+#undef GNG_ALLOC
+  std::vector<float2> tw(nm);
+  std::vector<float> dc(nm);
+  for (int m = 0; m < nm; ++m) {
+    const long double a = 6.283185307179586476925286766559005768L * (long double)m / (long double)nm;
+    tw[m] = make_float2((float)cosl(a), (float)(-sinl(a)));
+    const int f = m <= nm / 2 ? m : m - nm;
+    const double xx = M_PI * (double)f / (double)nm;
+    const double s = f == 0 ? 1.0 : std::sin(xx) / xx;
+    dc[m] = (float)(1.0 / (s * s * s * s));
+  }
+  hipError_t e = hipMemcpy(p->d_tw, tw.data(), sizeof(float2) * nm, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->d_deconv, dc.data(), sizeof(float) * nm, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  for (int i = 0; i <= kStages && e == hipSuccess; ++i) e = hipEventCreate(&p->ev[i]);
+  if (e != hipSuccess) return bail(fail(GNG_ERR_HIP, std::string("plan setup: ") + hipGetErrorString(e)));
+  *out = p;
+  return GNG_OK;
+}
+
+void gng_pm_destroy(gng_pm_plan* p) { free_plan(p); }
+
+int gng_pm_accelerations_f32(gng_pm_plan* p, const float* x, const float* y, const float* z, const float* mass,
+                             int64_t n, const int64_t* idx, int64_t k, double g, float* ax, float* ay, float* az) {
+  int rc = validate(p, n, idx, k);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(p->mu);
+  if ((rc = set_device(p))) return rc;
+  return solve_host_soa(p, x, y, z, mass, n, idx, k, g, ax, ay, az);
+}
+
+int gng_pm_accelerations_f64(gng_pm_plan* p, const double* pos, const double* mass, int64_t n, const int64_t* idx,
+                             int64_t k, double eps2, double g, double* out) {
+  (void)eps2;  // PmSolver ignores eps2 (crates/gadget-ng-pm/src/solver.rs:70-72)
+  int rc = validate(p, n, idx, k);
+  if (rc) return rc;
+  if (n > 0 && (!pos || !mass)) return fail(GNG_ERR_INVALID, "null input array");
+  if (k > 0 && !out) return fail(GNG_ERR_INVALID, "null output array");
+  std::vector<float> h(4 * (size_t)n + 3 * (size_t)k);
+  float *x = h.data(), *y = x + n, *z = y + n, *m = z + n, *ax = m + n, *ay = ax + k, *az = ay + k;
+  for (long long i = 0; i < n; ++i) {
+    x[i] = (float)pos[3 * i];
+    y[i] = (float)pos[3 * i + 1];
+    z[i] = (float)pos[3 * i + 2];
+    m[i] = (float)mass[i];
+  }
+  std::lock_guard<std::mutex> lk(p->mu);
+  if ((rc = set_device(p))) return rc;
+  rc = solve_host_soa(p, x, y, z, m, n, idx, k, g, ax, ay, az);
+  if (rc) return rc;
+  for (long long j = 0; j < k; ++j) {
+    out[3 * j] = ax[j];
+    out[3 * j + 1] = ay[j];
+    out[3 * j + 2] = az[j];
+  }
+  return GNG_OK;
+}
+
+int gng_pm_solve_device(gng_pm_plan* p, const float* d_x, const float* d_y, const float* d_z, const float* d_mass,
+                        int64_t n, const int64_t* d_idx, int64_t k, double g, float* d_ax, float* d_ay,
+                        float* d_az, void* stream) {
+  int rc = validate(p, n, d_idx, k);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(p->mu);
+  if ((rc = set_device(p))) return rc;
+  rc = run_pipeline(p, d_x, d_y, d_z, d_mass, n, reinterpret_cast<const long long*>(d_idx), k, g, d_ax, d_ay, d_az,
+                    reinterpret_cast<hipStream_t>(stream));
+  if (rc) return rc;
+  return collect_timing(p);
+}
+
+int gng_pm_density_fixed(gng_pm_plan* p, int64_t* out, int32_t* shift) {
+  if (!p || !out) return fail(GNG_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  int rc = set_device(p);
+  if (rc) return rc;
+  SolveScalars sc;
+  GNG_HIP(hipMemcpy(&sc, p->d_sc, sizeof(sc), hipMemcpyDeviceToHost));
+  GNG_HIP(hipMemcpy(out, p->d_fixed, sizeof(int64_t) * (size_t)p->nm3, hipMemcpyDeviceToHost));
+  float mmax;
+  std::memcpy(&mmax, &sc.mmax_bits, sizeof(float));
+  int e = 0;
+  std::frexp((double)p->last_n * (double)mmax, &e);
+  if (shift) *shift = 62 - e;
+  return GNG_OK;
+}
+
+int gng_pm_density(gng_pm_plan* p, double* out) {
+  if (!p || !out) return fail(GNG_ERR_INVALID, "null argument");
+  std::vector<int64_t> q((size_t)p->nm3);
+  int32_t s = 0;
+  int rc = gng_pm_density_fixed(p, q.data(), &s);
+  if (rc) return rc;
+  const double unit = std::ldexp(1.0, -s);
+  for (long long i = 0; i < p->nm3; ++i) out[i] = (double)q[i] * unit;
+  return GNG_OK;
+}
+
+int gng_pm_force_meshes(gng_pm_plan* p, float* out) {
+  if (!p || !out) return fail(GNG_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  int rc = set_device(p);
+  if (rc) return rc;
+  const size_t bytes = sizeof(float) * (size_t)p->nm3;
+  GNG_HIP(hipMemcpy(out, p->d_A, bytes, hipMemcpyDeviceToHost));
+  GNG_HIP(hipMemcpy(out + p->nm3, p->d_P, bytes, hipMemcpyDeviceToHost));
+  GNG_HIP(hipMemcpy(out + 2 * p->nm3, p->d_B, bytes, hipMemcpyDeviceToHost));
+  return GNG_OK;
+}
+
+int gng_pm_set_timing(gng_pm_plan* p, int enable) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  p->timing = enable != 0;
+  return GNG_OK;
+}
+
+int gng_pm_stage_times(gng_pm_plan* p, float* ms, int capacity) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  for (int i = 0; i < kStages && i < capacity; ++i) ms[i] = p->stage_ms[i];
+  return kStages;
+}
+
+// ---- reference-compatible entry points ----------------------------------
+
+void* hip_pm_create(int grid_size, float box_size) {
+  gng_pm_config cfg;
+  std::memset(&cfg, 0, sizeof(cfg));
+  cfg.grid_size = grid_size;
+  cfg.box_size = box_size;
+  cfg.device = -1;
+  gng_pm_plan* p = nullptr;
+  if (gng_pm_create(&cfg, &p) != GNG_OK) {
+    std::fprintf(stderr, "[gng_pm] hip_pm_create failed: %s\n", g_last_error.c_str());
+    return nullptr;
+  }
+  return p;
+}
+
+void hip_pm_destroy(void* h) { gng_pm_destroy(static_cast<gng_pm_plan*>(h)); }
+
+int hip_pm_solve(void* h, const float* x, const float* y, const float* z, const float* mass, float* ax, float* ay,
+                 float* az, int n, float eps2, float g, float r_split) {
+  (void)eps2;
+  if (!h || n <= 0) return 0;
+  gng_pm_plan* p = static_cast<gng_pm_plan*>(h);
+  std::lock_guard<std::mutex> lk(p->mu);
+  int rc = set_device(p);
+  if (rc) return rc;
+  // the reference passes r_split per call (pm_gravity.hip:363-367)
+  const double keep = p->cfg.r_split;
+  p->cfg.r_split = r_split > 0.0f ? (double)r_split : 0.0;
+  rc = solve_host_soa(p, x, y, z, mass, n, nullptr, n, (double)g, ax, ay, az);
+  p->cfg.r_split = keep;
+  if (rc) std::fprintf(stderr, "[gng_pm] hip_pm_solve failed: %s\n", g_last_error.c_str());
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,508 @@
+// pm_kernels.hpp -- the periodic PM force pipeline as HIP kernels for gfx950.
+//
+// Stages (reference semantics: crates/gadget-ng-pm/src/solver.rs:74-140):
+//   deposit    CIC mass per cell, 64-bit fixed point, integer atomics, so the
+//              mesh is bit-identical from run to run (cic.rs:37-93 semantics)
+//   fft_x_fwd  fixed point -> FP64 -> FP32 conversion fused with the real-to-
+//              complex x transform; output is the packed half spectrum
+//   fft_y_fwd  complex y transform
+//   fft_z_green  z transform, Green's function x filters x (optional) CIC
+//              deconvolution, then the inverse z transform of two fields:
+//              A = phi, B = -i n_z phi (fft_poisson.rs:509-549)
+//   fft_y_inv  inverse y transform: A -> A (for F_x), -i n_y A -> P (F_y),
+//              B -> B (F_z)
+//   fft_x_inv  -i n_x on the F_x field, complex-to-real x transform, in place
+//   interp     CIC gather of the three force meshes (cic.rs:100-170)
+//
+// Packed half spectrum.  The x transform of a real row of length N has N/2+1
+// values; the DC and Nyquist ones are real.  They are stored together as one
+// complex number in slot 0 (re = DC, im = Nyquist), so every row is N/2
+// complex values and the spectrum occupies exactly the bytes of the real
+// mesh.  After the y and z transforms, slot 0 holds Z = R0 + i RN where R0,
+// RN are the (Hermitian) spectra of the kx = 0 and kx = N/2 planes; the
+// Green's function, which differs between the two planes, is applied as
+//   out = (G0 + GN)/2 * Z(k) + (G0 - GN)/2 * conj(Z(-k)).
+// The gradient factors -i n_alpha are the same on both planes (zero for x).
+//
+// Nyquist rule.  The reference takes Re() of a full complex inverse, which
+// equals the Hermitian projection of F_alpha-hat; that projection is zero on
+// the n_alpha = N/2 plane and the identity elsewhere.  Here the derivative
+// uses n_alpha,eff = 0 on that plane while k^2 keeps n = +N/2
+// (fft_poisson.rs:724-730), so every self-conjugate mode is exactly zero and
+// the complex-to-real transform reproduces Re(Z2Z).
+//
+// Normalisation.  With M-hat the unnormalised DFT of the mass per cell and
+// n the integer wavenumber, F_alpha-hat / N^3 = -i n_alpha phi,
+//   phi = -(2 G / L^2) W(k) / |n|^2 * M-hat,
+// which equals the reference's rho = M / V_cell, phi = -4 pi G W rho-hat/k^2,
+// F = -i k phi, F = Re(IFFT(F-hat)) / N^3.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_device.hpp"
+
+namespace gng {
+
+// 1/L of the cell map as a float pair: inv_cell = hi + lo, nm / L exactly
+// enough that x * inv_cell keeps ~1e-12 relative accuracy in FP32 ops.
+struct CellMap {
+  float hi, lo;
+  int nm;
+};
+
+// Cell index and offset of one coordinate: c = (x * nm / L) mod nm,
+// i0 = floor(c), d = c - i0 (cic.rs:50-60), with the product carried as
+// p + e (error-free FMA split) so d is accurate to one FP32 ulp of 1.
+// Contraction is switched off here: fusing p - f into fma(x, hi, -f) would
+// count the product's rounding error twice and leave d wrong by up to
+// ulp(c), an error that grows with the mesh size.
+__device__ __forceinline__ void cell_coord(float x, const CellMap cm, int& i0, float& d) {
+#pragma clang fp contract(off)
+  const float p = x * cm.hi;
+  const float e = fmaf(x, cm.hi, -p) + x * cm.lo;
+  const float f = floorf(p);
+  float t = (p - f) + e;
+  int i = (int)f;
+  if (t < 0.0f) {
+    t += 1.0f;
+    i -= 1;
+  } else if (t >= 1.0f) {
+    t -= 1.0f;
+    i += 1;
+  }
+  i %= cm.nm;
+  if (i < 0) i += cm.nm;
+  i0 = i;
+  d = t;
+}
+
+// Per-solve scalars kept on the device (no host round trip).
+struct SolveScalars {
+  unsigned int mmax_bits;  // max |m| as float bits (non-negative floats order as integers)
+  unsigned int pad[63];
+};
+
+// Fixed-point exponent: contributions are w*m*2^s with N*max|m|*2^s < 2^62,
+// so no cell sum can overflow int64 whatever the clustering.
+__device__ __forceinline__ int fixed_shift(const SolveScalars* sc, long long ntot) {
+  const float mmax = __uint_as_float(sc->mmax_bits);
+  int e = 0;
+  (void)frexp((double)ntot * (double)mmax, &e);
+  return 62 - e;
+}
+
+__global__ void k_reset_scalars(SolveScalars* sc) {
+  if (threadIdx.x == 0) sc->mmax_bits = 0u;
+}
+
+// max |m| over the particles (deterministic: integer max of float bits).
+__global__ __launch_bounds__(256) void k_mass_max(const float* __restrict__ m, long long n, SolveScalars* sc) {
+  float v = 0.0f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    v = fmaxf(v, fabsf(m[i]));
+  // wave reduction
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  __shared__ float red[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float r = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = fmaxf(r, red[w]);
+    atomicMax(&sc->mmax_bits, __float_as_uint(r));
+  }
+}
+
+// Zero a buffer with vector stores (graph-safe replacement for a memset).
+__global__ __launch_bounds__(256) void k_zero(uint4* __restrict__ p, long long n16) {
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
+    p[i] = z;
+}
+
+// CIC deposit: one thread per particle, 8 fixed-point atomics into the int64
+// mesh (flat = iz*nm^2 + iy*nm + ix, cic.rs:8).  All 8 indices wrap.
+__global__ __launch_bounds__(256) void k_deposit_atomic(const float* __restrict__ x, const float* __restrict__ y,
+                                                        const float* __restrict__ z, const float* __restrict__ m,
+                                                        long long n, const CellMap cm, const SolveScalars* sc,
+                                                        unsigned long long* __restrict__ mesh) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = fixed_shift(sc, n);
+  int ix, iy, iz;
+  float dx, dy, dz;
+  cell_coord(x[i], cm, ix, dx);
+  cell_coord(y[i], cm, iy, dy);
+  cell_coord(z[i], cm, iz, dz);
+  const int nm = cm.nm;
+  const float mi = m[i];
+  const float wx[2] = {1.0f - dx, dx}, wy[2] = {1.0f - dy, dy}, wz[2] = {1.0f - dz, dz};
+  const int xs[2] = {ix, ix + 1 == nm ? 0 : ix + 1};
+  const int ys[2] = {iy, iy + 1 == nm ? 0 : iy + 1};
+  const int zs[2] = {iz, iz + 1 == nm ? 0 : iz + 1};
+  const size_t nm2 = (size_t)nm * nm;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const float w = mi * wx[c] * wy[b] * wz[a];
+        const long long q = __float2ll_rn(ldexpf(w, s));
+        atomicAdd(mesh + zs[a] * nm2 + (size_t)ys[b] * nm + xs[c], (unsigned long long)q);
+      }
+}
+
+// ---------------------------------------------------------------------------
+// x transforms (rows are contiguous; a workgroup owns ROWS whole rows)
+// ---------------------------------------------------------------------------
+
+template <int N>
+struct XCfg {
+  static constexpr int H = N / 2;
+  static constexpr int T = 256;
+  static constexpr int ROWS_RAW = (2048 / H) > 0 ? 2048 / H : 1;
+  static constexpr int LS = H + 1;  // odd stride against LDS bank conflicts
+};
+
+// Fixed point -> FP64 -> FP32 (mass per cell), then R2C along x.
+template <int N, int ROWS>
+__global__ __launch_bounds__(256) void k_fft_x_fwd(const long long* __restrict__ mesh, const SolveScalars* sc,
+                                                   long long ntot, float2* __restrict__ P,
+                                                   const float2* __restrict__ tw) {
+  constexpr int H = N / 2, T = 256, LS = XCfg<N>::LS;
+  __shared__ float2 buf[ROWS * LS];
+  const int tid = threadIdx.x;
+  const long long row0 = (long long)blockIdx.x * ROWS;
+  const double unit = ldexp(1.0, -fixed_shift(sc, ntot));
+  const longlong2* src = reinterpret_cast<const longlong2*>(mesh + row0 * N);
+  constexpr int TOT = ROWS * H;
+#pragma unroll
+  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) {
+      const longlong2 v = src[e];
+      buf[(e / H) * LS + (e % H)] = make_float2((float)((double)v.x * unit), (float)((double)v.y * unit));
+    }
+  }
+  __syncthreads();
+  fft_lds<H, false, ROWS, T, LS, 1, false, N>(buf, tw, tid);
+  float2* dst = P + row0 * H;
+#pragma unroll
+  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) {
+      const int r = e / H, k = e % H;
+      const float2 zk = buf[r * LS + k];
+      float2 out;
+      if (k == 0) {
+        out = make_float2(zk.x + zk.y, zk.x - zk.y);
+      } else {
+        const float2 zc = cconj(buf[r * LS + (H - k)]);
+        const float2 E = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+        // O = (zk - zc) / (2i)
+        const float2 O = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
+        out = cadd(E, cmul(tw[k], O));
+      }
+      dst[e] = out;
+    }
+  }
+}
+
+// C2R along x, in place, for field blockIdx.y (0: A with -i n_x, 1: P, 2: B).
+template <int N, int ROWS>
+__global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float2* __restrict__ P,
+                                                   float2* __restrict__ B, const float2* __restrict__ tw) {
+  constexpr int H = N / 2, T = 256, LS = XCfg<N>::LS;
+  __shared__ float2 buf[ROWS * LS];
+  const int tid = threadIdx.x;
+  const int field = blockIdx.y;
+  float2* data = field == 0 ? A : (field == 1 ? P : B);
+  const long long row0 = (long long)blockIdx.x * ROWS;
+  float2* row = data + row0 * H;
+  constexpr int TOT = ROWS * H;
+  constexpr int PER = (TOT + T - 1) / T;
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) {
+      float2 v = row[e];
+      if (field == 0) {
+        // -i n_x; slot 0 holds DC (n=0) and Nyquist (n_eff=0): both vanish
+        const int k = e % H;
+        const float nx = (float)k;
+        v = make_float2(nx * v.y, -nx * v.x);
+      }
+      buf[(e / H) * LS + (e % H)] = v;
+    }
+  }
+  __syncthreads();
+  float2 zr[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) {
+      const int r = e / H, k = e % H;
+      const float2 y = buf[r * LS + k];
+      if (k == 0) {
+        zr[p] = make_float2(y.x + y.y, y.x - y.y);
+      } else {
+        const float2 xc = cconj(buf[r * LS + (H - k)]);
+        const float2 s = cadd(y, xc);
+        const float2 d = csub(y, xc);
+        // Z = (X_k + conj X_{H-k}) + i (X_k - conj X_{H-k}) e^{+2 pi i k/N}
+        const float2 t = cmul(d, cconj(tw[k]));
+        zr[p] = make_float2(s.x - t.y, s.y + t.x);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) buf[(e / H) * LS + (e % H)] = zr[p];
+  }
+  __syncthreads();
+  fft_lds<H, true, ROWS, T, LS, 1, false, N>(buf, tw, tid);
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) row[e] = buf[(e / H) * LS + (e % H)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y / z transforms: a workgroup owns C adjacent kx columns of one plane
+// ---------------------------------------------------------------------------
+
+// Column tile width: the largest divisor of N/2 that is at most 8, so the
+// tiles cover every kx column exactly.
+__host__ __device__ constexpr int yz_cols(int h) {
+  int c = h < 8 ? h : 8;
+  while (h % c != 0) --c;
+  return c;
+}
+
+template <int N>
+struct YZCfg {
+  static constexpr int H = N / 2;
+  static constexpr int C = yz_cols(H);
+  static constexpr int T = 256;
+  static_assert(H % C == 0, "column tiles must cover the half spectrum");
+};
+
+template <int N, int C>
+__global__ __launch_bounds__(256) void k_fft_y_fwd(float2* __restrict__ P, const float2* __restrict__ tw) {
+  constexpr int H = N / 2, T = 256, IS = C + 1;
+  __shared__ float2 buf[N * IS];
+  const int tid = threadIdx.x;
+  float2* base = P + (size_t)blockIdx.y * N * H + (size_t)blockIdx.x * C;
+  constexpr int TOT = N * C;
+#pragma unroll
+  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) buf[(e / C) * IS + (e % C)] = base[(size_t)(e / C) * H + (e % C)];
+  }
+  __syncthreads();
+  fft_lds<N, false, C, T, 1, IS, true, N>(buf, tw, tid);
+#pragma unroll
+  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) base[(size_t)(e / C) * H + (e % C)] = buf[(e / C) * IS + (e % C)];
+  }
+}
+
+struct GreenParams {
+  float c;        // 2 G / L^2
+  float a;        // (2 pi / L)^2 (r_s^2 / 2 + eps^2): W = exp(-a |n|^2)
+  double c64, a64;
+  const float* deconv;  // 1/sinc^4(pi n / N) per axis index, or null
+};
+
+__device__ __forceinline__ int freq_of(int j, int n) { return j <= n / 2 ? j : j - n; }
+
+template <int N, int C>
+__global__ __launch_bounds__(256) void k_fft_z_green(const float2* __restrict__ P, float2* __restrict__ A,
+                                                     float2* __restrict__ B, const float2* __restrict__ tw,
+                                                     const GreenParams gp) {
+  constexpr int H = N / 2, T = 256, IS = 2 * C + 1;
+  __shared__ float2 buf[N * IS];
+  const int tid = threadIdx.x;
+  const int tile = blockIdx.x, ky = blockIdx.y;
+  const size_t plane = (size_t)N * H;
+  const float2* src = P + (size_t)ky * H + (size_t)tile * C;
+  constexpr int TOT = N * C;
+#pragma unroll
+  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) buf[(e / C) * IS + (e % C)] = src[(size_t)(e / C) * plane + (e % C)];
+  }
+  // partner column for the packed kx = 0 slot: Z(-ky, .) lives in row (N-ky)%N
+  const int kym = (N - ky) % N;
+  for (int z = tid; z < N; z += T)
+    buf[z * IS + C] = tile == 0 ? P[(size_t)z * plane + (size_t)kym * H] : make_float2(0.f, 0.f);
+  __syncthreads();
+  fft_lds<N, false, C + 1, T, 1, IS, true, N>(buf, tw, tid);
+
+  constexpr int PER = (TOT + T - 1) / T;
+  float2 ph[PER], pb[PER];
+  const int ny = freq_of(ky, N);
+  const float dy = gp.deconv ? gp.deconv[ky] : 1.0f;
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) {
+      const int i = e / C, c = e % C;
+      const int kx = tile * C + c;
+      const int nz = freq_of(i, N);
+      const float2 r = buf[i * IS + c];
+      float2 phi;
+      if (kx == 0) {
+        // packed DC / Nyquist planes: FP64 Green's factors
+        const double n2a = (double)(ny * ny + nz * nz);
+        const double n2b = n2a + (double)(N / 2) * (double)(N / 2);
+        double dd = 1.0, dn = 1.0;
+        if (gp.deconv) {
+          const double dyz = (double)dy * (double)gp.deconv[i];
+          dd = (double)gp.deconv[0] * dyz;
+          dn = (double)gp.deconv[N / 2] * dyz;
+        }
+        const double g0 = n2a == 0.0 ? 0.0 : -gp.c64 * exp(-gp.a64 * n2a) / n2a * dd;
+        const double gn = -gp.c64 * exp(-gp.a64 * n2b) / n2b * dn;
+        const float sp = (float)(0.5 * (g0 + gn)), sm = (float)(0.5 * (g0 - gn));
+        const float2 zm = cconj(buf[((N - i) % N) * IS + C]);
+        phi = make_float2(sp * r.x + sm * zm.x, sp * r.y + sm * zm.y);
+      } else {
+        const int n2 = kx * kx + ny * ny + nz * nz;
+        const float fn2 = (float)n2;
+        float g = -gp.c * expf(-gp.a * fn2) / fn2;
+        if (gp.deconv) g *= gp.deconv[kx] * dy * gp.deconv[i];
+        phi = make_float2(g * r.x, g * r.y);
+      }
+      const float nze = (i == N / 2) ? 0.0f : (float)nz;
+      ph[p] = phi;
+      pb[p] = make_float2(nze * phi.y, -nze * phi.x);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) {
+      const int i = e / C, c = e % C;
+      buf[i * IS + c] = ph[p];
+      buf[i * IS + C + 1 + c] = pb[p];
+    }
+  }
+  __syncthreads();
+  fft_lds<N, true, 2 * C + 1, T, 1, IS, true, N>(buf, tw, tid);
+  float2* da = A + (size_t)ky * H + (size_t)tile * C;
+  float2* db = B + (size_t)ky * H + (size_t)tile * C;
+#pragma unroll
+  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) {
+      const int i = e / C, c = e % C;
+      da[(size_t)i * plane + c] = buf[i * IS + c];
+      db[(size_t)i * plane + c] = buf[i * IS + C + 1 + c];
+    }
+  }
+}
+
+template <int N, int C>
+__global__ __launch_bounds__(256) void k_fft_y_inv(float2* __restrict__ A, float2* __restrict__ B,
+                                                   float2* __restrict__ P, const float2* __restrict__ tw) {
+  constexpr int H = N / 2, T = 256, IS = 2 * C + 1;
+  __shared__ float2 buf[N * IS];
+  const int tid = threadIdx.x;
+  const size_t off = (size_t)blockIdx.y * N * H + (size_t)blockIdx.x * C;
+  float2* a = A + off;
+  float2* b = B + off;
+  float2* pp = P + off;
+  constexpr int TOT = N * C;
+  constexpr int PER = (TOT + T - 1) / T;
+  float2 av[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) {
+      const int y = e / C, c = e % C;
+      av[p] = a[(size_t)y * H + c];
+      buf[y * IS + c] = av[p];
+      buf[y * IS + C + c] = b[(size_t)y * H + c];
+    }
+  }
+  __syncthreads();
+  fft_lds<N, true, 2 * C, T, 1, IS, true, N>(buf, tw, tid);
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) {
+      const int y = e / C, c = e % C;
+      a[(size_t)y * H + c] = buf[y * IS + c];
+      b[(size_t)y * H + c] = buf[y * IS + C + c];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) {
+      const int y = e / C, c = e % C;
+      const float nye = (y == N / 2) ? 0.0f : (float)freq_of(y, N);
+      buf[y * IS + c] = make_float2(nye * av[p].y, -nye * av[p].x);
+    }
+  }
+  __syncthreads();
+  fft_lds<N, true, C, T, 1, IS, true, N>(buf, tw, tid);
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) {
+      const int y = e / C, c = e % C;
+      pp[(size_t)y * H + c] = buf[y * IS + c];
+    }
+  }
+}
+
+// CIC gather of (Fx, Fy, Fz) at pos[idx[j]] (idx null: j itself).
+__global__ __launch_bounds__(256) void k_interp(const float* __restrict__ x, const float* __restrict__ y,
+                                                const float* __restrict__ z, const long long* __restrict__ idx,
+                                                long long k, const CellMap cm, const float* __restrict__ fx,
+                                                const float* __restrict__ fy, const float* __restrict__ fz,
+                                                float* __restrict__ ax, float* __restrict__ ay,
+                                                float* __restrict__ az) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  const long long i = idx ? idx[j] : j;
+  int ix, iy, iz;
+  float dx, dy, dz;
+  cell_coord(x[i], cm, ix, dx);
+  cell_coord(y[i], cm, iy, dy);
+  cell_coord(z[i], cm, iz, dz);
+  const int nm = cm.nm;
+  const float wx[2] = {1.0f - dx, dx}, wy[2] = {1.0f - dy, dy}, wz[2] = {1.0f - dz, dz};
+  const int xs[2] = {ix, ix + 1 == nm ? 0 : ix + 1};
+  const int ys[2] = {iy, iy + 1 == nm ? 0 : iy + 1};
+  const int zs[2] = {iz, iz + 1 == nm ? 0 : iz + 1};
+  const size_t nm2 = (size_t)nm * nm;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const float w = wx[c] * wy[b] * wz[a];
+        const size_t q = zs[a] * nm2 + (size_t)ys[b] * nm + xs[c];
+        sx = fmaf(fx[q], w, sx);
+        sy = fmaf(fy[q], w, sy);
+        sz = fmaf(fz[q], w, sz);
+      }
+  ax[j] = sx;
+  ay[j] = sy;
+  az[j] = sz;
+}
+
+}  // namespace gng

@@ -1,0 +1,405 @@
+"""gadget_ng_pm_hip -- Python host mirror of the MI355X PM solver's C ABI.
+
+The product is ``libgng_pm.so`` (HIP kernels for gfx950 behind the C ABI in
+``include/gng_pm.h``).  This module binds that ABI with ctypes and mirrors the
+reference's solver interface for this path so callers and tests read like the
+reference's own:
+
+* :class:`PmSolver` -- ``gadget_ng_pm::PmSolver`` (crates/gadget-ng-pm/src/solver.rs:23-140):
+  ``PmSolver(grid_size, box_size)``, public ``plummer_eps``, and
+  ``accelerations_for_indices(global_positions, global_masses, eps2, g,
+  global_indices, out)`` from the ``GravitySolver`` trait
+  (crates/gadget-ng-core/src/gravity.rs:14-24).
+* :class:`HipPmSolver` -- ``gadget_ng_hip::HipPmSolver``
+  (crates/gadget-ng-hip/src/pm_solver.rs:17-258): ``availability``,
+  ``is_available``, ``try_new[_checked]``, ``try_new_with_r_split[_checked]``,
+  ``try_accelerations_for_indices`` (raises :class:`HipExecutionError`) and
+  ``accelerations_for_indices`` (raises ``RuntimeError`` like the Rust panic).
+* :class:`TreePmLongRange` -- the long-range half of ``TreePmSolver``
+  (crates/gadget-ng-treepm/src/solver.rs:44-90): r_s = r_split or 2.5 cells.
+
+There is no CPU fallback: if the library is missing or no GPU is present the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+_ROOT = os.path.dirname(_PKG_DIR)  # gadget-ng_amd/
+LIB_PATH = os.path.join(_ROOT, "lib", "libgng_pm.so")
+CSRC = os.path.join(_ROOT, "csrc")
+REPO_ROOT = os.path.dirname(_ROOT)
+HEADER = os.path.join(REPO_ROOT, "include", "gng_pm.h")
+
+GNG_OK = 0
+GNG_ERR_INVALID = 1
+GNG_ERR_HIP = 2
+GNG_ERR_NO_DEVICE = 3
+GNG_ERR_OOM = 4
+GNG_ERR_UNSUPPORTED = 5
+
+STAGES = ("mass_scan_clear", "deposit", "fft_x_fwd", "fft_y_fwd", "fft_z_green", "fft_y_inv", "fft_x_inv",
+          "interp")
+
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def build(verbose: bool = False) -> str:
+    """Compile libgng_pm.so for gfx950 in-tree (always recompiles)."""
+    os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = LIB_PATH + ".tmp"
+    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, os.path.join(CSRC, "gng_pm.hip")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    if verbose and r.stderr:
+        print(r.stderr)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+class GngConfig(ctypes.Structure):
+    _fields_ = [("grid_size", ctypes.c_int32), ("deconvolve", ctypes.c_int32), ("box_size", ctypes.c_double),
+                ("r_split", ctypes.c_double), ("plummer_eps", ctypes.c_double), ("device", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 7)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libgng_pm.so (raises if it was never built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; "
+                               f"g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i64 = ctypes.c_int64
+        f32 = ctypes.c_float
+        d = ctypes.c_double
+        i32 = ctypes.c_int
+        L.gng_pm_abi_version.restype = i32
+        L.gng_last_error_string.restype = ctypes.c_char_p
+        L.gng_pm_mesh_supported.argtypes = [i32]
+        L.gng_pm_mesh_supported.restype = i32
+        L.gng_pm_device_count.restype = i32
+        L.gng_pm_create.argtypes = [ctypes.POINTER(GngConfig), ctypes.POINTER(P)]
+        L.gng_pm_create.restype = i32
+        L.gng_pm_destroy.argtypes = [P]
+        L.gng_pm_destroy.restype = None
+        L.gng_pm_accelerations_f64.argtypes = [P, P, P, i64, P, i64, d, d, P]
+        L.gng_pm_accelerations_f32.argtypes = [P, P, P, P, P, i64, P, i64, d, P, P, P]
+        L.gng_pm_solve_device.argtypes = [P, P, P, P, P, i64, P, i64, d, P, P, P, P]
+        L.gng_pm_density.argtypes = [P, P]
+        L.gng_pm_density_fixed.argtypes = [P, P, ctypes.POINTER(ctypes.c_int32)]
+        L.gng_pm_force_meshes.argtypes = [P, P]
+        L.gng_pm_set_timing.argtypes = [P, i32]
+        L.gng_pm_stage_times.argtypes = [P, P, i32]
+        for f in (L.gng_pm_accelerations_f64, L.gng_pm_accelerations_f32, L.gng_pm_solve_device, L.gng_pm_density,
+                  L.gng_pm_density_fixed, L.gng_pm_force_meshes, L.gng_pm_set_timing, L.gng_pm_stage_times):
+            f.restype = i32
+        L.hip_pm_create.argtypes = [i32, f32]
+        L.hip_pm_create.restype = P
+        L.hip_pm_destroy.argtypes = [P]
+        L.hip_pm_destroy.restype = None
+        L.hip_pm_solve.argtypes = [P, P, P, P, P, P, P, P, i32, f32, f32, f32]
+        L.hip_pm_solve.restype = i32
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().gng_last_error_string().decode(errors="replace")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class HipExecutionError(RuntimeError):
+    """A solve returned a nonzero code (crates/gadget-ng-hip/src/availability.rs)."""
+
+    def __init__(self, kernel: str, code: int, message: str = ""):
+        super().__init__(f"{kernel} failed with code {code}: {message}")
+        self.kernel = kernel
+        self.code = code
+
+
+@dataclass
+class HipAvailability:
+    """Build / runtime availability, as HipAvailability (availability.rs:7-40)."""
+    compiled: bool
+    runtime_available: bool
+    reason: str
+
+    def is_available(self) -> bool:
+        return self.compiled and self.runtime_available
+
+
+class HipUnavailable(RuntimeError):
+    def __init__(self, availability: HipAvailability):
+        super().__init__(f"HIP PM solver unavailable: {availability.reason}")
+        self.availability = availability
+
+
+class Plan:
+    """Owns one gng_pm_plan (device meshes for one mesh size and filter set)."""
+
+    def __init__(self, grid_size: int, box_size: float, r_split: float = 0.0, plummer_eps: float = 0.0,
+                 deconvolve: bool = False, device: int = -1):
+        cfg = GngConfig()
+        cfg.grid_size = int(grid_size)
+        cfg.deconvolve = int(bool(deconvolve))
+        cfg.box_size = float(box_size)
+        cfg.r_split = float(r_split)
+        cfg.plummer_eps = float(plummer_eps)
+        cfg.device = int(device)
+        h = ctypes.c_void_p()
+        rc = lib().gng_pm_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_pm_create", rc, last_error())
+        self._h = h
+        self.grid_size = int(grid_size)
+        self.box_size = float(box_size)
+        self.r_split = float(r_split)
+        self.plummer_eps = float(plummer_eps)
+        self.deconvolve = bool(deconvolve)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().gng_pm_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != GNG_OK:
+            raise HipExecutionError(what, rc, last_error())
+
+    def accelerations_f64(self, pos: np.ndarray, mass: np.ndarray, g: float, idx=None,
+                          eps2: float = 0.0) -> np.ndarray:
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        n = pos.shape[0]
+        if mass.shape[0] != n:
+            raise ValueError("positions and masses differ in length")
+        if idx is None:
+            k, ip = n, None
+        else:
+            idx = np.ascontiguousarray(idx, dtype=np.int64)
+            k, ip = idx.shape[0], _ptr(idx)
+        out = np.empty((k, 3), dtype=np.float64)
+        self._check(lib().gng_pm_accelerations_f64(self._h, _ptr(pos), _ptr(mass), n, ip, k, float(eps2), float(g),
+                                                   _ptr(out)), "gng_pm_accelerations_f64")
+        return out
+
+    def accelerations_f32(self, x, y, z, m, g: float, idx=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, z, m)]
+        n = arrs[0].shape[0]
+        if idx is None:
+            k, ip = n, None
+        else:
+            idx = np.ascontiguousarray(idx, dtype=np.int64)
+            k, ip = idx.shape[0], _ptr(idx)
+        out = np.empty((3, k), dtype=np.float32)
+        self._check(lib().gng_pm_accelerations_f32(self._h, *(_ptr(a) for a in arrs), n, ip, k, float(g),
+                                                   _ptr(out[0]), _ptr(out[1]), _ptr(out[2])),
+                    "gng_pm_accelerations_f32")
+        return out
+
+    def solve_device(self, x_ptr: int, y_ptr: int, z_ptr: int, m_ptr: int, n: int, ax_ptr: int, ay_ptr: int,
+                     az_ptr: int, g: float, stream: int = 0, idx_ptr: Optional[int] = None,
+                     k: Optional[int] = None):
+        """Enqueue a solve on raw device pointers (e.g. torch tensors' data_ptr())."""
+        k = n if k is None else k
+        self._check(lib().gng_pm_solve_device(self._h, x_ptr, y_ptr, z_ptr, m_ptr, n, idx_ptr, k, float(g), ax_ptr,
+                                              ay_ptr, az_ptr, stream or None), "gng_pm_solve_device")
+
+    def density(self) -> np.ndarray:
+        nm = self.grid_size
+        out = np.empty((nm, nm, nm), dtype=np.float64)
+        self._check(lib().gng_pm_density(self._h, _ptr(out)), "gng_pm_density")
+        return out
+
+    def density_fixed(self):
+        nm = self.grid_size
+        out = np.empty((nm, nm, nm), dtype=np.int64)
+        s = ctypes.c_int32()
+        self._check(lib().gng_pm_density_fixed(self._h, _ptr(out), ctypes.byref(s)), "gng_pm_density_fixed")
+        return out, int(s.value)
+
+    def force_meshes(self) -> np.ndarray:
+        """(3, nm, nm, nm) f32 force meshes of the last solve."""
+        nm = self.grid_size
+        out = np.empty((3, nm, nm, nm), dtype=np.float32)
+        self._check(lib().gng_pm_force_meshes(self._h, _ptr(out)), "gng_pm_force_meshes")
+        return out
+
+    def set_timing(self, on: bool = True):
+        self._check(lib().gng_pm_set_timing(self._h, int(on)), "gng_pm_set_timing")
+
+    def stage_times(self) -> dict:
+        ms = np.zeros(16, dtype=np.float32)
+        n = lib().gng_pm_stage_times(self._h, _ptr(ms), 16)
+        return {STAGES[i]: float(ms[i]) for i in range(min(n, len(STAGES)))}
+
+
+def _as_positions(global_positions) -> np.ndarray:
+    return np.ascontiguousarray(global_positions, dtype=np.float64).reshape(-1, 3)
+
+
+@dataclass
+class PmSolver:
+    """Drop-in for gadget_ng_pm::PmSolver on one GPU (solver.rs:23-58).
+
+    ``modified_gravity`` (f(R)) is not part of this solver; setting it raises.
+    ``deconvolve`` is an extension (False = reference semantics).
+    """
+    grid_size: int
+    box_size: float
+    plummer_eps: Optional[float] = None
+    deconvolve: bool = False
+    modified_gravity: Optional[object] = None
+    _plans: dict = field(default_factory=dict, repr=False)
+
+    def _plan(self, r_split: float = 0.0) -> Plan:
+        if self.modified_gravity is not None:
+            raise NotImplementedError("f(R) modified gravity is not implemented by the GPU PM solver")
+        eps = self.plummer_eps if (self.plummer_eps is not None and self.plummer_eps > 0.0) else 0.0
+        key = (self.grid_size, self.box_size, float(r_split), eps, self.deconvolve)
+        p = self._plans.get(key)
+        if p is None:
+            p = Plan(self.grid_size, self.box_size, r_split=r_split, plummer_eps=eps, deconvolve=self.deconvolve)
+            self._plans[key] = p
+        return p
+
+    def accelerations_for_indices(self, global_positions, global_masses, eps2: float, g: float, global_indices,
+                                  out: Optional[np.ndarray] = None) -> np.ndarray:
+        """GravitySolver::accelerations_for_indices: density from all N, output for the K indices."""
+        idx = np.asarray(global_indices, dtype=np.int64)
+        if out is not None and out.shape[0] != idx.shape[0]:
+            raise AssertionError("out and global_indices differ in length")  # solver.rs:83
+        res = self._plan().accelerations_f64(_as_positions(global_positions), global_masses, g, idx, eps2)
+        if out is not None:
+            out[...] = res
+            return out
+        return res
+
+
+@dataclass
+class TreePmLongRange:
+    """Long-range half of gadget_ng_treepm::TreePmSolver (treepm/src/solver.rs:44-90)."""
+    grid_size: int
+    box_size: float
+    r_split: float = 0.0
+    _pm: Optional[PmSolver] = field(default=None, repr=False)
+
+    def effective_r_split(self) -> float:
+        """treepm/src/solver.rs:56-62: r_split > 0 or 2.5 cells."""
+        return self.r_split if self.r_split > 0.0 else 2.5 * self.box_size / self.grid_size
+
+    def long_range_accelerations_for_indices(self, global_positions, global_masses, eps2: float, g: float,
+                                             global_indices) -> np.ndarray:
+        if self._pm is None:
+            self._pm = PmSolver(self.grid_size, self.box_size)
+        idx = np.asarray(global_indices, dtype=np.int64)
+        return self._pm._plan(self.effective_r_split()).accelerations_f64(
+            _as_positions(global_positions), global_masses, g, idx, eps2)
+
+
+class HipPmSolver:
+    """Mirror of gadget_ng_hip::HipPmSolver (crates/gadget-ng-hip/src/pm_solver.rs:17-258)."""
+
+    def __init__(self, grid_size: int, box_size: float, r_split: float, plan: Plan):
+        self._grid_size = int(grid_size)
+        self._r_split = float(r_split)
+        self._plan = plan
+
+    @staticmethod
+    def availability() -> HipAvailability:
+        if not os.path.exists(LIB_PATH):
+            return HipAvailability(False, False, "libgng_pm.so not built")
+        try:
+            L = lib()
+        except OSError as e:
+            return HipAvailability(False, False, f"cannot load libgng_pm.so: {e}")
+        if L.gng_pm_device_count() <= 0:
+            return HipAvailability(True, False, "no HIP device")
+        h = L.hip_pm_create(8, ctypes.c_float(1.0))
+        if not h:
+            return HipAvailability(True, False, "could not create a runtime handle: " + last_error())
+        L.hip_pm_destroy(h)
+        return HipAvailability(True, True, "ok")
+
+    @classmethod
+    def is_available(cls) -> bool:
+        return cls.availability().is_available()
+
+    @classmethod
+    def try_new(cls, grid_size: int, box_size: float):
+        try:
+            return cls.try_new_checked(grid_size, box_size)
+        except HipUnavailable:
+            return None
+
+    @classmethod
+    def try_new_checked(cls, grid_size: int, box_size: float):
+        return cls.try_new_with_r_split_checked(grid_size, box_size, 0.0)
+
+    @classmethod
+    def try_new_with_r_split(cls, grid_size: int, box_size: float, r_split: float):
+        try:
+            return cls.try_new_with_r_split_checked(grid_size, box_size, r_split)
+        except HipUnavailable:
+            return None
+
+    @classmethod
+    def try_new_with_r_split_checked(cls, grid_size: int, box_size: float, r_split: float):
+        av = cls.availability()
+        if not av.is_available():
+            raise HipUnavailable(av)
+        try:
+            plan = Plan(grid_size, box_size, r_split=max(float(r_split), 0.0))
+        except HipExecutionError as e:
+            raise HipUnavailable(HipAvailability(True, False, str(e))) from e
+        return cls(grid_size, box_size, r_split, plan)
+
+    def r_split(self) -> float:
+        return self._r_split
+
+    def grid_size(self) -> int:
+        return self._grid_size
+
+    def try_accelerations_for_indices(self, global_positions, global_masses, eps2: float, g: float,
+                                      global_indices, out: np.ndarray) -> None:
+        idx = np.asarray(global_indices, dtype=np.int64)
+        assert idx.shape[0] == out.shape[0]
+        if idx.shape[0] == 0:
+            return
+        out[...] = self._plan.accelerations_f64(_as_positions(global_positions), global_masses, g, idx, eps2)
+
+    def accelerations_for_indices(self, global_positions, global_masses, eps2: float, g: float, global_indices,
+                                  out: np.ndarray) -> None:
+        try:
+            self.try_accelerations_for_indices(global_positions, global_masses, eps2, g, global_indices, out)
+        except HipExecutionError as e:  # the Rust impl panics (pm_solver.rs:255)
+            raise RuntimeError(f"HipPmSolver::accelerations_for_indices failed: {e}") from e

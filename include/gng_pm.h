@@ -1,0 +1,149 @@
+/*
+ * gng_pm.h -- C ABI of the MI355X-native periodic PM gravity solver.
+ *
+ * One shared library, libgng_pm.so, built for gfx950 only.  All pointers are
+ * plain C pointers; no HIP or torch types appear in any signature (streams
+ * are passed as void*).
+ *
+ * Two families of entry points:
+ *
+ *  1. hip_pm_*   -- binary drop-ins for the symbols the reference's Rust crate
+ *                   links against (crates/gadget-ng-hip/src/ffi.rs:14-33,
+ *                   declared in crates/gadget-ng-hip/hip/pm_gravity.h:24-51),
+ *                   so HipPmSolver links unchanged.
+ *  2. gng_pm_*   -- the versioned API: configuration struct, integer error
+ *                   codes with gng_last_error_string(), an f64 Vec3 entry point
+ *                   that matches GravitySolver::accelerations_for_indices
+ *                   (crates/gadget-ng-core/src/gravity.rs:14-24), a device-
+ *                   resident entry point on a caller stream, the density mesh
+ *                   readback and per-stage timings.
+ *
+ * Semantics common to every solve (reference: crates/gadget-ng-pm/src/solver.rs:74-140):
+ *   - the density uses all n particles; accelerations are produced for the
+ *     k requested indices (or all n);
+ *   - positions outside [0, L) are wrapped periodically (cic.rs:50-52);
+ *   - eps2 is accepted and ignored, as PmSolver ignores it (solver.rs:70-72);
+ *   - the output is linear in g;
+ *   - r_split > 0 applies the long-range filter exp(-k^2 r_split^2 / 2)
+ *     (fft_poisson.rs:87-103, TreePM long range: treepm/src/solver.rs:83-90);
+ *   - plummer_eps > 0 applies exp(-k^2 eps^2) (fft_poisson.rs:106-122);
+ *   - deconvolve != 0 divides the Green's function by W_CIC(k)^2,
+ *     W_CIC = prod_alpha sinc^2(pi n_alpha / nm) (an extension; off = reference);
+ *   - the CIC density is accumulated in 64-bit fixed point, so it is
+ *     bit-identical from run to run.
+ * A handle is not reentrant: calls on one handle must not overlap.
+ */
+#ifndef GNG_PM_H
+#define GNG_PM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GNG_PM_ABI_VERSION 1
+
+/* Error codes (0 = success). */
+enum gng_status {
+  GNG_OK = 0,
+  GNG_ERR_INVALID = 1,     /* bad argument or unsupported size            */
+  GNG_ERR_HIP = 2,         /* HIP runtime error (see gng_last_error_string) */
+  GNG_ERR_NO_DEVICE = 3,   /* no usable GPU                                 */
+  GNG_ERR_OOM = 4,         /* device allocation failed                      */
+  GNG_ERR_UNSUPPORTED = 5  /* mesh size not compiled in                     */
+};
+
+/* ---- 1. reference-compatible entry points -------------------------------- */
+
+/* Replaces hip_pm_create (crates/gadget-ng-hip/hip/pm_gravity.h:24,
+ * pm_gravity.hip:278-334).  NULL on any failure. */
+void *hip_pm_create(int grid_size, float box_size);
+
+/* Replaces hip_pm_destroy (pm_gravity.h:29).  NULL-safe. */
+void hip_pm_destroy(void *h);
+
+/* Replaces hip_pm_solve (pm_gravity.h:45-51, pm_gravity.hip:363-484).
+ * Host SoA f32 arrays of length n.  Synchronous.  n <= 0 or h == NULL is a
+ * no-op returning 0.  Returns 0, or a nonzero error code. */
+int hip_pm_solve(void *h, const float *x, const float *y, const float *z, const float *mass, float *ax,
+                 float *ay, float *az, int n, float eps2, float g, float r_split);
+
+/* ---- 2. versioned API ----------------------------------------------------- */
+
+typedef struct gng_pm_config {
+  int32_t grid_size;    /* mesh side nm; 2^a 3^b, 4 <= nm <= 1024 (see gng_pm_mesh_supported) */
+  int32_t deconvolve;   /* 0 = reference semantics                              */
+  double box_size;      /* L, same units as positions                           */
+  double r_split;       /* <= 0: none.  TreePM long-range split radius           */
+  double plummer_eps;   /* <= 0: none                                           */
+  int32_t device;       /* HIP device ordinal, -1 = current                     */
+  int32_t reserved[7];
+} gng_pm_config;
+
+typedef struct gng_pm_plan gng_pm_plan;
+
+int gng_pm_abi_version(void);
+
+/* Message of the last error on the calling thread ("" if none). */
+const char *gng_last_error_string(void);
+
+/* 1 if nm is compiled in, else 0. */
+int gng_pm_mesh_supported(int grid_size);
+
+/* Number of visible GPUs (0 when none, never an error). */
+int gng_pm_device_count(void);
+
+/* Allocates the meshes (about 3.5 * 4 * nm^3 bytes) and the tables. */
+int gng_pm_create(const gng_pm_config *cfg, gng_pm_plan **out);
+
+void gng_pm_destroy(gng_pm_plan *p);
+
+/* Mirrors GravitySolver::accelerations_for_indices
+ * (crates/gadget-ng-core/src/gravity.rs:16-24, PmSolver at
+ * crates/gadget-ng-pm/src/solver.rs:74-140): pos is n x 3 f64 interleaved
+ * (the layout of Vec3), mass n f64, idx k int64 indices into [0, n) or NULL
+ * (then k must equal n), out k x 3 f64.  Host memory; synchronous.  The
+ * device computes in FP32 from f32-rounded inputs. */
+int gng_pm_accelerations_f64(gng_pm_plan *p, const double *pos, const double *mass, int64_t n,
+                             const int64_t *idx, int64_t k, double eps2, double g, double *out);
+
+/* Same on host SoA f32 arrays (the hip_pm_solve data layout); idx may be NULL. */
+int gng_pm_accelerations_f32(gng_pm_plan *p, const float *x, const float *y, const float *z,
+                             const float *mass, int64_t n, const int64_t *idx, int64_t k, double g,
+                             float *ax, float *ay, float *az);
+
+/* Device-resident solve: every pointer is device memory (SoA f32, idx int64
+ * or NULL), enqueued on `stream` (a hipStream_t, NULL = default stream); the
+ * call returns without synchronising.  Inputs must stay valid until the
+ * stream reaches the end of the solve. */
+int gng_pm_solve_device(gng_pm_plan *p, const float *d_x, const float *d_y, const float *d_z,
+                        const float *d_mass, int64_t n, const int64_t *d_idx, int64_t k, double g,
+                        float *d_ax, float *d_ay, float *d_az, void *stream);
+
+/* Density of the last solve as mass per cell (nm^3 f64, flat iz*nm^2+iy*nm+ix,
+ * the output of cic::assign, crates/gadget-ng-pm/src/cic.rs:260-292).
+ * Synchronous. */
+int gng_pm_density(gng_pm_plan *p, double *out);
+
+/* Raw fixed-point density of the last solve (nm^3 int64) and its scale:
+ * mass = q * 2^-shift.  Bit-identical for identical inputs. */
+int gng_pm_density_fixed(gng_pm_plan *p, int64_t *out, int32_t *shift);
+
+/* Force meshes of the last solve (3 x nm^3 f32: F_x, F_y, F_z, flat
+ * iz*nm^2+iy*nm+ix), the output of fft_poisson::solve_forces
+ * (crates/gadget-ng-pm/src/fft_poisson.rs:66-122).  Synchronous. */
+int gng_pm_force_meshes(gng_pm_plan *p, float *out);
+
+/* Per-stage GPU times of the last solve in milliseconds, measured with HIP
+ * events when timing is enabled (gng_pm_set_timing).  Stages, in order:
+ * 0 mass scan + clear, 1 deposit, 2 x forward, 3 y forward, 4 z + Green,
+ * 5 y inverse, 6 x inverse, 7 interpolation.  Returns the stage count. */
+int gng_pm_set_timing(gng_pm_plan *p, int enable);
+int gng_pm_stage_times(gng_pm_plan *p, float *ms, int capacity);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* GNG_PM_H */
