@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the MI355X PM gravity solver.
+
+Workload: 256^3 particles from a Zel'dovich displacement (P(k) ~ k^-2 up to
+the mesh Nyquist, 3-D rms displacement 0.5 cell) of a node-aligned lattice,
+box L = 256 Mpc/h, 512^3 mesh, equal f32 masses, G = 1, no filter.  One step
+is one full force evaluation: mass scan, fixed-point CIC deposit, R2C FFT,
+fused Green's function x gradient, C2R FFTs, CIC interpolation, with the
+particle arrays already resident in HBM.
+
+Printed JSON line (stdout, one line):
+  value         whole-job M particle-forces/s (N * steps / timed wall time)
+  roofline      dominant kernel's algorithmic bytes / its mean event-timed
+                duration inside the timed region, against 8 TB/s
+  cpu_baseline  the FP64 CPU oracle (lean R2C mode, all host cores) on the
+                same f32 inputs
+  rms_force_err / max_force_err  vs that FP64 result
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gadget-ng_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stage_bytes(n: int, nm: int) -> dict:
+    """Algorithmic bytes of each stage of one evaluation (packed half spectrum:
+    one complex field = nm^3/2 complex = 4 nm^3 bytes)."""
+    m = nm ** 3
+    return {
+        "mass_scan_clear": 4 * n,
+        "deposit": 16 * n + 4 * m,          # x, y, z, m in; f32 density out
+        "fft_x_fwd": 4 * m + 4 * m,         # density in, spectrum out
+        "fft_y_fwd": 8 * m,                 # spectrum in and out
+        "fft_z_green": 4 * m + 8 * m,       # spectrum in, A and B out
+        "fft_y_inv": 8 * m + 12 * m,        # A, B in; three fields out
+        "fft_x_inv": 12 * m + 12 * m,       # three fields in, three force meshes out
+        "interp": 12 * n + 12 * m + 12 * n,  # positions, force meshes in; accelerations out
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n-side", type=int, default=256)
+    ap.add_argument("--mesh", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the FP64 CPU oracle (and the error check)")
+    ap.add_argument("--cpu-runs", type=int, default=2, help="timed CPU oracle runs after one warm-up")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this PM solver has no multi-GPU path "
+              "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
+        return 2
+    if not 1 <= args.steps <= 64:
+        print("--steps must be in [1, 64] (the stage-timing ring holds 64 solves)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import gadget_ng_pm_hip as pm
+    from gadget_ng_pm_hip import workloads
+
+    nm, n_side = args.mesh, args.n_side
+    box = float(n_side)  # 1 Mpc/h lattice spacing
+    t0 = time.perf_counter()
+    parts = workloads.zeldovich_lattice(n_side=n_side, nm=nm, box=box, spectral_index=-2.0, rms_disp_cells=0.5,
+                                        seed=workloads.SEED_ZELDOVICH, workers=-1)
+    n = parts.n
+    log(f"[bench] workload generated in {time.perf_counter() - t0:.1f}s: {parts.name}, N={n}")
+
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(parts.x).to(dev)
+    y = torch.from_numpy(parts.y).to(dev)
+    z = torch.from_numpy(parts.z).to(dev)
+    m = torch.from_numpy(parts.m).to(dev)
+    ax = torch.empty(n, dtype=torch.float32, device=dev)
+    ay = torch.empty_like(ax)
+    az = torch.empty_like(ax)
+    plan = pm.Plan(nm, box, device=0)
+    stream = torch.cuda.current_stream(dev)
+    sptr = stream.cuda_stream
+
+    def step():
+        plan.solve_device(x.data_ptr(), y.data_ptr(), z.data_ptr(), m.data_ptr(), n, ax.data_ptr(), ay.data_ptr(),
+                          az.data_ptr(), 1.0, stream=sptr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    plan.set_timing(True)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t_start = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    gpu_ms = ev0.elapsed_time(ev1) / args.steps
+    stages = plan.stage_times()
+    plan.set_timing(False)
+    ms_per_step = wall * 1e3 / args.steps
+    value = n * args.steps / wall / 1e6
+    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step wall, {gpu_ms:.3f} ms/step GPU events, "
+        f"{value:.1f} M particle-forces/s")
+    log("[bench] stage ms: " + ", ".join(f"{k}={v:.4f}" for k, v in stages.items()))
+
+    sb = stage_bytes(n, nm)
+    dom = max(stages, key=stages.get)
+    dom_ms = stages[dom]
+    achieved = sb[dom] / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "algorithmic_bytes": sb[dom], "mean_ms": round(dom_ms, 5)}
+    b_alg = workloads.algorithmic_bytes(n, nm)
+    pipeline_gbps = b_alg / (gpu_ms * 1e-3) / 1e9
+
+    # drop-in rate through the host C ABI (H2D + D2H of f32 SoA included)
+    host_out = None
+    t_e2e = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        host_out = plan.accelerations_f32(parts.x, parts.y, parts.z, parts.m, 1.0)
+        t_e2e.append(time.perf_counter() - t1)
+    e2e_ms = statistics.median(t_e2e) * 1e3
+
+    acc = torch.stack([ax, ay, az], dim=1).cpu().numpy().astype(np.float64)
+    result = {
+        "metric": "M particle-forces/s + HBM GB/s at 256^3 part/512^3 mesh; rms force err vs FP64",
+        "value": round(value, 2),
+        "unit": "M particle-forces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"Zel'dovich lattice {n_side}^3 particles on {nm}^3 mesh, L={box:g} Mpc/h, "
+                               f"P(k)~k^-2, rms displacement 0.5 cell, seed 0x6E670002, G=1, no filter",
+                   "particles": n, "mesh": nm, "box": box, "precision": "FP32 mesh/FFT/interp, int64 deposit"},
+        "roofline": roofline,
+        "hbm_gbps": round(pipeline_gbps, 1),
+        "hbm_frac": round(pipeline_gbps / HBM_PEAK_GBPS, 4),
+        "algorithmic_bytes": b_alg,
+        "gpu_ms_per_step": round(gpu_ms, 4),
+        "stage_ms": {k: round(v, 5) for k, v in stages.items()},
+        "pcie_inclusive_ms": round(e2e_ms, 3),
+        "cpu_baseline": None,
+        "rms_force_err": None,
+        "max_force_err": None,
+    }
+
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # CPU baseline and checker only
+
+        oracle.build()
+        pos = parts.pos64()
+        mass = parts.mass64()
+        cores = oracle.get_threads()
+        t1 = time.perf_counter()
+        ref = oracle.pm_accelerations(pos, mass, box, nm, 1.0, mode=oracle.MODE_LEAN)
+        warm = time.perf_counter() - t1
+        times = []
+        for _ in range(args.cpu_runs):
+            t1 = time.perf_counter()
+            oracle.pm_accelerations(pos, mass, box, nm, 1.0, mode=oracle.MODE_LEAN)
+            times.append(time.perf_counter() - t1)
+        tc = statistics.median(times) if times else warm
+        result["cpu_baseline"] = {
+            "value": round(n / tc / 1e6, 3), "unit": "M particle-forces/s", "cores": cores, "kind": "port",
+            "sample": f"the full workload ({n} particles, {nm}^3 mesh), FP64 oracle lean R2C mode, "
+                      f"median of {len(times)} timed runs after 1 warm-up ({tc:.2f} s each)"}
+        d = acc - ref
+        nr = np.linalg.norm(ref, axis=1)
+        floor = 0.05 * nr.mean()
+        rel = np.linalg.norm(d, axis=1) / np.maximum(np.maximum(nr, floor), 1e-300)
+        result["rms_force_err"] = float(f"{np.sqrt((d ** 2).sum() / (ref ** 2).sum()):.4e}")
+        result["max_force_err"] = float(f"{rel.max():.4e}")
+        result["p99_force_err"] = float(f"{np.percentile(rel, 99):.4e}")
+        dh = host_out.T.astype(np.float64) - ref
+        result["rms_force_err_host_path"] = float(f"{np.sqrt((dh ** 2).sum() / (ref ** 2).sum()):.4e}")
+        log(f"[bench] CPU oracle {tc:.2f}s on {cores} threads; rms err {result['rms_force_err']:.3e}, "
+            f"max {result['max_force_err']:.3e}")
+
+    print(json.dumps(result), flush=True)
+    plan.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -38,6 +38,7 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kStages = 8;
+constexpr int kTimingSlots = 64;
 
 // ---------------------------------------------------------------------------
 // per-mesh-size kernel table
@@ -136,9 +137,11 @@ struct gng_pm_plan {
   hipStream_t stream = nullptr;
 
   long long last_n = 0;
+  // Stage timing: every solve records kStages+1 events into the next slot of
+  // a ring; nothing synchronises until gng_pm_stage_times() reads them.
   bool timing = false;
-  hipEvent_t ev[kStages + 1] = {};
-  float stage_ms[kStages] = {};
+  long long timed_solves = 0;
+  hipEvent_t ev[kTimingSlots][kStages + 1] = {};
 };
 
 namespace {
@@ -160,9 +163,10 @@ void free_plan(gng_pm_plan* p) {
   hipFree(p->d_sc);
   hipFree(p->d_part);
   hipFree(p->d_idx);
-  for (auto& e : p->ev)
-    if (e) hipEventDestroy(e);
-  if (p->stream) hipStreamDestroy(p->stream);
+  for (auto& slot : p->ev)
+    for (auto& e : slot)
+      if (e) (void)hipEventDestroy(e);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
 
@@ -191,7 +195,7 @@ GreenParams green_params(const gng_pm_plan* p, double g) {
 }
 
 inline void mark(gng_pm_plan* p, int i, hipStream_t s) {
-  if (p->timing) hipEventRecord(p->ev[i], s);
+  if (p->timing) (void)hipEventRecord(p->ev[p->timed_solves % kTimingSlots][i], s);
 }
 
 // The whole pipeline on device pointers.
@@ -239,14 +243,26 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
                        reinterpret_cast<const float*>(p->d_B), ax, ay, az);
   mark(p, 8, s);
   GNG_HIP(hipGetLastError());
+  if (p->timing) ++p->timed_solves;
   p->last_n = n;
   return GNG_OK;
 }
 
-int collect_timing(gng_pm_plan* p) {
-  if (!p->timing) return GNG_OK;
-  GNG_HIP(hipEventSynchronize(p->ev[kStages]));
-  for (int i = 0; i < kStages; ++i) GNG_HIP(hipEventElapsedTime(&p->stage_ms[i], p->ev[i], p->ev[i + 1]));
+// Average per-stage time over the solves still held in the timing ring.
+int read_timing(gng_pm_plan* p, float* ms) {
+  const long long held = p->timed_solves < kTimingSlots ? p->timed_solves : kTimingSlots;
+  for (int i = 0; i < kStages; ++i) ms[i] = 0.0f;
+  if (held == 0) return GNG_OK;
+  for (long long j = 0; j < held; ++j) {
+    hipEvent_t* ev = p->ev[(p->timed_solves - 1 - j) % kTimingSlots];
+    GNG_HIP(hipEventSynchronize(ev[kStages]));
+    for (int i = 0; i < kStages; ++i) {
+      float t = 0.0f;
+      GNG_HIP(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+      ms[i] += t;
+    }
+  }
+  for (int i = 0; i < kStages; ++i) ms[i] /= (float)held;
   return GNG_OK;
 }
 
@@ -308,7 +324,7 @@ int solve_host_soa(gng_pm_plan* p, const float* x, const float* y, const float* 
     GNG_HIP(hipMemcpyAsync(az, daz, kb, hipMemcpyDeviceToHost, s));
   }
   GNG_HIP(hipStreamSynchronize(s));
-  return collect_timing(p);
+  return GNG_OK;
 }
 
 }  // namespace
@@ -388,7 +404,8 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
   hipError_t e = hipMemcpy(p->d_tw, tw.data(), sizeof(float2) * nm, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_deconv, dc.data(), sizeof(float) * nm, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
-  for (int i = 0; i <= kStages && e == hipSuccess; ++i) e = hipEventCreate(&p->ev[i]);
+  for (int j = 0; j < kTimingSlots && e == hipSuccess; ++j)
+    for (int i = 0; i <= kStages && e == hipSuccess; ++i) e = hipEventCreate(&p->ev[j][i]);
   if (e != hipSuccess) return bail(fail(GNG_ERR_HIP, std::string("plan setup: ") + hipGetErrorString(e)));
   *out = p;
   return GNG_OK;
@@ -441,8 +458,7 @@ int gng_pm_solve_device(gng_pm_plan* p, const float* d_x, const float* d_y, cons
   if ((rc = set_device(p))) return rc;
   rc = run_pipeline(p, d_x, d_y, d_z, d_mass, n, reinterpret_cast<const long long*>(d_idx), k, g, d_ax, d_ay, d_az,
                     reinterpret_cast<hipStream_t>(stream));
-  if (rc) return rc;
-  return collect_timing(p);
+  return rc;
 }
 
 int gng_pm_density_fixed(gng_pm_plan* p, int64_t* out, int32_t* shift) {
@@ -486,13 +502,20 @@ int gng_pm_force_meshes(gng_pm_plan* p, float* out) {
 
 int gng_pm_set_timing(gng_pm_plan* p, int enable) {
   if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  std::lock_guard<std::mutex> lk(p->mu);
   p->timing = enable != 0;
+  p->timed_solves = 0;
   return GNG_OK;
 }
 
 int gng_pm_stage_times(gng_pm_plan* p, float* ms, int capacity) {
-  if (!p) return fail(GNG_ERR_INVALID, "null plan");
-  for (int i = 0; i < kStages && i < capacity; ++i) ms[i] = p->stage_ms[i];
+  if (!p || (!ms && capacity > 0)) return fail(GNG_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  int rc = set_device(p);
+  if (rc) return rc;
+  float t[kStages];
+  if ((rc = read_timing(p, t))) return -rc;
+  for (int i = 0; i < kStages && i < capacity; ++i) ms[i] = t[i];
   return kStages;
 }
 

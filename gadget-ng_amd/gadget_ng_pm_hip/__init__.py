@@ -258,8 +258,11 @@ class Plan:
         self._check(lib().gng_pm_set_timing(self._h, int(on)), "gng_pm_set_timing")
 
     def stage_times(self) -> dict:
+        """Mean per-stage milliseconds over the solves recorded since set_timing(True)."""
         ms = np.zeros(16, dtype=np.float32)
         n = lib().gng_pm_stage_times(self._h, _ptr(ms), 16)
+        if n < 0:
+            raise HipExecutionError("gng_pm_stage_times", -n, last_error())
         return {STAGES[i]: float(ms[i]) for i in range(min(n, len(STAGES)))}
 
 

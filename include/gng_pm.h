@@ -135,10 +135,13 @@ int gng_pm_density_fixed(gng_pm_plan *p, int64_t *out, int32_t *shift);
  * (crates/gadget-ng-pm/src/fft_poisson.rs:66-122).  Synchronous. */
 int gng_pm_force_meshes(gng_pm_plan *p, float *out);
 
-/* Per-stage GPU times of the last solve in milliseconds, measured with HIP
- * events when timing is enabled (gng_pm_set_timing).  Stages, in order:
- * 0 mass scan + clear, 1 deposit, 2 x forward, 3 y forward, 4 z + Green,
- * 5 y inverse, 6 x inverse, 7 interpolation.  Returns the stage count. */
+/* Per-stage GPU times in milliseconds, from HIP events recorded on the solve's
+ * stream between the stages of every solve issued while timing is enabled
+ * (gng_pm_set_timing resets the record), averaged over the most recent 64
+ * such solves.  Recording does not synchronise; gng_pm_stage_times waits for
+ * the recorded events.  Stages, in order: 0 mass scan + clear, 1 deposit,
+ * 2 x forward, 3 y forward, 4 z + Green, 5 y inverse, 6 x inverse,
+ * 7 interpolation.  Returns the stage count (negative on error). */
 int gng_pm_set_timing(gng_pm_plan *p, int enable);
 int gng_pm_stage_times(gng_pm_plan *p, float *ms, int capacity);
 
