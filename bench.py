@@ -4,7 +4,7 @@
 Workload: 256^3 particles from a Zel'dovich displacement (P(k) ~ k^-2 up to
 the mesh Nyquist, 3-D rms displacement 0.5 cell) of a node-aligned lattice,
 box L = 256 Mpc/h, 512^3 mesh, equal f32 masses, G = 1, no filter.  One step
-is one full force evaluation: mass scan, fixed-point CIC deposit, R2C FFT,
+is one full force evaluation: bucket sort, fixed-point CIC deposit, R2C FFT,
 fused Green's function x gradient, C2R FFTs, CIC interpolation, with the
 particle arrays already resident in HBM.
 
@@ -40,8 +40,9 @@ def stage_bytes(n: int, nm: int) -> dict:
     one complex field = nm^3/2 complex = 4 nm^3 bytes)."""
     m = nm ** 3
     return {
-        "mass_scan_clear": 4 * n,
-        "deposit": 16 * n + 4 * m,          # x, y, z, m in; f32 density out
+        "bucket": 16 * n + 8 * n,           # x, y, z, m in; bucket, rank out
+        "sort": 24 * n + 20 * n,            # x, y, z, m, bucket, rank in; float4 + index out
+        "deposit": 16 * n + 4 * m,          # sorted particles in; f32 density out
         "fft_x_fwd": 4 * m + 4 * m,         # density in, spectrum out
         "fft_y_fwd": 8 * m,                 # spectrum in and out
         "fft_z_green": 4 * m + 8 * m,       # spectrum in, A and B out
@@ -60,6 +61,9 @@ def main() -> int:
     ap.add_argument("--mesh", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the FP64 CPU oracle (and the error check)")
     ap.add_argument("--cpu-runs", type=int, default=2, help="timed CPU oracle runs after one warm-up")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (a kernel of known traffic for scaling "
+                         "profiler byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -71,7 +75,6 @@ def main() -> int:
         return 2
 
     import numpy as np
-    import torch
 
     import gadget_ng_pm_hip as pm
     from gadget_ng_pm_hip import workloads
@@ -84,40 +87,35 @@ def main() -> int:
     n = parts.n
     log(f"[bench] workload generated in {time.perf_counter() - t0:.1f}s: {parts.name}, N={n}")
 
-    dev = torch.device("cuda:0")
-    x = torch.from_numpy(parts.x).to(dev)
-    y = torch.from_numpy(parts.y).to(dev)
-    z = torch.from_numpy(parts.z).to(dev)
-    m = torch.from_numpy(parts.m).to(dev)
-    ax = torch.empty(n, dtype=torch.float32, device=dev)
-    ay = torch.empty_like(ax)
-    az = torch.empty_like(ax)
+    # inputs resident in HBM before the timed region (library-owned device buffers)
+    x, y, z, m = (pm.DeviceArray.from_host(a) for a in (parts.x, parts.y, parts.z, parts.m))
+    ax, ay, az = (pm.DeviceArray((n,), np.float32) for _ in range(3))
     plan = pm.Plan(nm, box, device=0)
-    stream = torch.cuda.current_stream(dev)
-    sptr = stream.cuda_stream
 
     def step():
-        plan.solve_device(x.data_ptr(), y.data_ptr(), z.data_ptr(), m.data_ptr(), n, ax.data_ptr(), ay.data_ptr(),
-                          az.data_ptr(), 1.0, stream=sptr)
+        plan.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, ax.ptr, ay.ptr, az.ptr, 1.0)
+
+    if args.copy_calibration:
+        src = pm.DeviceArray((1 << 28,), np.float32)
+        dst = pm.DeviceArray((1 << 28,), np.float32)
+        pm.device_copy(dst, src)
+        pm.device_synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    pm.device_synchronize()
 
     plan.set_timing(True)
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    pm.device_synchronize()
     t_start = time.perf_counter()
-    ev0.record(stream)
     for _ in range(args.steps):
         step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
+    pm.device_synchronize()
     wall = time.perf_counter() - t_start
-    gpu_ms = ev0.elapsed_time(ev1) / args.steps
     stages = plan.stage_times()
     plan.set_timing(False)
+    gpu_ms = stages.pop("total")
     ms_per_step = wall * 1e3 / args.steps
     value = n * args.steps / wall / 1e6
     log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step wall, {gpu_ms:.3f} ms/step GPU events, "
@@ -143,7 +141,7 @@ def main() -> int:
         t_e2e.append(time.perf_counter() - t1)
     e2e_ms = statistics.median(t_e2e) * 1e3
 
-    acc = torch.stack([ax, ay, az], dim=1).cpu().numpy().astype(np.float64)
+    acc = np.stack([ax.numpy(), ay.numpy(), az.numpy()], axis=1).astype(np.float64)
     result = {
         "metric": "M particle-forces/s + HBM GB/s at 256^3 part/512^3 mesh; rms force err vs FP64",
         "value": round(value, 2),

@@ -1,0 +1,280 @@
+// deposit.hpp -- deterministic tiled CIC deposit for gfx950.
+//
+// Reference semantics: crates/gadget-ng-pm/src/cic.rs:37-93 (mass per cell,
+// node-centred trilinear weights, periodic wrap of all 8 indices).
+//
+// Method:
+//   1. k_bucket_count  each particle's base cell (i0,j0,k0) gives a tile of
+//      T^3 cells and a class: bit a is set when the base cell is on the tile's
+//      last layer along axis a, i.e. when the particle also feeds the next
+//      tile along that axis.  Bucket = tile * 8 + class.  Counts use one
+//      atomic per run of equal buckets within a wavefront (particles that
+//      arrive in spatial order share buckets in runs), which also yields each
+//      particle's rank inside its bucket.  The max |m| for the fixed-point
+//      scale is reduced in the same pass.
+//   2. k_scan_*        exclusive scan of the bucket counts.
+//   3. k_scatter       particles to bucket order as float4 (x, y, z, m).
+//   4. k_deposit_tiles one workgroup per tile owns its T^3 cells outright.  It
+//      reads its own particles and, from the 7 lower neighbour tiles, only the
+//      classes that reach into it, and adds each of the 8 CIC targets exactly
+//      once over all tiles (target offset a lands in tile t + (a & class)).
+//      Weights become 64-bit fixed point (w * m * 2^s, N max|m| 2^s < 2^62)
+//      and accumulate with LDS integer atomics, so the sum is exact and
+//      order-independent; the tile is then converted in FP64 and written as
+//      FP32 mass per cell with plain stores.  No global atomics touch the mesh
+//      and no mesh clearing pass is needed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pm_kernels.hpp"
+
+namespace gng {
+
+struct TileMap {
+  int T;   // tile side in cells
+  int nt;  // tiles per axis (nm / T)
+};
+
+// Zero a buffer with vector stores (a fill kernel rather than a memset, so the
+// sequence stays safe to capture in a graph).
+__global__ __launch_bounds__(256) void k_zero(uint4* __restrict__ p, long long n16) {
+  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
+    p[i] = z;
+}
+
+__device__ __forceinline__ unsigned bucket_of_cell(int ix, int iy, int iz, const TileMap tm) {
+  const int tx = ix / tm.T, ty = iy / tm.T, tz = iz / tm.T;
+  const unsigned cls = (unsigned)(ix - tx * tm.T == tm.T - 1) | ((unsigned)(iy - ty * tm.T == tm.T - 1) << 1) |
+                       ((unsigned)(iz - tz * tm.T == tm.T - 1) << 2);
+  return (((unsigned)tz * tm.nt + ty) * tm.nt + tx) * 8u + cls;
+}
+
+__global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ x, const float* __restrict__ y,
+                                                      const float* __restrict__ z, const float* __restrict__ m,
+                                                      long long n, const CellMap cm, const TileMap tm,
+                                                      unsigned* __restrict__ count, unsigned* __restrict__ bucket,
+                                                      unsigned* __restrict__ rank, SolveScalars* sc) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool live = i < n;
+  unsigned b = 0xffffffffu;
+  float am = 0.0f;
+  if (live) {
+    int ix, iy, iz;
+    float d;
+    cell_coord(x[i], cm, ix, d);
+    cell_coord(y[i], cm, iy, d);
+    cell_coord(z[i], cm, iz, d);
+    b = bucket_of_cell(ix, iy, iz, tm);
+    am = fabsf(m[i]);
+  }
+  // runs of equal buckets: a lane heads a run when its bucket differs from the previous lane's
+  const unsigned prev = __shfl_up(b, 1);
+  const bool head = live && (lane == 0 || prev != b);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long below = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+  const int my_head = 63 - __clzll(heads & below);  // highest head at or below this lane
+  const unsigned long long above = heads & ~below;
+  const unsigned long long live_mask = __ballot(live);
+  const int next_head = above ? __ffsll((long long)above) - 1 : 64 - __clzll(live_mask);
+  unsigned base = 0;
+  if (head) base = atomicAdd(count + b, (unsigned)(next_head - lane));
+  base = __shfl(base, my_head);
+  if (live) {
+    bucket[i] = b;
+    rank[i] = base + (unsigned)(lane - my_head);
+  }
+  // max |m|: wave, then block, then one atomic per block only when it raises
+  // the running maximum (a plain read filters the common equal-mass case)
+  for (int off = 32; off > 0; off >>= 1) am = fmaxf(am, __shfl_xor(am, off));
+  __shared__ float wmax[4];
+  if (lane == 0) wmax[threadIdx.x >> 6] = am;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float bm = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    const unsigned bits = __float_as_uint(bm);
+    if (bm > 0.0f && bits > __atomic_load_n(&sc->mmax_bits, __ATOMIC_RELAXED)) atomicMax(&sc->mmax_bits, bits);
+  }
+}
+
+// Exclusive scan of `count` (nb entries) into `offset` (nb + 1 entries).
+// Phase 1: each 256-thread block scans 4096 entries, keeps local offsets and
+// its total; phase 2: one block scans the totals; phase 3: adds them back.
+constexpr int kScanItems = 16;
+constexpr int kScanBlock = 256 * kScanItems;
+
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* lds, unsigned& total) {
+  // 256 threads: wave scans then wave totals
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned s = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned t = __shfl_up(s, off);
+    if (lane >= off) s += t;
+  }
+  if (lane == 63) lds[wid] = s;
+  __syncthreads();
+  unsigned wbase = 0, tot = 0;
+  for (int w = 0; w < 4; ++w) {
+    const unsigned ws = lds[w];
+    if (w < wid) wbase += ws;
+    tot += ws;
+  }
+  __syncthreads();
+  total = tot;
+  return wbase + s - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_local(const unsigned* __restrict__ count, unsigned* __restrict__ offset,
+                                                    unsigned* __restrict__ partial, long long nb) {
+  __shared__ unsigned lds[4];
+  const long long base = (long long)blockIdx.x * kScanBlock + (long long)threadIdx.x * kScanItems;
+  unsigned v[kScanItems];
+  unsigned sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (base + k < nb) ? count[base + k] : 0u;
+    sum += v[k];
+  }
+  unsigned total;
+  unsigned run = block_exclusive_scan(sum, lds, total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < nb) offset[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_partials(unsigned* __restrict__ partial, int nblocks,
+                                                       unsigned* __restrict__ offset, long long nb) {
+  __shared__ unsigned lds[4];
+  unsigned carry = 0;
+  for (int b0 = 0; b0 < nblocks; b0 += 256) {
+    const int b = b0 + (int)threadIdx.x;
+    const unsigned v = b < nblocks ? partial[b] : 0u;
+    unsigned total;
+    const unsigned ex = block_exclusive_scan(v, lds, total);
+    if (b < nblocks) partial[b] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) offset[nb] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(unsigned* __restrict__ offset, const unsigned* __restrict__ partial,
+                                                  long long nb) {
+  const long long base = (long long)blockIdx.x * kScanBlock;
+  const unsigned add = partial[blockIdx.x];
+  for (int k = threadIdx.x; k < kScanBlock; k += 256)
+    if (base + k < nb) offset[base + k] += add;
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const float* __restrict__ x, const float* __restrict__ y,
+                                                 const float* __restrict__ z, const float* __restrict__ m, long long n,
+                                                 const unsigned* __restrict__ bucket, const unsigned* __restrict__ rank,
+                                                 const unsigned* __restrict__ offset, float4* __restrict__ sorted,
+                                                 unsigned* __restrict__ perm) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned slot = offset[bucket[i]] + rank[i];
+  sorted[slot] = make_float4(x[i], y[i], z[i], m[i]);
+  perm[slot] = (unsigned)i;
+}
+
+// The 14 bucket ranges a tile reads: (d, first class, end class).  d is the
+// neighbour offset (bit 0 x, bit 1 y, bit 2 z: the source tile is u - d) and
+// the classes are those containing every bit of d, merged where adjacent.
+__constant__ unsigned char kRangeD[14] = {0, 1, 1, 1, 1, 2, 2, 3, 3, 4, 5, 5, 6, 7};
+__constant__ unsigned char kRangeLo[14] = {0, 1, 3, 5, 7, 2, 6, 3, 7, 4, 5, 7, 6, 7};
+__constant__ unsigned char kRangeHi[14] = {8, 2, 4, 6, 8, 4, 8, 4, 8, 8, 6, 8, 8, 8};
+
+// One workgroup per tile; T is the tile side (T^3 int64 accumulators in LDS).
+// The 72 bucket offsets of the 8 source tiles are fetched in parallel, the 14
+// ranges are laid end to end, and all their particles are processed in one
+// flat loop, so the only serial latency is one offset load.
+template <int T>
+__global__ __launch_bounds__(256) void k_deposit_tiles(const float4* __restrict__ sorted,
+                                                       const unsigned* __restrict__ offset, const CellMap cm,
+                                                       const TileMap tm, const SolveScalars* sc, long long ntot,
+                                                       float* __restrict__ rho) {
+  constexpr int T3 = T * T * T;
+  __shared__ unsigned long long acc[T3];
+  __shared__ unsigned toff[8][9];
+  __shared__ unsigned rbeg[14], rpre[15];
+  const int tid = threadIdx.x;
+  for (int c = tid; c < T3; c += 256) acc[c] = 0ull;
+  const int nt = tm.nt;
+  const int tile = blockIdx.x;
+  const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
+  if (tid < 72) {
+    const int d = tid / 9, j = tid % 9;
+    const int sx = (d & 1) ? (ux == 0 ? nt - 1 : ux - 1) : ux;
+    const int sy = (d & 2) ? (uy == 0 ? nt - 1 : uy - 1) : uy;
+    const int sz = (d & 4) ? (uz == 0 ? nt - 1 : uz - 1) : uz;
+    toff[d][j] = offset[((unsigned)(sz * nt + sy) * nt + sx) * 8u + j];
+  }
+  __syncthreads();
+  if (tid < 64) {
+    unsigned len = 0;
+    if (tid < 14) {
+      const int d = kRangeD[tid];
+      rbeg[tid] = toff[d][kRangeLo[tid]];
+      len = toff[d][kRangeHi[tid]] - toff[d][kRangeLo[tid]];
+    }
+    unsigned incl = len;
+    for (int off = 1; off < 16; off <<= 1) {
+      const unsigned t = __shfl_up(incl, off);
+      if (tid >= off) incl += t;
+    }
+    if (tid < 14) rpre[tid + 1] = incl;
+    if (tid == 0) rpre[0] = 0;
+  }
+  __syncthreads();
+  const int s = fixed_shift(sc, ntot);
+  const unsigned total = rpre[14];
+  for (unsigned q = tid; q < total; q += 256) {
+    int r = 0;
+    while (q >= rpre[r + 1]) ++r;
+    const int d = kRangeD[r];
+    const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
+    const float4 p = sorted[rbeg[r] + (q - rpre[r])];
+    int ix, iy, iz;
+    float fx, fy, fz;
+    cell_coord(p.x, cm, ix, fx);
+    cell_coord(p.y, cm, iy, fy);
+    cell_coord(p.z, cm, iz, fz);
+    const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+    // local base index in the particle's own tile; its class follows from it
+    const int lx0 = ix % T, ly0 = iy % T, lz0 = iz % T;
+    const int cx = lx0 == T - 1, cy = ly0 == T - 1, cz = lz0 == T - 1;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      if ((a & cz) != dz) continue;
+      const int lz = dz ? 0 : lz0 + a;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if ((b & cy) != dy) continue;
+        const int ly = dy ? 0 : ly0 + b;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          if ((c & cx) != dx) continue;
+          const int lx = dx ? 0 : lx0 + c;
+          const float w = p.w * wx[c] * wy[b] * wz[a];
+          const long long qv = __float2ll_rn(ldexpf(w, s));
+          atomicAdd(acc + (lz * T + ly) * T + lx, (unsigned long long)qv);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const double unit = ldexp(1.0, -s);
+  const size_t nm2 = (size_t)cm.nm * cm.nm;
+  for (int c = tid; c < T3; c += 256) {
+    const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
+    const size_t g = (size_t)(uz * T + lz) * nm2 + (size_t)(uy * T + ly) * cm.nm + (ux * T + lx);
+    rho[g] = (float)((double)(long long)acc[c] * unit);
+  }
+}
+
+}  // namespace gng

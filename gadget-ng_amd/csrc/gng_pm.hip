@@ -5,6 +5,12 @@
 // and grows the particle buffers on demand (pm_gravity.hip:376-392).  Every
 // solve is a fixed sequence of launches on one stream; host entry points add
 // the H2D/D2H copies and a stream synchronisation.
+//
+// Mesh memory: three packed half-spectrum buffers P, A, B of 4 nm^3 bytes
+// each.  The deposit writes the FP32 mass per cell into P (or into a
+// separate buffer when the density is kept for readback) and every FFT pass
+// works in place or ping-pongs among P, A, B, ending with F_x in A, F_y in P
+// and F_z in B.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -16,6 +22,7 @@
 #include <vector>
 
 #include "../../include/gng_pm.h"
+#include "deposit.hpp"
 #include "pm_kernels.hpp"
 
 using namespace gng;
@@ -37,7 +44,9 @@ int fail(int code, const std::string& msg) {
                   std::string(#call) + ": " + hipGetErrorString(e_));                            \
   } while (0)
 
-constexpr int kStages = 8;
+// stages: 0 bucket count, 1 sort (scan + scatter), 2 deposit, 3 x forward,
+// 4 y forward, 5 z + Green, 6 y inverse, 7 x inverse, 8 interpolation
+constexpr int kStages = 9;
 constexpr int kTimingSlots = 64;
 
 // ---------------------------------------------------------------------------
@@ -61,12 +70,18 @@ constexpr int x_rows() {
   constexpr long long b = pow2_part((long long)N * N);
   return a < b ? a : (int)b;
 }
+// Deposit tile side: the largest of 16, 12, 8, 4 that divides nm.
+constexpr int deposit_tile(int nm) { return nm % 16 == 0 ? 16 : nm % 12 == 0 ? 12 : nm % 8 == 0 ? 8 : 4; }
 
 struct MeshOps {
   int nm;
   int x_rows;
   int yz_cols;
-  void (*x_fwd)(dim3, dim3, hipStream_t, const long long*, const SolveScalars*, long long, float2*, const float2*);
+  int yz_threads;
+  int tile;
+  void (*deposit)(dim3, dim3, hipStream_t, const float4*, const unsigned*, CellMap, TileMap, const SolveScalars*,
+                  long long, float*);
+  void (*x_fwd)(dim3, dim3, hipStream_t, const float*, float2*, const float2*);
   void (*y_fwd)(dim3, dim3, hipStream_t, float2*, const float2*);
   void (*z_green)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
   void (*y_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
@@ -77,19 +92,29 @@ template <int N>
 MeshOps make_ops() {
   constexpr int R = x_rows<N>();
   constexpr int C = YZCfg<N>::C;
+  constexpr int TY = YZCfg<N>::T;
+  constexpr int TD = deposit_tile(N);
+  static_assert(N % TD == 0, "deposit tiles must cover the mesh");
   MeshOps o;
   o.nm = N;
   o.x_rows = R;
   o.yz_cols = C;
-  o.x_fwd = [](dim3 g, dim3 b, hipStream_t s, const long long* m, const SolveScalars* sc, long long n, float2* P,
-               const float2* tw) { hipLaunchKernelGGL((k_fft_x_fwd<N, R>), g, b, 0, s, m, sc, n, P, tw); };
+  o.yz_threads = TY;
+  o.tile = TD;
+  o.deposit = [](dim3 g, dim3 b, hipStream_t s, const float4* srt, const unsigned* off, CellMap cm, TileMap tm,
+                 const SolveScalars* sc, long long n, float* rho) {
+    hipLaunchKernelGGL((k_deposit_tiles<TD>), g, b, 0, s, srt, off, cm, tm, sc, n, rho);
+  };
+  o.x_fwd = [](dim3 g, dim3 b, hipStream_t s, const float* rho, float2* P, const float2* tw) {
+    hipLaunchKernelGGL((k_fft_x_fwd<N, R>), g, b, 0, s, rho, P, tw);
+  };
   o.y_fwd = [](dim3 g, dim3 b, hipStream_t s, float2* P, const float2* tw) {
-    hipLaunchKernelGGL((k_fft_y_fwd<N, C>), g, b, 0, s, P, tw);
+    hipLaunchKernelGGL((k_fft_y_fwd<N, C, TY>), g, b, 0, s, P, tw);
   };
   o.z_green = [](dim3 g, dim3 b, hipStream_t s, const float2* P, float2* A, float2* B, const float2* tw,
-                 GreenParams gp) { hipLaunchKernelGGL((k_fft_z_green<N, C>), g, b, 0, s, P, A, B, tw, gp); };
+                 GreenParams gp) { hipLaunchKernelGGL((k_fft_z_green<N, C, TY>), g, b, 0, s, P, A, B, tw, gp); };
   o.y_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* B, float2* P, const float2* tw) {
-    hipLaunchKernelGGL((k_fft_y_inv<N, C>), g, b, 0, s, A, B, P, tw);
+    hipLaunchKernelGGL((k_fft_y_inv<N, C, TY>), g, b, 0, s, A, B, P, tw);
   };
   o.x_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* P, float2* B, const float2* tw) {
     hipLaunchKernelGGL((k_fft_x_inv<N, R>), g, b, 0, s, A, P, B, tw);
@@ -121,13 +146,29 @@ struct gng_pm_plan {
   std::mutex mu;
 
   long long nm3 = 0;
-  unsigned long long* d_fixed = nullptr;  // nm^3 int64 fixed-point mass
-  float2* d_P = nullptr;                  // three nm^3/2 complex meshes
+  float2* d_P = nullptr;  // three nm^3/2 complex meshes
   float2* d_A = nullptr;
   float2* d_B = nullptr;
+  float* d_rho = nullptr;  // kept density (only with keep_density)
+  bool keep_density = false;
   float2* d_tw = nullptr;  // nm-th roots of unity
   float* d_deconv = nullptr;
   SolveScalars* d_sc = nullptr;
+
+  // deposit buckets: nb = tiles * 8 counts, nb + 1 offsets, scan partials
+  TileMap tm{};
+  long long nb = 0;
+  int scan_blocks = 0;
+  unsigned* d_count = nullptr;
+  unsigned* d_offset = nullptr;
+  unsigned* d_partial = nullptr;
+
+  // per-particle work buffers (grown on demand)
+  long long work_cap = 0;
+  float4* d_sorted = nullptr;
+  unsigned* d_perm = nullptr;
+  unsigned* d_bucket = nullptr;
+  unsigned* d_rank = nullptr;
 
   // particle staging for the host entry points
   long long cap = 0;
@@ -137,6 +178,7 @@ struct gng_pm_plan {
   hipStream_t stream = nullptr;
 
   long long last_n = 0;
+  bool density_valid = false;
   // Stage timing: every solve records kStages+1 events into the next slot of
   // a ring; nothing synchronises until gng_pm_stage_times() reads them.
   bool timing = false;
@@ -151,18 +193,24 @@ int set_device(gng_pm_plan* p) {
   return GNG_OK;
 }
 
+void free_work(gng_pm_plan* p) {
+  (void)hipFree(p->d_sorted);
+  (void)hipFree(p->d_perm);
+  (void)hipFree(p->d_bucket);
+  (void)hipFree(p->d_rank);
+  p->d_sorted = nullptr;
+  p->d_perm = p->d_bucket = p->d_rank = nullptr;
+  p->work_cap = 0;
+}
+
 void free_plan(gng_pm_plan* p) {
   if (!p) return;
-  hipSetDevice(p->device);
-  hipFree(p->d_fixed);
-  hipFree(p->d_P);
-  hipFree(p->d_A);
-  hipFree(p->d_B);
-  hipFree(p->d_tw);
-  hipFree(p->d_deconv);
-  hipFree(p->d_sc);
-  hipFree(p->d_part);
-  hipFree(p->d_idx);
+  (void)hipSetDevice(p->device);
+  for (void* q : {(void*)p->d_P, (void*)p->d_A, (void*)p->d_B, (void*)p->d_rho, (void*)p->d_tw, (void*)p->d_deconv,
+                  (void*)p->d_sc, (void*)p->d_count, (void*)p->d_offset, (void*)p->d_partial, (void*)p->d_part,
+                  (void*)p->d_idx})
+    (void)hipFree(q);
+  free_work(p);
   for (auto& slot : p->ev)
     for (auto& e : slot)
       if (e) (void)hipEventDestroy(e);
@@ -198,60 +246,81 @@ inline void mark(gng_pm_plan* p, int i, hipStream_t s) {
   if (p->timing) (void)hipEventRecord(p->ev[p->timed_solves % kTimingSlots][i], s);
 }
 
+int ensure_work(gng_pm_plan* p, long long n) {
+  if (n <= p->work_cap) return GNG_OK;
+  free_work(p);
+  const size_t m = (size_t)n;
+  GNG_HIP(hipMalloc(&p->d_sorted, sizeof(float4) * m));
+  GNG_HIP(hipMalloc(&p->d_perm, sizeof(unsigned) * m));
+  GNG_HIP(hipMalloc(&p->d_bucket, sizeof(unsigned) * m));
+  GNG_HIP(hipMalloc(&p->d_rank, sizeof(unsigned) * m));
+  p->work_cap = n;
+  return GNG_OK;
+}
+
 // The whole pipeline on device pointers.
 int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z, const float* m, long long n,
                  const long long* idx, long long k, double g, float* ax, float* ay, float* az, hipStream_t s) {
+  int rc = ensure_work(p, n > 0 ? n : 1);
+  if (rc) return rc;
   const int nm = p->cfg.grid_size;
   const int H = nm / 2;
   const MeshOps& o = *p->ops;
   const CellMap cm = cell_map(p);
+  float* rho = p->keep_density ? p->d_rho : reinterpret_cast<float*>(p->d_P);
+
   mark(p, 0, s);
   hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, p->d_sc);
   {
-    long long blocks = (n + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_mass_max, dim3((unsigned)blocks), dim3(256), 0, s, m, n, p->d_sc);
+    const long long n16 = (p->nb + 3) / 4;  // counts padded to 16-byte words
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<uint4*>(p->d_count), n16);
   }
-  {
-    const long long n16 = p->nm3 * 8 / 16;
-    long long blocks = (n16 + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<uint4*>(p->d_fixed), n16);
-  }
-  mark(p, 1, s);
   if (n > 0)
-    hipLaunchKernelGGL(k_deposit_atomic, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z, m, n, cm,
-                       p->d_sc, p->d_fixed);
+    hipLaunchKernelGGL(k_bucket_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z, m, n, cm, p->tm,
+                       p->d_count, p->d_bucket, p->d_rank, p->d_sc);
+  mark(p, 1, s);
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_count, p->d_offset,
+                     p->d_partial, p->nb);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(256), 0, s, p->d_partial, p->scan_blocks, p->d_offset, p->nb);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_offset, p->d_partial, p->nb);
+  if (n > 0)
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z, m, n, p->d_bucket,
+                       p->d_rank, p->d_offset, p->d_sorted, p->d_perm);
   mark(p, 2, s);
-  const long long rows = (long long)nm * nm;
-  o.x_fwd(dim3((unsigned)(rows / o.x_rows)), dim3(256), s, reinterpret_cast<const long long*>(p->d_fixed), p->d_sc,
-          n, p->d_P, p->d_tw);
+  const unsigned ntiles = (unsigned)p->tm.nt * p->tm.nt * p->tm.nt;
+  o.deposit(dim3(ntiles), dim3(256), s, p->d_sorted, p->d_offset, cm, p->tm, p->d_sc, n, rho);
   mark(p, 3, s);
-  const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
-  o.y_fwd(yz_grid, dim3(256), s, p->d_P, p->d_tw);
+  const long long rows = (long long)nm * nm;
+  o.x_fwd(dim3((unsigned)(rows / o.x_rows)), dim3(256), s, rho, p->d_P, p->d_tw);
   mark(p, 4, s);
-  o.z_green(yz_grid, dim3(256), s, p->d_P, p->d_A, p->d_B, p->d_tw, green_params(p, g));
+  const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
+  const dim3 yz_block((unsigned)o.yz_threads);
+  o.y_fwd(yz_grid, yz_block, s, p->d_P, p->d_tw);
   mark(p, 5, s);
-  o.y_inv(yz_grid, dim3(256), s, p->d_A, p->d_B, p->d_P, p->d_tw);
+  o.z_green(yz_grid, yz_block, s, p->d_P, p->d_A, p->d_B, p->d_tw, green_params(p, g));
   mark(p, 6, s);
-  o.x_inv(dim3((unsigned)(rows / o.x_rows), 3), dim3(256), s, p->d_A, p->d_P, p->d_B, p->d_tw);
+  o.y_inv(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
   mark(p, 7, s);
+  o.x_inv(dim3((unsigned)(rows / o.x_rows), 3), dim3(256), s, p->d_A, p->d_P, p->d_B, p->d_tw);
+  mark(p, 8, s);
   if (k > 0)
     hipLaunchKernelGGL(k_interp, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, x, y, z, idx, k, cm,
                        reinterpret_cast<const float*>(p->d_A), reinterpret_cast<const float*>(p->d_P),
                        reinterpret_cast<const float*>(p->d_B), ax, ay, az);
-  mark(p, 8, s);
+  mark(p, 9, s);
   GNG_HIP(hipGetLastError());
   if (p->timing) ++p->timed_solves;
   p->last_n = n;
+  p->density_valid = p->keep_density;
   return GNG_OK;
 }
 
-// Average per-stage time over the solves still held in the timing ring.
+// Average per-stage time (ms[0..kStages-1]) and whole-solve time
+// (ms[kStages]) over the solves still held in the timing ring.
 int read_timing(gng_pm_plan* p, float* ms) {
   const long long held = p->timed_solves < kTimingSlots ? p->timed_solves : kTimingSlots;
-  for (int i = 0; i < kStages; ++i) ms[i] = 0.0f;
+  for (int i = 0; i <= kStages; ++i) ms[i] = 0.0f;
   if (held == 0) return GNG_OK;
   for (long long j = 0; j < held; ++j) {
     hipEvent_t* ev = p->ev[(p->timed_solves - 1 - j) % kTimingSlots];
@@ -261,22 +330,25 @@ int read_timing(gng_pm_plan* p, float* ms) {
       GNG_HIP(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
       ms[i] += t;
     }
+    float tot = 0.0f;
+    GNG_HIP(hipEventElapsedTime(&tot, ev[0], ev[kStages]));
+    ms[kStages] += tot;
   }
-  for (int i = 0; i < kStages; ++i) ms[i] /= (float)held;
+  for (int i = 0; i <= kStages; ++i) ms[i] /= (float)held;
   return GNG_OK;
 }
 
 int ensure_particles(gng_pm_plan* p, long long n, long long k, bool need_idx) {
   const long long need = n > k ? n : k;
   if (need > p->cap) {
-    hipFree(p->d_part);
+    (void)hipFree(p->d_part);
     p->d_part = nullptr;
     p->cap = 0;
     GNG_HIP(hipMalloc(&p->d_part, sizeof(float) * 7 * (size_t)need));
     p->cap = need;
   }
   if (need_idx && k > p->idx_cap) {
-    hipFree(p->d_idx);
+    (void)hipFree(p->d_idx);
     p->d_idx = nullptr;
     p->idx_cap = 0;
     GNG_HIP(hipMalloc(&p->d_idx, sizeof(long long) * (size_t)k));
@@ -289,7 +361,7 @@ int validate(gng_pm_plan* p, long long n, const void* idx, long long k) {
   if (!p) return fail(GNG_ERR_INVALID, "null plan");
   if (n < 0 || k < 0) return fail(GNG_ERR_INVALID, "negative particle count");
   if (!idx && k != n) return fail(GNG_ERR_INVALID, "idx is NULL but k != n");
-  if (n > (1LL << 31)) return fail(GNG_ERR_INVALID, "n above 2^31 is not supported");
+  if (n >= (1LL << 31)) return fail(GNG_ERR_INVALID, "n of 2^31 or more is not supported");
   return GNG_OK;
 }
 
@@ -365,7 +437,7 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
     }
     p->device = cfg->device;
   } else {
-    hipGetDevice(&p->device);
+    (void)hipGetDevice(&p->device);
   }
   auto bail = [&](int rc) {
     free_plan(p);
@@ -375,6 +447,10 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
   if (rc) return bail(rc);
   const int nm = cfg->grid_size;
   p->nm3 = (long long)nm * nm * nm;
+  p->tm.T = ops->tile;
+  p->tm.nt = nm / ops->tile;
+  p->nb = (long long)p->tm.nt * p->tm.nt * p->tm.nt * 8;
+  p->scan_blocks = (int)((p->nb + kScanBlock - 1) / kScanBlock);
   const size_t half = sizeof(float2) * (size_t)(p->nm3 / 2);
 #define GNG_ALLOC(ptr, bytes)                                                                             \
   do {                                                                                                    \
@@ -383,13 +459,15 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
       return bail(fail(e_ == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP,                             \
                        std::string("hipMalloc ") + #ptr + ": " + hipGetErrorString(e_)));                 \
   } while (0)
-  GNG_ALLOC(p->d_fixed, sizeof(unsigned long long) * (size_t)p->nm3);
   GNG_ALLOC(p->d_P, half);
   GNG_ALLOC(p->d_A, half);
   GNG_ALLOC(p->d_B, half);
   GNG_ALLOC(p->d_tw, sizeof(float2) * (size_t)nm);
   GNG_ALLOC(p->d_deconv, sizeof(float) * (size_t)nm);
   GNG_ALLOC(p->d_sc, sizeof(SolveScalars));
+  GNG_ALLOC(p->d_count, sizeof(unsigned) * (size_t)((p->nb + 3) / 4 * 4));
+  GNG_ALLOC(p->d_offset, sizeof(unsigned) * (size_t)(p->nb + 1));
+  GNG_ALLOC(p->d_partial, sizeof(unsigned) * (size_t)p->scan_blocks);
 #undef GNG_ALLOC
   std::vector<float2> tw(nm);
   std::vector<float> dc(nm);
@@ -417,6 +495,8 @@ int gng_pm_accelerations_f32(gng_pm_plan* p, const float* x, const float* y, con
                              int64_t n, const int64_t* idx, int64_t k, double g, float* ax, float* ay, float* az) {
   int rc = validate(p, n, idx, k);
   if (rc) return rc;
+  if (n > 0 && (!x || !y || !z || !mass)) return fail(GNG_ERR_INVALID, "null input array");
+  if (k > 0 && (!ax || !ay || !az)) return fail(GNG_ERR_INVALID, "null output array");
   std::lock_guard<std::mutex> lk(p->mu);
   if ((rc = set_device(p))) return rc;
   return solve_host_soa(p, x, y, z, mass, n, idx, k, g, ax, ay, az);
@@ -456,35 +536,31 @@ int gng_pm_solve_device(gng_pm_plan* p, const float* d_x, const float* d_y, cons
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(p->mu);
   if ((rc = set_device(p))) return rc;
-  rc = run_pipeline(p, d_x, d_y, d_z, d_mass, n, reinterpret_cast<const long long*>(d_idx), k, g, d_ax, d_ay, d_az,
-                    reinterpret_cast<hipStream_t>(stream));
-  return rc;
+  return run_pipeline(p, d_x, d_y, d_z, d_mass, n, reinterpret_cast<const long long*>(d_idx), k, g, d_ax, d_ay,
+                      d_az, reinterpret_cast<hipStream_t>(stream));
 }
 
-int gng_pm_density_fixed(gng_pm_plan* p, int64_t* out, int32_t* shift) {
-  if (!p || !out) return fail(GNG_ERR_INVALID, "null argument");
+int gng_pm_keep_density(gng_pm_plan* p, int enable) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
   std::lock_guard<std::mutex> lk(p->mu);
   int rc = set_device(p);
   if (rc) return rc;
-  SolveScalars sc;
-  GNG_HIP(hipMemcpy(&sc, p->d_sc, sizeof(sc), hipMemcpyDeviceToHost));
-  GNG_HIP(hipMemcpy(out, p->d_fixed, sizeof(int64_t) * (size_t)p->nm3, hipMemcpyDeviceToHost));
-  float mmax;
-  std::memcpy(&mmax, &sc.mmax_bits, sizeof(float));
-  int e = 0;
-  std::frexp((double)p->last_n * (double)mmax, &e);
-  if (shift) *shift = 62 - e;
+  if (enable && !p->d_rho) GNG_HIP(hipMalloc(&p->d_rho, sizeof(float) * (size_t)p->nm3));
+  p->keep_density = enable != 0;
+  p->density_valid = false;
   return GNG_OK;
 }
 
 int gng_pm_density(gng_pm_plan* p, double* out) {
   if (!p || !out) return fail(GNG_ERR_INVALID, "null argument");
-  std::vector<int64_t> q((size_t)p->nm3);
-  int32_t s = 0;
-  int rc = gng_pm_density_fixed(p, q.data(), &s);
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (!p->density_valid) return fail(GNG_ERR_INVALID, "no kept density: call gng_pm_keep_density(p, 1) first");
+  int rc = set_device(p);
   if (rc) return rc;
-  const double unit = std::ldexp(1.0, -s);
-  for (long long i = 0; i < p->nm3; ++i) out[i] = (double)q[i] * unit;
+  std::vector<float> f((size_t)p->nm3);
+  GNG_HIP(hipDeviceSynchronize());
+  GNG_HIP(hipMemcpy(f.data(), p->d_rho, sizeof(float) * (size_t)p->nm3, hipMemcpyDeviceToHost));
+  for (long long i = 0; i < p->nm3; ++i) out[i] = (double)f[i];
   return GNG_OK;
 }
 
@@ -493,6 +569,7 @@ int gng_pm_force_meshes(gng_pm_plan* p, float* out) {
   std::lock_guard<std::mutex> lk(p->mu);
   int rc = set_device(p);
   if (rc) return rc;
+  GNG_HIP(hipDeviceSynchronize());
   const size_t bytes = sizeof(float) * (size_t)p->nm3;
   GNG_HIP(hipMemcpy(out, p->d_A, bytes, hipMemcpyDeviceToHost));
   GNG_HIP(hipMemcpy(out + p->nm3, p->d_P, bytes, hipMemcpyDeviceToHost));
@@ -513,10 +590,39 @@ int gng_pm_stage_times(gng_pm_plan* p, float* ms, int capacity) {
   std::lock_guard<std::mutex> lk(p->mu);
   int rc = set_device(p);
   if (rc) return rc;
-  float t[kStages];
+  float t[kStages + 1];
   if ((rc = read_timing(p, t))) return -rc;
-  for (int i = 0; i < kStages && i < capacity; ++i) ms[i] = t[i];
+  for (int i = 0; i <= kStages && i < capacity; ++i) ms[i] = t[i];
   return kStages;
+}
+
+// ---- device-memory helpers ------------------------------------------------
+
+int gng_dev_alloc(void** ptr, uint64_t bytes) {
+  if (!ptr) return fail(GNG_ERR_INVALID, "null argument");
+  *ptr = nullptr;
+  GNG_HIP(hipMalloc(ptr, (size_t)bytes));
+  return GNG_OK;
+}
+
+int gng_dev_free(void* ptr) {
+  GNG_HIP(hipFree(ptr));
+  return GNG_OK;
+}
+
+int gng_dev_copy(void* dst, const void* src, uint64_t bytes, int kind) {
+  if (bytes == 0) return GNG_OK;
+  if (!dst || !src) return fail(GNG_ERR_INVALID, "null argument");
+  if (kind < 0 || kind > 2) return fail(GNG_ERR_INVALID, "bad copy kind");
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                        : hipMemcpyDeviceToDevice;
+  GNG_HIP(hipMemcpy(dst, src, (size_t)bytes, k));
+  return GNG_OK;
+}
+
+int gng_dev_synchronize(void) {
+  GNG_HIP(hipDeviceSynchronize());
+  return GNG_OK;
 }
 
 // ---- reference-compatible entry points ----------------------------------

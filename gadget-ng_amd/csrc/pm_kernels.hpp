@@ -1,10 +1,9 @@
 // pm_kernels.hpp -- the periodic PM force pipeline as HIP kernels for gfx950.
 //
 // Stages (reference semantics: crates/gadget-ng-pm/src/solver.rs:74-140):
-//   deposit    CIC mass per cell, 64-bit fixed point, integer atomics, so the
-//              mesh is bit-identical from run to run (cic.rs:37-93 semantics)
-//   fft_x_fwd  fixed point -> FP64 -> FP32 conversion fused with the real-to-
-//              complex x transform; output is the packed half spectrum
+//   deposit    CIC mass per cell in 64-bit fixed point, converted in FP64 and
+//              stored as FP32 (deposit.hpp; cic.rs:37-93 semantics)
+//   fft_x_fwd  real-to-complex x transform; output is the packed half spectrum
 //   fft_y_fwd  complex y transform
 //   fft_z_green  z transform, Green's function x filters x (optional) CIC
 //              deconvolution, then the inverse z transform of two fields:
@@ -96,64 +95,6 @@ __global__ void k_reset_scalars(SolveScalars* sc) {
   if (threadIdx.x == 0) sc->mmax_bits = 0u;
 }
 
-// max |m| over the particles (deterministic: integer max of float bits).
-__global__ __launch_bounds__(256) void k_mass_max(const float* __restrict__ m, long long n, SolveScalars* sc) {
-  float v = 0.0f;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    v = fmaxf(v, fabsf(m[i]));
-  // wave reduction
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
-  __shared__ float red[4];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float r = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = fmaxf(r, red[w]);
-    atomicMax(&sc->mmax_bits, __float_as_uint(r));
-  }
-}
-
-// Zero a buffer with vector stores (graph-safe replacement for a memset).
-__global__ __launch_bounds__(256) void k_zero(uint4* __restrict__ p, long long n16) {
-  const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
-    p[i] = z;
-}
-
-// CIC deposit: one thread per particle, 8 fixed-point atomics into the int64
-// mesh (flat = iz*nm^2 + iy*nm + ix, cic.rs:8).  All 8 indices wrap.
-__global__ __launch_bounds__(256) void k_deposit_atomic(const float* __restrict__ x, const float* __restrict__ y,
-                                                        const float* __restrict__ z, const float* __restrict__ m,
-                                                        long long n, const CellMap cm, const SolveScalars* sc,
-                                                        unsigned long long* __restrict__ mesh) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int s = fixed_shift(sc, n);
-  int ix, iy, iz;
-  float dx, dy, dz;
-  cell_coord(x[i], cm, ix, dx);
-  cell_coord(y[i], cm, iy, dy);
-  cell_coord(z[i], cm, iz, dz);
-  const int nm = cm.nm;
-  const float mi = m[i];
-  const float wx[2] = {1.0f - dx, dx}, wy[2] = {1.0f - dy, dy}, wz[2] = {1.0f - dz, dz};
-  const int xs[2] = {ix, ix + 1 == nm ? 0 : ix + 1};
-  const int ys[2] = {iy, iy + 1 == nm ? 0 : iy + 1};
-  const int zs[2] = {iz, iz + 1 == nm ? 0 : iz + 1};
-  const size_t nm2 = (size_t)nm * nm;
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const float w = mi * wx[c] * wy[b] * wz[a];
-        const long long q = __float2ll_rn(ldexpf(w, s));
-        atomicAdd(mesh + zs[a] * nm2 + (size_t)ys[b] * nm + xs[c], (unsigned long long)q);
-      }
-}
-
 // ---------------------------------------------------------------------------
 // x transforms (rows are contiguous; a workgroup owns ROWS whole rows)
 // ---------------------------------------------------------------------------
@@ -166,25 +107,21 @@ struct XCfg {
   static constexpr int LS = H + 1;  // odd stride against LDS bank conflicts
 };
 
-// Fixed point -> FP64 -> FP32 (mass per cell), then R2C along x.
+// R2C along x of the FP32 mass-per-cell rows.  rho and P may be the same
+// buffer (a row of N reals has the bytes of N/2 complex values and every
+// workgroup reads its rows completely before writing them).
 template <int N, int ROWS>
-__global__ __launch_bounds__(256) void k_fft_x_fwd(const long long* __restrict__ mesh, const SolveScalars* sc,
-                                                   long long ntot, float2* __restrict__ P,
-                                                   const float2* __restrict__ tw) {
+__global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, const float2* __restrict__ tw) {
   constexpr int H = N / 2, T = 256, LS = XCfg<N>::LS;
   __shared__ float2 buf[ROWS * LS];
   const int tid = threadIdx.x;
   const long long row0 = (long long)blockIdx.x * ROWS;
-  const double unit = ldexp(1.0, -fixed_shift(sc, ntot));
-  const longlong2* src = reinterpret_cast<const longlong2*>(mesh + row0 * N);
+  const float2* src = reinterpret_cast<const float2*>(rho + row0 * N);
   constexpr int TOT = ROWS * H;
 #pragma unroll
   for (int p = 0; p < (TOT + T - 1) / T; ++p) {
     const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) {
-      const longlong2 v = src[e];
-      buf[(e / H) * LS + (e % H)] = make_float2((float)((double)v.x * unit), (float)((double)v.y * unit));
-    }
+    if ((TOT % T == 0) || e < TOT) buf[(e / H) * LS + (e % H)] = src[e];
   }
   __syncthreads();
   fft_lds<H, false, ROWS, T, LS, 1, false, N>(buf, tw, tid);
@@ -276,10 +213,12 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float
 // y / z transforms: a workgroup owns C adjacent kx columns of one plane
 // ---------------------------------------------------------------------------
 
-// Column tile width: the largest divisor of N/2 that is at most 8, so the
-// tiles cover every kx column exactly.
+// Column tile width: the largest divisor of N/2 that is at most 16, so the
+// tiles cover every kx column exactly and each strided row segment is a whole
+// 128-byte cache line (narrower tiles leave half of every line to another
+// workgroup, which fetches it from HBM again).
 __host__ __device__ constexpr int yz_cols(int h) {
-  int c = h < 8 ? h : 8;
+  int c = h < 16 ? h : 16;
   while (h % c != 0) --c;
   return c;
 }
@@ -288,29 +227,46 @@ template <int N>
 struct YZCfg {
   static constexpr int H = N / 2;
   static constexpr int C = yz_cols(H);
-  static constexpr int T = 256;
+  static constexpr int T = N >= 512 ? 512 : 256;
   static_assert(H % C == 0, "column tiles must cover the half spectrum");
 };
 
-template <int N, int C>
-__global__ __launch_bounds__(256) void k_fft_y_fwd(float2* __restrict__ P, const float2* __restrict__ tw) {
-  constexpr int H = N / 2, T = 256, IS = C + 1;
-  __shared__ float2 buf[N * IS];
-  const int tid = threadIdx.x;
-  float2* base = P + (size_t)blockIdx.y * N * H + (size_t)blockIdx.x * C;
+// Load / store a C-column tile of a plane-strided field into LDS columns
+// [0, C) (element (i, c) at buf[i*IS + c]); `stride` is the distance in
+// complex values between consecutive i.
+// Element offsets stay 32-bit (a whole nm <= 1024 half spectrum has fewer
+// than 2^32 complex values), so addressing is one scalar base plus a 32-bit
+// vector offset per element.
+template <int N, int C, int T, int IS>
+__device__ __forceinline__ void tile_load(float2* buf, const float2* __restrict__ src, unsigned stride, int tid) {
   constexpr int TOT = N * C;
 #pragma unroll
   for (int p = 0; p < (TOT + T - 1) / T; ++p) {
     const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) buf[(e / C) * IS + (e % C)] = base[(size_t)(e / C) * H + (e % C)];
+    if ((TOT % T == 0) || e < TOT) buf[(e / C) * IS + (e % C)] = src[(unsigned)(e / C) * stride + (unsigned)(e % C)];
   }
-  __syncthreads();
-  fft_lds<N, false, C, T, 1, IS, true, N>(buf, tw, tid);
+}
+
+template <int N, int C, int T, int IS>
+__device__ __forceinline__ void tile_store(float2* __restrict__ dst, const float2* buf, unsigned stride, int tid) {
+  constexpr int TOT = N * C;
 #pragma unroll
   for (int p = 0; p < (TOT + T - 1) / T; ++p) {
     const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) base[(size_t)(e / C) * H + (e % C)] = buf[(e / C) * IS + (e % C)];
+    if ((TOT % T == 0) || e < TOT) dst[(unsigned)(e / C) * stride + (unsigned)(e % C)] = buf[(e / C) * IS + (e % C)];
   }
+}
+
+template <int N, int C, int T>
+__global__ __launch_bounds__(T) void k_fft_y_fwd(float2* __restrict__ P, const float2* __restrict__ tw) {
+  constexpr int H = N / 2, IS = C + 1;
+  __shared__ float2 buf[N * IS];
+  const int tid = threadIdx.x;
+  float2* base = P + (size_t)blockIdx.y * N * H + (size_t)blockIdx.x * C;
+  tile_load<N, C, T, IS>(buf, base, H, tid);
+  __syncthreads();
+  fft_lds<N, false, C, T, 1, IS, true, N>(buf, tw, tid);
+  tile_store<N, C, T, IS>(base, buf, H, tid);
 }
 
 struct GreenParams {
@@ -322,31 +278,28 @@ struct GreenParams {
 
 __device__ __forceinline__ int freq_of(int j, int n) { return j <= n / 2 ? j : j - n; }
 
-template <int N, int C>
-__global__ __launch_bounds__(256) void k_fft_z_green(const float2* __restrict__ P, float2* __restrict__ A,
-                                                     float2* __restrict__ B, const float2* __restrict__ tw,
-                                                     const GreenParams gp) {
-  constexpr int H = N / 2, T = 256, IS = 2 * C + 1;
+// z transform, Green's function, then the inverse z transforms of A = phi and
+// B = -i n_z phi, one after the other through the same LDS columns.
+template <int N, int C, int T>
+__global__ __launch_bounds__(T) void k_fft_z_green(const float2* __restrict__ P, float2* __restrict__ A,
+                                                   float2* __restrict__ B, const float2* __restrict__ tw,
+                                                   const GreenParams gp) {
+  constexpr int H = N / 2, IS = C + 1;
   __shared__ float2 buf[N * IS];
   const int tid = threadIdx.x;
   const int tile = blockIdx.x, ky = blockIdx.y;
-  const size_t plane = (size_t)N * H;
-  const float2* src = P + (size_t)ky * H + (size_t)tile * C;
-  constexpr int TOT = N * C;
-#pragma unroll
-  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) buf[(e / C) * IS + (e % C)] = src[(size_t)(e / C) * plane + (e % C)];
-  }
+  const unsigned plane = (unsigned)N * H;
+  tile_load<N, C, T, IS>(buf, P + (size_t)ky * H + (size_t)tile * C, plane, tid);
   // partner column for the packed kx = 0 slot: Z(-ky, .) lives in row (N-ky)%N
   const int kym = (N - ky) % N;
   for (int z = tid; z < N; z += T)
-    buf[z * IS + C] = tile == 0 ? P[(size_t)z * plane + (size_t)kym * H] : make_float2(0.f, 0.f);
+    buf[z * IS + C] = tile == 0 ? P[(unsigned)z * plane + (unsigned)kym * H] : make_float2(0.f, 0.f);
   __syncthreads();
   fft_lds<N, false, C + 1, T, 1, IS, true, N>(buf, tw, tid);
 
+  constexpr int TOT = N * C;
   constexpr int PER = (TOT + T - 1) / T;
-  float2 ph[PER], pb[PER];
+  float2 ph[PER];
   const int ny = freq_of(ky, N);
   const float dy = gp.deconv ? gp.deconv[ky] : 1.0f;
 #pragma unroll
@@ -380,40 +333,38 @@ __global__ __launch_bounds__(256) void k_fft_z_green(const float2* __restrict__ 
         if (gp.deconv) g *= gp.deconv[kx] * dy * gp.deconv[i];
         phi = make_float2(g * r.x, g * r.y);
       }
-      const float nze = (i == N / 2) ? 0.0f : (float)nz;
       ph[p] = phi;
-      pb[p] = make_float2(nze * phi.y, -nze * phi.x);
     }
   }
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int e = tid + p * T;
+    if ((TOT % T == 0) || e < TOT) buf[(e / C) * IS + (e % C)] = ph[p];
+  }
+  __syncthreads();
+  fft_lds<N, true, C, T, 1, IS, true, N>(buf, tw, tid);
+  tile_store<N, C, T, IS>(A + (size_t)ky * H + (size_t)tile * C, buf, plane, tid);
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
     if ((TOT % T == 0) || e < TOT) {
-      const int i = e / C, c = e % C;
-      buf[i * IS + c] = ph[p];
-      buf[i * IS + C + 1 + c] = pb[p];
+      const int i = e / C;
+      const float nze = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
+      buf[i * IS + (e % C)] = make_float2(nze * ph[p].y, -nze * ph[p].x);
     }
   }
   __syncthreads();
-  fft_lds<N, true, 2 * C + 1, T, 1, IS, true, N>(buf, tw, tid);
-  float2* da = A + (size_t)ky * H + (size_t)tile * C;
-  float2* db = B + (size_t)ky * H + (size_t)tile * C;
-#pragma unroll
-  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) {
-      const int i = e / C, c = e % C;
-      da[(size_t)i * plane + c] = buf[i * IS + c];
-      db[(size_t)i * plane + c] = buf[i * IS + C + 1 + c];
-    }
-  }
+  fft_lds<N, true, C, T, 1, IS, true, N>(buf, tw, tid);
+  tile_store<N, C, T, IS>(B + (size_t)ky * H + (size_t)tile * C, buf, plane, tid);
 }
 
-template <int N, int C>
-__global__ __launch_bounds__(256) void k_fft_y_inv(float2* __restrict__ A, float2* __restrict__ B,
-                                                   float2* __restrict__ P, const float2* __restrict__ tw) {
-  constexpr int H = N / 2, T = 256, IS = 2 * C + 1;
+// Inverse y transforms: A -> A, -i n_y A -> P, B -> B, one after the other.
+template <int N, int C, int T>
+__global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2* __restrict__ B,
+                                                 float2* __restrict__ P, const float2* __restrict__ tw) {
+  constexpr int H = N / 2, IS = C + 1;
   __shared__ float2 buf[N * IS];
   const int tid = threadIdx.x;
   const size_t off = (size_t)blockIdx.y * N * H + (size_t)blockIdx.x * C;
@@ -426,44 +377,34 @@ __global__ __launch_bounds__(256) void k_fft_y_inv(float2* __restrict__ A, float
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) {
-      const int y = e / C, c = e % C;
-      av[p] = a[(size_t)y * H + c];
-      buf[y * IS + c] = av[p];
-      buf[y * IS + C + c] = b[(size_t)y * H + c];
-    }
+    if ((TOT % T == 0) || e < TOT) av[p] = a[(unsigned)(e / C) * H + (unsigned)(e % C)];
   }
-  __syncthreads();
-  fft_lds<N, true, 2 * C, T, 1, IS, true, N>(buf, tw, tid);
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) {
-      const int y = e / C, c = e % C;
-      a[(size_t)y * H + c] = buf[y * IS + c];
-      b[(size_t)y * H + c] = buf[y * IS + C + c];
-    }
+    if ((TOT % T == 0) || e < TOT) buf[(e / C) * IS + (e % C)] = av[p];
   }
+  __syncthreads();
+  fft_lds<N, true, C, T, 1, IS, true, N>(buf, tw, tid);
+  tile_store<N, C, T, IS>(a, buf, H, tid);
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int e = tid + p * T;
     if ((TOT % T == 0) || e < TOT) {
-      const int y = e / C, c = e % C;
+      const int y = e / C;
       const float nye = (y == N / 2) ? 0.0f : (float)freq_of(y, N);
-      buf[y * IS + c] = make_float2(nye * av[p].y, -nye * av[p].x);
+      buf[y * IS + (e % C)] = make_float2(nye * av[p].y, -nye * av[p].x);
     }
   }
   __syncthreads();
   fft_lds<N, true, C, T, 1, IS, true, N>(buf, tw, tid);
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) {
-      const int y = e / C, c = e % C;
-      pp[(size_t)y * H + c] = buf[y * IS + c];
-    }
-  }
+  tile_store<N, C, T, IS>(pp, buf, H, tid);
+  __syncthreads();
+  tile_load<N, C, T, IS>(buf, b, H, tid);
+  __syncthreads();
+  fft_lds<N, true, C, T, 1, IS, true, N>(buf, tw, tid);
+  tile_store<N, C, T, IS>(b, buf, H, tid);
 }
 
 // CIC gather of (Fx, Fy, Fz) at pos[idx[j]] (idx null: j itself).

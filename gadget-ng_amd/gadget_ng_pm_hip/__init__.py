@@ -45,8 +45,7 @@ GNG_ERR_NO_DEVICE = 3
 GNG_ERR_OOM = 4
 GNG_ERR_UNSUPPORTED = 5
 
-STAGES = ("mass_scan_clear", "deposit", "fft_x_fwd", "fft_y_fwd", "fft_z_green", "fft_y_inv", "fft_x_inv",
-          "interp")
+STAGES = ("bucket", "sort", "deposit", "fft_x_fwd", "fft_y_fwd", "fft_z_green", "fft_y_inv", "fft_x_inv", "interp")
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -102,12 +101,18 @@ def lib():
         L.gng_pm_accelerations_f32.argtypes = [P, P, P, P, P, i64, P, i64, d, P, P, P]
         L.gng_pm_solve_device.argtypes = [P, P, P, P, P, i64, P, i64, d, P, P, P, P]
         L.gng_pm_density.argtypes = [P, P]
-        L.gng_pm_density_fixed.argtypes = [P, P, ctypes.POINTER(ctypes.c_int32)]
+        L.gng_pm_keep_density.argtypes = [P, i32]
         L.gng_pm_force_meshes.argtypes = [P, P]
         L.gng_pm_set_timing.argtypes = [P, i32]
         L.gng_pm_stage_times.argtypes = [P, P, i32]
         for f in (L.gng_pm_accelerations_f64, L.gng_pm_accelerations_f32, L.gng_pm_solve_device, L.gng_pm_density,
-                  L.gng_pm_density_fixed, L.gng_pm_force_meshes, L.gng_pm_set_timing, L.gng_pm_stage_times):
+                  L.gng_pm_keep_density, L.gng_pm_force_meshes, L.gng_pm_set_timing, L.gng_pm_stage_times):
+            f.restype = i32
+        L.gng_dev_alloc.argtypes = [ctypes.POINTER(P), ctypes.c_uint64]
+        L.gng_dev_free.argtypes = [P]
+        L.gng_dev_copy.argtypes = [P, P, ctypes.c_uint64, i32]
+        L.gng_dev_synchronize.argtypes = []
+        for f in (L.gng_dev_alloc, L.gng_dev_free, L.gng_dev_copy, L.gng_dev_synchronize):
             f.restype = i32
         L.hip_pm_create.argtypes = [i32, f32]
         L.hip_pm_create.restype = P
@@ -157,7 +162,7 @@ class Plan:
     """Owns one gng_pm_plan (device meshes for one mesh size and filter set)."""
 
     def __init__(self, grid_size: int, box_size: float, r_split: float = 0.0, plummer_eps: float = 0.0,
-                 deconvolve: bool = False, device: int = -1):
+                 deconvolve: bool = False, device: int = -1, keep_density: bool = False):
         cfg = GngConfig()
         cfg.grid_size = int(grid_size)
         cfg.deconvolve = int(bool(deconvolve))
@@ -170,6 +175,8 @@ class Plan:
         if rc != GNG_OK:
             raise HipExecutionError("gng_pm_create", rc, last_error())
         self._h = h
+        if keep_density:
+            self._check(lib().gng_pm_keep_density(h, 1), "gng_pm_keep_density")
         self.grid_size = int(grid_size)
         self.box_size = float(box_size)
         self.r_split = float(r_split)
@@ -235,17 +242,11 @@ class Plan:
                                               ay_ptr, az_ptr, stream or None), "gng_pm_solve_device")
 
     def density(self) -> np.ndarray:
+        """Mass per cell of the last solve (needs keep_density=True)."""
         nm = self.grid_size
         out = np.empty((nm, nm, nm), dtype=np.float64)
         self._check(lib().gng_pm_density(self._h, _ptr(out)), "gng_pm_density")
         return out
-
-    def density_fixed(self):
-        nm = self.grid_size
-        out = np.empty((nm, nm, nm), dtype=np.int64)
-        s = ctypes.c_int32()
-        self._check(lib().gng_pm_density_fixed(self._h, _ptr(out), ctypes.byref(s)), "gng_pm_density_fixed")
-        return out, int(s.value)
 
     def force_meshes(self) -> np.ndarray:
         """(3, nm, nm, nm) f32 force meshes of the last solve."""
@@ -258,12 +259,79 @@ class Plan:
         self._check(lib().gng_pm_set_timing(self._h, int(on)), "gng_pm_set_timing")
 
     def stage_times(self) -> dict:
-        """Mean per-stage milliseconds over the solves recorded since set_timing(True)."""
+        """Mean per-stage milliseconds over the solves recorded since set_timing(True),
+        plus 'total' (first to last event of a solve)."""
         ms = np.zeros(16, dtype=np.float32)
         n = lib().gng_pm_stage_times(self._h, _ptr(ms), 16)
         if n < 0:
             raise HipExecutionError("gng_pm_stage_times", -n, last_error())
-        return {STAGES[i]: float(ms[i]) for i in range(min(n, len(STAGES)))}
+        out = {STAGES[i]: float(ms[i]) for i in range(min(n, len(STAGES)))}
+        out["total"] = float(ms[n])
+        return out
+
+
+class DeviceArray:
+    """A device buffer owned through the library's own HIP runtime (no torch needed)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        rc = lib().gng_dev_alloc(ctypes.byref(p), max(self.nbytes, 1))
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_dev_alloc", rc, last_error())
+        self._p = p
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    @property
+    def ptr(self) -> int:
+        return int(self._p.value or 0)
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        rc = lib().gng_dev_copy(self._p, _ptr(a), self.nbytes, 0)
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_dev_copy", rc, last_error())
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        rc = lib().gng_dev_copy(_ptr(out), self._p, self.nbytes, 1)
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_dev_copy", rc, last_error())
+        return out
+
+    def free(self) -> None:
+        if getattr(self, "_p", None) is not None and self._p.value:
+            lib().gng_dev_free(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_copy(dst: DeviceArray, src: DeviceArray) -> None:
+    """Device-to-device copy of src into dst (same byte size)."""
+    assert dst.nbytes == src.nbytes
+    rc = lib().gng_dev_copy(dst._p, src._p, src.nbytes, 2)
+    if rc != GNG_OK:
+        raise HipExecutionError("gng_dev_copy", rc, last_error())
+
+
+def device_synchronize() -> None:
+    rc = lib().gng_dev_synchronize()
+    if rc != GNG_OK:
+        raise HipExecutionError("gng_dev_synchronize", rc, last_error())
 
 
 def _as_positions(global_positions) -> np.ndarray:

@@ -121,14 +121,18 @@ int gng_pm_solve_device(gng_pm_plan *p, const float *d_x, const float *d_y, cons
                         const float *d_mass, int64_t n, const int64_t *d_idx, int64_t k, double g,
                         float *d_ax, float *d_ay, float *d_az, void *stream);
 
-/* Density of the last solve as mass per cell (nm^3 f64, flat iz*nm^2+iy*nm+ix,
- * the output of cic::assign, crates/gadget-ng-pm/src/cic.rs:260-292).
- * Synchronous. */
-int gng_pm_density(gng_pm_plan *p, double *out);
+/* Keep the density of the following solves for gng_pm_density (costs one
+ * extra nm^3 f32 buffer; off by default, when the density is written straight
+ * into the spectrum buffer). */
+int gng_pm_keep_density(gng_pm_plan *p, int enable);
 
-/* Raw fixed-point density of the last solve (nm^3 int64) and its scale:
- * mass = q * 2^-shift.  Bit-identical for identical inputs. */
-int gng_pm_density_fixed(gng_pm_plan *p, int64_t *out, int32_t *shift);
+/* Density of the last solve as mass per cell (nm^3 values widened to f64,
+ * flat iz*nm^2+iy*nm+ix: the output of cic::assign,
+ * crates/gadget-ng-pm/src/cic.rs:260-292).  The deposit sums 64-bit fixed
+ * point and rounds once to f32, so the mesh is bit-identical for identical
+ * inputs in any particle order.  Needs gng_pm_keep_density(p, 1) before the
+ * solve.  Synchronous. */
+int gng_pm_density(gng_pm_plan *p, double *out);
 
 /* Force meshes of the last solve (3 x nm^3 f32: F_x, F_y, F_z, flat
  * iz*nm^2+iy*nm+ix), the output of fft_poisson::solve_forces
@@ -139,11 +143,24 @@ int gng_pm_force_meshes(gng_pm_plan *p, float *out);
  * stream between the stages of every solve issued while timing is enabled
  * (gng_pm_set_timing resets the record), averaged over the most recent 64
  * such solves.  Recording does not synchronise; gng_pm_stage_times waits for
- * the recorded events.  Stages, in order: 0 mass scan + clear, 1 deposit,
- * 2 x forward, 3 y forward, 4 z + Green, 5 y inverse, 6 x inverse,
- * 7 interpolation.  Returns the stage count (negative on error). */
+ * the recorded events.  Stages, in order: 0 bucket count (tile/class of
+ * every particle, max mass), 1 sort (scan + scatter), 2 deposit, 3 x forward,
+ * 4 y forward, 5 z + Green, 6 y inverse, 7 x inverse, 8 interpolation;
+ * ms[9], when capacity > 9, receives the mean whole-solve time.  Returns the
+ * stage count (negative on error). */
 int gng_pm_set_timing(gng_pm_plan *p, int enable);
 int gng_pm_stage_times(gng_pm_plan *p, float *ms, int capacity);
+
+/* ---- 3. device-memory helpers --------------------------------------------
+ * No counterpart in the reference.  For callers that hold no GPU runtime of
+ * their own (test and benchmark harnesses, FFI hosts) and need device-resident
+ * inputs for gng_pm_solve_device; they run on the HIP runtime this library
+ * links, so a process never mixes two runtimes.  kind: 0 host->device,
+ * 1 device->host, 2 device->device.  Synchronous. */
+int gng_dev_alloc(void **ptr, uint64_t bytes);
+int gng_dev_free(void *ptr);
+int gng_dev_copy(void *dst, const void *src, uint64_t bytes, int kind);
+int gng_dev_synchronize(void);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -1,0 +1,121 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol
+include/gng_pm.h declares, and keeps the reference's error contract
+(crates/gadget-ng-hip: hip_pm_create -> NULL on failure, hip_pm_solve with a
+NULL handle or n <= 0 is a no-op returning 0; HipPmSolver::try_new -> None).
+No compute call is made."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import gadget_ng_pm_hip as pm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "gng_pm.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:gng|hip)_\w+)\s*\(", text)))
+
+
+def has_device():
+    return pm.lib().gng_pm_device_count() > 0
+
+
+def test_header_declares_the_reference_symbols():
+    names = declared_functions()
+    for sym in ("hip_pm_create", "hip_pm_destroy", "hip_pm_solve"):
+        assert sym in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(pm.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_abi_version_and_mesh_table():
+    L = pm.lib()
+    assert L.gng_pm_abi_version() == 1
+    for nm in (4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 1024):
+        assert L.gng_pm_mesh_supported(nm) == 1, nm
+    for nm in (0, 1, 2, 3, 5, 7, 10, 100, 2048, -8):
+        assert L.gng_pm_mesh_supported(nm) == 0, nm
+
+
+def test_unsupported_mesh_is_rejected_before_touching_a_device():
+    cfg = pm.GngConfig()
+    cfg.grid_size = 100
+    cfg.box_size = 1.0
+    cfg.device = -1
+    h = ctypes.c_void_p()
+    rc = pm.lib().gng_pm_create(ctypes.byref(cfg), ctypes.byref(h))
+    assert rc == pm.GNG_ERR_UNSUPPORTED and not h.value
+    assert "not compiled in" in pm.last_error()
+
+
+def test_invalid_box_is_rejected():
+    cfg = pm.GngConfig()
+    cfg.grid_size = 16
+    cfg.box_size = -2.0
+    h = ctypes.c_void_p()
+    assert pm.lib().gng_pm_create(ctypes.byref(cfg), ctypes.byref(h)) == pm.GNG_ERR_INVALID
+
+
+def test_null_plan_calls_return_invalid():
+    L = pm.lib()
+    out = np.zeros(3)
+    assert L.gng_pm_accelerations_f64(None, None, None, 0, None, 0, 0.0, 1.0, out.ctypes.data) == pm.GNG_ERR_INVALID
+    assert L.gng_pm_density(None, None) == pm.GNG_ERR_INVALID
+    L.gng_pm_destroy(None)  # NULL-safe
+
+
+def test_hip_pm_solve_null_handle_is_noop():
+    """pm_gravity.hip:370: `if (!h || n <= 0) return 0;`"""
+    L = pm.lib()
+    a = np.zeros(4, dtype=np.float32)
+    p = a.ctypes.data
+    assert L.hip_pm_solve(None, p, p, p, p, p, p, p, 4, 0.0, 1.0, 0.0) == 0
+    assert L.hip_pm_solve(None, p, p, p, p, p, p, p, 0, 0.0, 1.0, 0.0) == 0
+    L.hip_pm_destroy(None)
+
+
+@pytest.mark.skipif("has_device()")
+def test_without_device_create_fails_cleanly():
+    L = pm.lib()
+    assert L.gng_pm_device_count() == 0
+    assert not L.hip_pm_create(16, ctypes.c_float(1.0))
+    cfg = pm.GngConfig()
+    cfg.grid_size = 16
+    cfg.box_size = 1.0
+    cfg.device = -1
+    h = ctypes.c_void_p()
+    assert L.gng_pm_create(ctypes.byref(cfg), ctypes.byref(h)) == pm.GNG_ERR_NO_DEVICE
+
+
+@pytest.mark.skipif("has_device()")
+def test_without_device_solver_is_unavailable():
+    av = pm.HipPmSolver.availability()
+    assert av.compiled and not av.runtime_available
+    assert not pm.HipPmSolver.is_available()
+    assert pm.HipPmSolver.try_new(16, 1.0) is None
+    assert pm.HipPmSolver.try_new_with_r_split(16, 1.0, 0.2) is None
+    with pytest.raises(pm.HipUnavailable):
+        pm.HipPmSolver.try_new_checked(16, 1.0)
+    with pytest.raises(pm.HipExecutionError):
+        pm.PmSolver(16, 1.0).accelerations_for_indices(np.zeros((1, 3)), np.ones(1), 0.0, 1.0, [0])
+
+
+def test_config_struct_layout_matches_header():
+    # int32, int32, double, double, double, int32, int32[7] -> 64 bytes with padding
+    assert ctypes.sizeof(pm.GngConfig) == 64
+    assert pm.GngConfig.box_size.offset == 8 and pm.GngConfig.device.offset == 32
+
+
+def test_treepm_effective_r_split():
+    lr = pm.TreePmLongRange(64, 128.0)
+    assert lr.effective_r_split() == pytest.approx(2.5 * 128.0 / 64)
+    assert pm.TreePmLongRange(64, 128.0, 0.7).effective_r_split() == 0.7
