@@ -170,4 +170,168 @@ __device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ 
   FftStages<N, 1, INV, LINES, T, LS, IS, LINE_FAST, TWN>::run(buf, tw, tid);
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident transform
+//
+// The radix plan of length N is the greedy sequence of fft_pick_radix (8s
+// first); REV runs it backwards.  Stage 0 takes its inputs from registers and
+// stage S-1 leaves its outputs in registers, both in the "edge layout" of
+// their radix R: butterfly slot b of the thread is (line l, j) and holds the
+// R elements at indices j + r*N/R, r < R.  Middle stages go through LDS.
+// Because the last stage's output layout equals the input layout of a first
+// stage with the same radix, a transform on the reversed plan can consume
+// another transform's outputs straight from registers.
+// ---------------------------------------------------------------------------
+
+__host__ __device__ constexpr int plan_len(int n) {
+  int s = 0, ns = 1;
+  while (ns < n) {
+    ns *= fft_pick_radix(n, ns);
+    ++s;
+  }
+  return s;
+}
+__host__ __device__ constexpr int plan_radix(int n, int s) {
+  int ns = 1;
+  for (int i = 0; i < s; ++i) ns *= fft_pick_radix(n, ns);
+  return fft_pick_radix(n, ns);
+}
+template <int N, bool REV>
+__host__ __device__ constexpr int radix_at(int s) {
+  return REV ? plan_radix(N, plan_len(N) - 1 - s) : plan_radix(N, s);
+}
+template <int N, bool REV>
+__host__ __device__ constexpr int ns_at(int s) {
+  int ns = 1;
+  for (int i = 0; i < s; ++i) ns *= radix_at<N, REV>(i);
+  return ns;
+}
+
+// Butterfly slots of a radix-R stage over LINES lines shared by T threads.
+template <int N, int R, int LINES, int T>
+struct Slots {
+  static constexpr int NB = N / R;
+  static constexpr int TOTAL = LINES * NB;
+  static constexpr int PER = (TOTAL + T - 1) / T;
+};
+
+template <int N, int R, int LINES, bool LF>
+__device__ __forceinline__ void slot_lj(int b, int& l, int& j) {
+  constexpr int NB = N / R;
+  l = LF ? b % LINES : b / NB;
+  j = LF ? b / LINES : b % NB;
+}
+
+// Twiddle then radix-R DFT of one butterfly of stage NS.
+template <int N, int R, int NS, bool INV, int TWN>
+__device__ __forceinline__ void butterfly(float2* v, int j, const float2* __restrict__ tw) {
+  if (NS > 1) {
+    constexpr int TSTEP = (N / (NS * R)) * (TWN / N);
+    const int k = j % NS;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      float2 w = tw[r * k * TSTEP];
+      if (INV) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+  }
+  Dft<R, INV>::run(v);
+}
+
+// Edge-layout element loop: f(p, r, line, index, valid) for every slot.
+template <int N, int R, int LINES, int T, bool LF, typename F>
+__device__ __forceinline__ void for_edge(int tid, F f) {
+  using S = Slots<N, R, LINES, T>;
+#pragma unroll
+  for (int p = 0; p < S::PER; ++p) {
+    const int b = tid + p * T;
+    const bool ok = (S::TOTAL % T == 0) || b < S::TOTAL;
+    int l = 0, j = 0;
+    if (ok) slot_lj<N, R, LINES, LF>(b, l, j);
+#pragma unroll
+    for (int r = 0; r < R; ++r) f(p, r, l, j + r * S::NB, ok);
+  }
+}
+
+// v is float2[Slots<N, R, LINES, T>::PER][R].
+template <int N, int R, int NS, bool INV, int LINES, int T, int LS, int IS, bool LF, int TWN, typename V>
+__device__ __forceinline__ void stage_regs_to_lds(V& v, float2* buf, const float2* __restrict__ tw, int tid) {
+  using S = Slots<N, R, LINES, T>;
+#pragma unroll
+  for (int p = 0; p < S::PER; ++p) {
+    const int b = tid + p * T;
+    if ((S::TOTAL % T == 0) || b < S::TOTAL) {
+      int l, j;
+      slot_lj<N, R, LINES, LF>(b, l, j);
+      butterfly<N, R, NS, INV, TWN>(v[p], j, tw);
+      const int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[l * LS + (base + r * NS) * IS] = v[p][r];
+    }
+  }
+}
+
+template <int N, int R, int NS, bool INV, int LINES, int T, int LS, int IS, bool LF, int TWN, typename V>
+__device__ __forceinline__ void stage_lds_to_regs(V& v, const float2* buf, const float2* __restrict__ tw, int tid) {
+  static_assert(NS * R == N, "only the last stage leaves its outputs in registers");
+  using S = Slots<N, R, LINES, T>;
+#pragma unroll
+  for (int p = 0; p < S::PER; ++p) {
+    const int b = tid + p * T;
+    if ((S::TOTAL % T == 0) || b < S::TOTAL) {
+      int l, j;
+      slot_lj<N, R, LINES, LF>(b, l, j);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[p][r] = buf[l * LS + (j + r * S::NB) * IS];
+      butterfly<N, R, NS, INV, TWN>(v[p], j, tw);
+    }
+  }
+}
+
+template <int N, bool REV, int S, bool INV, int LINES, int T, int LS, int IS, bool LF, int TWN>
+struct MiddleStages {
+  __device__ __forceinline__ static void run(float2* buf, const float2* __restrict__ tw, int tid) {
+    if constexpr (S < plan_len(N) - 1) {
+      fft_stage<N, radix_at<N, REV>(S), ns_at<N, REV>(S), INV, LINES, T, LS, IS, LF, TWN>(buf, tw, tid);
+      MiddleStages<N, REV, S + 1, INV, LINES, T, LS, IS, LF, TWN>::run(buf, tw, tid);
+    }
+  }
+};
+
+// Register-resident transform of LINES lines: vin in the edge layout of the
+// plan's first radix (consumed), vout in the edge layout of its last radix.
+// Uses buf as exchange space; begins with a barrier, so the caller need not
+// synchronise before it, and leaves LDS readable only after a barrier.
+template <int N, bool INV, bool REV, int LINES, int T, int LS, int IS, bool LF, int TWN>
+struct RegFft {
+  static constexpr int S = plan_len(N);
+  static constexpr int R0 = radix_at<N, REV>(0);
+  static constexpr int RL = radix_at<N, REV>(S - 1);
+  static constexpr int PER0 = Slots<N, R0, LINES, T>::PER;
+  static constexpr int PERL = Slots<N, RL, LINES, T>::PER;
+  static_assert(fft_size_ok(N), "FFT length must be 2^a 3^b");
+
+  __device__ __forceinline__ static void run(float2 (&vin)[PER0][R0], float2 (&vout)[PERL][RL], float2* buf,
+                                             const float2* __restrict__ tw, int tid) {
+    if constexpr (S == 1) {
+      using Sl = Slots<N, R0, LINES, T>;
+#pragma unroll
+      for (int p = 0; p < PER0; ++p) {
+        const int b = tid + p * T;
+        if ((Sl::TOTAL % T == 0) || b < Sl::TOTAL) {
+          Dft<R0, INV>::run(vin[p]);
+#pragma unroll
+          for (int r = 0; r < R0; ++r) vout[p][r] = vin[p][r];
+        }
+      }
+    } else {
+      __syncthreads();
+      stage_regs_to_lds<N, R0, 1, INV, LINES, T, LS, IS, LF, TWN>(vin, buf, tw, tid);
+      __syncthreads();
+      MiddleStages<N, REV, 1, INV, LINES, T, LS, IS, LF, TWN>::run(buf, tw, tid);
+      stage_lds_to_regs<N, RL, ns_at<N, REV>(S - 1), INV, LINES, T, LS, IS, LF, TWN>(vout, buf, tw, tid);
+    }
+  }
+};
+
 }  // namespace gng

@@ -84,6 +84,7 @@ struct MeshOps {
   void (*x_fwd)(dim3, dim3, hipStream_t, const float*, float2*, const float2*);
   void (*y_fwd)(dim3, dim3, hipStream_t, float2*, const float2*);
   void (*z_green)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
+  void (*z_green_col0)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
   void (*y_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
   void (*x_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
 };
@@ -113,6 +114,8 @@ MeshOps make_ops() {
   };
   o.z_green = [](dim3 g, dim3 b, hipStream_t s, const float2* P, float2* A, float2* B, const float2* tw,
                  GreenParams gp) { hipLaunchKernelGGL((k_fft_z_green<N, C, TY>), g, b, 0, s, P, A, B, tw, gp); };
+  o.z_green_col0 = [](dim3 g, dim3 b, hipStream_t s, const float2* P, float2* A, float2* B, const float2* tw,
+                      GreenParams gp) { hipLaunchKernelGGL((k_fft_z_green_col0<N>), g, b, 0, s, P, A, B, tw, gp); };
   o.y_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* B, float2* P, const float2* tw) {
     hipLaunchKernelGGL((k_fft_y_inv<N, C, TY>), g, b, 0, s, A, B, P, tw);
   };
@@ -298,7 +301,9 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   const dim3 yz_block((unsigned)o.yz_threads);
   o.y_fwd(yz_grid, yz_block, s, p->d_P, p->d_tw);
   mark(p, 5, s);
-  o.z_green(yz_grid, yz_block, s, p->d_P, p->d_A, p->d_B, p->d_tw, green_params(p, g));
+  const GreenParams gp = green_params(p, g);
+  o.z_green(yz_grid, yz_block, s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
+  o.z_green_col0(dim3((unsigned)(H + 1)), dim3(256), s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
   mark(p, 6, s);
   o.y_inv(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
   mark(p, 7, s);

@@ -109,23 +109,29 @@ struct XCfg {
 
 // R2C along x of the FP32 mass-per-cell rows.  rho and P may be the same
 // buffer (a row of N reals has the bytes of N/2 complex values and every
-// workgroup reads its rows completely before writing them).
+// workgroup reads its rows completely before writing them).  The rows are
+// read straight into the registers of the first FFT stage; the pairwise
+// post-processing (X_k needs Z_k and Z_{H-k}) exchanges through LDS.
 template <int N, int ROWS>
 __global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, const float2* __restrict__ tw) {
   constexpr int H = N / 2, T = 256, LS = XCfg<N>::LS;
+  using F = RegFft<H, false, false, ROWS, T, LS, 1, false, N>;
   __shared__ float2 buf[ROWS * LS];
   const int tid = threadIdx.x;
   const long long row0 = (long long)blockIdx.x * ROWS;
   const float2* src = reinterpret_cast<const float2*>(rho + row0 * N);
-  constexpr int TOT = ROWS * H;
-#pragma unroll
-  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) buf[(e / H) * LS + (e % H)] = src[e];
-  }
+  float2 vin[F::PER0][F::R0], vout[F::PERL][F::RL];
+  for_edge<H, F::R0, ROWS, T, false>(tid, [&](int p, int r, int l, int i, bool ok) {
+    if (ok) vin[p][r] = src[l * H + i];
+  });
+  F::run(vin, vout, buf, tw, tid);
   __syncthreads();
-  fft_lds<H, false, ROWS, T, LS, 1, false, N>(buf, tw, tid);
+  for_edge<H, F::RL, ROWS, T, false>(tid, [&](int p, int r, int l, int i, bool ok) {
+    if (ok) buf[l * LS + i] = vout[p][r];
+  });
+  __syncthreads();
   float2* dst = P + row0 * H;
+  constexpr int TOT = ROWS * H;
 #pragma unroll
   for (int p = 0; p < (TOT + T - 1) / T; ++p) {
     const int e = tid + p * T;
@@ -148,10 +154,13 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, 
 }
 
 // C2R along x, in place, for field blockIdx.y (0: A with -i n_x, 1: P, 2: B).
+// The pairwise pre-processing reads the row from LDS and feeds the first FFT
+// stage in registers; the last stage stores the real row directly.
 template <int N, int ROWS>
 __global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float2* __restrict__ P,
                                                    float2* __restrict__ B, const float2* __restrict__ tw) {
   constexpr int H = N / 2, T = 256, LS = XCfg<N>::LS;
+  using F = RegFft<H, true, false, ROWS, T, LS, 1, false, N>;
   __shared__ float2 buf[ROWS * LS];
   const int tid = threadIdx.x;
   const int field = blockIdx.y;
@@ -159,54 +168,38 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float
   const long long row0 = (long long)blockIdx.x * ROWS;
   float2* row = data + row0 * H;
   constexpr int TOT = ROWS * H;
-  constexpr int PER = (TOT + T - 1) / T;
 #pragma unroll
-  for (int p = 0; p < PER; ++p) {
+  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
     const int e = tid + p * T;
     if ((TOT % T == 0) || e < TOT) {
       float2 v = row[e];
       if (field == 0) {
         // -i n_x; slot 0 holds DC (n=0) and Nyquist (n_eff=0): both vanish
-        const int k = e % H;
-        const float nx = (float)k;
+        const float nx = (float)(e % H);
         v = make_float2(nx * v.y, -nx * v.x);
       }
       buf[(e / H) * LS + (e % H)] = v;
     }
   }
   __syncthreads();
-  float2 zr[PER];
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) {
-      const int r = e / H, k = e % H;
-      const float2 y = buf[r * LS + k];
-      if (k == 0) {
-        zr[p] = make_float2(y.x + y.y, y.x - y.y);
-      } else {
-        const float2 xc = cconj(buf[r * LS + (H - k)]);
-        const float2 s = cadd(y, xc);
-        const float2 d = csub(y, xc);
-        // Z = (X_k + conj X_{H-k}) + i (X_k - conj X_{H-k}) e^{+2 pi i k/N}
-        const float2 t = cmul(d, cconj(tw[k]));
-        zr[p] = make_float2(s.x - t.y, s.y + t.x);
-      }
+  float2 vin[F::PER0][F::R0], vout[F::PERL][F::RL];
+  for_edge<H, F::R0, ROWS, T, false>(tid, [&](int p, int r, int l, int k, bool ok) {
+    if (!ok) return;
+    const float2 y = buf[l * LS + k];
+    if (k == 0) {
+      vin[p][r] = make_float2(y.x + y.y, y.x - y.y);
+    } else {
+      const float2 xc = cconj(buf[l * LS + (H - k)]);
+      const float2 sm = cadd(y, xc);
+      // Z = (X_k + conj X_{H-k}) + i (X_k - conj X_{H-k}) e^{+2 pi i k/N}
+      const float2 t = cmul(csub(y, xc), cconj(tw[k]));
+      vin[p][r] = make_float2(sm.x - t.y, sm.y + t.x);
     }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) buf[(e / H) * LS + (e % H)] = zr[p];
-  }
-  __syncthreads();
-  fft_lds<H, true, ROWS, T, LS, 1, false, N>(buf, tw, tid);
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) row[e] = buf[(e / H) * LS + (e % H)];
-  }
+  });
+  F::run(vin, vout, buf, tw, tid);
+  for_edge<H, F::RL, ROWS, T, false>(tid, [&](int p, int r, int l, int i, bool ok) {
+    if (ok) row[l * H + i] = vout[p][r];
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -226,47 +219,34 @@ __host__ __device__ constexpr int yz_cols(int h) {
 template <int N>
 struct YZCfg {
   static constexpr int H = N / 2;
-  static constexpr int C = yz_cols(H);
-  static constexpr int T = N >= 512 ? 512 : 256;
+  // 16 columns, or 8 at N = 1024 so the thread rule below stays within 1024
+  static constexpr int C = N >= 1024 ? yz_cols(H < 8 ? H : 8) : yz_cols(H);
+  // one radix-8 butterfly (8 complex values) per thread and register array:
+  // C * N / 8 threads, rounded up to whole wavefronts, at most 1024.  More
+  // values per thread push the register count past 128 and halve occupancy.
+  static constexpr int T_RAW = C * N / 8;
+  static constexpr int T = T_RAW >= 1024 ? 1024 : (T_RAW <= 64 ? 64 : (T_RAW + 63) / 64 * 64);
   static_assert(H % C == 0, "column tiles must cover the half spectrum");
 };
 
 // Load / store a C-column tile of a plane-strided field into LDS columns
 // [0, C) (element (i, c) at buf[i*IS + c]); `stride` is the distance in
 // complex values between consecutive i.
-// Element offsets stay 32-bit (a whole nm <= 1024 half spectrum has fewer
-// than 2^32 complex values), so addressing is one scalar base plus a 32-bit
-// vector offset per element.
-template <int N, int C, int T, int IS>
-__device__ __forceinline__ void tile_load(float2* buf, const float2* __restrict__ src, unsigned stride, int tid) {
-  constexpr int TOT = N * C;
-#pragma unroll
-  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) buf[(e / C) * IS + (e % C)] = src[(unsigned)(e / C) * stride + (unsigned)(e % C)];
-  }
-}
-
-template <int N, int C, int T, int IS>
-__device__ __forceinline__ void tile_store(float2* __restrict__ dst, const float2* buf, unsigned stride, int tid) {
-  constexpr int TOT = N * C;
-#pragma unroll
-  for (int p = 0; p < (TOT + T - 1) / T; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) dst[(unsigned)(e / C) * stride + (unsigned)(e % C)] = buf[(e / C) * IS + (e % C)];
-  }
-}
-
 template <int N, int C, int T>
 __global__ __launch_bounds__(T) void k_fft_y_fwd(float2* __restrict__ P, const float2* __restrict__ tw) {
   constexpr int H = N / 2, IS = C + 1;
+  using F = RegFft<N, false, false, C, T, 1, IS, true, N>;
   __shared__ float2 buf[N * IS];
   const int tid = threadIdx.x;
   float2* base = P + (size_t)blockIdx.y * N * H + (size_t)blockIdx.x * C;
-  tile_load<N, C, T, IS>(buf, base, H, tid);
-  __syncthreads();
-  fft_lds<N, false, C, T, 1, IS, true, N>(buf, tw, tid);
-  tile_store<N, C, T, IS>(base, buf, H, tid);
+  float2 vin[F::PER0][F::R0], vout[F::PERL][F::RL];
+  for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    if (ok) vin[p][r] = base[(unsigned)i * H + l];
+  });
+  F::run(vin, vout, buf, tw, tid);
+  for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    if (ok) base[(unsigned)i * H + l] = vout[p][r];
+  });
 }
 
 struct GreenParams {
@@ -279,132 +259,168 @@ struct GreenParams {
 __device__ __forceinline__ int freq_of(int j, int n) { return j <= n / 2 ? j : j - n; }
 
 // z transform, Green's function, then the inverse z transforms of A = phi and
-// B = -i n_z phi, one after the other through the same LDS columns.
+// B = -i n_z phi, for kx >= 1.  The forward transform leaves rho-hat in
+// registers, the Green's function is applied there, and the inverse runs on
+// the reversed radix plan so it starts from those registers.  The packed
+// kx = 0 column is done by k_fft_z_green_col0.
 template <int N, int C, int T>
 __global__ __launch_bounds__(T) void k_fft_z_green(const float2* __restrict__ P, float2* __restrict__ A,
                                                    float2* __restrict__ B, const float2* __restrict__ tw,
                                                    const GreenParams gp) {
   constexpr int H = N / 2, IS = C + 1;
+  using FF = RegFft<N, false, false, C, T, 1, IS, true, N>;
+  using FI = RegFft<N, true, true, C, T, 1, IS, true, N>;
+  static_assert(FI::R0 == FF::RL && FI::PER0 == FF::PERL, "inverse must start in the forward output layout");
   __shared__ float2 buf[N * IS];
   const int tid = threadIdx.x;
   const int tile = blockIdx.x, ky = blockIdx.y;
   const unsigned plane = (unsigned)N * H;
-  tile_load<N, C, T, IS>(buf, P + (size_t)ky * H + (size_t)tile * C, plane, tid);
-  // partner column for the packed kx = 0 slot: Z(-ky, .) lives in row (N-ky)%N
-  const int kym = (N - ky) % N;
-  for (int z = tid; z < N; z += T)
-    buf[z * IS + C] = tile == 0 ? P[(unsigned)z * plane + (unsigned)kym * H] : make_float2(0.f, 0.f);
-  __syncthreads();
-  fft_lds<N, false, C + 1, T, 1, IS, true, N>(buf, tw, tid);
-
-  constexpr int TOT = N * C;
-  constexpr int PER = (TOT + T - 1) / T;
-  float2 ph[PER];
+  const size_t col0 = (size_t)ky * H + (size_t)tile * C;
+  float2 vin[FF::PER0][FF::R0], ph[FF::PERL][FF::RL];
+  for_edge<N, FF::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    if (ok) vin[p][r] = P[col0 + (unsigned)i * plane + l];
+  });
+  FF::run(vin, ph, buf, tw, tid);
   const int ny = freq_of(ky, N);
   const float dy = gp.deconv ? gp.deconv[ky] : 1.0f;
+  for_edge<N, FF::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    if (!ok) return;
+    const int kx = tile * C + l;
+    const int nz = freq_of(i, N);
+    const int n2 = kx * kx + ny * ny + nz * nz;
+    const float fn2 = (float)(n2 > 0 ? n2 : 1);
+    float g = kx == 0 ? 0.0f : -gp.c * expf(-gp.a * fn2) / fn2;
+    if (gp.deconv) g *= gp.deconv[kx] * dy * gp.deconv[i];
+    ph[p][r] = make_float2(g * ph[p][r].x, g * ph[p][r].y);
+  });
+  float2 w[FI::PER0][FI::R0];
 #pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) {
-      const int i = e / C, c = e % C;
-      const int kx = tile * C + c;
-      const int nz = freq_of(i, N);
-      const float2 r = buf[i * IS + c];
-      float2 phi;
-      if (kx == 0) {
-        // packed DC / Nyquist planes: FP64 Green's factors
-        const double n2a = (double)(ny * ny + nz * nz);
-        const double n2b = n2a + (double)(N / 2) * (double)(N / 2);
-        double dd = 1.0, dn = 1.0;
-        if (gp.deconv) {
-          const double dyz = (double)dy * (double)gp.deconv[i];
-          dd = (double)gp.deconv[0] * dyz;
-          dn = (double)gp.deconv[N / 2] * dyz;
-        }
-        const double g0 = n2a == 0.0 ? 0.0 : -gp.c64 * exp(-gp.a64 * n2a) / n2a * dd;
-        const double gn = -gp.c64 * exp(-gp.a64 * n2b) / n2b * dn;
-        const float sp = (float)(0.5 * (g0 + gn)), sm = (float)(0.5 * (g0 - gn));
-        const float2 zm = cconj(buf[((N - i) % N) * IS + C]);
-        phi = make_float2(sp * r.x + sm * zm.x, sp * r.y + sm * zm.y);
-      } else {
-        const int n2 = kx * kx + ny * ny + nz * nz;
-        const float fn2 = (float)n2;
-        float g = -gp.c * expf(-gp.a * fn2) / fn2;
-        if (gp.deconv) g *= gp.deconv[kx] * dy * gp.deconv[i];
-        phi = make_float2(g * r.x, g * r.y);
-      }
-      ph[p] = phi;
-    }
-  }
-  __syncthreads();
+  for (int p = 0; p < FI::PER0; ++p)
 #pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) buf[(e / C) * IS + (e % C)] = ph[p];
+    for (int r = 0; r < FI::R0; ++r) w[p][r] = ph[p][r];
+  {
+    float2 out[FI::PERL][FI::RL];
+    FI::run(w, out, buf, tw, tid);
+    for_edge<N, FI::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      if (ok && (tile | l)) A[col0 + (unsigned)i * plane + l] = out[p][r];
+    });
   }
-  __syncthreads();
-  fft_lds<N, true, C, T, 1, IS, true, N>(buf, tw, tid);
-  tile_store<N, C, T, IS>(A + (size_t)ky * H + (size_t)tile * C, buf, plane, tid);
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) {
-      const int i = e / C;
-      const float nze = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
-      buf[i * IS + (e % C)] = make_float2(nze * ph[p].y, -nze * ph[p].x);
-    }
+  for_edge<N, FI::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    const float nze = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
+    w[p][r] = make_float2(nze * ph[p][r].y, -nze * ph[p][r].x);
+  });
+  {
+    float2 out[FI::PERL][FI::RL];
+    FI::run(w, out, buf, tw, tid);
+    for_edge<N, FI::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      if (ok && (tile | l)) B[col0 + (unsigned)i * plane + l] = out[p][r];
+    });
   }
-  __syncthreads();
-  fft_lds<N, true, C, T, 1, IS, true, N>(buf, tw, tid);
-  tile_store<N, C, T, IS>(B + (size_t)ky * H + (size_t)tile * C, buf, plane, tid);
 }
 
-// Inverse y transforms: A -> A, -i n_y A -> P, B -> B, one after the other.
+// The packed kx = 0 column: slot 0 holds Z = R0 + i RN, the spectra of the
+// kx = 0 and kx = N/2 planes, whose Green's functions differ, so
+//   phi(ky, kz) = (G0 + GN)/2 Z(ky, kz) + (G0 - GN)/2 conj(Z(-ky, -kz)).
+// One workgroup per pair {ky, N - ky} transforms both z lines, combines them
+// with FP64 Green's factors and writes A and B of both.
+template <int N>
+__global__ __launch_bounds__(256) void k_fft_z_green_col0(const float2* __restrict__ P, float2* __restrict__ A,
+                                                          float2* __restrict__ B, const float2* __restrict__ tw,
+                                                          const GreenParams gp) {
+  constexpr int H = N / 2, T = 256, LS = N + 1;
+  __shared__ float2 buf[4 * LS];
+  const int tid = threadIdx.x;
+  const int ky = blockIdx.x, kym = (N - ky) % N;
+  const size_t plane = (size_t)N * H;
+  for (int e = tid; e < 2 * N; e += T) {
+    const int l = e / N, z = e % N;
+    buf[l * LS + z] = P[(size_t)z * plane + (size_t)(l ? kym : ky) * H];
+  }
+  __syncthreads();
+  fft_lds<N, false, 2, T, LS, 1, false, N>(buf, tw, tid);
+  constexpr int PER = (2 * N + T - 1) / T;
+  float2 ph[PER];
+  const int ny = freq_of(ky, N);  // |ny| is the same for both lines
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if (e < 2 * N) {
+      const int l = e / N, i = e % N;
+      const int nz = freq_of(i, N);
+      const double n2a = (double)(ny * ny + nz * nz);
+      const double n2b = n2a + (double)(N / 2) * (double)(N / 2);
+      double dd = 1.0, dn = 1.0;
+      if (gp.deconv) {
+        const double dyz = (double)gp.deconv[l ? kym : ky] * (double)gp.deconv[i];
+        dd = (double)gp.deconv[0] * dyz;
+        dn = (double)gp.deconv[N / 2] * dyz;
+      }
+      const double g0 = n2a == 0.0 ? 0.0 : -gp.c64 * exp(-gp.a64 * n2a) / n2a * dd;
+      const double gn = -gp.c64 * exp(-gp.a64 * n2b) / n2b * dn;
+      const float sp = (float)(0.5 * (g0 + gn)), sm = (float)(0.5 * (g0 - gn));
+      const float2 r = buf[l * LS + i];
+      const float2 zm = cconj(buf[(1 - l) * LS + (N - i) % N]);
+      ph[p] = make_float2(sp * r.x + sm * zm.x, sp * r.y + sm * zm.y);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int e = tid + p * T;
+    if (e < 2 * N) {
+      const int l = e / N, i = e % N;
+      const float nze = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
+      buf[l * LS + i] = ph[p];
+      buf[(2 + l) * LS + i] = make_float2(nze * ph[p].y, -nze * ph[p].x);
+    }
+  }
+  __syncthreads();
+  fft_lds<N, true, 4, T, LS, 1, false, N>(buf, tw, tid);
+  for (int e = tid; e < 4 * N; e += T) {
+    const int l = e / N, z = e % N;
+    if ((l & 1) && ky == kym) continue;  // self-paired row: written once
+    float2* dst = l < 2 ? A : B;
+    dst[(size_t)z * plane + (size_t)((l & 1) ? kym : ky) * H] = buf[l * LS + z];
+  }
+}
+
+// Inverse y transforms: A -> A, -i n_y A -> P, B -> B, one after the other,
+// each starting from registers and storing from registers.
 template <int N, int C, int T>
 __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2* __restrict__ B,
                                                  float2* __restrict__ P, const float2* __restrict__ tw) {
   constexpr int H = N / 2, IS = C + 1;
+  using F = RegFft<N, true, false, C, T, 1, IS, true, N>;
   __shared__ float2 buf[N * IS];
   const int tid = threadIdx.x;
   const size_t off = (size_t)blockIdx.y * N * H + (size_t)blockIdx.x * C;
   float2* a = A + off;
   float2* b = B + off;
   float2* pp = P + off;
-  constexpr int TOT = N * C;
-  constexpr int PER = (TOT + T - 1) / T;
-  float2 av[PER];
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) av[p] = a[(unsigned)(e / C) * H + (unsigned)(e % C)];
-  }
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) buf[(e / C) * IS + (e % C)] = av[p];
-  }
-  __syncthreads();
-  fft_lds<N, true, C, T, 1, IS, true, N>(buf, tw, tid);
-  tile_store<N, C, T, IS>(a, buf, H, tid);
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    const int e = tid + p * T;
-    if ((TOT % T == 0) || e < TOT) {
-      const int y = e / C;
-      const float nye = (y == N / 2) ? 0.0f : (float)freq_of(y, N);
-      buf[y * IS + (e % C)] = make_float2(nye * av[p].y, -nye * av[p].x);
-    }
-  }
-  __syncthreads();
-  fft_lds<N, true, C, T, 1, IS, true, N>(buf, tw, tid);
-  tile_store<N, C, T, IS>(pp, buf, H, tid);
-  __syncthreads();
-  tile_load<N, C, T, IS>(buf, b, H, tid);
-  __syncthreads();
-  fft_lds<N, true, C, T, 1, IS, true, N>(buf, tw, tid);
-  tile_store<N, C, T, IS>(b, buf, H, tid);
+  float2 av[F::PER0][F::R0], w[F::PER0][F::R0], out[F::PERL][F::RL];
+  for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    av[p][r] = ok ? a[(unsigned)i * H + l] : make_float2(0.f, 0.f);
+    w[p][r] = av[p][r];
+  });
+  F::run(w, out, buf, tw, tid);
+  for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    if (ok) a[(unsigned)i * H + l] = out[p][r];
+  });
+  for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    const float nye = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
+    w[p][r] = make_float2(nye * av[p][r].y, -nye * av[p][r].x);
+  });
+  F::run(w, out, buf, tw, tid);
+  for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    if (ok) pp[(unsigned)i * H + l] = out[p][r];
+  });
+  for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    if (ok) w[p][r] = b[(unsigned)i * H + l];
+  });
+  F::run(w, out, buf, tw, tid);
+  for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    if (ok) b[(unsigned)i * H + l] = out[p][r];
+  });
 }
 
 // CIC gather of (Fx, Fy, Fz) at pos[idx[j]] (idx null: j itself).
