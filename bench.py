@@ -52,11 +52,44 @@ def stage_bytes(n: int, nm: int) -> dict:
     }
 
 
+# kernel-name prefix of each stage's launch (the z stage also launches the
+# small k_x = 0 column kernel, counted with it)
+STAGE_KERNELS = {"bucket": ("k_bucket_count",), "sort": ("k_scan_local", "k_scan_partials", "k_scan_add", "k_scatter"),
+                 "deposit": ("k_deposit_tiles",), "fft_x_fwd": ("k_fft_x_fwd",), "fft_y_fwd": ("k_fft_y_fwd",),
+                 "fft_z_green": ("k_fft_z_green",), "fft_y_inv": ("k_fft_y_inv",), "fft_x_inv": ("k_fft_x_inv",),
+                 "interp": ("k_interp",)}
+
+
+def measured_traffic(stage: str, nm: int, workload: str):
+    """HBM bytes per launch of a stage's kernels (FETCH_SIZE + WRITE_SIZE,
+    each scaled by what the counter reads for a 1 GiB copy) from the
+    committed rocprofv3 summary of the same workload, or None."""
+    name = {"zeldovich": "pm_headline_256cubed_512mesh_traffic.json",
+            "clustered": "pm_clustered_512cubed_1024mesh_traffic.json"}.get(workload)
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
+        return None
+    try:
+        kern = json.load(open(path))["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    total, found = 0.0, False
+    for k, v in kern.items():
+        short = k.split("::")[-1]
+        if short.startswith(STAGE_KERNELS[stage]) and (f"<{nm}" in short or "<" not in short):
+            total += v["read_bytes"] + v["write_bytes"]
+            found = True
+    return int(total) if found else None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("zeldovich", "clustered"), default="zeldovich",
+                    help="zeldovich: 256^3 lattice on a 512^3 mesh (the headline); clustered: 512^3 particles, "
+                         "70%% lattice + 30%% in 2000 Plummer spheres, 1024^3 mesh, TreePM long range r_s = 1.25 cells")
     ap.add_argument("--n-side", type=int, default=256)
     ap.add_argument("--mesh", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the FP64 CPU oracle (and the error check)")
@@ -79,18 +112,28 @@ def main() -> int:
     import gadget_ng_pm_hip as pm
     from gadget_ng_pm_hip import workloads
 
-    nm, n_side = args.mesh, args.n_side
-    box = float(n_side)  # 1 Mpc/h lattice spacing
     t0 = time.perf_counter()
-    parts = workloads.zeldovich_lattice(n_side=n_side, nm=nm, box=box, spectral_index=-2.0, rms_disp_cells=0.5,
-                                        seed=workloads.SEED_ZELDOVICH, workers=-1)
+    r_split = 0.0
+    if args.workload == "zeldovich":
+        nm, n_side = args.mesh, args.n_side
+        box = float(n_side)  # 1 Mpc/h lattice spacing
+        parts = workloads.zeldovich_lattice(n_side=n_side, nm=nm, box=box, spectral_index=-2.0, rms_disp_cells=0.5,
+                                            seed=workloads.SEED_ZELDOVICH, workers=-1)
+        wl_desc = (f"Zel'dovich lattice {n_side}^3 particles on {nm}^3 mesh, L={box:g} Mpc/h, P(k)~k^-2, "
+                   f"rms displacement 0.5 cell, seed 0x6E670002, G=1, no filter")
+    else:
+        nm, box = 1024, 512.0
+        parts = workloads.clustered_box(workers=-1)
+        r_split = 1.25 * box / nm  # reference convention exp(-k^2 r_s^2 / 2)
+        wl_desc = ("512^3 particles: 454^3 Zel'dovich lattice + 40641064 in 2000 Plummer spheres (a=0.2 Mpc/h), "
+                   f"L=512 Mpc/h, 1024^3 mesh, TreePM long range r_s=1.25 cells, seed 0x6E670003, G=1")
     n = parts.n
     log(f"[bench] workload generated in {time.perf_counter() - t0:.1f}s: {parts.name}, N={n}")
 
     # inputs resident in HBM before the timed region (library-owned device buffers)
     x, y, z, m = (pm.DeviceArray.from_host(a) for a in (parts.x, parts.y, parts.z, parts.m))
     ax, ay, az = (pm.DeviceArray((n,), np.float32) for _ in range(3))
-    plan = pm.Plan(nm, box, device=0)
+    plan = pm.Plan(nm, box, r_split=r_split, device=0)
 
     def step():
         plan.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, ax.ptr, ay.ptr, az.ptr, 1.0)
@@ -127,7 +170,8 @@ def main() -> int:
     dom_ms = stages[dom]
     achieved = sb[dom] / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                "traffic": measured_traffic(dom, nm, args.workload),
                 "algorithmic_bytes": sb[dom], "mean_ms": round(dom_ms, 5)}
     b_alg = workloads.algorithmic_bytes(n, nm)
     pipeline_gbps = b_alg / (gpu_ms * 1e-3) / 1e9
@@ -143,7 +187,9 @@ def main() -> int:
 
     acc = np.stack([ax.numpy(), ay.numpy(), az.numpy()], axis=1).astype(np.float64)
     result = {
-        "metric": "M particle-forces/s + HBM GB/s at 256^3 part/512^3 mesh; rms force err vs FP64",
+        "metric": "M particle-forces/s + HBM GB/s at 256^3 part/512^3 mesh; rms force err vs FP64"
+                  if args.workload == "zeldovich" else "M particle-forces/s, TreePM long range, 512^3 clustered "
+                                                       "particles on a 1024^3 mesh; rms force err vs FP64",
         "value": round(value, 2),
         "unit": "M particle-forces/s",
         "n_gpus": 1,
@@ -155,9 +201,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"Zel'dovich lattice {n_side}^3 particles on {nm}^3 mesh, L={box:g} Mpc/h, "
-                               f"P(k)~k^-2, rms displacement 0.5 cell, seed 0x6E670002, G=1, no filter",
-                   "particles": n, "mesh": nm, "box": box, "precision": "FP32 mesh/FFT/interp, int64 deposit"},
+        "config": {"workload": wl_desc, "particles": n, "mesh": nm, "box": box, "r_split": r_split,
+                   "precision": "FP32 mesh/FFT/interp, int64 deposit"},
         "roofline": roofline,
         "hbm_gbps": round(pipeline_gbps, 1),
         "hbm_frac": round(pipeline_gbps / HBM_PEAK_GBPS, 4),
@@ -179,12 +224,12 @@ def main() -> int:
         mass = parts.mass64()
         cores = oracle.get_threads()
         t1 = time.perf_counter()
-        ref = oracle.pm_accelerations(pos, mass, box, nm, 1.0, mode=oracle.MODE_LEAN)
+        ref = oracle.pm_accelerations(pos, mass, box, nm, 1.0, r_split=r_split, mode=oracle.MODE_LEAN)
         warm = time.perf_counter() - t1
         times = []
         for _ in range(args.cpu_runs):
             t1 = time.perf_counter()
-            oracle.pm_accelerations(pos, mass, box, nm, 1.0, mode=oracle.MODE_LEAN)
+            oracle.pm_accelerations(pos, mass, box, nm, 1.0, r_split=r_split, mode=oracle.MODE_LEAN)
             times.append(time.perf_counter() - t1)
         tc = statistics.median(times) if times else warm
         result["cpu_baseline"] = {

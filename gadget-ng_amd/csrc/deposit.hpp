@@ -7,7 +7,7 @@
 //   1. k_bucket_count  each particle's base cell (i0,j0,k0) gives a tile of
 //      T^3 cells and a class: bit a is set when the base cell is on the tile's
 //      last layer along axis a, i.e. when the particle also feeds the next
-//      tile along that axis.  Bucket = tile * 8 + class.  Counts use one
+//      tile along that axis.  Bucket = Hilbert rank of the tile * 8 + class.  Counts use one
 //      atomic per run of equal buckets within a wavefront (particles that
 //      arrive in spatial order share buckets in runs), which also yields each
 //      particle's rank inside its bucket.  The max |m| for the fixed-point
@@ -31,9 +31,16 @@
 
 namespace gng {
 
+// Tiles are numbered along a 3-D Hilbert curve: rank[t] is the curve
+// position of the tile with linear index t = (tz*nt + ty)*nt + tx and
+// order[] the inverse, so bucket order (and the sorted particle order) walks
+// the box tile by face-adjacent tile, and the deposit launches its
+// workgroups in the same order.
 struct TileMap {
   int T;   // tile side in cells
   int nt;  // tiles per axis (nm / T)
+  const unsigned* rank;
+  const unsigned* order;
 };
 
 // Zero a buffer with vector stores (a fill kernel rather than a memset, so the
@@ -48,7 +55,7 @@ __device__ __forceinline__ unsigned bucket_of_cell(int ix, int iy, int iz, const
   const int tx = ix / tm.T, ty = iy / tm.T, tz = iz / tm.T;
   const unsigned cls = (unsigned)(ix - tx * tm.T == tm.T - 1) | ((unsigned)(iy - ty * tm.T == tm.T - 1) << 1) |
                        ((unsigned)(iz - tz * tm.T == tm.T - 1) << 2);
-  return (((unsigned)tz * tm.nt + ty) * tm.nt + tx) * 8u + cls;
+  return tm.rank[((unsigned)tz * tm.nt + ty) * tm.nt + tx] * 8u + cls;
 }
 
 __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ x, const float* __restrict__ y,
@@ -182,6 +189,29 @@ __global__ __launch_bounds__(256) void k_scatter(const float* __restrict__ x, co
   perm[slot] = (unsigned)i;
 }
 
+// LDS fixed-point add with wavefront aggregation: the lanes whose cell equals
+// the first valid lane's cell sum their contributions in registers and one
+// lane adds the total; the others add directly.  In a dense cluster most of a
+// wavefront hits one cell, so this turns 64 serialised atomics into one; with
+// scattered cells the ballot finds a single match and the reduction is skipped.
+// Integer sums keep the result independent of the grouping.
+__device__ __forceinline__ void lds_add_aggregated(unsigned long long* acc, int cell, long long q, bool valid) {
+  const unsigned long long vmask = __ballot(valid);
+  if (vmask == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)vmask) - 1;
+  const int lcell = __shfl(cell, leader);
+  const bool mine = valid && cell == lcell;
+  const unsigned long long mm = __ballot(mine);
+  if (__popcll(mm) > 1) {
+    long long v = mine ? q : 0ll;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == leader) atomicAdd(acc + lcell, (unsigned long long)v);
+    valid = valid && !mine;
+  }
+  if (valid) atomicAdd(acc + cell, (unsigned long long)q);
+}
+
 // The 14 bucket ranges a tile reads: (d, first class, end class).  d is the
 // neighbour offset (bit 0 x, bit 1 y, bit 2 z: the source tile is u - d) and
 // the classes are those containing every bit of d, merged where adjacent.
@@ -205,14 +235,14 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const float4* __restrict_
   const int tid = threadIdx.x;
   for (int c = tid; c < T3; c += 256) acc[c] = 0ull;
   const int nt = tm.nt;
-  const int tile = blockIdx.x;
+  const int tile = (int)tm.order[blockIdx.x];
   const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
   if (tid < 72) {
     const int d = tid / 9, j = tid % 9;
     const int sx = (d & 1) ? (ux == 0 ? nt - 1 : ux - 1) : ux;
     const int sy = (d & 2) ? (uy == 0 ? nt - 1 : uy - 1) : uy;
     const int sz = (d & 4) ? (uz == 0 ? nt - 1 : uz - 1) : uz;
-    toff[d][j] = offset[((unsigned)(sz * nt + sy) * nt + sx) * 8u + j];
+    toff[d][j] = offset[tm.rank[(unsigned)(sz * nt + sy) * nt + sx] * 8u + j];
   }
   __syncthreads();
   if (tid < 64) {
@@ -233,12 +263,18 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const float4* __restrict_
   __syncthreads();
   const int s = fixed_shift(sc, ntot);
   const unsigned total = rpre[14];
-  for (unsigned q = tid; q < total; q += 256) {
+  // every lane of a wavefront runs the same number of iterations (lanes past
+  // the end carry no particle), so the aggregation ballots see whole waves
+  const unsigned rounds = (total + 255u) / 256u;
+  for (unsigned it = 0; it < rounds; ++it) {
+    const unsigned q = it * 256u + (unsigned)tid;
+    const bool live = q < total;
     int r = 0;
-    while (q >= rpre[r + 1]) ++r;
+    if (live)
+      while (q >= rpre[r + 1]) ++r;
     const int d = kRangeD[r];
     const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-    const float4 p = sorted[rbeg[r] + (q - rpre[r])];
+    const float4 p = live ? sorted[rbeg[r] + (q - rpre[r])] : make_float4(0.f, 0.f, 0.f, 0.f);
     int ix, iy, iz;
     float fx, fy, fz;
     cell_coord(p.x, cm, ix, fx);
@@ -250,19 +286,17 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const float4* __restrict_
     const int cx = lx0 == T - 1, cy = ly0 == T - 1, cz = lz0 == T - 1;
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
-      if ((a & cz) != dz) continue;
       const int lz = dz ? 0 : lz0 + a;
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        if ((b & cy) != dy) continue;
         const int ly = dy ? 0 : ly0 + b;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-          if ((c & cx) != dx) continue;
           const int lx = dx ? 0 : lx0 + c;
+          const bool valid = live && (a & cz) == dz && (b & cy) == dy && (c & cx) == dx;
           const float w = p.w * wx[c] * wy[b] * wz[a];
-          const long long qv = __float2ll_rn(ldexpf(w, s));
-          atomicAdd(acc + (lz * T + ly) * T + lx, (unsigned long long)qv);
+          const long long qv = valid ? __float2ll_rn(ldexpf(w, s)) : 0ll;
+          lds_add_aggregated(acc, valid ? (lz * T + ly) * T + lx : 0, qv, valid);
         }
       }
     }

@@ -15,8 +15,10 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <utility>
 #include <new>
 #include <string>
 #include <vector>
@@ -24,25 +26,20 @@
 #include "../../include/gng_pm.h"
 #include "deposit.hpp"
 #include "pm_kernels.hpp"
+#include "status.hpp"
 
 using namespace gng;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+int gng::fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
 
-#define GNG_HIP(call)                                                                            \
-  do {                                                                                           \
-    hipError_t e_ = (call);                                                                      \
-    if (e_ != hipSuccess)                                                                        \
-      return fail(e_ == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP,                         \
-                  std::string(#call) + ": " + hipGetErrorString(e_));                            \
-  } while (0)
+namespace {
 
 // stages: 0 bucket count, 1 sort (scan + scatter), 2 deposit, 3 x forward,
 // 4 y forward, 5 z + Green, 6 y inverse, 7 x inverse, 8 interpolation
@@ -86,7 +83,7 @@ struct MeshOps {
   void (*z_green)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
   void (*z_green_col0)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
   void (*y_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
-  void (*x_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
+  void (*x_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, float2*, const float2*, int, float);
 };
 
 template <int N>
@@ -119,8 +116,9 @@ MeshOps make_ops() {
   o.y_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* B, float2* P, const float2* tw) {
     hipLaunchKernelGGL((k_fft_y_inv<N, C, TY>), g, b, 0, s, A, B, P, tw);
   };
-  o.x_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* P, float2* B, const float2* tw) {
-    hipLaunchKernelGGL((k_fft_x_inv<N, R>), g, b, 0, s, A, P, B, tw);
+  o.x_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* P, float2* B, float2* pot, const float2* tw,
+                int field0, float pot_scale) {
+    hipLaunchKernelGGL((k_fft_x_inv<N, R>), g, b, 0, s, A, P, B, pot, tw, field0, pot_scale);
   };
   return o;
 }
@@ -154,6 +152,9 @@ struct gng_pm_plan {
   float2* d_B = nullptr;
   float* d_rho = nullptr;  // kept density (only with keep_density)
   bool keep_density = false;
+  float* d_pot = nullptr;  // mesh potential (only with keep_potential)
+  bool keep_potential = false;
+  bool potential_valid = false;
   float2* d_tw = nullptr;  // nm-th roots of unity
   float* d_deconv = nullptr;
   SolveScalars* d_sc = nullptr;
@@ -165,6 +166,8 @@ struct gng_pm_plan {
   unsigned* d_count = nullptr;
   unsigned* d_offset = nullptr;
   unsigned* d_partial = nullptr;
+  unsigned* d_tile_rank = nullptr;   // tile -> Hilbert rank
+  unsigned* d_tile_order = nullptr;  // Hilbert rank -> tile
 
   // per-particle work buffers (grown on demand)
   long long work_cap = 0;
@@ -209,9 +212,9 @@ void free_work(gng_pm_plan* p) {
 void free_plan(gng_pm_plan* p) {
   if (!p) return;
   (void)hipSetDevice(p->device);
-  for (void* q : {(void*)p->d_P, (void*)p->d_A, (void*)p->d_B, (void*)p->d_rho, (void*)p->d_tw, (void*)p->d_deconv,
+  for (void* q : {(void*)p->d_P, (void*)p->d_A, (void*)p->d_B, (void*)p->d_rho, (void*)p->d_pot, (void*)p->d_tw, (void*)p->d_deconv,
                   (void*)p->d_sc, (void*)p->d_count, (void*)p->d_offset, (void*)p->d_partial, (void*)p->d_part,
-                  (void*)p->d_idx})
+                  (void*)p->d_idx, (void*)p->d_tile_rank, (void*)p->d_tile_order})
     (void)hipFree(q);
   free_work(p);
   for (auto& slot : p->ev)
@@ -307,7 +310,11 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   mark(p, 6, s);
   o.y_inv(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
   mark(p, 7, s);
-  o.x_inv(dim3((unsigned)(rows / o.x_rows), 3), dim3(256), s, p->d_A, p->d_P, p->d_B, p->d_tw);
+  if (p->keep_potential)
+    // the potential reads A before the force pass below overwrites it
+    o.x_inv(dim3((unsigned)(rows / o.x_rows), 1), dim3(256), s, p->d_A, p->d_P, p->d_B,
+            reinterpret_cast<float2*>(p->d_pot), p->d_tw, 3, (float)(p->cfg.box_size / (2.0 * M_PI)));
+  o.x_inv(dim3((unsigned)(rows / o.x_rows), 3), dim3(256), s, p->d_A, p->d_P, p->d_B, nullptr, p->d_tw, 0, 1.0f);
   mark(p, 8, s);
   if (k > 0)
     hipLaunchKernelGGL(k_interp, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, x, y, z, idx, k, cm,
@@ -318,6 +325,7 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   if (p->timing) ++p->timed_solves;
   p->last_n = n;
   p->density_valid = p->keep_density;
+  p->potential_valid = p->keep_potential;
   return GNG_OK;
 }
 
@@ -404,9 +412,43 @@ int solve_host_soa(gng_pm_plan* p, const float* x, const float* y, const float* 
   return GNG_OK;
 }
 
+// 3-D Hilbert key of integer coordinates with `bits` bits each: Skilling's
+// transform of the axes to the "transposed" Hilbert index (J. Skilling,
+// Programming the Hilbert curve, AIP Conf. Proc. 707, 381 (2004)), then the
+// bits interleaved with x most significant.  For bits = 1 the curve visits
+// (0,0,0) (0,0,1) (0,1,1) (0,1,0) (1,1,0) (1,1,1) (1,0,1) (1,0,0).
+uint64_t hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int bits) {
+  uint32_t X[3] = {x, y, z};
+  if (bits <= 0) return 0;
+  const uint32_t M = 1u << (bits - 1);
+  for (uint32_t Q = M; Q > 1; Q >>= 1) {
+    const uint32_t P = Q - 1;
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        const uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+  uint32_t t = 0;
+  for (uint32_t Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  for (int i = 0; i < 3; ++i) X[i] ^= t;
+  uint64_t key = 0;
+  for (int b = bits - 1; b >= 0; --b)
+    for (int i = 0; i < 3; ++i) key = (key << 1) | ((X[i] >> b) & 1u);
+  return key;
+}
+
 }  // namespace
 
 extern "C" {
+
+uint64_t gng_hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int32_t bits) { return hilbert_key3(x, y, z, bits); }
 
 int gng_pm_abi_version(void) { return GNG_PM_ABI_VERSION; }
 
@@ -473,7 +515,33 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
   GNG_ALLOC(p->d_count, sizeof(unsigned) * (size_t)((p->nb + 3) / 4 * 4));
   GNG_ALLOC(p->d_offset, sizeof(unsigned) * (size_t)(p->nb + 1));
   GNG_ALLOC(p->d_partial, sizeof(unsigned) * (size_t)p->scan_blocks);
+  GNG_ALLOC(p->d_tile_rank, sizeof(unsigned) * (size_t)(p->nb / 8));
+  GNG_ALLOC(p->d_tile_order, sizeof(unsigned) * (size_t)(p->nb / 8));
 #undef GNG_ALLOC
+  {
+    // Hilbert ranks of the tiles (keys on the enclosing power-of-two grid)
+    const int nt = p->tm.nt;
+    int bits = 0;
+    while ((1 << bits) < nt) ++bits;
+    std::vector<std::pair<uint64_t, unsigned>> keys;
+    keys.reserve((size_t)nt * nt * nt);
+    for (int tz = 0; tz < nt; ++tz)
+      for (int ty = 0; ty < nt; ++ty)
+        for (int tx = 0; tx < nt; ++tx)
+          keys.emplace_back(hilbert_key3(tx, ty, tz, bits), (unsigned)((tz * nt + ty) * nt + tx));
+    std::sort(keys.begin(), keys.end());
+    std::vector<unsigned> rank(keys.size()), order(keys.size());
+    for (size_t r = 0; r < keys.size(); ++r) {
+      order[r] = keys[r].second;
+      rank[keys[r].second] = (unsigned)r;
+    }
+    hipError_t e2 = hipMemcpy(p->d_tile_rank, rank.data(), sizeof(unsigned) * rank.size(), hipMemcpyHostToDevice);
+    if (e2 == hipSuccess)
+      e2 = hipMemcpy(p->d_tile_order, order.data(), sizeof(unsigned) * order.size(), hipMemcpyHostToDevice);
+    if (e2 != hipSuccess) return bail(fail(GNG_ERR_HIP, std::string("tile order upload: ") + hipGetErrorString(e2)));
+    p->tm.rank = p->d_tile_rank;
+    p->tm.order = p->d_tile_order;
+  }
   std::vector<float2> tw(nm);
   std::vector<float> dc(nm);
   for (int m = 0; m < nm; ++m) {
@@ -569,6 +637,45 @@ int gng_pm_density(gng_pm_plan* p, double* out) {
   return GNG_OK;
 }
 
+int gng_pm_keep_potential(gng_pm_plan* p, int enable) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  std::lock_guard<std::mutex> lk(p->mu);
+  int rc = set_device(p);
+  if (rc) return rc;
+  if (enable && !p->d_pot) GNG_HIP(hipMalloc(&p->d_pot, sizeof(float) * (size_t)p->nm3));
+  p->keep_potential = enable != 0;
+  p->potential_valid = false;
+  return GNG_OK;
+}
+
+int gng_pm_potential(gng_pm_plan* p, double* out) {
+  if (!p || !out) return fail(GNG_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (!p->potential_valid) return fail(GNG_ERR_INVALID, "no kept potential: call gng_pm_keep_potential(p, 1) first");
+  int rc = set_device(p);
+  if (rc) return rc;
+  std::vector<float> f((size_t)p->nm3);
+  GNG_HIP(hipDeviceSynchronize());
+  GNG_HIP(hipMemcpy(f.data(), p->d_pot, sizeof(float) * (size_t)p->nm3, hipMemcpyDeviceToHost));
+  for (long long i = 0; i < p->nm3; ++i) out[i] = (double)f[i];
+  return GNG_OK;
+}
+
+int gng_pm_device_meshes(gng_pm_plan* p, const float** d_mass, const float** d_potential) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (d_mass) {
+    if (!p->density_valid) return fail(GNG_ERR_INVALID, "no kept density: call gng_pm_keep_density(p, 1) first");
+    *d_mass = p->d_rho;
+  }
+  if (d_potential) {
+    if (!p->potential_valid)
+      return fail(GNG_ERR_INVALID, "no kept potential: call gng_pm_keep_potential(p, 1) first");
+    *d_potential = p->d_pot;
+  }
+  return GNG_OK;
+}
+
 int gng_pm_force_meshes(gng_pm_plan* p, float* out) {
   if (!p || !out) return fail(GNG_ERR_INVALID, "null argument");
   std::lock_guard<std::mutex> lk(p->mu);
@@ -601,7 +708,159 @@ int gng_pm_stage_times(gng_pm_plan* p, float* ms, int capacity) {
   return kStages;
 }
 
+// ---- ensembles of boxes ---------------------------------------------------
+
+}  // extern "C"
+
+struct gng_pm_batch {
+  std::vector<gng_pm_plan*> plans;
+  int device = 0;
+  int branches = 0;
+  std::mutex mu;
+  hipStream_t stream = nullptr;               // capture origin
+  std::vector<hipStream_t> side;              // fork branches
+  std::vector<hipEvent_t> fork_ev, join_ev;   // one per branch
+  hipEvent_t root_ev = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // key of the captured graph: pointers, offsets and g
+  std::vector<uintptr_t> key;
+};
+
+namespace {
+
+void free_batch(gng_pm_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto e : b->fork_ev) (void)hipEventDestroy(e);
+  for (auto e : b->join_ev) (void)hipEventDestroy(e);
+  if (b->root_ev) (void)hipEventDestroy(b->root_ev);
+  for (auto s : b->side) (void)hipStreamDestroy(s);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  for (auto p : b->plans) free_plan(p);
+  delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gng_pm_batch_create(int n_boxes, const gng_pm_config* cfgs, gng_pm_batch** out) {
+  if (!out) return fail(GNG_ERR_INVALID, "null output pointer");
+  *out = nullptr;
+  if (n_boxes <= 0 || !cfgs) return fail(GNG_ERR_INVALID, "need at least one box configuration");
+  gng_pm_batch* b = new (std::nothrow) gng_pm_batch();
+  if (!b) return fail(GNG_ERR_OOM, "host allocation failed");
+  for (int i = 0; i < n_boxes; ++i) {
+    gng_pm_plan* p = nullptr;
+    const int rc = gng_pm_create(&cfgs[i], &p);
+    if (rc) {
+      const std::string msg = "box " + std::to_string(i) + ": " + g_last_error;
+      free_batch(b);
+      return fail(rc, msg);
+    }
+    b->plans.push_back(p);
+  }
+  b->device = b->plans[0]->device;
+  b->branches = n_boxes < 8 ? n_boxes : 8;
+  hipError_t e = hipSetDevice(b->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->root_ev, hipEventDisableTiming);
+  for (int k = 0; k < b->branches && e == hipSuccess; ++k) {
+    hipStream_t s;
+    hipEvent_t f, j;
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&f, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+    if (e == hipSuccess) {
+      b->side.push_back(s);
+      b->fork_ev.push_back(f);
+      b->join_ev.push_back(j);
+    }
+  }
+  if (e != hipSuccess) {
+    free_batch(b);
+    return fail(GNG_ERR_HIP, std::string("batch setup: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return GNG_OK;
+}
+
+void gng_pm_batch_destroy(gng_pm_batch* b) { free_batch(b); }
+
+int gng_pm_batch_size(const gng_pm_batch* b) { return b ? (int)b->plans.size() : 0; }
+
+int gng_pm_batch_plan(gng_pm_batch* b, int i, gng_pm_plan** out) {
+  if (!b || !out || i < 0 || i >= (int)b->plans.size()) return fail(GNG_ERR_INVALID, "bad batch index");
+  *out = b->plans[i];
+  return GNG_OK;
+}
+
+int gng_pm_batch_solve_device(gng_pm_batch* b, const float* d_x, const float* d_y, const float* d_z,
+                              const float* d_mass, const int64_t* offsets, double g, float* d_ax, float* d_ay,
+                              float* d_az, void* stream) {
+  if (!b || !offsets) return fail(GNG_ERR_INVALID, "null argument");
+  const int nb = (int)b->plans.size();
+  for (int i = 0; i < nb; ++i)
+    if (offsets[i + 1] < offsets[i] || offsets[i] < 0) return fail(GNG_ERR_INVALID, "offsets must be non-decreasing");
+  std::lock_guard<std::mutex> lk(b->mu);
+  GNG_HIP(hipSetDevice(b->device));
+  std::vector<uintptr_t> key = {(uintptr_t)d_x, (uintptr_t)d_y, (uintptr_t)d_z, (uintptr_t)d_mass,
+                                (uintptr_t)d_ax, (uintptr_t)d_ay, (uintptr_t)d_az};
+  uint64_t gbits;
+  std::memcpy(&gbits, &g, sizeof(g));
+  key.push_back((uintptr_t)gbits);
+  for (int i = 0; i <= nb; ++i) key.push_back((uintptr_t)offsets[i]);
+  if (!b->exec || key != b->key) {
+    // (re)capture: every allocation happens before capture begins
+    for (int i = 0; i < nb; ++i) {
+      int rc = ensure_work(b->plans[i], offsets[i + 1] - offsets[i] > 0 ? offsets[i + 1] - offsets[i] : 1);
+      if (rc) return rc;
+      b->plans[i]->timing = false;
+    }
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    GNG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = GNG_OK;
+    hipError_t e = hipEventRecord(b->root_ev, b->stream);
+    for (int k = 0; k < b->branches && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->side[k], b->root_ev, 0);
+    for (int i = 0; i < nb && e == hipSuccess && rc == GNG_OK; ++i) {
+      const long long o = offsets[i], n = offsets[i + 1] - offsets[i];
+      rc = run_pipeline(b->plans[i], d_x + o, d_y + o, d_z + o, d_mass + o, n, nullptr, n, g, d_ax + o, d_ay + o,
+                        d_az + o, b->side[i % b->branches]);
+    }
+    for (int k = 0; k < b->branches && e == hipSuccess; ++k) {
+      e = hipEventRecord(b->join_ev[k], b->side[k]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join_ev[k], 0);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ee = hipStreamEndCapture(b->stream, &graph);
+    if (e != hipSuccess || ee != hipSuccess || rc != GNG_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (rc) return rc;
+      return fail(GNG_ERR_HIP, std::string("ensemble capture: ") + hipGetErrorString(e != hipSuccess ? e : ee));
+    }
+    b->graph = graph;
+    GNG_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->key = key;
+  }
+  GNG_HIP(hipGraphLaunch(b->exec, reinterpret_cast<hipStream_t>(stream)));
+  for (int i = 0; i < nb; ++i) b->plans[i]->last_n = offsets[i + 1] - offsets[i];
+  return GNG_OK;
+}
+
 // ---- device-memory helpers ------------------------------------------------
+
+namespace {
+__global__ void k_axpy(float* __restrict__ y, const float* __restrict__ x, long long n, float a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+}  // namespace
 
 int gng_dev_alloc(void** ptr, uint64_t bytes) {
   if (!ptr) return fail(GNG_ERR_INVALID, "null argument");
@@ -627,6 +886,16 @@ int gng_dev_copy(void* dst, const void* src, uint64_t bytes, int kind) {
 
 int gng_dev_synchronize(void) {
   GNG_HIP(hipDeviceSynchronize());
+  return GNG_OK;
+}
+
+int gng_dev_axpy(float* d_y, const float* d_x, int64_t n, float a, void* stream) {
+  if (n < 0) return fail(GNG_ERR_INVALID, "negative length");
+  if (n == 0) return GNG_OK;
+  if (!d_y || !d_x) return fail(GNG_ERR_INVALID, "null argument");
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                     d_y, d_x, (long long)n, a);
+  GNG_HIP(hipGetLastError());
   return GNG_OK;
 }
 

@@ -1,0 +1,865 @@
+// learned.hip -- learned sub-cell force correction: a 3-layer network of
+// periodic 3^3 convolutions on the PM mesh, trained and run on the MFMA units.
+//
+// Layers: conv(2->16) SiLU, conv(16->16) SiLU, conv(16->3).  Activations and
+// weights are bfloat16, products accumulate in FP32 with
+// v_mfma_f32_16x16x16_bf16.  The convolution is written as an implicit GEMM
+// per 16-cell x row:  D[cell][o] = sum_k A[cell][k] B[k][o], k = tap*CIN + c,
+// so one MFMA takes one tap of 16 channels (CIN = 16), four taps of 4
+// channels or eight taps of 2 channels.  A workgroup stages its 16x8x8 output
+// tile plus a one-cell periodic halo in LDS once and keeps every weight
+// fragment of the layer in registers.
+//
+// Backward: the data gradient is the same convolution with the taps mirrored
+// (t -> 26 - t) and the channel roles swapped; the weight gradient is a GEMM
+// whose K dimension runs over cells (dW[t][c][o] = sum_cells X[cell+off_t][c]
+// dZ[cell][o]), accumulated per workgroup and reduced over workgroups in a
+// fixed order, so gradients are deterministic.  The loss gradient reaches
+// the mesh through the transpose of the CIC interpolation, with runs of
+// particles that share a base cell combined inside the wavefront before the
+// atomics.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/gng_learned.h"
+#include "status.hpp"
+
+namespace gng {
+namespace net {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// forward / data-gradient tile: 16 x 8 x 8 output cells
+constexpr int TX = 16, TY = 8, TZ = 8;
+constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, HC = HX * HY * HZ;
+// weight-gradient tile: 16 x 4 x 4 cells
+constexpr int WX = 16, WY = 4, WZ = 4;
+constexpr int WHX = WX + 2, WHY = WY + 2, WHZ = WZ + 2, WHC = WHX * WHY * WHZ;
+constexpr int kWgradBlocks = 512;
+
+// parameter layout (floats)
+constexpr int P_W1 = 0, P_B1 = P_W1 + 27 * 2 * 16, P_W2 = P_B1 + 16, P_B2 = P_W2 + 27 * 16 * 16,
+              P_W3 = P_B2 + 16, P_B3 = P_W3 + 27 * 16 * 3, P_COUNT = P_B3 + 3;
+
+template <int CIN>
+struct KSteps {
+  static constexpr int v = (27 * CIN + 15) / 16;
+};
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// halo-cell offset of tap t in a box of row length hx and plane hx*hy
+__host__ __device__ constexpr int tap_off(int t, int hx, int hy) {
+  return ((t / 9 - 1) * hy + ((t / 3) % 3 - 1)) * hx + (t % 3 - 1);
+}
+
+// Stage input cells (x0-1 .. x0+HXv-2, ...) mod nm, CIN bf16 channels each,
+// into lds[((hz*HYv + hy)*HXv + hx)*CIN + c], moving 32-bit words.
+template <int CIN, int HXv, int HYv, int HZv>
+__device__ __forceinline__ void load_halo(unsigned short* lds, const unsigned short* __restrict__ in, int x0,
+                                          int y0, int z0, int nm) {
+  constexpr int WPC = CIN / 2;
+  constexpr int NW = HXv * HYv * HZv * WPC;
+  const unsigned* src = reinterpret_cast<const unsigned*>(in);
+  unsigned* dst = reinterpret_cast<unsigned*>(lds);
+  for (int e = threadIdx.x; e < NW; e += blockDim.x) {
+    const int c = e / WPC, w = e - c * WPC;
+    const int hx = c % HXv, hy = (c / HXv) % HYv, hz = c / (HXv * HYv);
+    int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+    gx += gx < 0 ? nm : 0;
+    gx -= gx >= nm ? nm : 0;
+    gy += gy < 0 ? nm : 0;
+    gy -= gy >= nm ? nm : 0;
+    gz += gz < 0 ? nm : 0;
+    gz -= gz >= nm ? nm : 0;
+    const long long gc = ((long long)gz * nm + gy) * nm + gx;
+    dst[e] = src[gc * WPC + w];
+  }
+}
+
+// A fragment of k-step s for the 16-cell row whose lane-m cell sits at halo
+// index `base`: lane (m, g) holds x[cell m + off_t][c] for k = 16s + 4g + j,
+// t = k / CIN, c = k % CIN, j = 0..3 (zero for t >= 27).
+template <int CIN>
+__device__ __forceinline__ bf16x4 a_frag(const unsigned short* tile, int base, int s, int g) {
+  if constexpr (CIN == 16) {
+    return *reinterpret_cast<const bf16x4*>(tile + (base + tap_off(s, HX, HY)) * 16 + 4 * g);
+  } else if constexpr (CIN == 4) {
+    const int t = 4 * s + g;
+    if (t >= 27) return bf16x4{0, 0, 0, 0};
+    return *reinterpret_cast<const bf16x4*>(tile + (base + tap_off(t, HX, HY)) * 4);
+  } else {
+    static_assert(CIN == 2, "CIN must be 2, 4 or 16");
+    const int t0 = 8 * s + 2 * g;
+    unsigned lo = 0, hi = 0;
+    if (t0 < 27) lo = *reinterpret_cast<const unsigned*>(tile + (base + tap_off(t0, HX, HY)) * 2);
+    if (t0 + 1 < 27) hi = *reinterpret_cast<const unsigned*>(tile + (base + tap_off(t0 + 1, HX, HY)) * 2);
+    bf16x4 v;
+    v[0] = (short)(lo & 0xffffu);
+    v[1] = (short)(lo >> 16);
+    v[2] = (short)(hi & 0xffffu);
+    v[3] = (short)(hi >> 16);
+    return v;
+  }
+}
+
+enum { MODE_HIDDEN = 0, MODE_OUT = 1, MODE_BWD = 2 };
+
+// One periodic 3^3 convolution over an nm^3 mesh (nm % 16 == 0), grid
+// (nm/16, (nm/8)^2, boxes), 256 threads.  Output channels n = lane % 16:
+//   MODE_HIDDEN  zout = conv + bias (f32, 16 ch), xout = bf16(silu(zout));
+//   MODE_OUT     zout = conv + bias (f32, 4 ch: the 3 outputs and a zero);
+//   MODE_BWD     xout = bf16(conv * silu'(zprev)) (16 ch).
+template <int CIN, int MODE>
+__global__ __launch_bounds__(256) void k_conv(const unsigned short* __restrict__ in,
+                                              const bf16x4* __restrict__ wfrag, const float* __restrict__ bias,
+                                              float* __restrict__ zout, unsigned short* __restrict__ xout,
+                                              const float* __restrict__ zprev, int nm) {
+  constexpr int KS = KSteps<CIN>::v;
+  __shared__ __attribute__((aligned(16))) unsigned short tile[HC * CIN];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+  const int nty = nm / TY;
+  const int x0 = blockIdx.x * TX, y0 = (blockIdx.y % nty) * TY, z0 = (blockIdx.y / nty) * TZ;
+  const long long M = (long long)nm * nm * nm;
+  const long long boff = (long long)blockIdx.z * M;
+
+  bf16x4 b[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) b[s] = wfrag[s * 64 + lane];
+  float bn = 0.0f;
+  if constexpr (MODE != MODE_BWD) bn = bias[m];
+
+  load_halo<CIN, HX, HY, HZ>(tile, in + boff * CIN, x0, y0, z0, nm);
+  __syncthreads();
+
+  // each wave: 16 of the tile's 64 rows, four independent accumulators at a time
+#pragma unroll 1
+  for (int q = 0; q < 4; ++q) {
+    f32x4 acc[4];
+    int base[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int r = wave * 16 + q * 4 + a;
+      const int ry = r % TY, rz = r / TY;
+      base[a] = ((rz + 1) * HY + (ry + 1)) * HX + (m + 1);
+      acc[a] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const bf16x4 av = a_frag<CIN>(tile, base[a], s, g);
+        acc[a] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b[s], acc[a], 0, 0, 0);
+      }
+    }
+    // lane holds D[cell 4g + j][channel m]
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int r = wave * 16 + q * 4 + a;
+      const int ry = r % TY, rz = r / TY;
+      const long long rowc = boff + ((long long)(z0 + rz) * nm + (y0 + ry)) * nm + x0 + 4 * g;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const long long c = rowc + j;
+        if constexpr (MODE == MODE_HIDDEN) {
+          const float z = acc[a][j] + bn;
+          zout[c * 16 + m] = z;
+          xout[c * 16 + m] = f2bf(silu(z));
+        } else if constexpr (MODE == MODE_OUT) {
+          if (m < 4) zout[c * 4 + m] = acc[a][j] + bn;
+        } else {
+          xout[c * 16 + m] = f2bf(acc[a][j] * silu_grad(zprev[c * 16 + m]));
+        }
+      }
+    }
+  }
+}
+
+// Weight gradient partials.  Columns n = t*CIN + c (27*CIN of them) plus a
+// bias column of ones; rows o = output channel (< COS).  Each workgroup walks
+// tiles in a grid-stride loop and writes its D[o][n] partial sums:
+//   partial[block][n / 16][o * 16 + n % 16].
+template <int CIN, int COS>
+__global__ __launch_bounds__(256) void k_conv_wgrad(const unsigned short* __restrict__ X,
+                                                    const unsigned short* __restrict__ dZ,
+                                                    float* __restrict__ partial, int nm, int boxes) {
+  constexpr int NCOL = 27 * CIN + 1, NB = (NCOL + 15) / 16, SPW = (NB + 3) / 4;
+  __shared__ __attribute__((aligned(16))) unsigned short xt[WHC * CIN];
+  __shared__ __attribute__((aligned(16))) unsigned short dt[WX * WY * WZ * COS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, g = lane >> 4;
+
+  int coff[SPW], cch[SPW], ckind[SPW];  // kind 0: tap column, 1: ones, 2: zero
+#pragma unroll
+  for (int q = 0; q < SPW; ++q) {
+    const int n = 16 * (wave + 4 * q) + m;
+    coff[q] = 0;
+    cch[q] = 0;
+    if (n < 27 * CIN) {
+      coff[q] = tap_off(n / CIN, WHX, WHY);
+      cch[q] = n % CIN;
+      ckind[q] = 0;
+    } else {
+      ckind[q] = n == 27 * CIN ? 1 : 2;
+    }
+  }
+  f32x4 acc[SPW];
+#pragma unroll
+  for (int q = 0; q < SPW; ++q) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  const int ntx = nm / WX, nty = nm / WY, ntz = nm / WZ;
+  const long long per_box = (long long)ntx * nty * ntz;
+  const long long tiles = per_box * boxes;
+  const long long M = (long long)nm * nm * nm;
+  for (long long tt = blockIdx.x; tt < tiles; tt += gridDim.x) {
+    const long long box = tt / per_box;
+    const long long rem = tt - box * per_box;
+    const int bx = (int)(rem % ntx), by = (int)((rem / ntx) % nty), bz = (int)(rem / ((long long)ntx * nty));
+    const int x0 = bx * WX, y0 = by * WY, z0 = bz * WZ;
+    __syncthreads();
+    load_halo<CIN, WHX, WHY, WHZ>(xt, X + box * M * CIN, x0, y0, z0, nm);
+    {
+      constexpr int WPC = COS / 2, NW = WX * WY * WZ * WPC;
+      const unsigned* src = reinterpret_cast<const unsigned*>(dZ + box * M * COS);
+      unsigned* dst = reinterpret_cast<unsigned*>(dt);
+      for (int e = threadIdx.x; e < NW; e += 256) {
+        const int c = e / WPC, w = e - c * WPC;
+        const int cx = c % WX, cy = (c / WX) % WY, cz = c / (WX * WY);
+        const long long gc = ((long long)(z0 + cz) * nm + (y0 + cy)) * nm + x0 + cx;
+        dst[e] = src[gc * WPC + w];
+      }
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int r = 0; r < WY * WZ; ++r) {
+      const int ry = r % WY, rz = r / WY;
+      bf16x4 av;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) av[j] = m < COS ? (short)dt[((rz * WY + ry) * WX + 4 * g + j) * COS + m] : (short)0;
+      const int hb = ((rz + 1) * WHY + (ry + 1)) * WHX + 1 + 4 * g;
+#pragma unroll
+      for (int q = 0; q < SPW; ++q) {
+        if (wave + 4 * q < NB) {
+          bf16x4 bv;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            short v = 0;
+            if (ckind[q] == 0) v = (short)xt[(hb + j + coff[q]) * CIN + cch[q]];
+            else if (ckind[q] == 1) v = (short)0x3F80;
+            bv[j] = v;
+          }
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < SPW; ++q) {
+    const int s = wave + 4 * q;
+    if (s < NB) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        partial[((long long)blockIdx.x * NB + s) * 256 + (4 * g + j) * 16 + m] = acc[q][j];
+    }
+  }
+}
+
+// dW[t][c][o] (t*CIN + c = column) and db[o] from the partials, summed over
+// workgroups in order in FP64.
+template <int CIN, int COUT>
+__global__ void k_wgrad_reduce(const float* __restrict__ partial, int nblocks, float* __restrict__ gw,
+                               float* __restrict__ gb) {
+  constexpr int NCOL = 27 * CIN + 1, NB = (NCOL + 15) / 16;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= NCOL * COUT) return;
+  const int col = e / COUT, o = e % COUT;
+  const int s = col / 16, nn = col % 16;
+  double sum = 0.0;
+  for (int b = 0; b < nblocks; ++b) sum += partial[((long long)b * NB + s) * 256 + o * 16 + nn];
+  if (col < 27 * CIN) gw[col * COUT + o] = (float)sum;
+  else gb[o] = (float)sum;
+}
+
+// Fragments of B[k][n] for one convolution from FP32 weights w[t][c][o]
+// (cin x cout real channels, cin_pad channels in the k layout).  With
+// mirror != 0 the fragments are those of the data-gradient convolution:
+// B[k = t*cin_pad + o][n = c] = w[26 - t][c][o].
+__global__ void k_pack(const float* __restrict__ w, int cin, int cout, int cin_pad, int ksteps, int mirror,
+                       bf16x4* __restrict__ frag) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ksteps * 64) return;
+  const int s = e / 64, l = e % 64, g = l / 16, n = l % 16;
+  bf16x4 v;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = 16 * s + 4 * g + j;
+    const int t = k / cin_pad, c = k % cin_pad;
+    float x = 0.0f;
+    if (!mirror) {
+      if (t < 27 && c < cin && n < cout) x = w[(t * cin + c) * cout + n];
+    } else {
+      // data gradient: input channels are the forward outputs (c < cout),
+      // outputs the forward inputs (n < cin)
+      if (t < 27 && c < cout && n < cin) x = w[((26 - t) * cin + n) * cout + c];
+    }
+    v[j] = (short)f2bf(x);
+  }
+  frag[e] = v;
+}
+
+__global__ void k_bias16(const float* __restrict__ b, int cout, float* __restrict__ b16) {
+  const int n = threadIdx.x;
+  if (n < 16) b16[n] = n < cout ? b[n] : 0.0f;
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ gr, float* __restrict__ m1,
+                       float* __restrict__ m2, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gv = gr[i];
+  const float a = b1 * m1[i] + (1.0f - b1) * gv;
+  const float v = b2 * m2[i] + (1.0f - b2) * gv * gv;
+  m1[i] = a;
+  m2[i] = v;
+  p[i] -= lr * (a * c1) / (sqrtf(v * c2) + eps);
+}
+
+__global__ void k_fill(float* __restrict__ p, long long n, float v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_d(double* p) { *p = 0.0; }
+
+// dz3 = bf16(dL/dy), 4 channels per cell
+__global__ void k_to_bf16(const float* __restrict__ a, unsigned short* __restrict__ b, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = f2bf(a[i]);
+}
+
+__global__ void k_input(const float* __restrict__ rho, const float* __restrict__ phi, long long M, float inv_rho,
+                        float inv_phi, unsigned* __restrict__ x0) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const unsigned lo = f2bf(log1pf(rho[i] * inv_rho));
+  const unsigned hi = f2bf(asinhf(phi[i] * inv_phi));
+  x0[i] = lo | (hi << 16);
+}
+
+struct Cic {
+  long long node[8];
+  float w[8];
+};
+
+// CIC nodes and weights of one position (cic.rs:100-170 weights; float cell
+// coordinate, all indices wrapped).
+__device__ __forceinline__ void cic_nodes(float x, float y, float z, float cs, int nm, long long boff, Cic& c) {
+  float fx = x * cs, fy = y * cs, fz = z * cs;
+  const float ix = floorf(fx), iy = floorf(fy), iz = floorf(fz);
+  const float dx = fx - ix, dy = fy - iy, dz = fz - iz;
+  int i0 = (int)ix % nm, j0 = (int)iy % nm, k0 = (int)iz % nm;
+  i0 += i0 < 0 ? nm : 0;
+  j0 += j0 < 0 ? nm : 0;
+  k0 += k0 < 0 ? nm : 0;
+  const int i1 = i0 + 1 == nm ? 0 : i0 + 1, j1 = j0 + 1 == nm ? 0 : j0 + 1, k1 = k0 + 1 == nm ? 0 : k0 + 1;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int kk = (a & 4) ? k1 : k0, jj = (a & 2) ? j1 : j0, ii = (a & 1) ? i1 : i0;
+    c.node[a] = boff + ((long long)kk * nm + jj) * nm + ii;
+    c.w[a] = ((a & 4) ? dz : 1.0f - dz) * ((a & 2) ? dy : 1.0f - dy) * ((a & 1) ? dx : 1.0f - dx);
+  }
+}
+
+// Loss and dL/dy.  Lanes whose particles share a base cell (training sets are
+// stored cell-sorted) form runs; a segmented suffix sum over each run leaves
+// the run total in its first lane, which alone issues the atomics.
+__global__ __launch_bounds__(256) void k_loss(const float4* __restrict__ pos, const float4* __restrict__ tgt,
+                                              long long n, const float4* __restrict__ y, float* __restrict__ gy,
+                                              double* __restrict__ loss, int nm, float cs, float out_scale,
+                                              float gscale) {
+  __shared__ double red[4];
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const long long M = (long long)nm * nm * nm;
+  const bool live = p < n;
+  Cic c;
+  float gx = 0.0f, gyv = 0.0f, gz = 0.0f;
+  double lp = 0.0;
+  long long key = -1 - lane;
+  if (live) {
+    const float4 q = pos[p];
+    const float4 t = tgt[p];
+    cic_nodes(q.x, q.y, q.z, cs, nm, (long long)q.w * M, c);
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const float4 v = y[c.node[a]];
+      px += c.w[a] * v.x;
+      py += c.w[a] * v.y;
+      pz += c.w[a] * v.z;
+    }
+    const float ex = out_scale * px - t.x, ey = out_scale * py - t.y, ez = out_scale * pz - t.z;
+    lp = (double)t.w * ((double)ex * ex + (double)ey * ey + (double)ez * ez);
+    gx = gscale * t.w * ex;
+    gyv = gscale * t.w * ey;
+    gz = gscale * t.w * ez;
+    key = c.node[0];
+  } else {
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      c.node[a] = 0;
+      c.w[a] = 0.0f;
+    }
+  }
+  // run boundaries
+  const long long kprev = __shfl_up(key, 1);
+  const bool head = lane == 0 || kprev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long above = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+  const int run_end = above ? __ffsll((long long)above) - 2 : 63;
+  float v[24];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    v[3 * a] = c.w[a] * gx;
+    v[3 * a + 1] = c.w[a] * gyv;
+    v[3 * a + 2] = c.w[a] * gz;
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const bool take = lane + off <= run_end;
+#pragma unroll
+    for (int e = 0; e < 24; ++e) {
+      const float o = __shfl_down(v[e], off);
+      if (take) v[e] += o;
+    }
+  }
+  if (live && head) {
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      float* dst = gy + c.node[a] * 4;
+      atomicAdd(dst, v[3 * a]);
+      atomicAdd(dst + 1, v[3 * a + 1]);
+      atomicAdd(dst + 2, v[3 * a + 2]);
+    }
+  }
+  // loss: wave then block reduction, one FP64 atomic per block
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) lp += __shfl_down(lp, off);
+  if (lane == 0) red[threadIdx.x >> 6] = lp;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(loss, red[0] + red[1] + red[2] + red[3]);
+}
+
+__global__ void k_apply(const float* __restrict__ x, const float* __restrict__ yy, const float* __restrict__ z,
+                        long long n, const float4* __restrict__ ymesh, int nm, float cs, float out_scale,
+                        float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  Cic c;
+  cic_nodes(x[p], yy[p], z[p], cs, nm, 0, c);
+  float px = 0.0f, py = 0.0f, pz = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const float4 v = ymesh[c.node[a]];
+    px += c.w[a] * v.x;
+    py += c.w[a] * v.y;
+    pz += c.w[a] * v.z;
+  }
+  ax[p] += out_scale * px;
+  ay[p] += out_scale * py;
+  az[p] += out_scale * pz;
+}
+
+}  // namespace net
+}  // namespace gng
+
+using namespace gng::net;
+using gng::fail;
+
+struct gng_net {
+  int nm = 0;
+  int max_boxes = 0;
+  int device = 0;
+  long long M = 0;
+  // weights
+  float* d_params = nullptr;
+  float* d_grad = nullptr;
+  float* d_m1 = nullptr;
+  float* d_m2 = nullptr;
+  int adam_t = 0;
+  bf16x4* d_f1 = nullptr;   // forward fragments, layer 1..3
+  bf16x4* d_f2 = nullptr;
+  bf16x4* d_f3 = nullptr;
+  bf16x4* d_g3 = nullptr;   // data-gradient fragments of layers 3 and 2
+  bf16x4* d_g2 = nullptr;
+  float* d_b16 = nullptr;   // 3 x 16 padded biases
+  // activations
+  unsigned short* d_x0 = nullptr;
+  float* d_z1 = nullptr;
+  unsigned short* d_x1 = nullptr;
+  float* d_z2 = nullptr;
+  unsigned short* d_x2 = nullptr;
+  float* d_y = nullptr;
+  float* d_gy = nullptr;
+  unsigned short* d_dz3 = nullptr;
+  unsigned short* d_dz2 = nullptr;
+  unsigned short* d_dz1 = nullptr;
+  float* d_partial = nullptr;
+  double* d_loss = nullptr;
+  long long loss_n = 0;
+};
+
+namespace {
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+void free_net(gng_net* t) {
+  void* ptrs[] = {t->d_params, t->d_grad, t->d_m1, t->d_m2, t->d_f1, t->d_f2, t->d_f3, t->d_g3, t->d_g2,
+                  t->d_b16, t->d_x0, t->d_z1, t->d_x1, t->d_z2, t->d_x2, t->d_y, t->d_gy, t->d_dz3,
+                  t->d_dz2, t->d_dz1, t->d_partial, t->d_loss};
+  for (void* p : ptrs) (void)hipFree(p);
+}
+
+struct BufInfo {
+  void* ptr;
+  long long bytes;
+};
+
+BufInfo buffer(const gng_net* t, int which) {
+  const long long cells = t->M * t->max_boxes;
+  switch (which) {
+    case GNG_NET_X0: return {t->d_x0, cells * 2 * 2};
+    case GNG_NET_Z1: return {t->d_z1, cells * 16 * 4};
+    case GNG_NET_X1: return {t->d_x1, cells * 16 * 2};
+    case GNG_NET_Z2: return {t->d_z2, cells * 16 * 4};
+    case GNG_NET_X2: return {t->d_x2, cells * 16 * 2};
+    case GNG_NET_Y: return {t->d_y, cells * 4 * 4};
+    case GNG_NET_GY: return {t->d_gy, cells * 4 * 4};
+    case GNG_NET_DZ3: return {t->d_dz3, cells * 4 * 2};
+    case GNG_NET_DZ2: return {t->d_dz2, cells * 16 * 2};
+    case GNG_NET_DZ1: return {t->d_dz1, cells * 16 * 2};
+    case GNG_NET_GRAD: return {t->d_grad, (long long)P_COUNT * 4};
+    case GNG_NET_PARAMS: return {t->d_params, (long long)P_COUNT * 4};
+    default: return {nullptr, -1};
+  }
+}
+
+int repack(gng_net* t, hipStream_t s) {
+  const float* p = t->d_params;
+  hipLaunchKernelGGL(k_pack, dim3(1), dim3(KSteps<2>::v * 64), 0, s, p + P_W1, 2, 16, 2, KSteps<2>::v, 0, t->d_f1);
+  hipLaunchKernelGGL(k_pack, dim3(27), dim3(64), 0, s, p + P_W2, 16, 16, 16, 27, 0, t->d_f2);
+  hipLaunchKernelGGL(k_pack, dim3(27), dim3(64), 0, s, p + P_W3, 16, 3, 16, 27, 0, t->d_f3);
+  hipLaunchKernelGGL(k_pack, dim3(KSteps<4>::v), dim3(64), 0, s, p + P_W3, 16, 3, 4, KSteps<4>::v, 1, t->d_g3);
+  hipLaunchKernelGGL(k_pack, dim3(27), dim3(64), 0, s, p + P_W2, 16, 16, 16, 27, 1, t->d_g2);
+  hipLaunchKernelGGL(k_bias16, dim3(1), dim3(64), 0, s, p + P_B1, 16, t->d_b16);
+  hipLaunchKernelGGL(k_bias16, dim3(1), dim3(64), 0, s, p + P_B2, 16, t->d_b16 + 16);
+  hipLaunchKernelGGL(k_bias16, dim3(1), dim3(64), 0, s, p + P_B3, 3, t->d_b16 + 32);
+  GNG_HIP(hipGetLastError());
+  return GNG_OK;
+}
+
+int check_boxes(const gng_net* t, int boxes) {
+  if (boxes < 1 || boxes > t->max_boxes) return fail(GNG_ERR_INVALID, "box count outside [1, max_boxes]");
+  return GNG_OK;
+}
+
+dim3 conv_grid(const gng_net* t, int boxes) {
+  return dim3((unsigned)(t->nm / TX), (unsigned)((t->nm / TY) * (t->nm / TZ)), (unsigned)boxes);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gng_net_param_count(void) { return P_COUNT; }
+
+int gng_net_create(int32_t grid_size, int32_t max_boxes, int32_t device, gng_net** out) {
+  if (!out) return fail(GNG_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (grid_size < 16 || grid_size % 16 != 0 || grid_size > 1024)
+    return fail(GNG_ERR_INVALID, "network mesh must be a multiple of 16 in [16, 1024]");
+  if (max_boxes < 1) return fail(GNG_ERR_INVALID, "max_boxes must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(GNG_ERR_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= ndev) return fail(GNG_ERR_INVALID, "device index out of range");
+  GNG_HIP(hipSetDevice(device));
+  gng_net* t = new (std::nothrow) gng_net();
+  if (!t) return fail(GNG_ERR_OOM, "host allocation failed");
+  t->nm = grid_size;
+  t->max_boxes = max_boxes;
+  t->device = device;
+  t->M = (long long)grid_size * grid_size * grid_size;
+  const long long cells = t->M * max_boxes;
+  hipError_t e = hipSuccess;
+  auto A = [&](auto** p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(p), bytes);
+  };
+  A(&t->d_params, P_COUNT * 4);
+  A(&t->d_grad, P_COUNT * 4);
+  A(&t->d_m1, P_COUNT * 4);
+  A(&t->d_m2, P_COUNT * 4);
+  A(&t->d_f1, KSteps<2>::v * 64 * 8);
+  A(&t->d_f2, 27 * 64 * 8);
+  A(&t->d_f3, 27 * 64 * 8);
+  A(&t->d_g3, KSteps<4>::v * 64 * 8);
+  A(&t->d_g2, 27 * 64 * 8);
+  A(&t->d_b16, 48 * 4);
+  A(&t->d_x0, cells * 4);
+  A(&t->d_z1, cells * 64);
+  A(&t->d_x1, cells * 32);
+  A(&t->d_z2, cells * 64);
+  A(&t->d_x2, cells * 32);
+  A(&t->d_y, cells * 16);
+  A(&t->d_gy, cells * 16);
+  A(&t->d_dz3, cells * 8);
+  A(&t->d_dz2, cells * 32);
+  A(&t->d_dz1, cells * 32);
+  A(&t->d_partial, (size_t)kWgradBlocks * 28 * 256 * 4);
+  A(&t->d_loss, 8);
+  if (e != hipSuccess) {
+    free_net(t);
+    delete t;
+    return fail(e == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP,
+                std::string("network buffers: ") + hipGetErrorString(e));
+  }
+  std::vector<float> zero(P_COUNT, 0.0f);
+  e = hipMemcpy(t->d_params, zero.data(), P_COUNT * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t->d_m1, zero.data(), P_COUNT * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t->d_m2, zero.data(), P_COUNT * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t->d_grad, zero.data(), P_COUNT * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    free_net(t);
+    delete t;
+    return fail(GNG_ERR_HIP, std::string("network init: ") + hipGetErrorString(e));
+  }
+  int rc = repack(t, nullptr);
+  if (rc == GNG_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(GNG_ERR_HIP, "network init sync failed");
+  if (rc != GNG_OK) {
+    free_net(t);
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return GNG_OK;
+}
+
+void gng_net_destroy(gng_net* net) {
+  if (!net) return;
+  (void)hipSetDevice(net->device);
+  (void)hipDeviceSynchronize();
+  free_net(net);
+  delete net;
+}
+
+int gng_net_set_params(gng_net* net, const float* host_params) {
+  if (!net || !host_params) return fail(GNG_ERR_INVALID, "null argument");
+  GNG_HIP(hipSetDevice(net->device));
+  GNG_HIP(hipMemcpy(net->d_params, host_params, P_COUNT * 4, hipMemcpyHostToDevice));
+  std::vector<float> zero(P_COUNT, 0.0f);
+  GNG_HIP(hipMemcpy(net->d_m1, zero.data(), P_COUNT * 4, hipMemcpyHostToDevice));
+  GNG_HIP(hipMemcpy(net->d_m2, zero.data(), P_COUNT * 4, hipMemcpyHostToDevice));
+  net->adam_t = 0;
+  int rc = repack(net, nullptr);
+  if (rc != GNG_OK) return rc;
+  GNG_HIP(hipDeviceSynchronize());
+  return GNG_OK;
+}
+
+int gng_net_get_params(gng_net* net, float* host_params) {
+  if (!net || !host_params) return fail(GNG_ERR_INVALID, "null argument");
+  GNG_HIP(hipSetDevice(net->device));
+  GNG_HIP(hipDeviceSynchronize());
+  GNG_HIP(hipMemcpy(host_params, net->d_params, P_COUNT * 4, hipMemcpyDeviceToHost));
+  return GNG_OK;
+}
+
+int64_t gng_net_buffer_bytes(const gng_net* net, int32_t which) {
+  if (!net) return -1;
+  return buffer(net, which).bytes;
+}
+
+int gng_net_device_buffer(gng_net* net, int32_t which, void** d_ptr) {
+  if (!net || !d_ptr) return fail(GNG_ERR_INVALID, "null argument");
+  const BufInfo b = buffer(net, which);
+  if (!b.ptr) return fail(GNG_ERR_INVALID, "bad buffer id");
+  *d_ptr = b.ptr;
+  return GNG_OK;
+}
+
+int gng_net_read(gng_net* net, int32_t which, void* host, int64_t bytes) {
+  if (!net || !host) return fail(GNG_ERR_INVALID, "null argument");
+  const BufInfo b = buffer(net, which);
+  if (!b.ptr || bytes < 0 || bytes > b.bytes) return fail(GNG_ERR_INVALID, "bad buffer id or size");
+  GNG_HIP(hipSetDevice(net->device));
+  GNG_HIP(hipDeviceSynchronize());
+  GNG_HIP(hipMemcpy(host, b.ptr, (size_t)bytes, hipMemcpyDeviceToHost));
+  return GNG_OK;
+}
+
+int gng_net_write(gng_net* net, int32_t which, const void* host, int64_t bytes) {
+  if (!net || !host) return fail(GNG_ERR_INVALID, "null argument");
+  const BufInfo b = buffer(net, which);
+  if (!b.ptr || bytes < 0 || bytes > b.bytes) return fail(GNG_ERR_INVALID, "bad buffer id or size");
+  if (which == GNG_NET_PARAMS) return gng_net_set_params(net, static_cast<const float*>(host));
+  GNG_HIP(hipSetDevice(net->device));
+  GNG_HIP(hipDeviceSynchronize());
+  GNG_HIP(hipMemcpy(b.ptr, host, (size_t)bytes, hipMemcpyHostToDevice));
+  return GNG_OK;
+}
+
+int gng_net_input(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
+                  float phi_scale, void* stream) {
+  if (!net || !d_rho || !d_phi) return fail(GNG_ERR_INVALID, "null argument");
+  if (box < 0 || box >= net->max_boxes) return fail(GNG_ERR_INVALID, "box slot out of range");
+  if (!(rho_mean > 0.0f) || !(phi_scale > 0.0f)) return fail(GNG_ERR_INVALID, "scales must be positive");
+  GNG_HIP(hipSetDevice(net->device));
+  hipLaunchKernelGGL(k_input, dim3((unsigned)((net->M + 255) / 256)), dim3(256), 0, as_stream(stream), d_rho, d_phi,
+                     net->M, 1.0f / rho_mean, 1.0f / phi_scale,
+                     reinterpret_cast<unsigned*>(net->d_x0) + (long long)box * net->M);
+  GNG_HIP(hipGetLastError());
+  return GNG_OK;
+}
+
+int gng_net_forward(gng_net* net, int32_t boxes, void* stream) {
+  if (!net) return fail(GNG_ERR_INVALID, "null network");
+  int rc = check_boxes(net, boxes);
+  if (rc != GNG_OK) return rc;
+  GNG_HIP(hipSetDevice(net->device));
+  hipStream_t s = as_stream(stream);
+  const dim3 g = conv_grid(net, boxes);
+  hipLaunchKernelGGL((k_conv<2, MODE_HIDDEN>), g, dim3(256), 0, s, net->d_x0, net->d_f1, net->d_b16, net->d_z1,
+                     net->d_x1, (const float*)nullptr, net->nm);
+  hipLaunchKernelGGL((k_conv<16, MODE_HIDDEN>), g, dim3(256), 0, s, net->d_x1, net->d_f2, net->d_b16 + 16,
+                     net->d_z2, net->d_x2, (const float*)nullptr, net->nm);
+  hipLaunchKernelGGL((k_conv<16, MODE_OUT>), g, dim3(256), 0, s, net->d_x2, net->d_f3, net->d_b16 + 32, net->d_y,
+                     (unsigned short*)nullptr, (const float*)nullptr, net->nm);
+  GNG_HIP(hipGetLastError());
+  return GNG_OK;
+}
+
+int gng_net_loss_grad(gng_net* net, const float* d_pos4, const float* d_tgt4, int64_t n, double box_size,
+                      float out_scale, void* stream) {
+  if (!net || !d_pos4 || !d_tgt4) return fail(GNG_ERR_INVALID, "null argument");
+  if (n <= 0) return fail(GNG_ERR_INVALID, "no training particles");
+  if (!(box_size > 0.0)) return fail(GNG_ERR_INVALID, "box_size must be positive");
+  GNG_HIP(hipSetDevice(net->device));
+  hipStream_t s = as_stream(stream);
+  const long long cells = net->M * net->max_boxes;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((cells * 4 + 255) / 256)), dim3(256), 0, s, net->d_gy, cells * 4, 0.0f);
+  hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(1), 0, s, net->d_loss);
+  const float cs = (float)((double)net->nm / box_size);
+  const float gscale = (float)(2.0 * out_scale / (double)n);
+  hipLaunchKernelGGL(k_loss, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(d_pos4), reinterpret_cast<const float4*>(d_tgt4), (long long)n,
+                     reinterpret_cast<const float4*>(net->d_y), net->d_gy, net->d_loss, net->nm, cs, out_scale, gscale);
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((cells * 4 + 255) / 256)), dim3(256), 0, s, net->d_gy, net->d_dz3,
+                     cells * 4);
+  GNG_HIP(hipGetLastError());
+  net->loss_n = n;
+  return GNG_OK;
+}
+
+int gng_net_backward(gng_net* net, int32_t boxes, void* stream) {
+  if (!net) return fail(GNG_ERR_INVALID, "null network");
+  int rc = check_boxes(net, boxes);
+  if (rc != GNG_OK) return rc;
+  GNG_HIP(hipSetDevice(net->device));
+  hipStream_t s = as_stream(stream);
+  const int nm = net->nm;
+  const dim3 g = conv_grid(net, boxes);
+  const long long wtiles = (long long)(nm / WX) * (nm / WY) * (nm / WZ) * boxes;
+  const int wb = (int)(wtiles < kWgradBlocks ? wtiles : kWgradBlocks);
+  float* gr = net->d_grad;
+  // layer 3: dW3 from x2 and dz3, then dz2 = conv^T(dz3) * silu'(z2)
+  hipLaunchKernelGGL((k_conv_wgrad<16, 4>), dim3(wb), dim3(256), 0, s, net->d_x2, net->d_dz3, net->d_partial, nm,
+                     boxes);
+  hipLaunchKernelGGL((k_wgrad_reduce<16, 3>), dim3((433 * 3 + 255) / 256), dim3(256), 0, s, net->d_partial, wb,
+                     gr + P_W3, gr + P_B3);
+  hipLaunchKernelGGL((k_conv<4, MODE_BWD>), g, dim3(256), 0, s, net->d_dz3, net->d_g3, (const float*)nullptr,
+                     (float*)nullptr, net->d_dz2, net->d_z2, nm);
+  // layer 2
+  hipLaunchKernelGGL((k_conv_wgrad<16, 16>), dim3(wb), dim3(256), 0, s, net->d_x1, net->d_dz2, net->d_partial, nm,
+                     boxes);
+  hipLaunchKernelGGL((k_wgrad_reduce<16, 16>), dim3((433 * 16 + 255) / 256), dim3(256), 0, s, net->d_partial, wb,
+                     gr + P_W2, gr + P_B2);
+  hipLaunchKernelGGL((k_conv<16, MODE_BWD>), g, dim3(256), 0, s, net->d_dz2, net->d_g2, (const float*)nullptr,
+                     (float*)nullptr, net->d_dz1, net->d_z1, nm);
+  // layer 1
+  hipLaunchKernelGGL((k_conv_wgrad<2, 16>), dim3(wb), dim3(256), 0, s, net->d_x0, net->d_dz1, net->d_partial, nm,
+                     boxes);
+  hipLaunchKernelGGL((k_wgrad_reduce<2, 16>), dim3((55 * 16 + 255) / 256), dim3(256), 0, s, net->d_partial, wb,
+                     gr + P_W1, gr + P_B1);
+  GNG_HIP(hipGetLastError());
+  return GNG_OK;
+}
+
+int gng_net_adam(gng_net* net, float lr, float beta1, float beta2, float eps, void* stream) {
+  if (!net) return fail(GNG_ERR_INVALID, "null network");
+  if (!(lr >= 0.0f) || !(beta1 >= 0.0f && beta1 < 1.0f) || !(beta2 >= 0.0f && beta2 < 1.0f) || !(eps > 0.0f))
+    return fail(GNG_ERR_INVALID, "bad Adam hyper-parameters");
+  GNG_HIP(hipSetDevice(net->device));
+  hipStream_t s = as_stream(stream);
+  net->adam_t += 1;
+  const float c1 = (float)(1.0 / (1.0 - std::pow((double)beta1, net->adam_t)));
+  const float c2 = (float)(1.0 / (1.0 - std::pow((double)beta2, net->adam_t)));
+  hipLaunchKernelGGL(k_adam, dim3((P_COUNT + 255) / 256), dim3(256), 0, s, net->d_params, net->d_grad, net->d_m1,
+                     net->d_m2, P_COUNT, lr, beta1, beta2, eps, c1, c2);
+  GNG_HIP(hipGetLastError());
+  return repack(net, s);
+}
+
+int gng_net_train_step(gng_net* net, int32_t boxes, const float* d_pos4, const float* d_tgt4, int64_t n,
+                       double box_size, float out_scale, float lr, void* stream) {
+  int rc = gng_net_forward(net, boxes, stream);
+  if (rc == GNG_OK) rc = gng_net_loss_grad(net, d_pos4, d_tgt4, n, box_size, out_scale, stream);
+  if (rc == GNG_OK) rc = gng_net_backward(net, boxes, stream);
+  if (rc == GNG_OK) rc = gng_net_adam(net, lr, 0.9f, 0.999f, 1e-8f, stream);
+  return rc;
+}
+
+int gng_net_loss(gng_net* net, double* loss_out) {
+  if (!net || !loss_out) return fail(GNG_ERR_INVALID, "null argument");
+  if (net->loss_n <= 0) return fail(GNG_ERR_INVALID, "no loss has been computed");
+  GNG_HIP(hipSetDevice(net->device));
+  GNG_HIP(hipDeviceSynchronize());
+  double v = 0.0;
+  GNG_HIP(hipMemcpy(&v, net->d_loss, 8, hipMemcpyDeviceToHost));
+  *loss_out = v / (double)net->loss_n;
+  return GNG_OK;
+}
+
+int gng_net_apply(gng_net* net, int32_t box, const float* d_x, const float* d_y, const float* d_z, int64_t n,
+                  double box_size, float out_scale, float* d_ax, float* d_ay, float* d_az, void* stream) {
+  if (!net || !d_x || !d_y || !d_z || !d_ax || !d_ay || !d_az) return fail(GNG_ERR_INVALID, "null argument");
+  if (box < 0 || box >= net->max_boxes) return fail(GNG_ERR_INVALID, "box slot out of range");
+  if (n < 0) return fail(GNG_ERR_INVALID, "negative particle count");
+  if (!(box_size > 0.0)) return fail(GNG_ERR_INVALID, "box_size must be positive");
+  if (n == 0) return GNG_OK;
+  GNG_HIP(hipSetDevice(net->device));
+  const float cs = (float)((double)net->nm / box_size);
+  hipLaunchKernelGGL(k_apply, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), d_x, d_y, d_z,
+                     (long long)n, reinterpret_cast<const float4*>(net->d_y) + (long long)box * net->M, net->nm, cs,
+                     out_scale, d_ax, d_ay, d_az);
+  GNG_HIP(hipGetLastError());
+  return GNG_OK;
+}
+
+}  // extern "C"

@@ -158,15 +158,19 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, 
 // stage in registers; the last stage stores the real row directly.
 template <int N, int ROWS>
 __global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float2* __restrict__ P,
-                                                   float2* __restrict__ B, const float2* __restrict__ tw) {
+                                                   float2* __restrict__ B, float2* __restrict__ pot,
+                                                   const float2* __restrict__ tw, int field0, float pot_scale) {
   constexpr int H = N / 2, T = 256, LS = XCfg<N>::LS;
   using F = RegFft<H, true, false, ROWS, T, LS, 1, false, N>;
   __shared__ float2 buf[ROWS * LS];
   const int tid = threadIdx.x;
-  const int field = blockIdx.y;
-  float2* data = field == 0 ? A : (field == 1 ? P : B);
+  // fields 0..2: F_x from A (times -i n_x), F_y from P, F_z from B, in place;
+  // field 3: the potential from A (no derivative) into pot, scaled
+  const int field = blockIdx.y + field0;
+  float2* data = field == 0 ? A : (field == 1 ? P : (field == 2 ? B : A));
   const long long row0 = (long long)blockIdx.x * ROWS;
   float2* row = data + row0 * H;
+  float2* orow = (field == 3 ? pot : data) + row0 * H;
   constexpr int TOT = ROWS * H;
 #pragma unroll
   for (int p = 0; p < (TOT + T - 1) / T; ++p) {
@@ -197,8 +201,9 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float
     }
   });
   F::run(vin, vout, buf, tw, tid);
+  const float sc = field == 3 ? pot_scale : 1.0f;
   for_edge<H, F::RL, ROWS, T, false>(tid, [&](int p, int r, int l, int i, bool ok) {
-    if (ok) row[l * H + i] = vout[p][r];
+    if (ok) orow[l * H + i] = make_float2(vout[p][r].x * sc, vout[p][r].y * sc);
   });
 }
 

@@ -47,6 +47,7 @@ GNG_ERR_UNSUPPORTED = 5
 
 STAGES = ("bucket", "sort", "deposit", "fft_x_fwd", "fft_y_fwd", "fft_z_green", "fft_y_inv", "fft_x_inv", "interp")
 
+SOURCES = ("gng_pm.hip", "learned.hip", "shortrange.hip")
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -56,7 +57,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, os.path.join(CSRC, "gng_pm.hip")]
+    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp] + [os.path.join(CSRC, f) for f in SOURCES]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
@@ -112,7 +113,58 @@ def lib():
         L.gng_dev_free.argtypes = [P]
         L.gng_dev_copy.argtypes = [P, P, ctypes.c_uint64, i32]
         L.gng_dev_synchronize.argtypes = []
-        for f in (L.gng_dev_alloc, L.gng_dev_free, L.gng_dev_copy, L.gng_dev_synchronize):
+        L.gng_dev_axpy.argtypes = [P, P, i64, f32, P]
+        for f in (L.gng_dev_alloc, L.gng_dev_free, L.gng_dev_copy, L.gng_dev_synchronize, L.gng_dev_axpy):
+            f.restype = i32
+        L.gng_pm_batch_create.argtypes = [i32, ctypes.POINTER(GngConfig), ctypes.POINTER(P)]
+        L.gng_pm_batch_create.restype = i32
+        L.gng_pm_batch_destroy.argtypes = [P]
+        L.gng_pm_batch_destroy.restype = None
+        L.gng_pm_batch_size.argtypes = [P]
+        L.gng_pm_batch_size.restype = i32
+        L.gng_pm_batch_plan.argtypes = [P, i32, ctypes.POINTER(P)]
+        L.gng_pm_batch_plan.restype = i32
+        L.gng_pm_batch_solve_device.argtypes = [P, P, P, P, P, P, d, P, P, P, P]
+        L.gng_pm_batch_solve_device.restype = i32
+        L.gng_hilbert_key3.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]
+        L.gng_hilbert_key3.restype = ctypes.c_uint64
+        L.gng_pm_keep_potential.argtypes = [P, i32]
+        L.gng_pm_potential.argtypes = [P, P]
+        L.gng_pm_device_meshes.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
+        for f in (L.gng_pm_keep_potential, L.gng_pm_potential, L.gng_pm_device_meshes):
+            f.restype = i32
+        # short range and learned correction (include/gng_learned.h)
+        L.gng_sr_create.argtypes = [i32, ctypes.POINTER(P)]
+        L.gng_sr_create.restype = i32
+        L.gng_sr_destroy.argtypes = [P]
+        L.gng_sr_destroy.restype = None
+        L.gng_sr_accelerations.argtypes = [P, P, P, P, P, i64, d, d, d, d, d, i32, P, P, P, P]
+        L.gng_sr_accelerations.restype = i32
+        L.gng_sr_last_pair_count.argtypes = [P]
+        L.gng_sr_last_pair_count.restype = i64
+        L.gng_net_param_count.restype = i32
+        L.gng_net_create.argtypes = [i32, i32, i32, ctypes.POINTER(P)]
+        L.gng_net_destroy.argtypes = [P]
+        L.gng_net_destroy.restype = None
+        L.gng_net_set_params.argtypes = [P, P]
+        L.gng_net_get_params.argtypes = [P, P]
+        L.gng_net_buffer_bytes.argtypes = [P, i32]
+        L.gng_net_buffer_bytes.restype = i64
+        L.gng_net_read.argtypes = [P, i32, P, i64]
+        L.gng_net_write.argtypes = [P, i32, P, i64]
+        L.gng_net_device_buffer.argtypes = [P, i32, ctypes.POINTER(P)]
+        L.gng_net_device_buffer.restype = i32
+        L.gng_net_input.argtypes = [P, i32, P, P, f32, f32, P]
+        L.gng_net_forward.argtypes = [P, i32, P]
+        L.gng_net_loss_grad.argtypes = [P, P, P, i64, d, f32, P]
+        L.gng_net_backward.argtypes = [P, i32, P]
+        L.gng_net_adam.argtypes = [P, f32, f32, f32, f32, P]
+        L.gng_net_train_step.argtypes = [P, i32, P, P, i64, d, f32, f32, P]
+        L.gng_net_loss.argtypes = [P, ctypes.POINTER(d)]
+        L.gng_net_apply.argtypes = [P, i32, P, P, P, i64, d, f32, P, P, P, P]
+        for f in (L.gng_net_create, L.gng_net_set_params, L.gng_net_get_params, L.gng_net_read, L.gng_net_write,
+                  L.gng_net_input, L.gng_net_forward, L.gng_net_loss_grad, L.gng_net_backward, L.gng_net_adam,
+                  L.gng_net_train_step, L.gng_net_loss, L.gng_net_apply):
             f.restype = i32
         L.hip_pm_create.argtypes = [i32, f32]
         L.hip_pm_create.restype = P
@@ -122,6 +174,11 @@ def lib():
         L.hip_pm_solve.restype = i32
         _lib = L
     return _lib
+
+
+def hilbert_key3(x: int, y: int, z: int, bits: int) -> int:
+    """3-D Hilbert key used to order the deposit tiles (host function, no GPU)."""
+    return int(lib().gng_hilbert_key3(x, y, z, bits))
 
 
 def last_error() -> str:
@@ -162,7 +219,8 @@ class Plan:
     """Owns one gng_pm_plan (device meshes for one mesh size and filter set)."""
 
     def __init__(self, grid_size: int, box_size: float, r_split: float = 0.0, plummer_eps: float = 0.0,
-                 deconvolve: bool = False, device: int = -1, keep_density: bool = False):
+                 deconvolve: bool = False, device: int = -1, keep_density: bool = False,
+                 keep_potential: bool = False):
         cfg = GngConfig()
         cfg.grid_size = int(grid_size)
         cfg.deconvolve = int(bool(deconvolve))
@@ -177,6 +235,8 @@ class Plan:
         self._h = h
         if keep_density:
             self._check(lib().gng_pm_keep_density(h, 1), "gng_pm_keep_density")
+        if keep_potential:
+            self._check(lib().gng_pm_keep_potential(h, 1), "gng_pm_keep_potential")
         self.grid_size = int(grid_size)
         self.box_size = float(box_size)
         self.r_split = float(r_split)
@@ -247,6 +307,19 @@ class Plan:
         out = np.empty((nm, nm, nm), dtype=np.float64)
         self._check(lib().gng_pm_density(self._h, _ptr(out)), "gng_pm_density")
         return out
+
+    def potential(self) -> np.ndarray:
+        """Mesh potential of the last solve (needs keep_potential=True)."""
+        nm = self.grid_size
+        out = np.empty((nm, nm, nm), dtype=np.float64)
+        self._check(lib().gng_pm_potential(self._h, _ptr(out)), "gng_pm_potential")
+        return out
+
+    def device_meshes(self):
+        """(mass-per-cell, potential) device pointers of the last solve."""
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        self._check(lib().gng_pm_device_meshes(self._h, ctypes.byref(a), ctypes.byref(b)), "gng_pm_device_meshes")
+        return int(a.value), int(b.value)
 
     def force_meshes(self) -> np.ndarray:
         """(3, nm, nm, nm) f32 force meshes of the last solve."""
@@ -332,6 +405,47 @@ def device_synchronize() -> None:
     rc = lib().gng_dev_synchronize()
     if rc != GNG_OK:
         raise HipExecutionError("gng_dev_synchronize", rc, last_error())
+
+
+class Batch:
+    """An ensemble of independent periodic boxes solved together (one HIP graph)."""
+
+    def __init__(self, boxes, device: int = -1):
+        """boxes: sequence of dicts with grid_size, box_size and optional r_split,
+        plummer_eps, deconvolve."""
+        cfgs = (GngConfig * len(boxes))()
+        for c, b in zip(cfgs, boxes):
+            c.grid_size = int(b["grid_size"])
+            c.box_size = float(b["box_size"])
+            c.r_split = float(b.get("r_split", 0.0))
+            c.plummer_eps = float(b.get("plummer_eps", 0.0))
+            c.deconvolve = int(bool(b.get("deconvolve", False)))
+            c.device = int(device)
+        h = ctypes.c_void_p()
+        rc = lib().gng_pm_batch_create(len(boxes), cfgs, ctypes.byref(h))
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_pm_batch_create", rc, last_error())
+        self._h = h
+        self.n_boxes = len(boxes)
+
+    def solve_device(self, x: int, y: int, z: int, m: int, offsets, g: float, ax: int, ay: int, az: int,
+                     stream: int = 0):
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        assert off.shape[0] == self.n_boxes + 1
+        rc = lib().gng_pm_batch_solve_device(self._h, x, y, z, m, _ptr(off), float(g), ax, ay, az, stream or None)
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_pm_batch_solve_device", rc, last_error())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().gng_pm_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _as_positions(global_positions) -> np.ndarray:

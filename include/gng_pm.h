@@ -139,6 +139,22 @@ int gng_pm_density(gng_pm_plan *p, double *out);
  * (crates/gadget-ng-pm/src/fft_poisson.rs:66-122).  Synchronous. */
 int gng_pm_force_meshes(gng_pm_plan *p, float *out);
 
+/* Keep the mesh potential of the following solves (one extra nm^3 f32 buffer
+ * and one extra x pass).  The potential is the field whose gradient the
+ * solve returns, Phi_hat = -4 pi G W rho_hat / k^2 with the same filters and
+ * deconvolution (fft_poisson.rs:509-549 before the -i k factor); k = 0 is
+ * dropped.  No reference counterpart: the learned correction reads it. */
+int gng_pm_keep_potential(gng_pm_plan *p, int enable);
+
+/* Potential of the last solve (nm^3 values widened to f64, flat
+ * iz*nm^2+iy*nm+ix).  Synchronous. */
+int gng_pm_potential(gng_pm_plan *p, double *out);
+
+/* Device pointers of the kept mass-per-cell and potential meshes of the last
+ * solve (either output pointer may be NULL).  They stay valid until the plan
+ * is destroyed and are overwritten by the next solve. */
+int gng_pm_device_meshes(gng_pm_plan *p, const float **d_mass, const float **d_potential);
+
 /* Per-stage GPU times in milliseconds, from HIP events recorded on the solve's
  * stream between the stages of every solve issued while timing is enabled
  * (gng_pm_set_timing resets the record), averaged over the most recent 64
@@ -151,6 +167,37 @@ int gng_pm_force_meshes(gng_pm_plan *p, float *out);
 int gng_pm_set_timing(gng_pm_plan *p, int enable);
 int gng_pm_stage_times(gng_pm_plan *p, float *ms, int capacity);
 
+/* ---- ensembles of independent boxes --------------------------------------
+ * No single reference function: the reference solves one box per call
+ * (PmSolver::accelerations_for_indices); an ensemble is a set of such calls
+ * run together.  Each box has its own configuration (mesh size, box size,
+ * filters).  The first solve with a given set of pointers, offsets and g
+ * captures the pipelines of all boxes into one HIP graph, with boxes spread
+ * over parallel branches so small meshes run side by side; later solves with
+ * the same arguments replay it. */
+typedef struct gng_pm_batch gng_pm_batch;
+
+int gng_pm_batch_create(int n_boxes, const gng_pm_config *cfgs, gng_pm_batch **out);
+void gng_pm_batch_destroy(gng_pm_batch *b);
+int gng_pm_batch_size(const gng_pm_batch *b);
+/* The plan of box i (for per-box readback such as gng_pm_force_meshes). */
+int gng_pm_batch_plan(gng_pm_batch *b, int i, gng_pm_plan **out);
+
+/* Device-resident ensemble solve.  The particles of box i are entries
+ * [offsets[i], offsets[i+1]) of the concatenated SoA f32 device arrays
+ * (offsets is a host array of n_boxes + 1 values); accelerations are written
+ * at the same entries.  Enqueued on `stream`; returns without synchronising. */
+int gng_pm_batch_solve_device(gng_pm_batch *b, const float *d_x, const float *d_y, const float *d_z,
+                              const float *d_mass, const int64_t *offsets, double g, float *d_ax,
+                              float *d_ay, float *d_az, void *stream);
+
+/* 3-D Hilbert key of (x, y, z), `bits` bits per axis, x most significant in
+ * the interleave (Skilling's transform; the curve of
+ * crates/gadget-ng-parallel/src/sfc.rs:95-130, used there for domains).  The
+ * deposit orders its tiles, and hence the sorted particles, by this key.
+ * Pure host function; needs no device. */
+uint64_t gng_hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int32_t bits);
+
 /* ---- 3. device-memory helpers --------------------------------------------
  * No counterpart in the reference.  For callers that hold no GPU runtime of
  * their own (test and benchmark harnesses, FFI hosts) and need device-resident
@@ -161,6 +208,8 @@ int gng_dev_alloc(void **ptr, uint64_t bytes);
 int gng_dev_free(void *ptr);
 int gng_dev_copy(void *dst, const void *src, uint64_t bytes, int kind);
 int gng_dev_synchronize(void);
+/* d_y[i] += a * d_x[i] for n floats on `stream` (asynchronous). */
+int gng_dev_axpy(float *d_y, const float *d_x, int64_t n, float a, void *stream);
 
 #ifdef __cplusplus
 } /* extern "C" */

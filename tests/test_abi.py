@@ -15,10 +15,16 @@ import gadget_ng_pm_hip as pm
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
+HEADERS = ("gng_pm.h", "gng_learned.h")
+
+
 def declared_functions():
-    text = open(os.path.join(ROOT, "include", "gng_pm.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:gng|hip)_\w+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b((?:gng|hip)_\w+)\s*\(", text))
+    return sorted(names)
 
 
 def has_device():
@@ -109,6 +115,26 @@ def test_without_device_solver_is_unavailable():
         pm.PmSolver(16, 1.0).accelerations_for_indices(np.zeros((1, 3)), np.ones(1), 0.0, 1.0, [0])
 
 
+def test_learned_symbols_declared_and_parameter_count():
+    names = declared_functions()
+    for sym in ("gng_sr_accelerations", "gng_net_forward", "gng_net_backward", "gng_net_train_step",
+                "gng_pm_keep_potential", "gng_dev_axpy"):
+        assert sym in names
+    # w1[27][2][16] b1[16] w2[27][16][16] b2[16] w3[27][16][3] b3[3]
+    assert pm.lib().gng_net_param_count() == 27 * 2 * 16 + 16 + 27 * 16 * 16 + 16 + 27 * 16 * 3 + 3
+
+
+@pytest.mark.skipif("has_device()")
+def test_learned_objects_need_a_device():
+    L = pm.lib()
+    h = ctypes.c_void_p()
+    assert L.gng_net_create(16, 1, 0, ctypes.byref(h)) == pm.GNG_ERR_NO_DEVICE and not h.value
+    assert L.gng_sr_create(0, ctypes.byref(h)) == pm.GNG_ERR_NO_DEVICE and not h.value
+    assert L.gng_net_create(24, 1, 0, ctypes.byref(h)) == pm.GNG_ERR_INVALID
+    L.gng_net_destroy(None)
+    L.gng_sr_destroy(None)
+
+
 def test_config_struct_layout_matches_header():
     # int32, int32, double, double, double, int32, int32[7] -> 64 bytes with padding
     assert ctypes.sizeof(pm.GngConfig) == 64
@@ -119,3 +145,23 @@ def test_treepm_effective_r_split():
     lr = pm.TreePmLongRange(64, 128.0)
     assert lr.effective_r_split() == pytest.approx(2.5 * 128.0 / 64)
     assert pm.TreePmLongRange(64, 128.0, 0.7).effective_r_split() == 0.7
+
+
+def test_hilbert_key_first_order_matches_reference_doc():
+    """crates/gadget-ng-parallel/src/sfc.rs:113-116 lists the p = 1 visiting order."""
+    order = [(0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), (1, 0, 0)]
+    assert [pm.hilbert_key3(*c, 1) for c in order] == list(range(8))
+
+
+@pytest.mark.parametrize("bits", [2, 3, 4, 5])
+def test_hilbert_curve_is_a_face_adjacent_bijection(bits):
+    n = 1 << bits
+    keys = {}
+    for z in range(n):
+        for y in range(n):
+            for x in range(n):
+                keys[pm.hilbert_key3(x, y, z, bits)] = (x, y, z)
+    assert sorted(keys) == list(range(n ** 3))
+    for k in range(n ** 3 - 1):
+        a, b = keys[k], keys[k + 1]
+        assert sum(abs(u - v) for u, v in zip(a, b)) == 1, (k, a, b)

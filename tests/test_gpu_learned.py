@@ -1,0 +1,285 @@
+"""HIP short range, mesh potential and the BF16-MFMA convolution network
+against the oracle (oracle/learned_oracle.py).
+
+Kernel-level checks feed each kernel the GPU's own inputs (the bf16 tensors
+it stored), so the only differences left are FP32 summation order and the
+rounding of a stored bfloat16; the end-to-end check compares the whole chain
+with the oracle run in bf16 emulation."""
+import math
+
+import numpy as np
+import pytest
+
+import learned_oracle as lo
+from helpers import rms_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_max(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def bf16_ulp_ok(got, ref, max_ulps=1, frac=0.999):
+    """got and ref are bf16-representable; count elements more than max_ulps apart."""
+    got = np.asarray(got, np.float64).ravel()
+    ref = np.asarray(ref, np.float64).ravel()
+    ulp = np.maximum(np.abs(ref), 1e-30) * 2.0 ** -7
+    close = np.abs(got - ref) <= max_ulps * ulp + 1e-30
+    return close.mean() >= frac and np.all(np.abs(got - ref) <= 4 * ulp + 1e-6 * np.abs(ref).max())
+
+
+def _params(seed, big_out=True):
+    p = lo.init_params(seed)
+    g = np.random.default_rng(seed + 1)
+    if big_out:
+        p["w3"] *= 10.0
+    for k in ("b1", "b2", "b3"):
+        p[k] = g.normal(scale=0.2, size=p[k].shape)
+    return {k: v.astype(np.float32) for k, v in p.items()}
+
+
+@pytest.fixture(scope="module")
+def learned(pm):
+    from gadget_ng_pm_hip import learned as L
+    return L
+
+
+# ---------------------------------------------------------------------------
+# potential
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("nm,r_split", [(32, 0.0), (48, 0.7)])
+def test_potential_matches_oracle(pm, nm, r_split):
+    print(f"case potential nm={nm} r_split={r_split}", flush=True)
+    g = np.random.default_rng(nm)
+    L = 10.0
+    pos = g.uniform(0, L, (4000, 3)).astype(np.float32)
+    m = g.uniform(0.5, 1.5, 4000).astype(np.float32)
+    plan = pm.Plan(nm, L, r_split=r_split, device=0, keep_density=True, keep_potential=True)
+    plan.accelerations_f32(pos[:, 0], pos[:, 1], pos[:, 2], m, 1.0)
+    mass = plan.density()
+    phi = plan.potential()
+    ref = lo.potential_mesh(mass / (L / nm) ** 3, 1.0, L, r_split=r_split)
+    assert rel_max(phi, ref) < 2e-5
+
+
+# ---------------------------------------------------------------------------
+# short range
+# ---------------------------------------------------------------------------
+
+def _clustered(n_uniform, n_sphere, L, seed):
+    g = np.random.default_rng(seed)
+    u = g.uniform(0, L, (n_uniform, 3))
+    cen = g.uniform(0, L, (3, 3))
+    s = cen[g.integers(0, 3, n_sphere)] + g.normal(scale=0.3, size=(n_sphere, 3))
+    pos = np.mod(np.concatenate([u, s]), L).astype(np.float32)
+    pos[pos >= L] = 0.0
+    m = g.uniform(0.5, 1.5, len(pos)).astype(np.float32)
+    return pos, m
+
+
+@pytest.mark.parametrize("kernel", ["exact", "reference"])
+@pytest.mark.parametrize("L", [20.0, 7.5])
+def test_short_range_matches_direct_sum(learned, kernel, L):
+    """L = 7.5 with r_cut = 2.5 gives the smallest chaining mesh (3^3 cells)."""
+    print(f"case short range kernel={kernel} L={L}", flush=True)
+    pos, m = _clustered(2000, 1000, L, 7)
+    rs, rc, eps2 = 0.5, 2.5, 0.01
+    sr = learned.ShortRange(device=0)
+    got = sr.accelerations(pos, m, L, rs, rc, eps2=eps2, kernel=kernel)
+    ref = lo.short_range_direct(pos.astype(np.float64), m.astype(np.float64), L, rs, rc, eps2=eps2, kernel=kernel)
+    assert rms_rel(got, ref) < 1e-5
+    assert sr.last_pair_count() > len(pos)
+
+
+def test_treepm_long_plus_short_is_newton(pm, learned):
+    g = np.random.default_rng(5)
+    L, n, nm = 1.0, 12, 64
+    pos = g.uniform(0, L, (n, 3)).astype(np.float32).astype(np.float64)
+    m = g.uniform(0.5, 1.5, n)
+    ref = lo.ewald_accelerations(pos, m, L)
+    tp = learned.TreePmSolver(nm, L, r_split=4 * L / nm, kernel="exact", deconvolve=True, device=0)
+    a = tp.accelerations_for_indices(pos, m, 0.0, 1.0, np.arange(n))
+    assert rms_rel(a, ref) < 3e-3
+    tpr = learned.TreePmSolver(nm, L, r_split=4 * L / nm, kernel="reference", deconvolve=True, device=0)
+    assert rms_rel(tpr.accelerations_for_indices(pos, m, 0.0, 1.0, np.arange(n)), ref) > 0.1
+
+
+def test_short_range_empty_and_errors(learned, pm):
+    sr = learned.ShortRange(device=0)
+    z = pm.DeviceArray((4,), np.float32)
+    sr.accelerations_device(z.ptr, z.ptr, z.ptr, z.ptr, 0, 10.0, 0.5, 2.5, 0.0, 1.0, 0, z.ptr, z.ptr, z.ptr)
+    with pytest.raises(pm.HipExecutionError):
+        sr.accelerations_device(z.ptr, z.ptr, z.ptr, z.ptr, 4, 5.0, 0.5, 2.5, 0.0, 1.0, 0, z.ptr, z.ptr, z.ptr)
+
+
+# ---------------------------------------------------------------------------
+# network kernels
+# ---------------------------------------------------------------------------
+
+def _forward(learned, nm, boxes, seed):
+    g = np.random.default_rng(seed)
+    x0 = lo.bf16_round(g.normal(size=(boxes, nm, nm, nm, 2)).astype(np.float32))
+    p = _params(seed)
+    net = learned.Net(nm, boxes, device=0)
+    net.set_params(p)
+    net.write("x0", x0)
+    net.forward(boxes)
+    return net, p, x0
+
+
+def test_conv_forward_kernels(learned):
+    nm, boxes = 16, 2
+    print(f"case conv forward nm={nm} boxes={boxes}", flush=True)
+    net, p, x0 = _forward(learned, nm, boxes, 11)
+    wb = {k: lo.bf16_round(p[k]).astype(np.float64) for k in ("w1", "w2", "w3")}
+    z1, x1, z2, x2, y = (net.read(k) for k in ("z1", "x1", "z2", "x2", "y"))
+    assert np.array_equal(net.read("x0"), x0)
+    for b in range(boxes):
+        assert rel_max(z1[b], lo.conv3d(x0[b], wb["w1"], p["b1"])) < 1e-5
+        assert bf16_ulp_ok(x1[b], lo.bf16_round(lo.silu(z1[b].astype(np.float64))))
+        assert rel_max(z2[b], lo.conv3d(x1[b], wb["w2"], p["b2"])) < 1e-5
+        assert bf16_ulp_ok(x2[b], lo.bf16_round(lo.silu(z2[b].astype(np.float64))))
+        assert rel_max(y[b][..., :3], lo.conv3d(x2[b], wb["w3"], p["b3"])) < 1e-5
+        assert np.all(y[b][..., 3] == 0.0)
+
+
+def _train_particles(nm, boxes, seed, L):
+    g = np.random.default_rng(seed)
+    per = 3000
+    pos = g.uniform(0, L, (boxes * per, 3)).astype(np.float32)
+    pos[::7] = pos[::7] // (L / nm) * (L / nm) + 0.3  # runs of particles in shared cells
+    bidx = np.repeat(np.arange(boxes), per)
+    c = np.floor(pos.astype(np.float64) * (nm / L)).astype(np.int64) % nm
+    key = bidx * nm ** 3 + (c[:, 2] * nm + c[:, 1]) * nm + c[:, 0]
+    order = np.argsort(key, kind="stable")
+    pos, bidx = pos[order], bidx[order]
+    pos4 = np.concatenate([pos, bidx[:, None].astype(np.float32)], axis=1)
+    tgt4 = np.concatenate([g.normal(size=(len(pos), 3)), g.uniform(0.1, 2.0, (len(pos), 1))], axis=1)
+    return pos4.astype(np.float32), tgt4.astype(np.float32)
+
+
+def test_loss_and_mesh_gradient(learned, pm):
+    nm, boxes, L, osc = 16, 2, 16.0, 1.7
+    print(f"case loss nm={nm}", flush=True)
+    net, p, x0 = _forward(learned, nm, boxes, 12)
+    pos4, tgt4 = _train_particles(nm, boxes, 3, L)
+    dp, dt = pm.DeviceArray.from_host(pos4), pm.DeviceArray.from_host(tgt4)
+    net.loss_grad(dp.ptr, dt.ptr, len(pos4), L, osc)
+    loss = net.loss()
+    y = net.read("y").astype(np.float64)
+    gy = net.read("gy")
+    ref_loss = 0.0
+    for b in range(boxes):
+        sel = pos4[:, 3] == b
+        lb, gb = lo.weighted_loss(y[b][..., :3], pos4[sel, :3].astype(np.float64), L, tgt4[sel, :3], tgt4[sel, 3],
+                                  osc, len(pos4))
+        ref_loss += lb
+        assert rel_max(gy[b][..., :3], gb) < 1e-5
+        assert np.all(gy[b][..., 3] == 0.0)
+    assert loss == pytest.approx(ref_loss, rel=1e-5)
+    assert np.array_equal(net.read("dz3"), lo.bf16_round(gy))
+
+
+def test_backward_kernels(learned, pm):
+    nm, boxes, L, osc = 16, 2, 16.0, 1.7
+    print(f"case backward nm={nm}", flush=True)
+    net, p, x0 = _forward(learned, nm, boxes, 13)
+    pos4, tgt4 = _train_particles(nm, boxes, 4, L)
+    dp, dt = pm.DeviceArray.from_host(pos4), pm.DeviceArray.from_host(tgt4)
+    net.loss_grad(dp.ptr, dt.ptr, len(pos4), L, osc)
+    net.backward(boxes)
+    from gadget_ng_pm_hip.learned import unpack_params
+    gr = unpack_params(net.read("grad"))
+    wb = {k: lo.bf16_round(p[k]).astype(np.float64) for k in ("w1", "w2", "w3")}
+    z1, x1, z2, x2 = (net.read(k).astype(np.float64) for k in ("z1", "x1", "z2", "x2"))
+    dz3, dz2, dz1 = (net.read(k).astype(np.float64) for k in ("dz3", "dz2", "dz1"))
+    ref = {k: 0.0 for k in gr}
+    for b in range(boxes):
+        d3 = dz3[b][..., :3]
+        w, bb = lo.conv3d_backward_weights(x2[b], d3)
+        ref["w3"] = ref["w3"] + w
+        ref["b3"] = ref["b3"] + bb
+        assert bf16_ulp_ok(dz2[b], lo.bf16_round(lo.conv3d_backward_data(d3, wb["w3"]) * lo.silu_grad(z2[b])))
+        w, bb = lo.conv3d_backward_weights(x1[b], dz2[b])
+        ref["w2"] = ref["w2"] + w
+        ref["b2"] = ref["b2"] + bb
+        assert bf16_ulp_ok(dz1[b], lo.bf16_round(lo.conv3d_backward_data(dz2[b], wb["w2"]) * lo.silu_grad(z1[b])))
+        w, bb = lo.conv3d_backward_weights(x0[b].astype(np.float64), dz1[b])
+        ref["w1"] = ref["w1"] + w
+        ref["b1"] = ref["b1"] + bb
+    for k in gr:
+        assert rel_max(gr[k], ref[k]) < 2e-5, k
+
+
+def test_gradients_end_to_end_against_oracle(learned, pm):
+    """Whole chain (forward, loss, backward) against the oracle run with the
+    same bfloat16 rounding points."""
+    nm, boxes, L, osc = 16, 1, 16.0, 1.3
+    net, p, x0 = _forward(learned, nm, boxes, 14)
+    pos4, tgt4 = _train_particles(nm, boxes, 5, L)
+    dp, dt = pm.DeviceArray.from_host(pos4), pm.DeviceArray.from_host(tgt4)
+    net.loss_grad(dp.ptr, dt.ptr, len(pos4), L, osc)
+    net.backward(boxes)
+    from gadget_ng_pm_hip.learned import unpack_params
+    gr = unpack_params(net.read("grad"))
+    fw = lo.net_forward(x0[0], p, bf16=True)
+    _, dy = lo.weighted_loss(fw["y"], pos4[:, :3].astype(np.float64), L, tgt4[:, :3], tgt4[:, 3], osc, len(pos4))
+    ref = lo.net_backward(fw, dy, bf16=True)
+    for k in gr:
+        a = np.asarray(gr[k], np.float64).ravel()
+        b = np.asarray(ref[k], np.float64).ravel()
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+        assert cos > 0.9999, (k, cos)
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-2, k
+
+
+def test_adam_step_matches_formula(learned, pm):
+    nm, boxes, L, osc = 16, 1, 16.0, 1.3
+    net, p, x0 = _forward(learned, nm, boxes, 15)
+    pos4, tgt4 = _train_particles(nm, boxes, 6, L)
+    dp, dt = pm.DeviceArray.from_host(pos4), pm.DeviceArray.from_host(tgt4)
+    from gadget_ng_pm_hip.learned import pack_params
+    before = pack_params(p).astype(np.float64)
+    net.loss_grad(dp.ptr, dt.ptr, len(pos4), L, osc)
+    net.backward(boxes)
+    gr = net.read("grad").astype(np.float64)
+    net.adam(1e-3)
+    after = pack_params(net.params()).astype(np.float64)
+    # first step: m = (1-b1) g, v = (1-b2) g^2, bias-corrected -> g / (|g| + eps)
+    ref = before - 1e-3 * gr / (np.abs(gr) + 1e-8)
+    assert np.allclose(after, ref, rtol=1e-5, atol=1e-7)
+
+
+def test_training_fits_a_teacher_network(learned, pm):
+    """Targets made by a fixed random network: Adam on the HIP kernels drives
+    the loss down by an order of magnitude."""
+    nm, boxes, L, osc = 16, 2, 16.0, 1.0
+    print("case teacher training", flush=True)
+    g = np.random.default_rng(21)
+    x0 = lo.bf16_round(g.normal(size=(boxes, nm, nm, nm, 2)).astype(np.float32))
+    teacher = _params(100, big_out=True)
+    net = learned.Net(nm, boxes, device=0)
+    net.set_params(teacher)
+    net.write("x0", x0)
+    net.forward(boxes)
+    yt = net.read("y").astype(np.float64)
+    pos4, _ = _train_particles(nm, boxes, 7, L)
+    tgt = np.zeros((len(pos4), 4), np.float32)
+    for b in range(boxes):
+        sel = pos4[:, 3] == b
+        tgt[sel, :3] = osc * lo.cic_gather(yt[b][..., :3], pos4[sel, :3].astype(np.float64), L)
+    tgt[:, 3] = 1.0
+    dp, dt = pm.DeviceArray.from_host(pos4), pm.DeviceArray.from_host(tgt)
+    net.set_params(lo.init_params(7))
+    net.forward(boxes)
+    net.loss_grad(dp.ptr, dt.ptr, len(pos4), L, osc)
+    first = net.loss()
+    for _ in range(300):
+        net.train_step(boxes, dp.ptr, dt.ptr, len(pos4), L, osc, 3e-3)
+    net.forward(boxes)
+    net.loss_grad(dp.ptr, dt.ptr, len(pos4), L, osc)
+    last = net.loss()
+    print(f"teacher loss {first:.4e} -> {last:.4e}", flush=True)
+    assert last < 0.1 * first
