@@ -87,13 +87,18 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("zeldovich", "clustered"), default="zeldovich",
+    ap.add_argument("--workload", choices=("zeldovich", "clustered", "learned"), default="zeldovich",
                     help="zeldovich: 256^3 lattice on a 512^3 mesh (the headline); clustered: 512^3 particles, "
                          "70%% lattice + 30%% in 2000 Plummer spheres, 1024^3 mesh, TreePM long range r_s = 1.25 cells")
     ap.add_argument("--n-side", type=int, default=256)
     ap.add_argument("--mesh", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the FP64 CPU oracle (and the error check)")
     ap.add_argument("--cpu-runs", type=int, default=2, help="timed CPU oracle runs after one warm-up")
+    ap.add_argument("--no-fp64-form", action="store_true",
+                    help="skip timing the FP64 GPU form of the reference algorithm")
+    ap.add_argument("--train-boxes", type=int, default=200, help="learned: training seeds 1..N (64^3 boxes)")
+    ap.add_argument("--train-steps", type=int, default=2000, help="learned: Adam steps")
+    ap.add_argument("--eval-boxes", type=int, default=2, help="learned: held-out 256^3 seeds from 1001")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a kernel of known traffic for scaling "
                          "profiler byte counters)")
@@ -106,6 +111,9 @@ def main() -> int:
     if not 1 <= args.steps <= 64:
         print("--steps must be in [1, 64] (the stage-timing ring holds 64 solves)", file=sys.stderr)
         return 2
+
+    if args.workload == "learned":
+        return bench_learned(args)
 
     import numpy as np
 
@@ -215,6 +223,42 @@ def main() -> int:
         "max_force_err": None,
     }
 
+    if not args.no_fp64_form and nm & (nm - 1) == 0:
+        # the reference algorithm in FP64 on the same GPU: device-resident
+        # (kernels only) and through its host entry point (f64 copies in and
+        # out on every call, as the reference's GPU crates do)
+        plan.close()
+        p64 = pm.Plan64(nm, box, r_split=r_split, device=0)
+        pos64 = np.ascontiguousarray(parts.pos64())
+        d_pos = pm.DeviceArray.from_host(pos64)
+        d_mass = pm.DeviceArray.from_host(parts.mass64())
+        d_out = pm.DeviceArray((n, 3), np.float64)
+        p64.solve_device(d_pos.ptr, d_mass.ptr, n, d_out.ptr, 1.0)
+        pm.device_synchronize()
+        t_dev = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            p64.solve_device(d_pos.ptr, d_mass.ptr, n, d_out.ptr, 1.0)
+            pm.device_synchronize()
+            t_dev.append(time.perf_counter() - t1)
+        a64 = d_out.numpy()
+        for a in (d_pos, d_mass, d_out):
+            a.free()
+        t_host = []
+        for _ in range(2):
+            t1 = time.perf_counter()
+            p64.accelerations(pos64, parts.mass64(), 1.0)
+            t_host.append(time.perf_counter() - t1)
+        p64.close()
+        ms_dev = statistics.median(t_dev) * 1e3
+        ms_host = statistics.median(t_host) * 1e3
+        result["fp64_gpu_form"] = {
+            "ms_device": round(ms_dev, 3), "ms_host_round_trip": round(ms_host, 3),
+            "fp32_speedup_device": round(ms_dev / ms_per_step, 2),
+            "fp32_speedup_vs_host_round_trip": round(ms_host / ms_per_step, 2),
+            "fp32_rms_vs_fp64_form": float(f"{np.sqrt(((acc - a64) ** 2).sum() / (a64 ** 2).sum()):.4e}")}
+        log(f"[bench] FP64 GPU form: {ms_dev:.2f} ms device-resident, {ms_host:.2f} ms with host copies")
+
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle  # CPU baseline and checker only
@@ -250,6 +294,85 @@ def main() -> int:
 
     print(json.dumps(result), flush=True)
     plan.close()
+    return 0
+
+
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+NET_FLOP_PER_CELL = 2 * 27 * (2 * 16 + 16 * 16 + 16 * 3)  # 18 144
+
+
+def bench_learned(args) -> int:
+    """Learned sub-cell correction (config-5 recipe): train on seeded 64^3
+    boxes made by the method itself, evaluate on held-out 256^3 boxes against
+    TreePM (long range + exact short range), and time the network against the
+    short-range sum it stands in for."""
+    import numpy as np
+
+    import gadget_ng_pm_hip as pm
+    from gadget_ng_pm_hip import learned
+
+    lc = learned.LearnedCorrection(cells=64, batch=8)
+    t0 = time.perf_counter()
+    info = lc.build_training_set(range(1, args.train_boxes + 1), workers=-1)
+    t_data = time.perf_counter() - t0
+    log(f"[learned] {args.train_boxes} training boxes in {t_data:.1f} s: {info}")
+    t0 = time.perf_counter()
+    hist = lc.train(steps=args.train_steps, lr=2e-3, log_every=max(1, args.train_steps // 10),
+                    log=lambda m: log("[learned] " + m))
+    t_train = time.perf_counter() - t0
+    evals = [lc.evaluate(1001 + i, cells=256, workers=-1) for i in range(args.eval_boxes)]
+    for e in evals:
+        log(f"[learned] seed {e['seed']}: PM field rms {e['pm']['field_rms_rel']:.4f} -> "
+            f"{e['pm_net']['field_rms_rel']:.4f}, field median {e['pm']['field_median_rel']:.5f} -> "
+            f"{e['pm_net']['field_median_rel']:.5f}")
+    # network inference roofline at 256^3: repeated forwards on the last evaluation's input
+    big = learned.LearnedCorrection(cells=256, batch=1)
+    big.net.set_params(lc.net.params())
+    pm.device_synchronize()
+    big.net.forward(1)
+    pm.device_synchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        big.net.forward(1)
+    pm.device_synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    cells = 256 ** 3
+    achieved = NET_FLOP_PER_CELL * cells / t_fwd / 1e12
+
+    def mean(key, which):
+        return float(np.mean([e[which][key] for e in evals]))
+
+    t_sr = float(np.mean([e["time_short_range_s"] for e in evals]))
+    t_net = float(np.mean([e["time_net_s"] for e in evals]))
+    result = {
+        "metric": "learned sub-cell correction, 256^3 clustered particles on a 256^3 mesh: held-out field-particle "
+                  "rms force error vs TreePM",
+        "value": round(mean("field_rms_rel", "pm_net"), 5),
+        "unit": "relative rms force error",
+        "n_gpus": 1, "steps": args.train_steps, "warmup": 0, "ms_per_step": round(t_train * 1e3 / args.train_steps, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "config-5 recipe: 70% Zel'dovich lattice + 30% in Plummer spheres (a = 0.2 Mpc/h), one "
+                               "particle per (Mpc/h)^3; train on 64^3 boxes (seeds 1..N), evaluate on 256^3 boxes "
+                               "(seeds 1001..)",
+                   "train_boxes": args.train_boxes, "eval_boxes": args.eval_boxes,
+                   "treepm": "r_s = 1.25 cells, CIC-deconvolved long range + exact short range within 5 r_s, "
+                             "eps = 0.05"},
+        "without_net": {k: round(mean(k, "pm"), 5) for k in ("field_rms_rel", "field_median_rel", "field_mean_rel",
+                                                               "rms_rel", "median_rel", "dense_median_rel")},
+        "with_net": {k: round(mean(k, "pm_net"), 5) for k in ("field_rms_rel", "field_median_rel", "field_mean_rel",
+                                                                "rms_rel", "median_rel", "dense_median_rel")},
+        "cost_s": {"network_forward_and_apply": round(t_net, 5), "short_range_sum": round(t_sr, 5),
+                   "pm": round(float(np.mean([e["time_pm_s"] for e in evals])), 5),
+                   "treepm_long_range": round(float(np.mean([e["time_treepm_long_s"] for e in evals])), 5)},
+        "roofline": {"bound": "mfma", "kernel": "network forward (3 convolutions)",
+                     "achieved": round(achieved, 2), "peak": BF16_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / BF16_DENSE_PEAK_TFLOPS, 5), "traffic": None,
+                     "flop_per_cell": NET_FLOP_PER_CELL, "mean_ms": round(t_fwd * 1e3, 4)},
+        "training": {"data_s": round(t_data, 2), "train_s": round(t_train, 2), "loss_first": hist[0],
+                     "loss_last": hist[-1], "f0": lc.f0},
+    }
+    print(json.dumps(result), flush=True)
     return 0
 
 

@@ -355,12 +355,35 @@ __global__ void k_to_bf16(const float* __restrict__ a, unsigned short* __restric
   if (i < n) b[i] = f2bf(a[i]);
 }
 
-__global__ void k_input(const float* __restrict__ rho, const float* __restrict__ phi, long long M, float inv_rho,
-                        float inv_phi, unsigned* __restrict__ x0) {
+// Periodic 7-point running mean along one axis (stride 1, nm or nm^2).
+constexpr int kHalfWidth = 3;
+__global__ void k_boxmean(const float* __restrict__ in, float* __restrict__ out, int nm, long long stride) {
+  const long long M = (long long)nm * nm * nm;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
-  const unsigned lo = f2bf(log1pf(rho[i] * inv_rho));
-  const unsigned hi = f2bf(asinhf(phi[i] * inv_phi));
+  const int c = (int)((i / stride) % nm);
+  const long long base = i - (long long)c * stride;
+  float s = 0.0f;
+#pragma unroll
+  for (int d = -kHalfWidth; d <= kHalfWidth; ++d) {
+    int j = c + d;
+    j += j < 0 ? nm : 0;
+    j -= j >= nm ? nm : 0;
+    s += in[base + (long long)j * stride];
+  }
+  out[i] = s * (1.0f / (2 * kHalfWidth + 1));
+}
+
+// x0 = (mass / rho_mean - 1, (phi - box7(phi)) / phi_scale): the potential
+// minus its 7^3 box mean keeps the part set by mass within a few cells and
+// drops the box-scale modes, which carry no sub-cell information.
+__global__ void k_input(const float* __restrict__ rho, const float* __restrict__ phi,
+                        const float* __restrict__ phi_mean, long long M, float inv_rho, float inv_phi,
+                        unsigned* __restrict__ x0) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const unsigned lo = f2bf(rho[i] * inv_rho - 1.0f);
+  const unsigned hi = f2bf((phi[i] - phi_mean[i]) * inv_phi);
   x0[i] = lo | (hi << 16);
 }
 
@@ -394,20 +417,22 @@ __device__ __forceinline__ void cic_nodes(float x, float y, float z, float cs, i
 __global__ __launch_bounds__(256) void k_loss(const float4* __restrict__ pos, const float4* __restrict__ tgt,
                                               long long n, const float4* __restrict__ y, float* __restrict__ gy,
                                               double* __restrict__ loss, int nm, float cs, float out_scale,
-                                              float gscale) {
+                                              float gscale, int max_boxes) {
   __shared__ double red[4];
   const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const long long M = (long long)nm * nm * nm;
-  const bool live = p < n;
+  float4 q = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+  if (p < n) q = pos[p];
+  // records whose box index is not a valid slot are ignored
+  const bool live = p < n && q.w >= 0.0f && q.w < (float)max_boxes;
   Cic c;
   float gx = 0.0f, gyv = 0.0f, gz = 0.0f;
   double lp = 0.0;
   long long key = -1 - lane;
   if (live) {
-    const float4 q = pos[p];
     const float4 t = tgt[p];
-    cic_nodes(q.x, q.y, q.z, cs, nm, (long long)q.w * M, c);
+    cic_nodes(q.x, q.y, q.z, cs, nm, (long long)(int)q.w * M, c);
     float px = 0.0f, py = 0.0f, pz = 0.0f;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
@@ -525,6 +550,7 @@ struct gng_net {
   float* d_partial = nullptr;
   double* d_loss = nullptr;
   long long loss_n = 0;
+  float* d_tmp = nullptr;  // 2 x M floats for the input's box means
 };
 
 namespace {
@@ -534,7 +560,7 @@ hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 void free_net(gng_net* t) {
   void* ptrs[] = {t->d_params, t->d_grad, t->d_m1, t->d_m2, t->d_f1, t->d_f2, t->d_f3, t->d_g3, t->d_g2,
                   t->d_b16, t->d_x0, t->d_z1, t->d_x1, t->d_z2, t->d_x2, t->d_y, t->d_gy, t->d_dz3,
-                  t->d_dz2, t->d_dz1, t->d_partial, t->d_loss};
+                  t->d_dz2, t->d_dz1, t->d_partial, t->d_loss, t->d_tmp};
   for (void* p : ptrs) (void)hipFree(p);
 }
 
@@ -634,6 +660,7 @@ int gng_net_create(int32_t grid_size, int32_t max_boxes, int32_t device, gng_net
   A(&t->d_dz1, cells * 32);
   A(&t->d_partial, (size_t)kWgradBlocks * 28 * 256 * 4);
   A(&t->d_loss, 8);
+  A(&t->d_tmp, t->M * 2 * 4);
   if (e != hipSuccess) {
     free_net(t);
     delete t;
@@ -731,8 +758,15 @@ int gng_net_input(gng_net* net, int32_t box, const float* d_rho, const float* d_
   if (box < 0 || box >= net->max_boxes) return fail(GNG_ERR_INVALID, "box slot out of range");
   if (!(rho_mean > 0.0f) || !(phi_scale > 0.0f)) return fail(GNG_ERR_INVALID, "scales must be positive");
   GNG_HIP(hipSetDevice(net->device));
-  hipLaunchKernelGGL(k_input, dim3((unsigned)((net->M + 255) / 256)), dim3(256), 0, as_stream(stream), d_rho, d_phi,
-                     net->M, 1.0f / rho_mean, 1.0f / phi_scale,
+  hipStream_t s = as_stream(stream);
+  const dim3 g((unsigned)((net->M + 255) / 256));
+  const int nm = net->nm;
+  float* t0 = net->d_tmp;
+  float* t1 = net->d_tmp + net->M;
+  hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, d_phi, t0, nm, 1LL);
+  hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, t0, t1, nm, (long long)nm);
+  hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, t1, t0, nm, (long long)nm * nm);
+  hipLaunchKernelGGL(k_input, g, dim3(256), 0, s, d_rho, d_phi, t0, net->M, 1.0f / rho_mean, 1.0f / phi_scale,
                      reinterpret_cast<unsigned*>(net->d_x0) + (long long)box * net->M);
   GNG_HIP(hipGetLastError());
   return GNG_OK;
@@ -769,7 +803,8 @@ int gng_net_loss_grad(gng_net* net, const float* d_pos4, const float* d_tgt4, in
   const float gscale = (float)(2.0 * out_scale / (double)n);
   hipLaunchKernelGGL(k_loss, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                      reinterpret_cast<const float4*>(d_pos4), reinterpret_cast<const float4*>(d_tgt4), (long long)n,
-                     reinterpret_cast<const float4*>(net->d_y), net->d_gy, net->d_loss, net->nm, cs, out_scale, gscale);
+                     reinterpret_cast<const float4*>(net->d_y), net->d_gy, net->d_loss, net->nm, cs, out_scale, gscale,
+                     net->max_boxes);
   hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((cells * 4 + 255) / 256)), dim3(256), 0, s, net->d_gy, net->d_dz3,
                      cells * 4);
   GNG_HIP(hipGetLastError());

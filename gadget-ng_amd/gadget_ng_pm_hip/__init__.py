@@ -47,7 +47,7 @@ GNG_ERR_UNSUPPORTED = 5
 
 STAGES = ("bucket", "sort", "deposit", "fft_x_fwd", "fft_y_fwd", "fft_z_green", "fft_y_inv", "fft_x_inv", "interp")
 
-SOURCES = ("gng_pm.hip", "learned.hip", "shortrange.hip")
+SOURCES = ("gng_pm.hip", "learned.hip", "shortrange.hip", "pm64.hip")
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -165,6 +165,13 @@ def lib():
         for f in (L.gng_net_create, L.gng_net_set_params, L.gng_net_get_params, L.gng_net_read, L.gng_net_write,
                   L.gng_net_input, L.gng_net_forward, L.gng_net_loss_grad, L.gng_net_backward, L.gng_net_adam,
                   L.gng_net_train_step, L.gng_net_loss, L.gng_net_apply):
+            f.restype = i32
+        L.gng_pm64_create.argtypes = [ctypes.POINTER(GngConfig), ctypes.POINTER(P)]
+        L.gng_pm64_destroy.argtypes = [P]
+        L.gng_pm64_destroy.restype = None
+        L.gng_pm64_accelerations.argtypes = [P, P, P, i64, P, i64, d, P]
+        L.gng_pm64_solve_device.argtypes = [P, P, P, i64, P, i64, d, P, P]
+        for f in (L.gng_pm64_create, L.gng_pm64_accelerations, L.gng_pm64_solve_device):
             f.restype = i32
         L.hip_pm_create.argtypes = [i32, f32]
         L.hip_pm_create.restype = P
@@ -341,6 +348,59 @@ class Plan:
         out = {STAGES[i]: float(ms[i]) for i in range(min(n, len(STAGES)))}
         out["total"] = float(ms[n])
         return out
+
+
+class Plan64:
+    """The FP64 GPU form of the reference algorithm (the baseline the FP32
+    solver is compared with): same configuration as Plan, powers of two only."""
+
+    def __init__(self, grid_size: int, box_size: float, r_split: float = 0.0, plummer_eps: float = 0.0,
+                 deconvolve: bool = False, device: int = -1):
+        cfg = GngConfig()
+        cfg.grid_size = int(grid_size)
+        cfg.deconvolve = int(bool(deconvolve))
+        cfg.box_size = float(box_size)
+        cfg.r_split = float(r_split)
+        cfg.plummer_eps = float(plummer_eps)
+        cfg.device = int(device)
+        h = ctypes.c_void_p()
+        rc = lib().gng_pm64_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_pm64_create", rc, last_error())
+        self._h = h
+        self.grid_size = int(grid_size)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().gng_pm64_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def accelerations(self, pos, mass, g: float, idx=None) -> np.ndarray:
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        n = pos.shape[0]
+        if idx is None:
+            k, ip = n, None
+        else:
+            idx = np.ascontiguousarray(idx, dtype=np.int64)
+            k, ip = idx.shape[0], _ptr(idx)
+        out = np.empty((k, 3), dtype=np.float64)
+        rc = lib().gng_pm64_accelerations(self._h, _ptr(pos), _ptr(mass), n, ip, k, float(g), _ptr(out))
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_pm64_accelerations", rc, last_error())
+        return out
+
+    def solve_device(self, pos_ptr: int, mass_ptr: int, n: int, out_ptr: int, g: float, stream: int = 0) -> None:
+        rc = lib().gng_pm64_solve_device(self._h, pos_ptr, mass_ptr, int(n), None, int(n), float(g), out_ptr,
+                                         stream or None)
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_pm64_solve_device", rc, last_error())
 
 
 class DeviceArray:

@@ -306,25 +306,37 @@ class Sample:
     f_pm: np.ndarray                # (n, 3) float32, plain PM
     f_true: np.ndarray              # (n, 3) float32, TreePM (long range + exact short range)
     pos: np.ndarray                 # (n, 3) float32
+    in_field: np.ndarray            # (n,) bool, base cell below the dense threshold
     times: Dict[str, float] = field(default_factory=dict)
 
 
-def force_errors(f: np.ndarray, f_true: np.ndarray, n_lattice: int) -> Dict[str, float]:
-    """rms_rel = sqrt(sum|f - t|^2 / sum|t|^2) over all particles, and the
-    per-particle relative error |f - t| / |t| (median and mean) for all,
-    lattice and sphere particles."""
+def force_errors(f: np.ndarray, f_true: np.ndarray, n_lattice: int, field=None) -> Dict[str, float]:
+    """rms_rel = sqrt(sum|f - t|^2 / sum|t|^2) and the per-particle relative
+    error |f - t| / |t| (median, mean) over all particles, the lattice and
+    sphere members, and the field (particles in cells below the dense
+    threshold) and dense populations when `field` (bool mask) is given."""
     f = np.asarray(f, np.float64)
     t = np.asarray(f_true, np.float64)
     d = np.linalg.norm(f - t, axis=1)
     tn = np.maximum(np.linalg.norm(t, axis=1), 1e-30)
     r = d / tn
-    out = {"rms_rel": float(np.sqrt(np.sum(d * d) / np.sum(tn * tn))),
-           "median_rel": float(np.median(r)), "mean_rel": float(np.mean(r))}
+
+    def pop(sel, tag):
+        if sel is None or not np.any(sel):
+            return {}
+        return {f"{tag}rms_rel": float(np.sqrt(np.sum(d[sel] ** 2) / np.sum(tn[sel] ** 2))),
+                f"{tag}median_rel": float(np.median(r[sel])), f"{tag}mean_rel": float(np.mean(r[sel])),
+                f"{tag}count": int(np.count_nonzero(sel))}
+
+    out = pop(np.ones(len(r), bool), "")
     if 0 < n_lattice < len(r):
-        out["lattice_median_rel"] = float(np.median(r[:n_lattice]))
-        out["lattice_rms_rel"] = float(np.sqrt(np.sum(d[:n_lattice] ** 2) / np.sum(tn[:n_lattice] ** 2)))
-        out["sphere_median_rel"] = float(np.median(r[n_lattice:]))
-        out["sphere_rms_rel"] = float(np.sqrt(np.sum(d[n_lattice:] ** 2) / np.sum(tn[n_lattice:] ** 2)))
+        lat = np.zeros(len(r), bool)
+        lat[:n_lattice] = True
+        out.update(pop(lat, "lattice_"))
+        out.update(pop(~lat, "sphere_"))
+    if field is not None:
+        out.update(pop(field, "field_"))
+        out.update(pop(~field, "dense_"))
     return out
 
 
@@ -337,13 +349,15 @@ class LearnedCorrection:
     force (long range with r_s = r_split_cells cells, CIC-deconvolved, plus
     the exact-split short range summed within 5 r_s, Plummer softening eps)
     and the network input (log mass, asinh potential) from the same PM solve.
-    Training minimises  mean_p |F_PM + out_scale CIC(y) - F_TreePM|^2 /
-    (|F_TreePM|^2 + f0^2)  with Adam, f0 = out_scale = the median TreePM force
-    magnitude of the first training box.
+    Training minimises  mean_p w_p |F_PM + out_scale CIC(y) - F_TreePM|^2  with
+    Adam, w_p = 1 / (|F_TreePM|^2 + f0^2) for field particles (base cell below
+    (1 + dense_contrast) times the mean mass) and 0 in denser cells, where the
+    residual depends on the arrangement inside the cell; f0 = out_scale = the
+    median TreePM force of the field particles of the first training box.
     """
 
     def __init__(self, cells: int = 64, r_split_cells: float = 1.25, eps: float = 0.05, g: float = 1.0,
-                 device: int = 0, batch: int = 8):
+                 device: int = 0, batch: int = 8, dense_contrast: float = 8.0, phi_scale: float = 1.0):
         self.cells = int(cells)
         self.nm = self.cells
         self.L = float(cells)
@@ -358,7 +372,12 @@ class LearnedCorrection:
         self.sr = ShortRange(device=device)
         self.net = Net(self.nm, self.batch, device=device)
         self.f0: Optional[float] = None
-        self.phi_scale = self.g  # G * (mean mass per cell = 1) / (cell = 1)
+        # potential unit: phi_scale * G * (mean mass per cell) / (cell size)
+        self.phi_scale = float(phi_scale) * self.g
+        # particles whose base cell holds more than (1 + dense_contrast) times
+        # the mean mass are left out of the loss: their residual is set by the
+        # arrangement inside the cell, which the mesh does not resolve
+        self.dense_contrast = float(dense_contrast)
         self._groups = []
 
     # -- per-box forces -----------------------------------------------------
@@ -393,7 +412,10 @@ class LearnedCorrection:
         rho_mean = float(np.sum(p.m, dtype=np.float64)) / self.nm ** 3
         f_pm, f_tp, times = self._forces(*dev, p.n, self.net if net is None else net, slot, rho_mean)
         pos = np.stack([p.x, p.y, p.z], axis=1)
-        return Sample(p.n, n_lat, self.L, dev, f_pm, f_tp, pos, times)
+        mass = self.pm.density()
+        c = np.floor(pos.astype(np.float64) * (self.nm / self.L)).astype(np.int64) % self.nm
+        fld = mass[c[:, 2], c[:, 1], c[:, 0]] <= (1.0 + self.dense_contrast) * rho_mean
+        return Sample(p.n, n_lat, self.L, dev, f_pm, f_tp, pos, fld, times)
 
     def _records(self, s: Sample, slot: int):
         """Training records sorted by base cell: pos4 (x, y, z, slot), tgt4
@@ -407,7 +429,7 @@ class LearnedCorrection:
         tgt4 = np.empty((s.n, 4), np.float32)
         tgt4[:, :3] = (s.f_true - s.f_pm)[order]
         tn2 = np.sum(s.f_true.astype(np.float64) ** 2, axis=1)[order]
-        tgt4[:, 3] = 1.0 / (tn2 + self.f0 ** 2)
+        tgt4[:, 3] = np.where(s.in_field[order], 1.0 / (tn2 + self.f0 ** 2), 0.0)
         return pos4, tgt4
 
     def build_training_set(self, seeds, workers: int = -1, log=None) -> Dict[str, float]:
@@ -427,7 +449,7 @@ class LearnedCorrection:
                 t_gpu += s.times["pm_s"] + s.times["treepm_s"]
                 pairs += s.times["sr_pairs"]
                 if self.f0 is None:
-                    self.f0 = float(np.median(np.linalg.norm(s.f_true.astype(np.float64), axis=1)))
+                    self.f0 = float(np.median(np.linalg.norm(s.f_true[s.in_field].astype(np.float64), axis=1)))
                 p4, t4 = self._records(s, j)
                 pos_l.append(p4)
                 tgt_l.append(t4)
@@ -484,6 +506,8 @@ class LearnedCorrection:
         if other is not self:
             net.set_params(self.net.params())
             other.f0 = self.f0
+            other.phi_scale = self.phi_scale
+            other.dense_contrast = self.dense_contrast
         s = other.sample(seed, net=net, slot=0, workers=workers)
         n = s.n
         corr = [DeviceArray.from_host(np.ascontiguousarray(s.f_pm[:, a])) for a in range(3)]
@@ -515,7 +539,7 @@ class LearnedCorrection:
         device_synchronize()
         t_sr = time.perf_counter() - t0
         return {"seed": seed, "cells": cells, "n": n,
-                "pm": force_errors(s.f_pm, s.f_true, s.n_lattice),
-                "pm_net": force_errors(f_corr, s.f_true, s.n_lattice),
+                "pm": force_errors(s.f_pm, s.f_true, s.n_lattice, s.in_field),
+                "pm_net": force_errors(f_corr, s.f_true, s.n_lattice, s.in_field),
                 "time_pm_s": t_pm, "time_net_s": t_net, "time_treepm_long_s": t_lr, "time_short_range_s": t_sr,
                 "sr_pairs": other.sr.last_pair_count()}

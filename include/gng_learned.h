@@ -100,8 +100,9 @@ int gng_net_device_buffer(gng_net* net, int32_t which, void** d_ptr);
 
 /* Input slot `box` from a PM solve's device meshes (FP32 mass per cell and
  * potential, grid_size^3 each, gng_pm_device_meshes):
- * x0 = (log(1 + mass/rho_mean), asinh(phi/phi_scale)), rho_mean = mean mass
- * per cell. */
+ * x0 = (mass/rho_mean - 1, (phi - box7(phi))/phi_scale), rho_mean = mean mass
+ * per cell, box7 = the periodic 7^3 box mean.  Both channels are linear in
+ * the mass, as the PM force error is, and independent of the box size. */
 int gng_net_input(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
                   float phi_scale, void* stream);
 

@@ -198,6 +198,25 @@ int gng_pm_batch_solve_device(gng_pm_batch *b, const float *d_x, const float *d_
  * Pure host function; needs no device. */
 uint64_t gng_hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int32_t bits);
 
+/* ---- FP64 GPU form --------------------------------------------------------
+ * The reference algorithm moved to the GPU in FP64, kept as the baseline the
+ * FP32 solver is measured against (not a product path): per-particle FP64
+ * atomic deposit (cic.rs:37-93), full complex-to-complex 3-D FFT in three 1-D
+ * passes, the spectral kernel with the Nyquist index at +nm/2
+ * (fft_poisson.rs:509-549, 724-730), three inverse FFTs, Re()/nm^3 and FP64
+ * interpolation (cic.rs:100-170).  Mesh sides: powers of two, 4..1024; cfg
+ * filters and deconvolve as above.  About 48 nm^3 bytes of device memory. */
+typedef struct gng_pm64_plan gng_pm64_plan;
+int gng_pm64_create(const gng_pm_config *cfg, gng_pm64_plan **out);
+void gng_pm64_destroy(gng_pm64_plan *p);
+/* Host f64 in/out (pos n x 3 interleaved, idx k or NULL), with the H2D and
+ * D2H copies of every call.  Synchronous. */
+int gng_pm64_accelerations(gng_pm64_plan *p, const double *pos, const double *mass, int64_t n,
+                           const int64_t *idx, int64_t k, double g, double *out);
+/* Device-resident variant on `stream` (asynchronous). */
+int gng_pm64_solve_device(gng_pm64_plan *p, const double *d_pos, const double *d_mass, int64_t n,
+                          const int64_t *d_idx, int64_t k, double g, double *d_out, void *stream);
+
 /* ---- 3. device-memory helpers --------------------------------------------
  * No counterpart in the reference.  For callers that hold no GPU runtime of
  * their own (test and benchmark harnesses, FFI hosts) and need device-resident

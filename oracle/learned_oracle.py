@@ -261,13 +261,21 @@ def potential_mesh(density, g: float, box: float, r_split: float = 0.0):
     return np.fft.ifftn(ph).real
 
 
+def box_mean(a, half: int = 3):
+    """Periodic (2 half + 1)^3 box mean of a mesh."""
+    out = np.asarray(a, np.float64)
+    for ax in range(3):
+        out = sum(np.roll(out, s, axis=ax) for s in range(-half, half + 1)) / (2 * half + 1)
+    return out
+
+
 def net_input(mass, potential, rho_mean: float, phi_scale: float):
-    """Channel 0 = log(1 + mass/rho_mean), channel 1 = asinh(Phi / phi_scale)
-    (mass per cell and its mean; both channels stay O(10) from voids to the
-    cores of Plummer spheres)."""
+    """Channel 0 = mass/rho_mean - 1 (density contrast), channel 1 =
+    (Phi - box7(Phi)) / phi_scale (the potential of the mass within a few
+    cells); both linear in the mass, as the PM force error is."""
     x = np.empty(mass.shape + (2,))
-    x[..., 0] = np.log1p(np.asarray(mass, np.float64) / rho_mean)
-    x[..., 1] = np.arcsinh(np.asarray(potential, np.float64) / phi_scale)
+    x[..., 0] = np.asarray(mass, np.float64) / rho_mean - 1.0
+    x[..., 1] = (np.asarray(potential, np.float64) - box_mean(potential)) / phi_scale
     return x
 
 

@@ -283,3 +283,18 @@ def test_training_fits_a_teacher_network(learned, pm):
     last = net.loss()
     print(f"teacher loss {first:.4e} -> {last:.4e}", flush=True)
     assert last < 0.1 * first
+
+
+def test_net_input_matches_oracle(learned, pm):
+    nm = 32
+    g = np.random.default_rng(31)
+    mass = g.gamma(2.0, 0.5, (nm, nm, nm)).astype(np.float32)
+    phi = (g.normal(size=(nm, nm, nm)) * 50 + 1000).astype(np.float32)
+    net = learned.Net(nm, 2, device=0)
+    dm, dp = pm.DeviceArray.from_host(mass), pm.DeviceArray.from_host(phi)
+    net.input_from_meshes(1, dm.ptr, dp.ptr, 1.0, 3.0)
+    got = net.read("x0")[1]
+    ref = lo.bf16_round(lo.net_input(mass, phi, 1.0, 3.0).astype(np.float32))
+    assert bf16_ulp_ok(got[..., 0], ref[..., 0], frac=1.0)
+    # the high-passed potential differs from a float64 evaluation by FP32 rounding of phi ~ 1e3
+    assert np.abs(got[..., 1] - ref[..., 1]).max() <= 2e-3 * np.abs(ref[..., 1]).max() + 1e-4

@@ -19,5 +19,5 @@ def test_learned_correction_beats_plain_pm_on_held_out_boxes(pm):
         ev = lc.evaluate(seed, workers=8)
         print(seed, "PM", ev["pm"], flush=True)
         print(seed, "PM+net", ev["pm_net"], flush=True)
-        assert ev["pm_net"]["mean_rel"] < ev["pm"]["mean_rel"]
-        assert ev["pm_net"]["lattice_median_rel"] < ev["pm"]["lattice_median_rel"]
+        assert ev["pm_net"]["field_mean_rel"] < ev["pm"]["field_mean_rel"]
+        assert ev["pm_net"]["field_median_rel"] < ev["pm"]["field_median_rel"]

@@ -209,6 +209,22 @@ def test_short_range_cut_and_softening():
     assert a2[0, 0] < 0
 
 
+def test_box_mean_and_net_input():
+    g = np.random.default_rng(15)
+    a = g.normal(size=(9, 9, 9))
+    # brute-force 7^3 periodic mean at a few cells
+    m = lo.box_mean(a)
+    for c in [(0, 0, 0), (8, 4, 1), (3, 3, 7)]:
+        idx = [(np.arange(-3, 4) + c[k]) % 9 for k in range(3)]
+        assert m[c] == pytest.approx(a[np.ix_(*idx)].mean(), rel=1e-12)
+    # a constant potential offset does not change the input
+    mass = g.uniform(0, 2, (9, 9, 9))
+    x1 = lo.net_input(mass, a, 1.0, 2.0)
+    x2 = lo.net_input(mass, a + 123.0, 1.0, 2.0)
+    assert np.allclose(x1, x2, atol=1e-12)
+    assert np.allclose(x1[..., 0], mass - 1.0)
+
+
 def test_potential_single_mode_closed_form():
     nm, box, g_, amp = 16, 3.0, 1.7, 0.4
     x = (np.arange(nm) + 0.0) * box / nm
