@@ -357,7 +357,7 @@ class LearnedCorrection:
     """
 
     def __init__(self, cells: int = 64, r_split_cells: float = 1.25, eps: float = 0.05, g: float = 1.0,
-                 device: int = 0, batch: int = 8, dense_contrast: float = 8.0, phi_scale: float = 1.0):
+                 device: int = 0, batch: int = 8, dense_contrast: float = 8.0, phi_scale: float = 100.0):
         self.cells = int(cells)
         self.nm = self.cells
         self.L = float(cells)
