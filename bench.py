@@ -87,7 +87,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("zeldovich", "clustered", "learned"), default="zeldovich",
+    ap.add_argument("--workload", choices=("zeldovich", "clustered", "ensemble", "learned"), default="zeldovich",
                     help="zeldovich: 256^3 lattice on a 512^3 mesh (the headline); clustered: 512^3 particles, "
                          "70%% lattice + 30%% in 2000 Plummer spheres, 1024^3 mesh, TreePM long range r_s = 1.25 cells")
     ap.add_argument("--n-side", type=int, default=256)
@@ -114,6 +114,8 @@ def main() -> int:
 
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
 
     import numpy as np
 
@@ -223,6 +225,24 @@ def main() -> int:
         "max_force_err": None,
     }
 
+    # a plain 1 GiB device-to-device copy: the practical HBM ceiling on this box
+    cbytes = 1 << 30
+    ca, cb = pm.DeviceArray((cbytes,), np.uint8), pm.DeviceArray((cbytes,), np.uint8)
+    pm.device_copy(cb, ca)
+    pm.device_synchronize()
+    t_copy = []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        pm.device_copy(cb, ca)
+        pm.device_synchronize()  # device-to-device copies return before they finish
+        t_copy.append(time.perf_counter() - t1)
+    ca.free()
+    cb.free()
+    copy_gbps = 2 * cbytes / min(t_copy) / 1e9
+    result["copy_gbps"] = round(copy_gbps, 1)
+    result["hbm_frac_of_copy"] = round(pipeline_gbps / copy_gbps, 4)
+    result["roofline"]["frac_of_copy"] = round(achieved / copy_gbps, 4)
+
     if not args.no_fp64_form and nm & (nm - 1) == 0:
         # the reference algorithm in FP64 on the same GPU: device-resident
         # (kernels only) and through its host entry point (f64 copies in and
@@ -291,9 +311,118 @@ def main() -> int:
         result["rms_force_err_host_path"] = float(f"{np.sqrt((dh ** 2).sum() / (ref ** 2).sum()):.4e}")
         log(f"[bench] CPU oracle {tc:.2f}s on {cores} threads; rms err {result['rms_force_err']:.3e}, "
             f"max {result['max_force_err']:.3e}")
+        # calibration against the reference's only published PM timings (AP-08 input,
+        # serial CPU PM: 1.83 ms at grid 32, 22.4 ms at grid 64), with this GPU beside it
+        ap = {}
+        threads = oracle.get_threads()
+        oracle.set_threads(1)
+        for grid, published in ((32, 1.83), (64, 22.4)):
+            q = workloads.ap08_curve(grid)
+            p64, m64 = q.pos64(), q.mass64()
+            oracle.pm_accelerations(p64, m64, q.box, grid, 1.0, mode=oracle.MODE_FAITHFUL,
+                                    deposit_mode=oracle.DEPOSIT_SERIAL)
+            tt = []
+            for _ in range(5):
+                t1 = time.perf_counter()
+                oracle.pm_accelerations(p64, m64, q.box, grid, 1.0, mode=oracle.MODE_FAITHFUL,
+                                        deposit_mode=oracle.DEPOSIT_SERIAL)
+                tt.append(time.perf_counter() - t1)
+            gp = pm.Plan(grid, q.box, device=0)
+            gp.accelerations_f32(q.x, q.y, q.z, q.m, 1.0)
+            tg = []
+            for _ in range(20):
+                t1 = time.perf_counter()
+                gp.accelerations_f32(q.x, q.y, q.z, q.m, 1.0)
+                tg.append(time.perf_counter() - t1)
+            gp.close()
+            ap[f"grid{grid}"] = {"n": q.n, "reference_published_cpu_ms": published,
+                                 "oracle_serial_faithful_ms": round(statistics.median(tt) * 1e3, 3),
+                                 "gpu_host_entry_ms": round(statistics.median(tg) * 1e3, 4)}
+        oracle.set_threads(threads)
+        result["ap08_calibration"] = ap
 
     print(json.dumps(result), flush=True)
     plan.close()
+    return 0
+
+
+def bench_ensemble(args) -> int:
+    """64 independent boxes of uneven size solved together
+    through one captured hipGraph; compared with the same boxes solved one
+    after another on one stream, and checked per box against the FP64 oracle."""
+    import numpy as np
+
+    import gadget_ng_pm_hip as pm
+    from gadget_ng_pm_hip import workloads
+
+    boxes, off, x, y, z, m = workloads.ensemble(workers=-1)
+    n = int(off[-1])
+    d = [pm.DeviceArray.from_host(a) for a in (x, y, z, m)]
+    out = [pm.DeviceArray((n,), np.float32) for _ in range(3)]
+    batch = pm.Batch(boxes, device=0)
+
+    def graph_step():
+        batch.solve_device(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, off, 1.0, out[0].ptr, out[1].ptr, out[2].ptr)
+
+    for _ in range(args.warmup):
+        graph_step()
+    pm.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph_step()
+    pm.device_synchronize()
+    wall = time.perf_counter() - t0
+    acc = np.stack([o.numpy() for o in out], axis=1).astype(np.float64)
+
+    # the same boxes, one plan each, launched one after another without a graph
+    plans = [pm.Plan(b["grid_size"], b["box_size"], device=0) for b in boxes]
+    fl = 4
+
+    def serial_step():
+        for i, p in enumerate(plans):
+            o0 = int(off[i]) * fl
+            p.solve_device(d[0].ptr + o0, d[1].ptr + o0, d[2].ptr + o0, d[3].ptr + o0, int(off[i + 1] - off[i]),
+                           out[0].ptr + o0, out[1].ptr + o0, out[2].ptr + o0, 1.0)
+
+    serial_step()
+    pm.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        serial_step()
+    pm.device_synchronize()
+    wall_serial = time.perf_counter() - t0
+
+    worst = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # checker only
+
+        oracle.build()
+        worst = 0.0
+        for i, b in enumerate(boxes):
+            p = b["particles"]
+            ref = oracle.pm_accelerations(p.pos64(), p.mass64(), b["box_size"], b["grid_size"], 1.0)
+            got = acc[off[i]:off[i + 1]]
+            worst = max(worst, float(np.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum())))
+    ms = wall * 1e3 / args.steps
+    b_alg = sum(workloads.algorithmic_bytes(int(off[i + 1] - off[i]), b["grid_size"]) for i, b in enumerate(boxes))
+    sizes = {}
+    for b in boxes:
+        sizes[b["grid_size"]] = sizes.get(b["grid_size"], 0) + 1
+    result = {
+        "metric": "M particle-forces/s, ensemble of 64 uneven periodic boxes (config 4) in one hipGraph",
+        "value": round(n * args.steps / wall / 1e6, 2), "unit": "M particle-forces/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "64 boxes, N_side from {32,48,64,96,128}, mesh = 2 N_side, L = N_side Mpc/h, "
+                               "Zel'dovich P(k)~k^n, n in [-2.5,-1.5], seeds 0x6E670004 + box",
+                   "particles": n, "meshes": {str(k): v for k, v in sorted(sizes.items())}},
+        "hbm_gbps": round(b_alg / (ms * 1e-3) / 1e9, 1),
+        "serial_ms_per_step": round(wall_serial * 1e3 / args.steps, 4),
+        "graph_speedup": round(wall_serial / wall, 3),
+        "worst_box_rms_force_err": worst,
+    }
+    print(json.dumps(result), flush=True)
     return 0
 
 
@@ -329,12 +458,12 @@ def bench_learned(args) -> int:
     big = learned.LearnedCorrection(cells=256, batch=1)
     big.net.set_params(lc.net.params())
     pm.device_synchronize()
-    big.net.forward(1)
+    big.net.infer(1)
     pm.device_synchronize()
     reps = 10
     t0 = time.perf_counter()
     for _ in range(reps):
-        big.net.forward(1)
+        big.net.infer(1)
     pm.device_synchronize()
     t_fwd = (time.perf_counter() - t0) / reps
     cells = 256 ** 3

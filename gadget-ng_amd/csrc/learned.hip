@@ -3,12 +3,13 @@
 //
 // Layers: conv(2->16) SiLU, conv(16->16) SiLU, conv(16->3).  Activations and
 // weights are bfloat16, products accumulate in FP32 with
-// v_mfma_f32_16x16x16_bf16.  The convolution is written as an implicit GEMM
+// v_mfma_f32_16x16x32_bf16.  The convolution is written as an implicit GEMM
 // per 16-cell x row:  D[cell][o] = sum_k A[cell][k] B[k][o], k = tap*CIN + c,
-// so one MFMA takes one tap of 16 channels (CIN = 16), four taps of 4
-// channels or eight taps of 2 channels.  A workgroup stages its 16x8x8 output
-// tile plus a one-cell periodic halo in LDS once and keeps every weight
-// fragment of the layer in registers.
+// so one MFMA takes two taps of 16 channels (CIN = 16), eight taps of 4
+// channels or sixteen taps of 2 channels.  A workgroup stages its 16x8x8
+// output tile plus a one-cell periodic halo in LDS once and keeps every
+// weight fragment of the layer in registers.  Inference skips the FP32
+// pre-activations that only the backward pass reads.
 //
 // Backward: the data gradient is the same convolution with the taps mirrored
 // (t -> 26 - t) and the channel roles swapped; the weight gradient is a GEMM
@@ -23,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -33,6 +35,9 @@ namespace gng {
 namespace net {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // forward / data-gradient tile: 16 x 8 x 8 output cells
@@ -47,9 +52,10 @@ constexpr int kWgradBlocks = 512;
 constexpr int P_W1 = 0, P_B1 = P_W1 + 27 * 2 * 16, P_W2 = P_B1 + 16, P_B2 = P_W2 + 27 * 16 * 16,
               P_W3 = P_B2 + 16, P_B3 = P_W3 + 27 * 16 * 3, P_COUNT = P_B3 + 3;
 
+// k-steps of 32 (forward and data-gradient convolutions)
 template <int CIN>
 struct KSteps {
-  static constexpr int v = (27 * CIN + 15) / 16;
+  static constexpr int v = (27 * CIN + 31) / 32;
 };
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
@@ -72,13 +78,20 @@ __host__ __device__ constexpr int tap_off(int t, int hx, int hy) {
 
 // Stage input cells (x0-1 .. x0+HXv-2, ...) mod nm, CIN bf16 channels each,
 // into lds[((hz*HYv + hy)*HXv + hx)*CIN + c], moving 32-bit words.
+template <int CIN>
+struct HaloWord {  // 16-byte words for 16 channels, one word per cell otherwise
+  using T = typename std::conditional<CIN == 16, u32x4, typename std::conditional<CIN == 4, u32x2, unsigned>::type>::type;
+  static constexpr int per_cell = CIN == 16 ? 2 : 1;
+};
+
 template <int CIN, int HXv, int HYv, int HZv>
 __device__ __forceinline__ void load_halo(unsigned short* lds, const unsigned short* __restrict__ in, int x0,
                                           int y0, int z0, int nm) {
-  constexpr int WPC = CIN / 2;
+  using W = typename HaloWord<CIN>::T;
+  constexpr int WPC = HaloWord<CIN>::per_cell;
   constexpr int NW = HXv * HYv * HZv * WPC;
-  const unsigned* src = reinterpret_cast<const unsigned*>(in);
-  unsigned* dst = reinterpret_cast<unsigned*>(lds);
+  const W* src = reinterpret_cast<const W*>(in);
+  W* dst = reinterpret_cast<W*>(lds);
   for (int e = threadIdx.x; e < NW; e += blockDim.x) {
     const int c = e / WPC, w = e - c * WPC;
     const int hx = c % HXv, hy = (c / HXv) % HYv, hz = c / (HXv * HYv);
@@ -95,41 +108,53 @@ __device__ __forceinline__ void load_halo(unsigned short* lds, const unsigned sh
 }
 
 // A fragment of k-step s for the 16-cell row whose lane-m cell sits at halo
-// index `base`: lane (m, g) holds x[cell m + off_t][c] for k = 16s + 4g + j,
-// t = k / CIN, c = k % CIN, j = 0..3 (zero for t >= 27).
+// index `base`: lane (m, g) holds x[cell m + off_t][c] for k = 32s + 8g + j,
+// t = k / CIN, c = k % CIN, j = 0..7 (zero for t >= 27).
 template <int CIN>
-__device__ __forceinline__ bf16x4 a_frag(const unsigned short* tile, int base, int s, int g) {
+__device__ __forceinline__ bf16x8 a_frag(const unsigned short* tile, int base, int s, int g) {
+  u32x4 v = {0u, 0u, 0u, 0u};
   if constexpr (CIN == 16) {
-    return *reinterpret_cast<const bf16x4*>(tile + (base + tap_off(s, HX, HY)) * 16 + 4 * g);
+    // taps 2s and 2s + 1, channel halves by g & 1
+    const int t = 2 * s + (g >> 1);
+    const int off = (g >> 1) ? tap_off(2 * s + 1, HX, HY) : tap_off(2 * s, HX, HY);
+    if (t < 27) v = *reinterpret_cast<const u32x4*>(tile + (base + off) * 16 + 8 * (g & 1));
   } else if constexpr (CIN == 4) {
-    const int t = 4 * s + g;
-    if (t >= 27) return bf16x4{0, 0, 0, 0};
-    return *reinterpret_cast<const bf16x4*>(tile + (base + tap_off(t, HX, HY)) * 4);
+    const int t0 = 8 * s + 2 * g;
+    if (t0 < 27) {
+      const u32x2 a = *reinterpret_cast<const u32x2*>(tile + (base + tap_off(t0, HX, HY)) * 4);
+      v.x = a.x;
+      v.y = a.y;
+    }
+    if (t0 + 1 < 27) {
+      const u32x2 b = *reinterpret_cast<const u32x2*>(tile + (base + tap_off(t0 + 1, HX, HY)) * 4);
+      v.z = b.x;
+      v.w = b.y;
+    }
   } else {
     static_assert(CIN == 2, "CIN must be 2, 4 or 16");
-    const int t0 = 8 * s + 2 * g;
-    unsigned lo = 0, hi = 0;
-    if (t0 < 27) lo = *reinterpret_cast<const unsigned*>(tile + (base + tap_off(t0, HX, HY)) * 2);
-    if (t0 + 1 < 27) hi = *reinterpret_cast<const unsigned*>(tile + (base + tap_off(t0 + 1, HX, HY)) * 2);
-    bf16x4 v;
-    v[0] = (short)(lo & 0xffffu);
-    v[1] = (short)(lo >> 16);
-    v[2] = (short)(hi & 0xffffu);
-    v[3] = (short)(hi >> 16);
-    return v;
+    const int t0 = 16 * s + 4 * g;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (t0 + q < 27) v[q] = *reinterpret_cast<const unsigned*>(tile + (base + tap_off(t0 + q, HX, HY)) * 2);
   }
+  return __builtin_bit_cast(bf16x8, v);
 }
 
-enum { MODE_HIDDEN = 0, MODE_OUT = 1, MODE_BWD = 2 };
+enum { MODE_HIDDEN = 0, MODE_OUT = 1, MODE_BWD = 2, MODE_INFER = 3 };
 
 // One periodic 3^3 convolution over an nm^3 mesh (nm % 16 == 0), grid
-// (nm/16, (nm/8)^2, boxes), 256 threads.  Output channels n = lane % 16:
+// (nm/16, (nm/8)^2, boxes), kConvThreads threads (8 waves, 8 of the tile's 64
+// rows each).  Output channels n = lane % 16:
 //   MODE_HIDDEN  zout = conv + bias (f32, 16 ch), xout = bf16(silu(zout));
+//   MODE_INFER   xout = bf16(silu(conv + bias)) only;
 //   MODE_OUT     zout = conv + bias (f32, 4 ch: the 3 outputs and a zero);
 //   MODE_BWD     xout = bf16(conv * silu'(zprev)) (16 ch).
+constexpr int kConvThreads = 512;
+constexpr int kRowsPerWave = TY * TZ / (kConvThreads / 64);
+
 template <int CIN, int MODE>
-__global__ __launch_bounds__(256) void k_conv(const unsigned short* __restrict__ in,
-                                              const bf16x4* __restrict__ wfrag, const float* __restrict__ bias,
+__global__ __launch_bounds__(kConvThreads) void k_conv(const unsigned short* __restrict__ in,
+                                              const u32x4* __restrict__ wfrag, const float* __restrict__ bias,
                                               float* __restrict__ zout, unsigned short* __restrict__ xout,
                                               const float* __restrict__ zprev, int nm) {
   constexpr int KS = KSteps<CIN>::v;
@@ -141,23 +166,23 @@ __global__ __launch_bounds__(256) void k_conv(const unsigned short* __restrict__
   const long long M = (long long)nm * nm * nm;
   const long long boff = (long long)blockIdx.z * M;
 
-  bf16x4 b[KS];
+  bf16x8 b[KS];
 #pragma unroll
-  for (int s = 0; s < KS; ++s) b[s] = wfrag[s * 64 + lane];
+  for (int s = 0; s < KS; ++s) b[s] = __builtin_bit_cast(bf16x8, wfrag[s * 64 + lane]);
   float bn = 0.0f;
   if constexpr (MODE != MODE_BWD) bn = bias[m];
 
   load_halo<CIN, HX, HY, HZ>(tile, in + boff * CIN, x0, y0, z0, nm);
   __syncthreads();
 
-  // each wave: 16 of the tile's 64 rows, four independent accumulators at a time
+  // each wave: kRowsPerWave of the tile's rows, four independent accumulators at a time
 #pragma unroll 1
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < kRowsPerWave / 4; ++q) {
     f32x4 acc[4];
     int base[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      const int r = wave * 16 + q * 4 + a;
+      const int r = wave * kRowsPerWave + q * 4 + a;
       const int ry = r % TY, rz = r / TY;
       base[a] = ((rz + 1) * HY + (ry + 1)) * HX + (m + 1);
       acc[a] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -166,14 +191,14 @@ __global__ __launch_bounds__(256) void k_conv(const unsigned short* __restrict__
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
-        const bf16x4 av = a_frag<CIN>(tile, base[a], s, g);
-        acc[a] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b[s], acc[a], 0, 0, 0);
+        const bf16x8 av = a_frag<CIN>(tile, base[a], s, g);
+        acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[s], acc[a], 0, 0, 0);
       }
     }
     // lane holds D[cell 4g + j][channel m]
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      const int r = wave * 16 + q * 4 + a;
+      const int r = wave * kRowsPerWave + q * 4 + a;
       const int ry = r % TY, rz = r / TY;
       const long long rowc = boff + ((long long)(z0 + rz) * nm + (y0 + ry)) * nm + x0 + 4 * g;
 #pragma unroll
@@ -183,6 +208,8 @@ __global__ __launch_bounds__(256) void k_conv(const unsigned short* __restrict__
           const float z = acc[a][j] + bn;
           zout[c * 16 + m] = z;
           xout[c * 16 + m] = f2bf(silu(z));
+        } else if constexpr (MODE == MODE_INFER) {
+          xout[c * 16 + m] = f2bf(silu(acc[a][j] + bn));
         } else if constexpr (MODE == MODE_OUT) {
           if (m < 4) zout[c * 4 + m] = acc[a][j] + bn;
         } else {
@@ -303,14 +330,14 @@ __global__ void k_wgrad_reduce(const float* __restrict__ partial, int nblocks, f
 // mirror != 0 the fragments are those of the data-gradient convolution:
 // B[k = t*cin_pad + o][n = c] = w[26 - t][c][o].
 __global__ void k_pack(const float* __restrict__ w, int cin, int cout, int cin_pad, int ksteps, int mirror,
-                       bf16x4* __restrict__ frag) {
+                       u32x4* __restrict__ frag) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ksteps * 64) return;
   const int s = e / 64, l = e % 64, g = l / 16, n = l % 16;
-  bf16x4 v;
+  unsigned short v[8];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int k = 16 * s + 4 * g + j;
+  for (int j = 0; j < 8; ++j) {
+    const int k = 32 * s + 8 * g + j;
     const int t = k / cin_pad, c = k % cin_pad;
     float x = 0.0f;
     if (!mirror) {
@@ -320,9 +347,12 @@ __global__ void k_pack(const float* __restrict__ w, int cin, int cout, int cin_p
       // outputs the forward inputs (n < cin)
       if (t < 27 && c < cout && n < cin) x = w[((26 - t) * cin + n) * cout + c];
     }
-    v[j] = (short)f2bf(x);
+    v[j] = f2bf(x);
   }
-  frag[e] = v;
+  u32x4 o;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) o[q] = (unsigned)v[2 * q] | ((unsigned)v[2 * q + 1] << 16);
+  frag[e] = o;
 }
 
 __global__ void k_bias16(const float* __restrict__ b, int cout, float* __restrict__ b16) {
@@ -530,11 +560,11 @@ struct gng_net {
   float* d_m1 = nullptr;
   float* d_m2 = nullptr;
   int adam_t = 0;
-  bf16x4* d_f1 = nullptr;   // forward fragments, layer 1..3
-  bf16x4* d_f2 = nullptr;
-  bf16x4* d_f3 = nullptr;
-  bf16x4* d_g3 = nullptr;   // data-gradient fragments of layers 3 and 2
-  bf16x4* d_g2 = nullptr;
+  u32x4* d_f1 = nullptr;   // forward fragments (8 bf16 per lane and k-step), layer 1..3
+  u32x4* d_f2 = nullptr;
+  u32x4* d_f3 = nullptr;
+  u32x4* d_g3 = nullptr;   // data-gradient fragments of layers 3 and 2
+  u32x4* d_g2 = nullptr;
   float* d_b16 = nullptr;   // 3 x 16 padded biases
   // activations
   unsigned short* d_x0 = nullptr;
@@ -590,11 +620,12 @@ BufInfo buffer(const gng_net* t, int which) {
 
 int repack(gng_net* t, hipStream_t s) {
   const float* p = t->d_params;
-  hipLaunchKernelGGL(k_pack, dim3(1), dim3(KSteps<2>::v * 64), 0, s, p + P_W1, 2, 16, 2, KSteps<2>::v, 0, t->d_f1);
-  hipLaunchKernelGGL(k_pack, dim3(27), dim3(64), 0, s, p + P_W2, 16, 16, 16, 27, 0, t->d_f2);
-  hipLaunchKernelGGL(k_pack, dim3(27), dim3(64), 0, s, p + P_W3, 16, 3, 16, 27, 0, t->d_f3);
-  hipLaunchKernelGGL(k_pack, dim3(KSteps<4>::v), dim3(64), 0, s, p + P_W3, 16, 3, 4, KSteps<4>::v, 1, t->d_g3);
-  hipLaunchKernelGGL(k_pack, dim3(27), dim3(64), 0, s, p + P_W2, 16, 16, 16, 27, 1, t->d_g2);
+  constexpr int K2 = KSteps<2>::v, K4 = KSteps<4>::v, K16 = KSteps<16>::v;
+  hipLaunchKernelGGL(k_pack, dim3(K2), dim3(64), 0, s, p + P_W1, 2, 16, 2, K2, 0, t->d_f1);
+  hipLaunchKernelGGL(k_pack, dim3(K16), dim3(64), 0, s, p + P_W2, 16, 16, 16, K16, 0, t->d_f2);
+  hipLaunchKernelGGL(k_pack, dim3(K16), dim3(64), 0, s, p + P_W3, 16, 3, 16, K16, 0, t->d_f3);
+  hipLaunchKernelGGL(k_pack, dim3(K4), dim3(64), 0, s, p + P_W3, 16, 3, 4, K4, 1, t->d_g3);
+  hipLaunchKernelGGL(k_pack, dim3(K16), dim3(64), 0, s, p + P_W2, 16, 16, 16, K16, 1, t->d_g2);
   hipLaunchKernelGGL(k_bias16, dim3(1), dim3(64), 0, s, p + P_B1, 16, t->d_b16);
   hipLaunchKernelGGL(k_bias16, dim3(1), dim3(64), 0, s, p + P_B2, 16, t->d_b16 + 16);
   hipLaunchKernelGGL(k_bias16, dim3(1), dim3(64), 0, s, p + P_B3, 3, t->d_b16 + 32);
@@ -642,11 +673,11 @@ int gng_net_create(int32_t grid_size, int32_t max_boxes, int32_t device, gng_net
   A(&t->d_grad, P_COUNT * 4);
   A(&t->d_m1, P_COUNT * 4);
   A(&t->d_m2, P_COUNT * 4);
-  A(&t->d_f1, KSteps<2>::v * 64 * 8);
-  A(&t->d_f2, 27 * 64 * 8);
-  A(&t->d_f3, 27 * 64 * 8);
-  A(&t->d_g3, KSteps<4>::v * 64 * 8);
-  A(&t->d_g2, 27 * 64 * 8);
+  A(&t->d_f1, KSteps<2>::v * 64 * 16);
+  A(&t->d_f2, KSteps<16>::v * 64 * 16);
+  A(&t->d_f3, KSteps<16>::v * 64 * 16);
+  A(&t->d_g3, KSteps<4>::v * 64 * 16);
+  A(&t->d_g2, KSteps<16>::v * 64 * 16);
   A(&t->d_b16, 48 * 4);
   A(&t->d_x0, cells * 4);
   A(&t->d_z1, cells * 64);
@@ -779,11 +810,28 @@ int gng_net_forward(gng_net* net, int32_t boxes, void* stream) {
   GNG_HIP(hipSetDevice(net->device));
   hipStream_t s = as_stream(stream);
   const dim3 g = conv_grid(net, boxes);
-  hipLaunchKernelGGL((k_conv<2, MODE_HIDDEN>), g, dim3(256), 0, s, net->d_x0, net->d_f1, net->d_b16, net->d_z1,
+  hipLaunchKernelGGL((k_conv<2, MODE_HIDDEN>), g, dim3(kConvThreads), 0, s, net->d_x0, net->d_f1, net->d_b16, net->d_z1,
                      net->d_x1, (const float*)nullptr, net->nm);
-  hipLaunchKernelGGL((k_conv<16, MODE_HIDDEN>), g, dim3(256), 0, s, net->d_x1, net->d_f2, net->d_b16 + 16,
+  hipLaunchKernelGGL((k_conv<16, MODE_HIDDEN>), g, dim3(kConvThreads), 0, s, net->d_x1, net->d_f2, net->d_b16 + 16,
                      net->d_z2, net->d_x2, (const float*)nullptr, net->nm);
-  hipLaunchKernelGGL((k_conv<16, MODE_OUT>), g, dim3(256), 0, s, net->d_x2, net->d_f3, net->d_b16 + 32, net->d_y,
+  hipLaunchKernelGGL((k_conv<16, MODE_OUT>), g, dim3(kConvThreads), 0, s, net->d_x2, net->d_f3, net->d_b16 + 32, net->d_y,
+                     (unsigned short*)nullptr, (const float*)nullptr, net->nm);
+  GNG_HIP(hipGetLastError());
+  return GNG_OK;
+}
+
+int gng_net_infer(gng_net* net, int32_t boxes, void* stream) {
+  if (!net) return fail(GNG_ERR_INVALID, "null network");
+  int rc = check_boxes(net, boxes);
+  if (rc != GNG_OK) return rc;
+  GNG_HIP(hipSetDevice(net->device));
+  hipStream_t s = as_stream(stream);
+  const dim3 g = conv_grid(net, boxes);
+  hipLaunchKernelGGL((k_conv<2, MODE_INFER>), g, dim3(kConvThreads), 0, s, net->d_x0, net->d_f1, net->d_b16, (float*)nullptr,
+                     net->d_x1, (const float*)nullptr, net->nm);
+  hipLaunchKernelGGL((k_conv<16, MODE_INFER>), g, dim3(kConvThreads), 0, s, net->d_x1, net->d_f2, net->d_b16 + 16,
+                     (float*)nullptr, net->d_x2, (const float*)nullptr, net->nm);
+  hipLaunchKernelGGL((k_conv<16, MODE_OUT>), g, dim3(kConvThreads), 0, s, net->d_x2, net->d_f3, net->d_b16 + 32, net->d_y,
                      (unsigned short*)nullptr, (const float*)nullptr, net->nm);
   GNG_HIP(hipGetLastError());
   return GNG_OK;
@@ -828,14 +876,14 @@ int gng_net_backward(gng_net* net, int32_t boxes, void* stream) {
                      boxes);
   hipLaunchKernelGGL((k_wgrad_reduce<16, 3>), dim3((433 * 3 + 255) / 256), dim3(256), 0, s, net->d_partial, wb,
                      gr + P_W3, gr + P_B3);
-  hipLaunchKernelGGL((k_conv<4, MODE_BWD>), g, dim3(256), 0, s, net->d_dz3, net->d_g3, (const float*)nullptr,
+  hipLaunchKernelGGL((k_conv<4, MODE_BWD>), g, dim3(kConvThreads), 0, s, net->d_dz3, net->d_g3, (const float*)nullptr,
                      (float*)nullptr, net->d_dz2, net->d_z2, nm);
   // layer 2
   hipLaunchKernelGGL((k_conv_wgrad<16, 16>), dim3(wb), dim3(256), 0, s, net->d_x1, net->d_dz2, net->d_partial, nm,
                      boxes);
   hipLaunchKernelGGL((k_wgrad_reduce<16, 16>), dim3((433 * 16 + 255) / 256), dim3(256), 0, s, net->d_partial, wb,
                      gr + P_W2, gr + P_B2);
-  hipLaunchKernelGGL((k_conv<16, MODE_BWD>), g, dim3(256), 0, s, net->d_dz2, net->d_g2, (const float*)nullptr,
+  hipLaunchKernelGGL((k_conv<16, MODE_BWD>), g, dim3(kConvThreads), 0, s, net->d_dz2, net->d_g2, (const float*)nullptr,
                      (float*)nullptr, net->d_dz1, net->d_z1, nm);
   // layer 1
   hipLaunchKernelGGL((k_conv_wgrad<2, 16>), dim3(wb), dim3(256), 0, s, net->d_x0, net->d_dz1, net->d_partial, nm,

@@ -156,6 +156,8 @@ def lib():
         L.gng_net_device_buffer.restype = i32
         L.gng_net_input.argtypes = [P, i32, P, P, f32, f32, P]
         L.gng_net_forward.argtypes = [P, i32, P]
+        L.gng_net_infer.argtypes = [P, i32, P]
+        L.gng_net_infer.restype = i32
         L.gng_net_loss_grad.argtypes = [P, P, P, i64, d, f32, P]
         L.gng_net_backward.argtypes = [P, i32, P]
         L.gng_net_adam.argtypes = [P, f32, f32, f32, f32, P]

@@ -256,6 +256,10 @@ class Net:
     def forward(self, boxes: int = 1, stream: int = 0) -> None:
         _check(lib().gng_net_forward(self._h, int(boxes), stream or None), "gng_net_forward")
 
+    def infer(self, boxes: int = 1, stream: int = 0) -> None:
+        """Forward pass without the pre-activations (inference)."""
+        _check(lib().gng_net_infer(self._h, int(boxes), stream or None), "gng_net_infer")
+
     def loss_grad(self, pos4: int, tgt4: int, n: int, box_size: float, out_scale: float, stream: int = 0) -> None:
         _check(lib().gng_net_loss_grad(self._h, pos4, tgt4, int(n), float(box_size), float(out_scale),
                                        stream or None), "gng_net_loss_grad")
@@ -514,7 +518,7 @@ class LearnedCorrection:
         x, y, z, m = s.dev
         device_synchronize()
         t0 = time.perf_counter()
-        net.forward(1)
+        net.infer(1)
         net.apply(0, x.ptr, y.ptr, z.ptr, n, other.L, self.f0, corr[0].ptr, corr[1].ptr, corr[2].ptr)
         device_synchronize()
         t_net = time.perf_counter() - t0

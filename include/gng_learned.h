@@ -22,7 +22,7 @@
  * on the CIC density/potential mesh of a PM solve; its 3-channel output mesh
  * is CIC-interpolated to the particles and added to the PM acceleration.
  * Weights and activations are bfloat16, products accumulate in FP32 on the
- * MFMA units; the master weights, gradients and Adam moments are FP32.  Mesh
+ * MFMA units (v_mfma_f32_16x16x32_bf16); the master weights, gradients and Adam moments are FP32.  Mesh
  * layout: [box][iz][iy][ix][channel], channels last.  A network object holds
  * the weights and the work buffers for up to max_boxes meshes of one size
  * (grid_size a multiple of 16).
@@ -106,7 +106,12 @@ int gng_net_device_buffer(gng_net* net, int32_t which, void** d_ptr);
 int gng_net_input(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
                   float phi_scale, void* stream);
 
+/* Forward pass for training: also keeps the FP32 pre-activations that the
+ * backward pass reads. */
 int gng_net_forward(gng_net* net, int32_t boxes, void* stream);
+/* Forward pass for inference: writes only the activations and the output
+ * mesh (GNG_NET_Z1/Z2 are left untouched). */
+int gng_net_infer(gng_net* net, int32_t boxes, void* stream);
 /* Loss and dL/dy for n training particles (uses the output of the last
  * forward); the loss is kept on the device until gng_net_loss() reads it. */
 int gng_net_loss_grad(gng_net* net, const float* d_pos4, const float* d_tgt4, int64_t n, double box_size,

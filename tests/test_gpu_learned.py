@@ -298,3 +298,15 @@ def test_net_input_matches_oracle(learned, pm):
     assert bf16_ulp_ok(got[..., 0], ref[..., 0], frac=1.0)
     # the high-passed potential differs from a float64 evaluation by FP32 rounding of phi ~ 1e3
     assert np.abs(got[..., 1] - ref[..., 1]).max() <= 2e-3 * np.abs(ref[..., 1]).max() + 1e-4
+
+
+def test_inference_pass_equals_training_forward(learned):
+    nm, boxes = 32, 1
+    net, p, x0 = _forward(learned, nm, boxes, 16)
+    y_train, x1_train = net.read("y"), net.read("x1")
+    z1 = net.read("z1")
+    net.write("y", np.zeros_like(y_train))
+    net.infer(boxes)
+    assert np.array_equal(net.read("y"), y_train)
+    assert np.array_equal(net.read("x1"), x1_train)
+    assert np.array_equal(net.read("z1"), z1)  # inference leaves the pre-activations alone
