@@ -82,6 +82,7 @@ struct MeshOps {
   void (*y_fwd)(dim3, dim3, hipStream_t, float2*, const float2*);
   void (*z_green)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
   void (*z_green_col0)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
+  void (*z_green_persist)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
   void (*y_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
   void (*x_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, float2*, const float2*, int, float);
 };
@@ -113,6 +114,10 @@ MeshOps make_ops() {
                  GreenParams gp) { hipLaunchKernelGGL((k_fft_z_green<N, C, TY>), g, b, 0, s, P, A, B, tw, gp); };
   o.z_green_col0 = [](dim3 g, dim3 b, hipStream_t s, const float2* P, float2* A, float2* B, const float2* tw,
                       GreenParams gp) { hipLaunchKernelGGL((k_fft_z_green_col0<N>), g, b, 0, s, P, A, B, tw, gp); };
+  o.z_green_persist = [](dim3 g, dim3 b, hipStream_t s, const float2* P, float2* A, float2* B, const float2* tw,
+                         GreenParams gp) {
+    hipLaunchKernelGGL((k_fft_z_green_persist<N, C, TY>), g, b, 0, s, P, A, B, tw, gp);
+  };
   o.y_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* B, float2* P, const float2* tw) {
     hipLaunchKernelGGL((k_fft_y_inv<N, C, TY>), g, b, 0, s, A, B, P, tw);
   };
@@ -144,6 +149,7 @@ struct gng_pm_plan {
   gng_pm_config cfg{};
   const MeshOps* ops = nullptr;
   int device = 0;
+  int cus = 0;  // compute units of the device (persistent launches)
   std::mutex mu;
 
   long long nm3 = 0;
@@ -305,7 +311,15 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   o.y_fwd(yz_grid, yz_block, s, p->d_P, p->d_tw);
   mark(p, 5, s);
   const GreenParams gp = green_params(p, g);
-  o.z_green(yz_grid, yz_block, s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
+  {
+    // persistent z pass at nm = 512, where it fits the register file of one
+    // 1024-thread workgroup per CU (other sizes spill or lose occupancy)
+    const long long items = (long long)(H / o.yz_cols) * nm;
+    if (nm == 512 && p->cus > 0 && items >= 4LL * p->cus)
+      o.z_green_persist(dim3((unsigned)p->cus), yz_block, s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
+    else
+      o.z_green(yz_grid, yz_block, s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
+  }
   o.z_green_col0(dim3((unsigned)(H + 1)), dim3(256), s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
   mark(p, 6, s);
   o.y_inv(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
@@ -492,6 +506,7 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
   };
   int rc = set_device(p);
   if (rc) return bail(rc);
+  if (hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess) p->cus = 0;
   const int nm = cfg->grid_size;
   p->nm3 = (long long)nm * nm * nm;
   p->tm.T = ops->tile;

@@ -113,11 +113,16 @@ struct XCfg {
 // read straight into the registers of the first FFT stage; the pairwise
 // post-processing (X_k needs Z_k and Z_{H-k}) exchanges through LDS.
 template <int N, int ROWS>
-__global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, const float2* __restrict__ tw) {
+__global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, const float2* __restrict__ tw_g) {
   constexpr int H = N / 2, T = 256, LS = XCfg<N>::LS;
   using F = RegFft<H, false, false, ROWS, T, LS, 1, false, N>;
   __shared__ float2 buf[ROWS * LS];
   const int tid = threadIdx.x;
+  // twiddles in LDS: the FFT stages then read them with LDS instructions, so
+  // no global-load wait of a stage also waits for unrelated outstanding loads
+  __shared__ float2 twl[N];
+  for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
+  const float2* tw = twl;
   const long long row0 = (long long)blockIdx.x * ROWS;
   const float2* src = reinterpret_cast<const float2*>(rho + row0 * N);
   float2 vin[F::PER0][F::R0], vout[F::PERL][F::RL];
@@ -159,11 +164,16 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, 
 template <int N, int ROWS>
 __global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float2* __restrict__ P,
                                                    float2* __restrict__ B, float2* __restrict__ pot,
-                                                   const float2* __restrict__ tw, int field0, float pot_scale) {
+                                                   const float2* __restrict__ tw_g, int field0, float pot_scale) {
   constexpr int H = N / 2, T = 256, LS = XCfg<N>::LS;
   using F = RegFft<H, true, false, ROWS, T, LS, 1, false, N>;
   __shared__ float2 buf[ROWS * LS];
   const int tid = threadIdx.x;
+  // twiddles in LDS: the FFT stages then read them with LDS instructions, so
+  // no global-load wait of a stage also waits for unrelated outstanding loads
+  __shared__ float2 twl[N];
+  for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
+  const float2* tw = twl;
   // fields 0..2: F_x from A (times -i n_x), F_y from P, F_z from B, in place;
   // field 3: the potential from A (no derivative) into pot, scaled
   const int field = blockIdx.y + field0;
@@ -238,11 +248,16 @@ struct YZCfg {
 // [0, C) (element (i, c) at buf[i*IS + c]); `stride` is the distance in
 // complex values between consecutive i.
 template <int N, int C, int T>
-__global__ __launch_bounds__(T) void k_fft_y_fwd(float2* __restrict__ P, const float2* __restrict__ tw) {
+__global__ __launch_bounds__(T) void k_fft_y_fwd(float2* __restrict__ P, const float2* __restrict__ tw_g) {
   constexpr int H = N / 2, IS = C + 1;
   using F = RegFft<N, false, false, C, T, 1, IS, true, N>;
   __shared__ float2 buf[N * IS];
   const int tid = threadIdx.x;
+  // twiddles in LDS: the FFT stages then read them with LDS instructions, so
+  // no global-load wait of a stage also waits for unrelated outstanding loads
+  __shared__ float2 twl[N];
+  for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
+  const float2* tw = twl;
   float2* base = P + (size_t)blockIdx.y * N * H + (size_t)blockIdx.x * C;
   float2 vin[F::PER0][F::R0], vout[F::PERL][F::RL];
   for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
@@ -270,7 +285,7 @@ __device__ __forceinline__ int freq_of(int j, int n) { return j <= n / 2 ? j : j
 // kx = 0 column is done by k_fft_z_green_col0.
 template <int N, int C, int T>
 __global__ __launch_bounds__(T) void k_fft_z_green(const float2* __restrict__ P, float2* __restrict__ A,
-                                                   float2* __restrict__ B, const float2* __restrict__ tw,
+                                                   float2* __restrict__ B, const float2* __restrict__ tw_g,
                                                    const GreenParams gp) {
   constexpr int H = N / 2, IS = C + 1;
   using FF = RegFft<N, false, false, C, T, 1, IS, true, N>;
@@ -278,6 +293,11 @@ __global__ __launch_bounds__(T) void k_fft_z_green(const float2* __restrict__ P,
   static_assert(FI::R0 == FF::RL && FI::PER0 == FF::PERL, "inverse must start in the forward output layout");
   __shared__ float2 buf[N * IS];
   const int tid = threadIdx.x;
+  // twiddles in LDS: the FFT stages then read them with LDS instructions, so
+  // no global-load wait of a stage also waits for unrelated outstanding loads
+  __shared__ float2 twl[N];
+  for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
+  const float2* tw = twl;
   const int tile = blockIdx.x, ky = blockIdx.y;
   const unsigned plane = (unsigned)N * H;
   const size_t col0 = (size_t)ky * H + (size_t)tile * C;
@@ -320,6 +340,92 @@ __global__ __launch_bounds__(T) void k_fft_z_green(const float2* __restrict__ P,
     for_edge<N, FI::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
       if (ok && (tile | l)) B[col0 + (unsigned)i * plane + l] = out[p][r];
     });
+  }
+}
+
+// Persistent form of k_fft_z_green for large meshes: gridDim.x workgroups (one
+// per CU) walk the (column tile, ky) items in a grid-stride loop.  Each
+// workgroup loads the next item's column values into registers right after
+// the forward transform of the current item has consumed its own, so that
+// load is in flight while the Green's function and both inverse transforms
+// run.  One workgroup fills a CU's registers, so without this the CU idles
+// through every item's load latency.  To make room for the prefetched values,
+// phi waits in LDS (each thread's own slots) while field A is transformed.
+template <int N, int C, int T>
+__global__ __launch_bounds__(T) void k_fft_z_green_persist(const float2* __restrict__ P, float2* __restrict__ A,
+                                                           float2* __restrict__ B, const float2* __restrict__ tw_g,
+                                                           const GreenParams gp) {
+  constexpr int H = N / 2, IS = C + 1, TILES = H / C;
+  using FF = RegFft<N, false, false, C, T, 1, IS, true, N>;
+  using FI = RegFft<N, true, true, C, T, 1, IS, true, N>;
+  static_assert(FI::R0 == FF::RL && FI::PER0 == FF::PERL, "inverse must start in the forward output layout");
+  __shared__ float2 buf[N * IS];
+  constexpr int KEEP = FF::PERL * FF::RL;
+  __shared__ float2 keep[KEEP * T];
+  const int tid = threadIdx.x;
+  // twiddles in LDS: the FFT stages then read them with LDS instructions, so
+  // no global-load wait of a stage also waits for unrelated outstanding loads
+  __shared__ float2 twl[N];
+  for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
+  const float2* tw = twl;
+  const unsigned plane = (unsigned)N * H;
+  const int items = TILES * N;
+  float2 vin[FF::PER0][FF::R0];
+  int item = blockIdx.x;
+  if (item < items) {
+    const size_t c0 = (size_t)(item / TILES) * H + (size_t)(item % TILES) * C;
+    for_edge<N, FF::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      if (ok) vin[p][r] = P[c0 + (unsigned)i * plane + l];
+    });
+  }
+  for (; item < items; item += gridDim.x) {
+    const int tile = item % TILES, ky = item / TILES;
+    const size_t col0 = (size_t)ky * H + (size_t)tile * C;
+    const int ny = freq_of(ky, N);
+    const float dy = gp.deconv ? gp.deconv[ky] : 1.0f;
+    float2 ph[FF::PERL][FF::RL];
+    FF::run(vin, ph, buf, tw, tid);
+    const int next = item + (int)gridDim.x;
+    if (next < items) {
+      const size_t c1 = (size_t)(next / TILES) * H + (size_t)(next % TILES) * C;
+      for_edge<N, FF::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        if (ok) vin[p][r] = P[c1 + (unsigned)i * plane + l];
+      });
+    }
+    for_edge<N, FF::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      if (!ok) return;
+      const int kx = tile * C + l;
+      const int nz = freq_of(i, N);
+      const int n2 = kx * kx + ny * ny + nz * nz;
+      const float fn2 = (float)(n2 > 0 ? n2 : 1);
+      float g = kx == 0 ? 0.0f : -gp.c * expf(-gp.a * fn2) / fn2;
+      if (gp.deconv) g *= gp.deconv[kx] * dy * gp.deconv[i];
+      ph[p][r] = make_float2(g * ph[p][r].x, g * ph[p][r].y);
+    });
+#pragma unroll
+    for (int p = 0; p < FF::PERL; ++p)
+#pragma unroll
+      for (int r = 0; r < FF::RL; ++r) keep[(p * FF::RL + r) * T + tid] = ph[p][r];
+    {
+      float2 out[FI::PERL][FI::RL];
+      FI::run(ph, out, buf, tw, tid);
+      for_edge<N, FI::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        if (ok && (tile | l)) A[col0 + (unsigned)i * plane + l] = out[p][r];
+      });
+    }
+    float2 w[FI::PER0][FI::R0];
+    for_edge<N, FI::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      const float nze = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
+      const float2 v = keep[(p * FF::RL + r) * T + tid];
+      w[p][r] = make_float2(nze * v.y, -nze * v.x);
+    });
+    {
+      float2 out[FI::PERL][FI::RL];
+      FI::run(w, out, buf, tw, tid);
+      for_edge<N, FI::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        if (ok && (tile | l)) B[col0 + (unsigned)i * plane + l] = out[p][r];
+      });
+    }
   }
 }
 
@@ -393,11 +499,16 @@ __global__ __launch_bounds__(256) void k_fft_z_green_col0(const float2* __restri
 // each starting from registers and storing from registers.
 template <int N, int C, int T>
 __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2* __restrict__ B,
-                                                 float2* __restrict__ P, const float2* __restrict__ tw) {
+                                                 float2* __restrict__ P, const float2* __restrict__ tw_g) {
   constexpr int H = N / 2, IS = C + 1;
   using F = RegFft<N, true, false, C, T, 1, IS, true, N>;
   __shared__ float2 buf[N * IS];
   const int tid = threadIdx.x;
+  // twiddles in LDS: the FFT stages then read them with LDS instructions, so
+  // no global-load wait of a stage also waits for unrelated outstanding loads
+  __shared__ float2 twl[N];
+  for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
+  const float2* tw = twl;
   const size_t off = (size_t)blockIdx.y * N * H + (size_t)blockIdx.x * C;
   float2* a = A + off;
   float2* b = B + off;
