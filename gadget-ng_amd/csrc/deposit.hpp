@@ -58,40 +58,56 @@ __device__ __forceinline__ unsigned bucket_of_cell(int ix, int iy, int iz, const
   return tm.rank[((unsigned)tz * tm.nt + ty) * tm.nt + tx] * 8u + cls;
 }
 
+// Each thread classifies kBucketPer particles (block stride 256, so every
+// wavefront still sees 64 consecutive particles) and issues all their counter
+// atomics before it uses any returned value: the atomics' round trips overlap
+// instead of each one stalling its wavefront.
+constexpr int kBucketPer = 4;
+
 __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ x, const float* __restrict__ y,
                                                       const float* __restrict__ z, const float* __restrict__ m,
                                                       long long n, const CellMap cm, const TileMap tm,
                                                       unsigned* __restrict__ count, unsigned* __restrict__ bucket,
                                                       unsigned* __restrict__ rank, SolveScalars* sc) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const bool live = i < n;
-  unsigned b = 0xffffffffu;
+  const long long i0 = (long long)blockIdx.x * (256 * kBucketPer) + threadIdx.x;
+  unsigned b[kBucketPer], base[kBucketPer];
+  int my_head[kBucketPer];
   float am = 0.0f;
-  if (live) {
-    int ix, iy, iz;
-    float d;
-    cell_coord(x[i], cm, ix, d);
-    cell_coord(y[i], cm, iy, d);
-    cell_coord(z[i], cm, iz, d);
-    b = bucket_of_cell(ix, iy, iz, tm);
-    am = fabsf(m[i]);
+#pragma unroll
+  for (int k = 0; k < kBucketPer; ++k) {
+    const long long i = i0 + 256LL * k;
+    const bool live = i < n;
+    b[k] = 0xffffffffu;
+    if (live) {
+      int ix, iy, iz;
+      float d;
+      cell_coord(x[i], cm, ix, d);
+      cell_coord(y[i], cm, iy, d);
+      cell_coord(z[i], cm, iz, d);
+      b[k] = bucket_of_cell(ix, iy, iz, tm);
+      am = fmaxf(am, fabsf(m[i]));
+    }
+    // runs of equal buckets: a lane heads a run when its bucket differs from the previous lane's
+    const unsigned prev = __shfl_up(b[k], 1);
+    const bool head = live && (lane == 0 || prev != b[k]);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long below = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+    my_head[k] = 63 - __clzll(heads & below);  // highest head at or below this lane
+    const unsigned long long above = heads & ~below;
+    const unsigned long long live_mask = __ballot(live);
+    const int next_head = above ? __ffsll((long long)above) - 1 : 64 - __clzll(live_mask);
+    base[k] = 0;
+    if (head) base[k] = atomicAdd(count + b[k], (unsigned)(next_head - lane));
   }
-  // runs of equal buckets: a lane heads a run when its bucket differs from the previous lane's
-  const unsigned prev = __shfl_up(b, 1);
-  const bool head = live && (lane == 0 || prev != b);
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long below = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
-  const int my_head = 63 - __clzll(heads & below);  // highest head at or below this lane
-  const unsigned long long above = heads & ~below;
-  const unsigned long long live_mask = __ballot(live);
-  const int next_head = above ? __ffsll((long long)above) - 1 : 64 - __clzll(live_mask);
-  unsigned base = 0;
-  if (head) base = atomicAdd(count + b, (unsigned)(next_head - lane));
-  base = __shfl(base, my_head);
-  if (live) {
-    bucket[i] = b;
-    rank[i] = base + (unsigned)(lane - my_head);
+#pragma unroll
+  for (int k = 0; k < kBucketPer; ++k) {
+    const long long i = i0 + 256LL * k;
+    const unsigned bb = __shfl(base[k], my_head[k] < 0 ? 0 : my_head[k]);
+    if (i < n) {
+      bucket[i] = b[k];
+      rank[i] = bb + (unsigned)(lane - my_head[k]);
+    }
   }
   // max |m|: wave, then block, then one atomic per block only when it raises
   // the running maximum (a plain read filters the common equal-mass case)
@@ -222,9 +238,13 @@ __constant__ unsigned char kRangeHi[14] = {8, 2, 4, 6, 8, 4, 8, 4, 8, 8, 6, 8, 8
 // One workgroup per tile; T is the tile side (T^3 int64 accumulators in LDS).
 // The 72 bucket offsets of the 8 source tiles are fetched in parallel, the 14
 // ranges are laid end to end, and all their particles are processed in one
-// flat loop, so the only serial latency is one offset load.
+// flat loop, so the only serial latency is one offset load.  Eight wavefronts
+// per workgroup: the LDS accumulators cap a CU at four tiles, and eight waves
+// each keep twice as many particle loads and atomics in flight as four.
+constexpr int kDepositThreads = 512;
+
 template <int T>
-__global__ __launch_bounds__(256) void k_deposit_tiles(const float4* __restrict__ sorted,
+__global__ __launch_bounds__(kDepositThreads) void k_deposit_tiles(const float4* __restrict__ sorted,
                                                        const unsigned* __restrict__ offset, const CellMap cm,
                                                        const TileMap tm, const SolveScalars* sc, long long ntot,
                                                        float* __restrict__ rho) {
@@ -233,7 +253,7 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const float4* __restrict_
   __shared__ unsigned toff[8][9];
   __shared__ unsigned rbeg[14], rpre[15];
   const int tid = threadIdx.x;
-  for (int c = tid; c < T3; c += 256) acc[c] = 0ull;
+  for (int c = tid; c < T3; c += kDepositThreads) acc[c] = 0ull;
   const int nt = tm.nt;
   const int tile = (int)tm.order[blockIdx.x];
   const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
@@ -265,9 +285,9 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const float4* __restrict_
   const unsigned total = rpre[14];
   // every lane of a wavefront runs the same number of iterations (lanes past
   // the end carry no particle), so the aggregation ballots see whole waves
-  const unsigned rounds = (total + 255u) / 256u;
+  const unsigned rounds = (total + kDepositThreads - 1u) / kDepositThreads;
   for (unsigned it = 0; it < rounds; ++it) {
-    const unsigned q = it * 256u + (unsigned)tid;
+    const unsigned q = it * kDepositThreads + (unsigned)tid;
     const bool live = q < total;
     int r = 0;
     if (live)
@@ -304,7 +324,7 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const float4* __restrict_
   __syncthreads();
   const double unit = ldexp(1.0, -s);
   const size_t nm2 = (size_t)cm.nm * cm.nm;
-  for (int c = tid; c < T3; c += 256) {
+  for (int c = tid; c < T3; c += kDepositThreads) {
     const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
     const size_t g = (size_t)(uz * T + lz) * nm2 + (size_t)(uy * T + ly) * cm.nm + (ux * T + lx);
     rho[g] = (float)((double)(long long)acc[c] * unit);

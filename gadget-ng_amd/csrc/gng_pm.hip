@@ -289,7 +289,8 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
                        reinterpret_cast<uint4*>(p->d_count), n16);
   }
   if (n > 0)
-    hipLaunchKernelGGL(k_bucket_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z, m, n, cm, p->tm,
+    hipLaunchKernelGGL(k_bucket_count, dim3((unsigned)((n + 256 * kBucketPer - 1) / (256 * kBucketPer))), dim3(256), 0,
+                       s, x, y, z, m, n, cm, p->tm,
                        p->d_count, p->d_bucket, p->d_rank, p->d_sc);
   mark(p, 1, s);
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_count, p->d_offset,
@@ -301,7 +302,7 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
                        p->d_rank, p->d_offset, p->d_sorted, p->d_perm);
   mark(p, 2, s);
   const unsigned ntiles = (unsigned)p->tm.nt * p->tm.nt * p->tm.nt;
-  o.deposit(dim3(ntiles), dim3(256), s, p->d_sorted, p->d_offset, cm, p->tm, p->d_sc, n, rho);
+  o.deposit(dim3(ntiles), dim3(kDepositThreads), s, p->d_sorted, p->d_offset, cm, p->tm, p->d_sc, n, rho);
   mark(p, 3, s);
   const long long rows = (long long)nm * nm;
   o.x_fwd(dim3((unsigned)(rows / o.x_rows)), dim3(256), s, rho, p->d_P, p->d_tw);

@@ -513,11 +513,18 @@ __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2*
   float2* a = A + off;
   float2* b = B + off;
   float2* pp = P + off;
-  float2 av[F::PER0][F::R0], w[F::PER0][F::R0], out[F::PERL][F::RL];
+  float2 av[F::PER0][F::R0], bv[F::PER0][F::R0], w[F::PER0][F::R0], out[F::PERL][F::RL];
   for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
     av[p][r] = ok ? a[(unsigned)i * H + l] : make_float2(0.f, 0.f);
-    w[p][r] = av[p][r];
   });
+  // field B is loaded now, so its load is in flight during both A transforms
+  for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    bv[p][r] = ok ? b[(unsigned)i * H + l] : make_float2(0.f, 0.f);
+  });
+#pragma unroll
+  for (int p = 0; p < F::PER0; ++p)
+#pragma unroll
+    for (int r = 0; r < F::R0; ++r) w[p][r] = av[p][r];
   F::run(w, out, buf, tw, tid);
   for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
     if (ok) a[(unsigned)i * H + l] = out[p][r];
@@ -530,10 +537,7 @@ __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2*
   for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
     if (ok) pp[(unsigned)i * H + l] = out[p][r];
   });
-  for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
-    if (ok) w[p][r] = b[(unsigned)i * H + l];
-  });
-  F::run(w, out, buf, tw, tid);
+  F::run(bv, out, buf, tw, tid);
   for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
     if (ok) b[(unsigned)i * H + l] = out[p][r];
   });
