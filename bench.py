@@ -360,6 +360,13 @@ def bench_ensemble(args) -> int:
     d = [pm.DeviceArray.from_host(a) for a in (x, y, z, m)]
     out = [pm.DeviceArray((n,), np.float32) for _ in range(3)]
     batch = pm.Batch(boxes, device=0)
+    t0 = time.perf_counter()
+    batch.autotune(reps=3)
+    t_tune = time.perf_counter() - t0
+    tuned = {}
+    for i, b in enumerate(boxes):
+        tuned[str(b["grid_size"])] = batch.tuning(i)
+    log(f"[bench] autotuned {len(tuned)} mesh sizes in {t_tune:.1f} s: {tuned}")
 
     def graph_step():
         batch.solve_device(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, off, 1.0, out[0].ptr, out[1].ptr, out[2].ptr)
@@ -418,6 +425,7 @@ def bench_ensemble(args) -> int:
                                "Zel'dovich P(k)~k^n, n in [-2.5,-1.5], seeds 0x6E670004 + box",
                    "particles": n, "meshes": {str(k): v for k, v in sorted(sizes.items())}},
         "hbm_gbps": round(b_alg / (ms * 1e-3) / 1e9, 1),
+        "autotune": {"seconds": round(t_tune, 2), "choices": tuned},
         "serial_ms_per_step": round(wall_serial * 1e3 / args.steps, 4),
         "graph_speedup": round(wall_serial / wall, 3),
         "worst_box_rms_force_err": worst,

@@ -243,8 +243,8 @@ __constant__ unsigned char kRangeHi[14] = {8, 2, 4, 6, 8, 4, 8, 4, 8, 8, 6, 8, 8
 // each keep twice as many particle loads and atomics in flight as four.
 constexpr int kDepositThreads = 512;
 
-template <int T>
-__global__ __launch_bounds__(kDepositThreads) void k_deposit_tiles(const float4* __restrict__ sorted,
+template <int T, int NT = kDepositThreads>
+__global__ __launch_bounds__(NT) void k_deposit_tiles(const float4* __restrict__ sorted,
                                                        const unsigned* __restrict__ offset, const CellMap cm,
                                                        const TileMap tm, const SolveScalars* sc, long long ntot,
                                                        float* __restrict__ rho) {
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit_tiles(const float4*
   __shared__ unsigned toff[8][9];
   __shared__ unsigned rbeg[14], rpre[15];
   const int tid = threadIdx.x;
-  for (int c = tid; c < T3; c += kDepositThreads) acc[c] = 0ull;
+  for (int c = tid; c < T3; c += NT) acc[c] = 0ull;
   const int nt = tm.nt;
   const int tile = (int)tm.order[blockIdx.x];
   const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
@@ -285,9 +285,9 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit_tiles(const float4*
   const unsigned total = rpre[14];
   // every lane of a wavefront runs the same number of iterations (lanes past
   // the end carry no particle), so the aggregation ballots see whole waves
-  const unsigned rounds = (total + kDepositThreads - 1u) / kDepositThreads;
+  const unsigned rounds = (total + NT - 1u) / NT;
   for (unsigned it = 0; it < rounds; ++it) {
-    const unsigned q = it * kDepositThreads + (unsigned)tid;
+    const unsigned q = it * NT + (unsigned)tid;
     const bool live = q < total;
     int r = 0;
     if (live)
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit_tiles(const float4*
   __syncthreads();
   const double unit = ldexp(1.0, -s);
   const size_t nm2 = (size_t)cm.nm * cm.nm;
-  for (int c = tid; c < T3; c += kDepositThreads) {
+  for (int c = tid; c < T3; c += NT) {
     const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
     const size_t g = (size_t)(uz * T + lz) * nm2 + (size_t)(uy * T + ly) * cm.nm + (ux * T + lx);
     rho[g] = (float)((double)(long long)acc[c] * unit);

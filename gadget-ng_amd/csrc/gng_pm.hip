@@ -16,7 +16,9 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <utility>
 #include <new>
@@ -70,14 +72,108 @@ constexpr int x_rows() {
 // Deposit tile side: the largest of 16, 12, 8, 4 that divides nm.
 constexpr int deposit_tile(int nm) { return nm % 16 == 0 ? 16 : nm % 12 == 0 ? 12 : nm % 8 == 0 ? 8 : 4; }
 
+// Launch choices with more than one compiled variant.  Defaults: the
+// persistent z pass at nm = 512 (elsewhere it spills or loses occupancy) and
+// 512-thread deposit workgroups; gng_pm_autotune() times the alternatives.
+struct Tuning {
+  int z_persistent = 0;
+  int deposit_wide = 1;
+  int tuned = 0;  // 1 when measured (or read from the tuning cache)
+};
+
+Tuning default_tuning(int nm) {
+  Tuning t;
+  t.z_persistent = nm == 512;
+  t.deposit_wide = 1;
+  return t;
+}
+
+// Process-wide cache of measured choices, keyed by (device architecture, nm).
+// When GNG_PM_TUNE_FILE names a file, it is read once ("arch nm z_persistent
+// deposit_wide" per line) and rewritten after every new measurement, so later
+// processes reuse the decisions.
+std::mutex g_tune_mu;
+std::map<std::pair<std::string, int>, Tuning> g_tune;
+bool g_tune_loaded = false;
+
+std::string device_arch(int dev) {
+  hipDeviceProp_t pr;
+  if (hipGetDeviceProperties(&pr, dev) != hipSuccess) return "unknown";
+  std::string a = pr.gcnArchName;
+  const size_t c = a.find(':');
+  return c == std::string::npos ? a : a.substr(0, c);
+}
+
+void load_tune_file_locked() {
+  g_tune_loaded = true;
+  const char* f = std::getenv("GNG_PM_TUNE_FILE");
+  if (!f || !*f) return;
+  FILE* fp = std::fopen(f, "r");
+  if (!fp) return;
+  char arch[64];
+  int nm, zp, dw;
+  while (std::fscanf(fp, "%63s %d %d %d", arch, &nm, &zp, &dw) == 4) {
+    Tuning t;
+    t.z_persistent = zp != 0;
+    t.deposit_wide = dw != 0;
+    t.tuned = 1;
+    g_tune[{std::string(arch), nm}] = t;
+  }
+  std::fclose(fp);
+}
+
+Tuning cached_tuning(int dev, int nm) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (!g_tune_loaded) load_tune_file_locked();
+  auto it = g_tune.find({device_arch(dev), nm});
+  return it == g_tune.end() ? default_tuning(nm) : it->second;
+}
+
+void store_tuning(int dev, int nm, const Tuning& t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (!g_tune_loaded) load_tune_file_locked();
+  g_tune[{device_arch(dev), nm}] = t;
+  const char* f = std::getenv("GNG_PM_TUNE_FILE");
+  if (!f || !*f) return;
+  FILE* fp = std::fopen(f, "w");
+  if (!fp) return;
+  for (const auto& kv : g_tune)
+    std::fprintf(fp, "%s %d %d %d\n", kv.first.first.c_str(), kv.first.second, kv.second.z_persistent,
+                 kv.second.deposit_wide);
+  std::fclose(fp);
+}
+
+// Uniform pseudo-random particles in [0, box)^3 with unit masses, for timing
+// (a 32-bit integer hash of the index per coordinate).
+__global__ void k_synth_particles(float* __restrict__ x, float* __restrict__ y, float* __restrict__ z,
+                                  float* __restrict__ m, long long n, float box) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned h = (unsigned)i * 0x9E3779B9u + 0x7F4A7C15u;
+  float c[3];
+  for (int a = 0; a < 3; ++a) {
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    c[a] = (float)(h >> 8) * (1.0f / 16777216.0f) * box;
+  }
+  x[i] = c[0] < box ? c[0] : 0.0f;
+  y[i] = c[1] < box ? c[1] : 0.0f;
+  z[i] = c[2] < box ? c[2] : 0.0f;
+  m[i] = 1.0f;
+}
+
 struct MeshOps {
   int nm;
   int x_rows;
   int yz_cols;
   int yz_threads;
   int tile;
-  void (*deposit)(dim3, dim3, hipStream_t, const float4*, const unsigned*, CellMap, TileMap, const SolveScalars*,
-                  long long, float*);
+  // deposit[v]: workgroups of 256 (v = 0) or 512 (v = 1) threads
+  void (*deposit[2])(dim3, dim3, hipStream_t, const float4*, const unsigned*, CellMap, TileMap, const SolveScalars*,
+                     long long, float*);
   void (*x_fwd)(dim3, dim3, hipStream_t, const float*, float2*, const float2*);
   void (*y_fwd)(dim3, dim3, hipStream_t, float2*, const float2*);
   void (*z_green)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
@@ -100,10 +196,15 @@ MeshOps make_ops() {
   o.yz_cols = C;
   o.yz_threads = TY;
   o.tile = TD;
-  o.deposit = [](dim3 g, dim3 b, hipStream_t s, const float4* srt, const unsigned* off, CellMap cm, TileMap tm,
-                 const SolveScalars* sc, long long n, float* rho) {
-    hipLaunchKernelGGL((k_deposit_tiles<TD>), g, b, 0, s, srt, off, cm, tm, sc, n, rho);
+  o.deposit[0] = [](dim3 g, dim3 b, hipStream_t s, const float4* srt, const unsigned* off, CellMap cm, TileMap tm,
+                    const SolveScalars* sc, long long n, float* rho) {
+    hipLaunchKernelGGL((k_deposit_tiles<TD, 256>), g, b, 0, s, srt, off, cm, tm, sc, n, rho);
   };
+  o.deposit[1] = [](dim3 g, dim3 b, hipStream_t s, const float4* srt, const unsigned* off, CellMap cm, TileMap tm,
+                    const SolveScalars* sc, long long n, float* rho) {
+    hipLaunchKernelGGL((k_deposit_tiles<TD, 512>), g, b, 0, s, srt, off, cm, tm, sc, n, rho);
+  };
+
   o.x_fwd = [](dim3 g, dim3 b, hipStream_t s, const float* rho, float2* P, const float2* tw) {
     hipLaunchKernelGGL((k_fft_x_fwd<N, R>), g, b, 0, s, rho, P, tw);
   };
@@ -150,6 +251,7 @@ struct gng_pm_plan {
   const MeshOps* ops = nullptr;
   int device = 0;
   int cus = 0;  // compute units of the device (persistent launches)
+  Tuning tune{};
   std::mutex mu;
 
   long long nm3 = 0;
@@ -302,7 +404,8 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
                        p->d_rank, p->d_offset, p->d_sorted, p->d_perm);
   mark(p, 2, s);
   const unsigned ntiles = (unsigned)p->tm.nt * p->tm.nt * p->tm.nt;
-  o.deposit(dim3(ntiles), dim3(kDepositThreads), s, p->d_sorted, p->d_offset, cm, p->tm, p->d_sc, n, rho);
+  o.deposit[p->tune.deposit_wide](dim3(ntiles), dim3(p->tune.deposit_wide ? 512 : 256), s, p->d_sorted, p->d_offset,
+                                   cm, p->tm, p->d_sc, n, rho);
   mark(p, 3, s);
   const long long rows = (long long)nm * nm;
   o.x_fwd(dim3((unsigned)(rows / o.x_rows)), dim3(256), s, rho, p->d_P, p->d_tw);
@@ -313,10 +416,8 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   mark(p, 5, s);
   const GreenParams gp = green_params(p, g);
   {
-    // persistent z pass at nm = 512, where it fits the register file of one
-    // 1024-thread workgroup per CU (other sizes spill or lose occupancy)
-    const long long items = (long long)(H / o.yz_cols) * nm;
-    if (nm == 512 && p->cus > 0 && items >= 4LL * p->cus)
+    // persistent z pass (one workgroup per CU) or one workgroup per item
+    if (p->tune.z_persistent && p->cus > 0)
       o.z_green_persist(dim3((unsigned)p->cus), yz_block, s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
     else
       o.z_green(yz_grid, yz_block, s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
@@ -508,6 +609,7 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
   int rc = set_device(p);
   if (rc) return bail(rc);
   if (hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess) p->cus = 0;
+  p->tune = cached_tuning(p->device, cfg->grid_size);
   const int nm = cfg->grid_size;
   p->nm3 = (long long)nm * nm * nm;
   p->tm.T = ops->tile;
@@ -650,6 +752,88 @@ int gng_pm_density(gng_pm_plan* p, double* out) {
   GNG_HIP(hipDeviceSynchronize());
   GNG_HIP(hipMemcpy(f.data(), p->d_rho, sizeof(float) * (size_t)p->nm3, hipMemcpyDeviceToHost));
   for (long long i = 0; i < p->nm3; ++i) out[i] = (double)f[i];
+  return GNG_OK;
+}
+
+int gng_pm_autotune(gng_pm_plan* p, int reps) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  if (reps < 1 || reps > 100) return fail(GNG_ERR_INVALID, "reps must be in [1, 100]");
+  std::lock_guard<std::mutex> lk(p->mu);
+  int rc = set_device(p);
+  if (rc) return rc;
+  const int nm = p->cfg.grid_size;
+  long long n = (long long)nm * nm * nm / 8;
+  if (n < 4096) n = 4096;
+  if ((rc = ensure_particles(p, n, n, false))) return rc;
+  const size_t cap = (size_t)p->cap;
+  float* d = p->d_part;
+  hipStream_t s = p->stream;
+  hipLaunchKernelGGL(k_synth_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, d + cap, d + 2 * cap,
+                     d + 3 * cap, n, (float)p->cfg.box_size);
+  GNG_HIP(hipGetLastError());
+  const bool timing = p->timing;
+  p->timing = false;
+  hipEvent_t e0, e1;
+  GNG_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(GNG_ERR_HIP, "event creation failed");
+  }
+  Tuning best = p->tune;
+  float best_ms = 1e30f;
+  rc = GNG_OK;
+  for (int zp = 0; zp < 2 && rc == GNG_OK; ++zp)
+    for (int dw = 0; dw < 2 && rc == GNG_OK; ++dw) {
+      p->tune.z_persistent = zp;
+      p->tune.deposit_wide = dw;
+      rc = run_pipeline(p, d, d + cap, d + 2 * cap, d + 3 * cap, n, nullptr, n, 1.0, d + 4 * cap, d + 5 * cap,
+                        d + 6 * cap, s);  // warm-up
+      float t_min = 1e30f;
+      for (int r = 0; r < reps && rc == GNG_OK; ++r) {
+        if (hipEventRecord(e0, s) != hipSuccess) rc = fail(GNG_ERR_HIP, "event record failed");
+        if (rc == GNG_OK)
+          rc = run_pipeline(p, d, d + cap, d + 2 * cap, d + 3 * cap, n, nullptr, n, 1.0, d + 4 * cap, d + 5 * cap,
+                            d + 6 * cap, s);
+        float ms = 0.0f;
+        if (rc == GNG_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          rc = fail(GNG_ERR_HIP, "event timing failed");
+        if (rc == GNG_OK && ms < t_min) t_min = ms;
+      }
+      if (rc == GNG_OK && t_min < best_ms) {
+        best_ms = t_min;
+        best = p->tune;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p->timing = timing;
+  p->density_valid = false;
+  p->potential_valid = false;
+  if (rc != GNG_OK) {
+    p->tune = cached_tuning(p->device, nm);
+    return rc;
+  }
+  best.tuned = 1;
+  p->tune = best;
+  store_tuning(p->device, nm, best);
+  return GNG_OK;
+}
+
+int gng_pm_set_tuning(gng_pm_plan* p, int z_persistent, int deposit_wide) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  std::lock_guard<std::mutex> lk(p->mu);
+  p->tune.z_persistent = z_persistent != 0;
+  p->tune.deposit_wide = deposit_wide != 0;
+  p->tune.tuned = 0;
+  return GNG_OK;
+}
+
+int gng_pm_tuning(const gng_pm_plan* p, int* z_persistent, int* deposit_wide, int* tuned) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  if (z_persistent) *z_persistent = p->tune.z_persistent;
+  if (deposit_wide) *deposit_wide = p->tune.deposit_wide;
+  if (tuned) *tuned = p->tune.tuned;
   return GNG_OK;
 }
 
@@ -811,6 +995,33 @@ int gng_pm_batch_size(const gng_pm_batch* b) { return b ? (int)b->plans.size() :
 int gng_pm_batch_plan(gng_pm_batch* b, int i, gng_pm_plan** out) {
   if (!b || !out || i < 0 || i >= (int)b->plans.size()) return fail(GNG_ERR_INVALID, "bad batch index");
   *out = b->plans[i];
+  return GNG_OK;
+}
+
+int gng_pm_batch_autotune(gng_pm_batch* b, int reps) {
+  if (!b) return fail(GNG_ERR_INVALID, "null batch");
+  std::lock_guard<std::mutex> lk(b->mu);
+  // one measurement per distinct mesh side, shared by the boxes of that side
+  std::map<int, Tuning> done;
+  for (gng_pm_plan* p : b->plans) {
+    const int nm = p->cfg.grid_size;
+    auto it = done.find(nm);
+    if (it != done.end()) {
+      std::lock_guard<std::mutex> pl(p->mu);
+      p->tune = it->second;
+      continue;
+    }
+    const int rc = gng_pm_autotune(p, reps);
+    if (rc != GNG_OK) return rc;
+    done[nm] = p->tune;
+  }
+  // the captured graph holds the old launch choices
+  (void)hipSetDevice(b->device);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->key.clear();
   return GNG_OK;
 }
 

@@ -128,6 +128,12 @@ def lib():
         L.gng_pm_batch_solve_device.restype = i32
         L.gng_hilbert_key3.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]
         L.gng_hilbert_key3.restype = ctypes.c_uint64
+        L.gng_pm_autotune.argtypes = [P, i32]
+        L.gng_pm_tuning.argtypes = [P, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.gng_pm_batch_autotune.argtypes = [P, i32]
+        L.gng_pm_set_tuning.argtypes = [P, i32, i32]
+        for f in (L.gng_pm_autotune, L.gng_pm_tuning, L.gng_pm_batch_autotune, L.gng_pm_set_tuning):
+            f.restype = i32
         L.gng_pm_keep_potential.argtypes = [P, i32]
         L.gng_pm_potential.argtypes = [P, P]
         L.gng_pm_device_meshes.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
@@ -317,6 +323,22 @@ class Plan:
         self._check(lib().gng_pm_density(self._h, _ptr(out)), "gng_pm_density")
         return out
 
+    def autotune(self, reps: int = 5) -> dict:
+        """Time the compiled launch variants on synthetic particles and keep the fastest."""
+        self._check(lib().gng_pm_autotune(self._h, int(reps)), "gng_pm_autotune")
+        return self.tuning()
+
+    def set_tuning(self, z_persistent: bool, deposit_threads: int) -> None:
+        if deposit_threads not in (256, 512):
+            raise ValueError("deposit_threads must be 256 or 512")
+        self._check(lib().gng_pm_set_tuning(self._h, int(bool(z_persistent)), int(deposit_threads == 512)),
+                    "gng_pm_set_tuning")
+
+    def tuning(self) -> dict:
+        z, d, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(lib().gng_pm_tuning(self._h, ctypes.byref(z), ctypes.byref(d), ctypes.byref(t)), "gng_pm_tuning")
+        return {"z_persistent": bool(z.value), "deposit_threads": 512 if d.value else 256, "tuned": bool(t.value)}
+
     def potential(self) -> np.ndarray:
         """Mesh potential of the last solve (needs keep_potential=True)."""
         nm = self.grid_size
@@ -489,6 +511,22 @@ class Batch:
             raise HipExecutionError("gng_pm_batch_create", rc, last_error())
         self._h = h
         self.n_boxes = len(boxes)
+
+    def autotune(self, reps: int = 3) -> None:
+        rc = lib().gng_pm_batch_autotune(self._h, int(reps))
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_pm_batch_autotune", rc, last_error())
+
+    def tuning(self, i: int) -> dict:
+        p = ctypes.c_void_p()
+        rc = lib().gng_pm_batch_plan(self._h, int(i), ctypes.byref(p))
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_pm_batch_plan", rc, last_error())
+        z, d, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        rc = lib().gng_pm_tuning(p, ctypes.byref(z), ctypes.byref(d), ctypes.byref(t))
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_pm_tuning", rc, last_error())
+        return {"z_persistent": bool(z.value), "deposit_threads": 512 if d.value else 256, "tuned": bool(t.value)}
 
     def solve_device(self, x: int, y: int, z: int, m: int, offsets, g: float, ax: int, ay: int, az: int,
                      stream: int = 0):

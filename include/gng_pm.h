@@ -139,6 +139,23 @@ int gng_pm_density(gng_pm_plan *p, double *out);
  * (crates/gadget-ng-pm/src/fft_poisson.rs:66-122).  Synchronous. */
 int gng_pm_force_meshes(gng_pm_plan *p, float *out);
 
+/* Launch-choice autotuner.  Some stages have more than one compiled variant
+ * (the z + Green pass: one workgroup per column item, or one persistent
+ * workgroup per CU with next-item prefetch; the deposit: 256- or 512-thread
+ * tile workgroups).  gng_pm_autotune times every combination on `reps` whole
+ * solves of nm^3/8 synthetic uniform particles on the plan's own buffers and
+ * keeps the fastest for this plan and for plans created later in the process
+ * with the same device architecture and mesh side.  With the environment
+ * variable GNG_PM_TUNE_FILE set, decisions are also read from and written to
+ * that file.  Untuned plans use the persistent z pass at nm = 512 and the
+ * 512-thread deposit.  No reference counterpart.  Synchronous; the density and
+ * potential of earlier solves are invalidated. */
+int gng_pm_autotune(gng_pm_plan *p, int reps);
+int gng_pm_tuning(const gng_pm_plan *p, int *z_persistent, int *deposit_wide, int *tuned);
+/* Sets the choices explicitly (for reproducible runs and for testing every
+ * variant); the plan then reports tuned = 0. */
+int gng_pm_set_tuning(gng_pm_plan *p, int z_persistent, int deposit_wide);
+
 /* Keep the mesh potential of the following solves (one extra nm^3 f32 buffer
  * and one extra x pass).  The potential is the field whose gradient the
  * solve returns, Phi_hat = -4 pi G W rho_hat / k^2 with the same filters and
@@ -187,6 +204,9 @@ int gng_pm_batch_plan(gng_pm_batch *b, int i, gng_pm_plan **out);
  * [offsets[i], offsets[i+1]) of the concatenated SoA f32 device arrays
  * (offsets is a host array of n_boxes + 1 values); accelerations are written
  * at the same entries.  Enqueued on `stream`; returns without synchronising. */
+/* Autotunes one plan per distinct mesh side and applies the choice to every
+ * box of that side; the next solve captures a new graph. */
+int gng_pm_batch_autotune(gng_pm_batch *b, int reps);
 int gng_pm_batch_solve_device(gng_pm_batch *b, const float *d_x, const float *d_y, const float *d_z,
                               const float *d_mass, const int64_t *offsets, double g, float *d_ax,
                               float *d_ay, float *d_az, void *stream);

@@ -1,0 +1,59 @@
+"""Every compiled launch variant gives the oracle's forces; the autotuner
+picks one of them and the choice carries over to new plans."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import rms_rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("nm", [8, 16, 24, 64, 96, 128, 256])
+def test_every_variant_matches_oracle(pm, oracle_mod, nm):
+    print(f"case variants nm={nm}", flush=True)
+    g = np.random.default_rng(nm)
+    L = 10.0
+    n = min(nm ** 3 // 4, 400000)
+    pos = g.uniform(0, L, (n, 3)).astype(np.float32).astype(np.float64)
+    m = g.uniform(0.5, 1.5, n)
+    ref = oracle_mod.pm_accelerations(pos, m, L, nm, r_split=0.3)
+    plan = pm.Plan(nm, L, r_split=0.3, device=0)
+    for zp in (False, True):
+        for dt in (256, 512):
+            plan.set_tuning(zp, dt)
+            got = plan.accelerations_f64(pos, m, 1.0)
+            assert rms_rel(got, ref) < 1e-5, (zp, dt)
+            assert plan.tuning() == {"z_persistent": zp, "deposit_threads": dt, "tuned": False}
+
+
+def test_autotune_picks_a_variant_and_new_plans_reuse_it(pm, oracle_mod, tmp_path):
+    nm, L = 64, 20.0
+    plan = pm.Plan(nm, L, device=0)
+    t = plan.autotune(reps=2)
+    assert t["tuned"]
+    again = pm.Plan(nm, L, device=0).tuning()
+    assert again == t
+    g = np.random.default_rng(3)
+    pos = g.uniform(0, L, (5000, 3)).astype(np.float32).astype(np.float64)
+    m = np.ones(5000)
+    assert rms_rel(plan.accelerations_f64(pos, m, 1.0), oracle_mod.pm_accelerations(pos, m, L, nm)) < 1e-5
+
+
+def test_batch_autotune_recaptures(pm, oracle_mod):
+    from gadget_ng_pm_hip import workloads
+    boxes, off, x, y, z, m = workloads.ensemble(n_boxes=4, sides=(8, 16), workers=-1)
+    batch = pm.Batch(boxes)
+    d = [pm.DeviceArray.from_host(a) for a in (x, y, z, m)]
+    out = [pm.DeviceArray((x.shape[0],), np.float32) for _ in range(3)]
+    batch.solve_device(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, off, 1.0, out[0].ptr, out[1].ptr, out[2].ptr)
+    batch.autotune(reps=1)
+    assert all(batch.tuning(i)["tuned"] for i in range(len(boxes)))
+    batch.solve_device(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, off, 1.0, out[0].ptr, out[1].ptr, out[2].ptr)
+    pm.device_synchronize()
+    a = np.stack([o.numpy() for o in out], axis=1).astype(np.float64)
+    for b, bx in enumerate(boxes):
+        p = bx["particles"]
+        ref = oracle_mod.pm_accelerations(p.pos64(), p.mass64(), bx["box_size"], bx["grid_size"], 1.0)
+        assert rms_rel(a[off[b]:off[b + 1]], ref) <= 1e-5
