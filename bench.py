@@ -347,7 +347,7 @@ def main() -> int:
 
 
 def bench_ensemble(args) -> int:
-    """64 independent boxes of uneven size solved together
+    """Ensemble workload: 64 independent boxes of uneven size solved together
     through one captured hipGraph; compared with the same boxes solved one
     after another on one stream, and checked per box against the FP64 oracle."""
     import numpy as np
@@ -417,7 +417,7 @@ def bench_ensemble(args) -> int:
     for b in boxes:
         sizes[b["grid_size"]] = sizes.get(b["grid_size"], 0) + 1
     result = {
-        "metric": "M particle-forces/s, ensemble of 64 uneven periodic boxes (config 4) in one hipGraph",
+        "metric": "M particle-forces/s, ensemble of 64 uneven periodic boxes (32^3..128^3 particles) in one hipGraph",
         "value": round(n * args.steps / wall / 1e6, 2), "unit": "M particle-forces/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -439,7 +439,7 @@ NET_FLOP_PER_CELL = 2 * 27 * (2 * 16 + 16 * 16 + 16 * 3)  # 18 144
 
 
 def bench_learned(args) -> int:
-    """Learned sub-cell correction (config-5 recipe): train on seeded 64^3
+    """Learned sub-cell correction (clustered boxes at one particle per cell): train on seeded 64^3
     boxes made by the method itself, evaluate on held-out 256^3 boxes against
     TreePM (long range + exact short range), and time the network against the
     short-range sum it stands in for."""
@@ -489,7 +489,7 @@ def bench_learned(args) -> int:
         "unit": "relative rms force error",
         "n_gpus": 1, "steps": args.train_steps, "warmup": 0, "ms_per_step": round(t_train * 1e3 / args.train_steps, 4),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
-        "config": {"workload": "config-5 recipe: 70% Zel'dovich lattice + 30% in Plummer spheres (a = 0.2 Mpc/h), one "
+        "config": {"workload": "clustered boxes: 70% Zel'dovich lattice + 30% in Plummer spheres (a = 0.2 Mpc/h), one "
                                "particle per (Mpc/h)^3; train on 64^3 boxes (seeds 1..N), evaluate on 256^3 boxes "
                                "(seeds 1001..)",
                    "train_boxes": args.train_boxes, "eval_boxes": args.eval_boxes,
