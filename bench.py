@@ -94,6 +94,7 @@ def main() -> int:
     ap.add_argument("--mesh", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the FP64 CPU oracle (and the error check)")
     ap.add_argument("--cpu-runs", type=int, default=2, help="timed CPU oracle runs after one warm-up")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch choices")
     ap.add_argument("--no-fp64-form", action="store_true",
                     help="skip timing the FP64 GPU form of the reference algorithm")
     ap.add_argument("--train-boxes", type=int, default=200, help="learned: training seeds 1..N (64^3 boxes)")
@@ -144,6 +145,9 @@ def main() -> int:
     x, y, z, m = (pm.DeviceArray.from_host(a) for a in (parts.x, parts.y, parts.z, parts.m))
     ax, ay, az = (pm.DeviceArray((n,), np.float32) for _ in range(3))
     plan = pm.Plan(nm, box, r_split=r_split, device=0)
+    t1 = time.perf_counter()
+    tuning = plan.autotune(reps=3) if not args.no_autotune else plan.tuning()
+    log(f"[bench] launch choices ({time.perf_counter() - t1:.2f} s): {tuning}")
 
     def step():
         plan.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, ax.ptr, ay.ptr, az.ptr, 1.0)
@@ -212,7 +216,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl_desc, "particles": n, "mesh": nm, "box": box, "r_split": r_split,
-                   "precision": "FP32 mesh/FFT/interp, int64 deposit"},
+                   "precision": "FP32 mesh/FFT/interp, int64 deposit", "launch_choices": tuning},
         "roofline": roofline,
         "hbm_gbps": round(pipeline_gbps, 1),
         "hbm_frac": round(pipeline_gbps / HBM_PEAK_GBPS, 4),

@@ -78,6 +78,8 @@ constexpr int deposit_tile(int nm) { return nm % 16 == 0 ? 16 : nm % 12 == 0 ? 1
 struct Tuning {
   int z_persistent = 0;
   int deposit_wide = 1;
+  int y_fwd_persistent = 0;
+  int y_inv_persistent = 0;
   int tuned = 0;  // 1 when measured (or read from the tuning cache)
 };
 
@@ -88,10 +90,15 @@ Tuning default_tuning(int nm) {
   return t;
 }
 
+constexpr int kKnobs = 4;
+int& knob(Tuning& t, int k) {
+  return k == 0 ? t.z_persistent : k == 1 ? t.deposit_wide : k == 2 ? t.y_fwd_persistent : t.y_inv_persistent;
+}
+
 // Process-wide cache of measured choices, keyed by (device architecture, nm).
 // When GNG_PM_TUNE_FILE names a file, it is read once ("arch nm z_persistent
-// deposit_wide" per line) and rewritten after every new measurement, so later
-// processes reuse the decisions.
+// deposit_wide y_fwd_persistent y_inv_persistent" per line) and rewritten after
+// every new measurement, so later processes reuse the decisions.
 std::mutex g_tune_mu;
 std::map<std::pair<std::string, int>, Tuning> g_tune;
 bool g_tune_loaded = false;
@@ -111,11 +118,10 @@ void load_tune_file_locked() {
   FILE* fp = std::fopen(f, "r");
   if (!fp) return;
   char arch[64];
-  int nm, zp, dw;
-  while (std::fscanf(fp, "%63s %d %d %d", arch, &nm, &zp, &dw) == 4) {
+  int nm, v[kKnobs];
+  while (std::fscanf(fp, "%63s %d %d %d %d %d", arch, &nm, &v[0], &v[1], &v[2], &v[3]) == 2 + kKnobs) {
     Tuning t;
-    t.z_persistent = zp != 0;
-    t.deposit_wide = dw != 0;
+    for (int k = 0; k < kKnobs; ++k) knob(t, k) = v[k] != 0;
     t.tuned = 1;
     g_tune[{std::string(arch), nm}] = t;
   }
@@ -138,8 +144,8 @@ void store_tuning(int dev, int nm, const Tuning& t) {
   FILE* fp = std::fopen(f, "w");
   if (!fp) return;
   for (const auto& kv : g_tune)
-    std::fprintf(fp, "%s %d %d %d\n", kv.first.first.c_str(), kv.first.second, kv.second.z_persistent,
-                 kv.second.deposit_wide);
+    std::fprintf(fp, "%s %d %d %d %d %d\n", kv.first.first.c_str(), kv.first.second, kv.second.z_persistent,
+                 kv.second.deposit_wide, kv.second.y_fwd_persistent, kv.second.y_inv_persistent);
   std::fclose(fp);
 }
 
@@ -179,6 +185,8 @@ struct MeshOps {
   void (*z_green)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
   void (*z_green_col0)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
   void (*z_green_persist)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
+  void (*y_fwd_persist)(dim3, dim3, hipStream_t, float2*, const float2*);
+  void (*y_inv_persist)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
   void (*y_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
   void (*x_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, float2*, const float2*, int, float);
 };
@@ -218,6 +226,12 @@ MeshOps make_ops() {
   o.z_green_persist = [](dim3 g, dim3 b, hipStream_t s, const float2* P, float2* A, float2* B, const float2* tw,
                          GreenParams gp) {
     hipLaunchKernelGGL((k_fft_z_green_persist<N, C, TY>), g, b, 0, s, P, A, B, tw, gp);
+  };
+  o.y_fwd_persist = [](dim3 g, dim3 b, hipStream_t s, float2* P, const float2* tw) {
+    hipLaunchKernelGGL((k_fft_y_fwd_persist<N, C, TY>), g, b, 0, s, P, tw);
+  };
+  o.y_inv_persist = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* B, float2* P, const float2* tw) {
+    hipLaunchKernelGGL((k_fft_y_inv_persist<N, C, TY>), g, b, 0, s, A, B, P, tw);
   };
   o.y_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* B, float2* P, const float2* tw) {
     hipLaunchKernelGGL((k_fft_y_inv<N, C, TY>), g, b, 0, s, A, B, P, tw);
@@ -412,7 +426,10 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   mark(p, 4, s);
   const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
   const dim3 yz_block((unsigned)o.yz_threads);
-  o.y_fwd(yz_grid, yz_block, s, p->d_P, p->d_tw);
+  if (p->tune.y_fwd_persistent && p->cus > 0)
+    o.y_fwd_persist(dim3((unsigned)p->cus), yz_block, s, p->d_P, p->d_tw);
+  else
+    o.y_fwd(yz_grid, yz_block, s, p->d_P, p->d_tw);
   mark(p, 5, s);
   const GreenParams gp = green_params(p, g);
   {
@@ -424,7 +441,10 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   }
   o.z_green_col0(dim3((unsigned)(H + 1)), dim3(256), s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
   mark(p, 6, s);
-  o.y_inv(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
+  if (p->tune.y_inv_persistent && p->cus > 0)
+    o.y_inv_persist(dim3((unsigned)p->cus), yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
+  else
+    o.y_inv(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
   mark(p, 7, s);
   if (p->keep_potential)
     // the potential reads A before the force pass below overwrites it
@@ -779,32 +799,41 @@ int gng_pm_autotune(gng_pm_plan* p, int reps) {
     (void)hipEventDestroy(e0);
     return fail(GNG_ERR_HIP, "event creation failed");
   }
+  // time one configuration: best of `reps` solves after a warm-up
+  auto measure = [&](float& out_ms) -> int {
+    int r2 = run_pipeline(p, d, d + cap, d + 2 * cap, d + 3 * cap, n, nullptr, n, 1.0, d + 4 * cap, d + 5 * cap,
+                          d + 6 * cap, s);
+    float t_min = 1e30f;
+    for (int r = 0; r < reps && r2 == GNG_OK; ++r) {
+      if (hipEventRecord(e0, s) != hipSuccess) r2 = fail(GNG_ERR_HIP, "event record failed");
+      if (r2 == GNG_OK)
+        r2 = run_pipeline(p, d, d + cap, d + 2 * cap, d + 3 * cap, n, nullptr, n, 1.0, d + 4 * cap, d + 5 * cap,
+                          d + 6 * cap, s);
+      float ms = 0.0f;
+      if (r2 == GNG_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        r2 = fail(GNG_ERR_HIP, "event timing failed");
+      if (r2 == GNG_OK && ms < t_min) t_min = ms;
+    }
+    out_ms = t_min;
+    return r2;
+  };
+  // coordinate descent from the defaults: each knob in turn keeps its faster value
+  p->tune = default_tuning(nm);
   Tuning best = p->tune;
   float best_ms = 1e30f;
-  rc = GNG_OK;
-  for (int zp = 0; zp < 2 && rc == GNG_OK; ++zp)
-    for (int dw = 0; dw < 2 && rc == GNG_OK; ++dw) {
-      p->tune.z_persistent = zp;
-      p->tune.deposit_wide = dw;
-      rc = run_pipeline(p, d, d + cap, d + 2 * cap, d + 3 * cap, n, nullptr, n, 1.0, d + 4 * cap, d + 5 * cap,
-                        d + 6 * cap, s);  // warm-up
-      float t_min = 1e30f;
-      for (int r = 0; r < reps && rc == GNG_OK; ++r) {
-        if (hipEventRecord(e0, s) != hipSuccess) rc = fail(GNG_ERR_HIP, "event record failed");
-        if (rc == GNG_OK)
-          rc = run_pipeline(p, d, d + cap, d + 2 * cap, d + 3 * cap, n, nullptr, n, 1.0, d + 4 * cap, d + 5 * cap,
-                            d + 6 * cap, s);
-        float ms = 0.0f;
-        if (rc == GNG_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-          rc = fail(GNG_ERR_HIP, "event timing failed");
-        if (rc == GNG_OK && ms < t_min) t_min = ms;
-      }
-      if (rc == GNG_OK && t_min < best_ms) {
-        best_ms = t_min;
-        best = p->tune;
-      }
+  rc = measure(best_ms);
+  for (int k = 0; k < kKnobs && rc == GNG_OK; ++k) {
+    Tuning trial = best;
+    knob(trial, k) = !knob(trial, k);
+    p->tune = trial;
+    float ms = 1e30f;
+    rc = measure(ms);
+    if (rc == GNG_OK && ms < best_ms) {
+      best_ms = ms;
+      best = trial;
     }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   p->timing = timing;
@@ -820,20 +849,25 @@ int gng_pm_autotune(gng_pm_plan* p, int reps) {
   return GNG_OK;
 }
 
-int gng_pm_set_tuning(gng_pm_plan* p, int z_persistent, int deposit_wide) {
-  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+int gng_pm_set_tuning(gng_pm_plan* p, const gng_pm_tuning_t* t) {
+  if (!p || !t) return fail(GNG_ERR_INVALID, "null argument");
   std::lock_guard<std::mutex> lk(p->mu);
-  p->tune.z_persistent = z_persistent != 0;
-  p->tune.deposit_wide = deposit_wide != 0;
+  p->tune.z_persistent = t->z_persistent != 0;
+  p->tune.deposit_wide = t->deposit_wide != 0;
+  p->tune.y_fwd_persistent = t->y_fwd_persistent != 0;
+  p->tune.y_inv_persistent = t->y_inv_persistent != 0;
   p->tune.tuned = 0;
   return GNG_OK;
 }
 
-int gng_pm_tuning(const gng_pm_plan* p, int* z_persistent, int* deposit_wide, int* tuned) {
-  if (!p) return fail(GNG_ERR_INVALID, "null plan");
-  if (z_persistent) *z_persistent = p->tune.z_persistent;
-  if (deposit_wide) *deposit_wide = p->tune.deposit_wide;
-  if (tuned) *tuned = p->tune.tuned;
+int gng_pm_get_tuning(const gng_pm_plan* p, gng_pm_tuning_t* t) {
+  if (!p || !t) return fail(GNG_ERR_INVALID, "null argument");
+  std::memset(t, 0, sizeof(*t));
+  t->z_persistent = p->tune.z_persistent;
+  t->deposit_wide = p->tune.deposit_wide;
+  t->y_fwd_persistent = p->tune.y_fwd_persistent;
+  t->y_inv_persistent = p->tune.y_inv_persistent;
+  t->tuned = p->tune.tuned;
   return GNG_OK;
 }
 

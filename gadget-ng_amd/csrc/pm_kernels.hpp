@@ -543,6 +543,116 @@ __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2*
   });
 }
 
+// Persistent forms of the y passes: gridDim.x workgroups walk the (column
+// tile, z plane) items in a grid-stride loop and load the next item's field
+// while the current item is transformed.  In the inverse pass the current
+// item's A values wait in LDS (each thread's own slots) for the -i n_y field,
+// which leaves registers for B and for the next item's A.
+template <int N, int C, int T>
+__global__ __launch_bounds__(T) void k_fft_y_fwd_persist(float2* __restrict__ P, const float2* __restrict__ tw_g) {
+  constexpr int H = N / 2, IS = C + 1, TILES = H / C;
+  using F = RegFft<N, false, false, C, T, 1, IS, true, N>;
+  __shared__ float2 buf[N * IS];
+  const int tid = threadIdx.x;
+  __shared__ float2 twl[N];
+  for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
+  const float2* tw = twl;
+  const int items = TILES * N;
+  float2 vin[F::PER0][F::R0];
+  int item = blockIdx.x;
+  if (item < items) {
+    const float2* b0 = P + (size_t)(item / TILES) * N * H + (size_t)(item % TILES) * C;
+    for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      if (ok) vin[p][r] = b0[(unsigned)i * H + l];
+    });
+  }
+  for (; item < items; item += gridDim.x) {
+    float2* base = P + (size_t)(item / TILES) * N * H + (size_t)(item % TILES) * C;
+    float2 vout[F::PERL][F::RL];
+    F::run(vin, vout, buf, tw, tid);
+    const int next = item + (int)gridDim.x;
+    if (next < items) {
+      const float2* b1 = P + (size_t)(next / TILES) * N * H + (size_t)(next % TILES) * C;
+      for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        if (ok) vin[p][r] = b1[(unsigned)i * H + l];
+      });
+    }
+    for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      if (ok) base[(unsigned)i * H + l] = vout[p][r];
+    });
+  }
+}
+
+template <int N, int C, int T>
+__global__ __launch_bounds__(T) void k_fft_y_inv_persist(float2* __restrict__ A, float2* __restrict__ B,
+                                                         float2* __restrict__ P, const float2* __restrict__ tw_g) {
+  constexpr int H = N / 2, IS = C + 1, TILES = H / C;
+  using F = RegFft<N, true, false, C, T, 1, IS, true, N>;
+  __shared__ float2 buf[N * IS];
+  constexpr int KEEP = F::PER0 * F::R0;
+  __shared__ float2 keep[KEEP * T];
+  const int tid = threadIdx.x;
+  __shared__ float2 twl[N];
+  for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
+  const float2* tw = twl;
+  const int items = TILES * N;
+  float2 av[F::PER0][F::R0];
+  int item = blockIdx.x;
+  if (item < items) {
+    const float2* a0 = A + (size_t)(item / TILES) * N * H + (size_t)(item % TILES) * C;
+    for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      av[p][r] = ok ? a0[(unsigned)i * H + l] : make_float2(0.f, 0.f);
+    });
+  }
+  for (; item < items; item += gridDim.x) {
+    const size_t off = (size_t)(item / TILES) * N * H + (size_t)(item % TILES) * C;
+    float2* a = A + off;
+    float2* b = B + off;
+    float2* pp = P + off;
+    float2 bv[F::PER0][F::R0];
+    for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      bv[p][r] = ok ? b[(unsigned)i * H + l] : make_float2(0.f, 0.f);
+    });
+#pragma unroll
+    for (int p = 0; p < F::PER0; ++p)
+#pragma unroll
+      for (int r = 0; r < F::R0; ++r) keep[(p * F::R0 + r) * T + tid] = av[p][r];
+    {
+      float2 out[F::PERL][F::RL];
+      F::run(av, out, buf, tw, tid);
+      const int next = item + (int)gridDim.x;
+      if (next < items) {
+        const float2* a1 = A + (size_t)(next / TILES) * N * H + (size_t)(next % TILES) * C;
+        for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+          av[p][r] = ok ? a1[(unsigned)i * H + l] : make_float2(0.f, 0.f);
+        });
+      }
+      for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        if (ok) a[(unsigned)i * H + l] = out[p][r];
+      });
+    }
+    {
+      float2 w[F::PER0][F::R0], out[F::PERL][F::RL];
+      for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        const float nye = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
+        const float2 v = keep[(p * F::R0 + r) * T + tid];
+        w[p][r] = make_float2(nye * v.y, -nye * v.x);
+      });
+      F::run(w, out, buf, tw, tid);
+      for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        if (ok) pp[(unsigned)i * H + l] = out[p][r];
+      });
+    }
+    {
+      float2 out[F::PERL][F::RL];
+      F::run(bv, out, buf, tw, tid);
+      for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        if (ok) b[(unsigned)i * H + l] = out[p][r];
+      });
+    }
+  }
+}
+
 // CIC gather of (Fx, Fy, Fz) at pos[idx[j]] (idx null: j itself).
 __global__ __launch_bounds__(256) void k_interp(const float* __restrict__ x, const float* __restrict__ y,
                                                 const float* __restrict__ z, const long long* __restrict__ idx,

@@ -73,6 +73,17 @@ class GngConfig(ctypes.Structure):
                 ("reserved", ctypes.c_int32 * 7)]
 
 
+class GngTuning(ctypes.Structure):
+    _fields_ = [("z_persistent", ctypes.c_int32), ("deposit_wide", ctypes.c_int32),
+                ("y_fwd_persistent", ctypes.c_int32), ("y_inv_persistent", ctypes.c_int32),
+                ("tuned", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+
+    def as_dict(self) -> dict:
+        return {"z_persistent": bool(self.z_persistent), "deposit_threads": 512 if self.deposit_wide else 256,
+                "y_fwd_persistent": bool(self.y_fwd_persistent), "y_inv_persistent": bool(self.y_inv_persistent),
+                "tuned": bool(self.tuned)}
+
+
 _lib = None
 
 
@@ -129,10 +140,10 @@ def lib():
         L.gng_hilbert_key3.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]
         L.gng_hilbert_key3.restype = ctypes.c_uint64
         L.gng_pm_autotune.argtypes = [P, i32]
-        L.gng_pm_tuning.argtypes = [P, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.gng_pm_get_tuning.argtypes = [P, ctypes.POINTER(GngTuning)]
         L.gng_pm_batch_autotune.argtypes = [P, i32]
-        L.gng_pm_set_tuning.argtypes = [P, i32, i32]
-        for f in (L.gng_pm_autotune, L.gng_pm_tuning, L.gng_pm_batch_autotune, L.gng_pm_set_tuning):
+        L.gng_pm_set_tuning.argtypes = [P, ctypes.POINTER(GngTuning)]
+        for f in (L.gng_pm_autotune, L.gng_pm_get_tuning, L.gng_pm_batch_autotune, L.gng_pm_set_tuning):
             f.restype = i32
         L.gng_pm_keep_potential.argtypes = [P, i32]
         L.gng_pm_potential.argtypes = [P, P]
@@ -328,16 +339,21 @@ class Plan:
         self._check(lib().gng_pm_autotune(self._h, int(reps)), "gng_pm_autotune")
         return self.tuning()
 
-    def set_tuning(self, z_persistent: bool, deposit_threads: int) -> None:
+    def set_tuning(self, z_persistent: bool = False, deposit_threads: int = 512, y_fwd_persistent: bool = False,
+                   y_inv_persistent: bool = False) -> None:
         if deposit_threads not in (256, 512):
             raise ValueError("deposit_threads must be 256 or 512")
-        self._check(lib().gng_pm_set_tuning(self._h, int(bool(z_persistent)), int(deposit_threads == 512)),
-                    "gng_pm_set_tuning")
+        t = GngTuning()
+        t.z_persistent = int(bool(z_persistent))
+        t.deposit_wide = int(deposit_threads == 512)
+        t.y_fwd_persistent = int(bool(y_fwd_persistent))
+        t.y_inv_persistent = int(bool(y_inv_persistent))
+        self._check(lib().gng_pm_set_tuning(self._h, ctypes.byref(t)), "gng_pm_set_tuning")
 
     def tuning(self) -> dict:
-        z, d, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        self._check(lib().gng_pm_tuning(self._h, ctypes.byref(z), ctypes.byref(d), ctypes.byref(t)), "gng_pm_tuning")
-        return {"z_persistent": bool(z.value), "deposit_threads": 512 if d.value else 256, "tuned": bool(t.value)}
+        t = GngTuning()
+        self._check(lib().gng_pm_get_tuning(self._h, ctypes.byref(t)), "gng_pm_get_tuning")
+        return t.as_dict()
 
     def potential(self) -> np.ndarray:
         """Mesh potential of the last solve (needs keep_potential=True)."""
@@ -522,11 +538,11 @@ class Batch:
         rc = lib().gng_pm_batch_plan(self._h, int(i), ctypes.byref(p))
         if rc != GNG_OK:
             raise HipExecutionError("gng_pm_batch_plan", rc, last_error())
-        z, d, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        rc = lib().gng_pm_tuning(p, ctypes.byref(z), ctypes.byref(d), ctypes.byref(t))
+        t = GngTuning()
+        rc = lib().gng_pm_get_tuning(p, ctypes.byref(t))
         if rc != GNG_OK:
-            raise HipExecutionError("gng_pm_tuning", rc, last_error())
-        return {"z_persistent": bool(z.value), "deposit_threads": 512 if d.value else 256, "tuned": bool(t.value)}
+            raise HipExecutionError("gng_pm_get_tuning", rc, last_error())
+        return t.as_dict()
 
     def solve_device(self, x: int, y: int, z: int, m: int, offsets, g: float, ax: int, ay: int, az: int,
                      stream: int = 0):

@@ -139,22 +139,32 @@ int gng_pm_density(gng_pm_plan *p, double *out);
  * (crates/gadget-ng-pm/src/fft_poisson.rs:66-122).  Synchronous. */
 int gng_pm_force_meshes(gng_pm_plan *p, float *out);
 
-/* Launch-choice autotuner.  Some stages have more than one compiled variant
- * (the z + Green pass: one workgroup per column item, or one persistent
- * workgroup per CU with next-item prefetch; the deposit: 256- or 512-thread
- * tile workgroups).  gng_pm_autotune times every combination on `reps` whole
- * solves of nm^3/8 synthetic uniform particles on the plan's own buffers and
- * keeps the fastest for this plan and for plans created later in the process
- * with the same device architecture and mesh side.  With the environment
- * variable GNG_PM_TUNE_FILE set, decisions are also read from and written to
- * that file.  Untuned plans use the persistent z pass at nm = 512 and the
- * 512-thread deposit.  No reference counterpart.  Synchronous; the density and
- * potential of earlier solves are invalidated. */
+/* Launch-choice autotuner.  Some stages have more than one compiled variant:
+ * the y forward, z + Green and y inverse passes run either one workgroup per
+ * column item or one persistent workgroup per CU that prefetches its next
+ * item; the deposit runs 256- or 512-thread tile workgroups.
+ * gng_pm_autotune times whole solves of nm^3/8 synthetic uniform particles on
+ * the plan's own buffers (best of `reps` after a warm-up), flips one choice at
+ * a time from the defaults and keeps each flip that is faster.  The result
+ * applies to this plan and to plans created later in the process with the
+ * same device architecture and mesh side; with GNG_PM_TUNE_FILE set it is also
+ * read from and written to that file.  Untuned plans use the persistent z pass
+ * at nm = 512 only and the 512-thread deposit.  No reference counterpart.
+ * Synchronous; the density and potential of earlier solves are invalidated. */
+typedef struct gng_pm_tuning_t {
+  int32_t z_persistent;      /* z + Green pass: persistent workgroups          */
+  int32_t deposit_wide;      /* deposit: 512-thread (1) or 256-thread (0)       */
+  int32_t y_fwd_persistent;  /* y forward pass: persistent workgroups          */
+  int32_t y_inv_persistent;  /* y inverse pass: persistent workgroups          */
+  int32_t tuned;             /* 1 when measured or read from the cache (get)  */
+  int32_t reserved[3];
+} gng_pm_tuning_t;
+
 int gng_pm_autotune(gng_pm_plan *p, int reps);
-int gng_pm_tuning(const gng_pm_plan *p, int *z_persistent, int *deposit_wide, int *tuned);
-/* Sets the choices explicitly (for reproducible runs and for testing every
- * variant); the plan then reports tuned = 0. */
-int gng_pm_set_tuning(gng_pm_plan *p, int z_persistent, int deposit_wide);
+int gng_pm_get_tuning(const gng_pm_plan *p, gng_pm_tuning_t *out);
+/* Sets the choices explicitly (reproducible runs, testing every variant);
+ * the plan then reports tuned = 0. */
+int gng_pm_set_tuning(gng_pm_plan *p, const gng_pm_tuning_t *choices);
 
 /* Keep the mesh potential of the following solves (one extra nm^3 f32 buffer
  * and one extra x pass).  The potential is the field whose gradient the

@@ -20,12 +20,13 @@ def test_every_variant_matches_oracle(pm, oracle_mod, nm):
     m = g.uniform(0.5, 1.5, n)
     ref = oracle_mod.pm_accelerations(pos, m, L, nm, r_split=0.3)
     plan = pm.Plan(nm, L, r_split=0.3, device=0)
-    for zp in (False, True):
-        for dt in (256, 512):
-            plan.set_tuning(zp, dt)
-            got = plan.accelerations_f64(pos, m, 1.0)
-            assert rms_rel(got, ref) < 1e-5, (zp, dt)
-            assert plan.tuning() == {"z_persistent": zp, "deposit_threads": dt, "tuned": False}
+    for zp, dt, yf, yi in [(False, 256, False, False), (True, 512, True, True), (False, 512, True, False),
+                           (True, 256, False, True)]:
+        plan.set_tuning(zp, dt, yf, yi)
+        got = plan.accelerations_f64(pos, m, 1.0)
+        assert rms_rel(got, ref) < 1e-5, (zp, dt, yf, yi)
+        assert plan.tuning() == {"z_persistent": zp, "deposit_threads": dt, "y_fwd_persistent": yf,
+                                 "y_inv_persistent": yi, "tuned": False}
 
 
 def test_autotune_picks_a_variant_and_new_plans_reuse_it(pm, oracle_mod, tmp_path):
