@@ -244,6 +244,21 @@ struct YZCfg {
   static_assert(H % C == 0, "column tiles must cover the half spectrum");
 };
 
+// Column tiles narrower than a 128-byte line (C = 8 at nm = 1024) share each
+// line with the neighbouring tile.  Workgroups are dealt to the 8 XCDs round
+// robin, and each XCD has its own L2, so tile pairs (2k, 2k+1) are given to
+// workgroup indices 8 apart: both halves of a line are then fetched through
+// one L2 at about the same time instead of once through each of two.
+template <int C, int TILES>
+__device__ __forceinline__ int line_tile(int b) {
+  if constexpr (C * 8 >= 128 || TILES % 16 != 0) {
+    return b;
+  } else {
+    const int q = b >> 4, r = b & 15;
+    return (q << 4) + ((r & 7) << 1) + (r >> 3);
+  }
+}
+
 // Load / store a C-column tile of a plane-strided field into LDS columns
 // [0, C) (element (i, c) at buf[i*IS + c]); `stride` is the distance in
 // complex values between consecutive i.
@@ -258,7 +273,7 @@ __global__ __launch_bounds__(T) void k_fft_y_fwd(float2* __restrict__ P, const f
   __shared__ float2 twl[N];
   for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
   const float2* tw = twl;
-  float2* base = P + (size_t)blockIdx.y * N * H + (size_t)blockIdx.x * C;
+  float2* base = P + (size_t)blockIdx.y * N * H + (size_t)line_tile<C, H / C>(blockIdx.x) * C;
   float2 vin[F::PER0][F::R0], vout[F::PERL][F::RL];
   for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
     if (ok) vin[p][r] = base[(unsigned)i * H + l];
@@ -298,7 +313,7 @@ __global__ __launch_bounds__(T) void k_fft_z_green(const float2* __restrict__ P,
   __shared__ float2 twl[N];
   for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
   const float2* tw = twl;
-  const int tile = blockIdx.x, ky = blockIdx.y;
+  const int tile = line_tile<C, H / C>(blockIdx.x), ky = blockIdx.y;
   const unsigned plane = (unsigned)N * H;
   const size_t col0 = (size_t)ky * H + (size_t)tile * C;
   float2 vin[FF::PER0][FF::R0], ph[FF::PERL][FF::RL];
@@ -373,13 +388,13 @@ __global__ __launch_bounds__(T) void k_fft_z_green_persist(const float2* __restr
   float2 vin[FF::PER0][FF::R0];
   int item = blockIdx.x;
   if (item < items) {
-    const size_t c0 = (size_t)(item / TILES) * H + (size_t)(item % TILES) * C;
+    const size_t c0 = (size_t)(item / TILES) * H + (size_t)line_tile<C, TILES>(item % TILES) * C;
     for_edge<N, FF::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
       if (ok) vin[p][r] = P[c0 + (unsigned)i * plane + l];
     });
   }
   for (; item < items; item += gridDim.x) {
-    const int tile = item % TILES, ky = item / TILES;
+    const int tile = line_tile<C, TILES>(item % TILES), ky = item / TILES;
     const size_t col0 = (size_t)ky * H + (size_t)tile * C;
     const int ny = freq_of(ky, N);
     const float dy = gp.deconv ? gp.deconv[ky] : 1.0f;
@@ -387,7 +402,7 @@ __global__ __launch_bounds__(T) void k_fft_z_green_persist(const float2* __restr
     FF::run(vin, ph, buf, tw, tid);
     const int next = item + (int)gridDim.x;
     if (next < items) {
-      const size_t c1 = (size_t)(next / TILES) * H + (size_t)(next % TILES) * C;
+      const size_t c1 = (size_t)(next / TILES) * H + (size_t)line_tile<C, TILES>(next % TILES) * C;
       for_edge<N, FF::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
         if (ok) vin[p][r] = P[c1 + (unsigned)i * plane + l];
       });
@@ -509,7 +524,7 @@ __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2*
   __shared__ float2 twl[N];
   for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
   const float2* tw = twl;
-  const size_t off = (size_t)blockIdx.y * N * H + (size_t)blockIdx.x * C;
+  const size_t off = (size_t)blockIdx.y * N * H + (size_t)line_tile<C, H / C>(blockIdx.x) * C;
   float2* a = A + off;
   float2* b = B + off;
   float2* pp = P + off;
@@ -561,18 +576,18 @@ __global__ __launch_bounds__(T) void k_fft_y_fwd_persist(float2* __restrict__ P,
   float2 vin[F::PER0][F::R0];
   int item = blockIdx.x;
   if (item < items) {
-    const float2* b0 = P + (size_t)(item / TILES) * N * H + (size_t)(item % TILES) * C;
+    const float2* b0 = P + (size_t)(item / TILES) * N * H + (size_t)line_tile<C, TILES>(item % TILES) * C;
     for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
       if (ok) vin[p][r] = b0[(unsigned)i * H + l];
     });
   }
   for (; item < items; item += gridDim.x) {
-    float2* base = P + (size_t)(item / TILES) * N * H + (size_t)(item % TILES) * C;
+    float2* base = P + (size_t)(item / TILES) * N * H + (size_t)line_tile<C, TILES>(item % TILES) * C;
     float2 vout[F::PERL][F::RL];
     F::run(vin, vout, buf, tw, tid);
     const int next = item + (int)gridDim.x;
     if (next < items) {
-      const float2* b1 = P + (size_t)(next / TILES) * N * H + (size_t)(next % TILES) * C;
+      const float2* b1 = P + (size_t)(next / TILES) * N * H + (size_t)line_tile<C, TILES>(next % TILES) * C;
       for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
         if (ok) vin[p][r] = b1[(unsigned)i * H + l];
       });
@@ -599,13 +614,13 @@ __global__ __launch_bounds__(T) void k_fft_y_inv_persist(float2* __restrict__ A,
   float2 av[F::PER0][F::R0];
   int item = blockIdx.x;
   if (item < items) {
-    const float2* a0 = A + (size_t)(item / TILES) * N * H + (size_t)(item % TILES) * C;
+    const float2* a0 = A + (size_t)(item / TILES) * N * H + (size_t)line_tile<C, TILES>(item % TILES) * C;
     for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
       av[p][r] = ok ? a0[(unsigned)i * H + l] : make_float2(0.f, 0.f);
     });
   }
   for (; item < items; item += gridDim.x) {
-    const size_t off = (size_t)(item / TILES) * N * H + (size_t)(item % TILES) * C;
+    const size_t off = (size_t)(item / TILES) * N * H + (size_t)line_tile<C, TILES>(item % TILES) * C;
     float2* a = A + off;
     float2* b = B + off;
     float2* pp = P + off;
@@ -622,7 +637,7 @@ __global__ __launch_bounds__(T) void k_fft_y_inv_persist(float2* __restrict__ A,
       F::run(av, out, buf, tw, tid);
       const int next = item + (int)gridDim.x;
       if (next < items) {
-        const float2* a1 = A + (size_t)(next / TILES) * N * H + (size_t)(next % TILES) * C;
+        const float2* a1 = A + (size_t)(next / TILES) * N * H + (size_t)line_tile<C, TILES>(next % TILES) * C;
         for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
           av[p][r] = ok ? a1[(unsigned)i * H + l] : make_float2(0.f, 0.f);
         });
