@@ -236,6 +236,7 @@ __constant__ unsigned char kRangeLo[14] = {0, 1, 3, 5, 7, 2, 6, 3, 7, 4, 5, 7, 6
 __constant__ unsigned char kRangeHi[14] = {8, 2, 4, 6, 8, 4, 8, 4, 8, 8, 6, 8, 8, 8};
 
 // One workgroup per tile; T is the tile side (T^3 int64 accumulators in LDS).
+// Cell (x, y, z) of the mass mesh is rho[z*rho_plane + y*nm + x].
 // The 72 bucket offsets of the 8 source tiles are fetched in parallel, the 14
 // ranges are laid end to end, and all their particles are processed in one
 // flat loop, so the only serial latency is one offset load.  Eight wavefronts
@@ -247,7 +248,7 @@ template <int T, int NT = kDepositThreads>
 __global__ __launch_bounds__(NT) void k_deposit_tiles(const float4* __restrict__ sorted,
                                                        const unsigned* __restrict__ offset, const CellMap cm,
                                                        const TileMap tm, const SolveScalars* sc, long long ntot,
-                                                       float* __restrict__ rho) {
+                                                       float* __restrict__ rho, long long rho_plane) {
   constexpr int T3 = T * T * T;
   __shared__ unsigned long long acc[T3];
   __shared__ unsigned toff[8][9];
@@ -323,10 +324,9 @@ __global__ __launch_bounds__(NT) void k_deposit_tiles(const float4* __restrict__
   }
   __syncthreads();
   const double unit = ldexp(1.0, -s);
-  const size_t nm2 = (size_t)cm.nm * cm.nm;
   for (int c = tid; c < T3; c += NT) {
     const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
-    const size_t g = (size_t)(uz * T + lz) * nm2 + (size_t)(uy * T + ly) * cm.nm + (ux * T + lx);
+    const size_t g = (size_t)(uz * T + lz) * rho_plane + (size_t)(uy * T + ly) * cm.nm + (ux * T + lx);
     rho[g] = (float)((double)(long long)acc[c] * unit);
   }
 }

@@ -6,8 +6,9 @@
 // solve is a fixed sequence of launches on one stream; host entry points add
 // the H2D/D2H copies and a stream synchronisation.
 //
-// Mesh memory: three packed half-spectrum buffers P, A, B of 4 nm^3 bytes
-// each.  The deposit writes the FP32 mass per cell into P (or into a
+// Mesh memory: three packed half-spectrum buffers P, A, B of nm z planes,
+// mesh_plane(nm) complex values apart (4 nm^3 bytes each, plus a small pad
+// per plane from nm = 512 on).  The deposit writes the FP32 mass per cell into P (or into a
 // separate buffer when the density is kept for readback) and every FFT pass
 // works in place or ping-pongs among P, A, B, ending with F_x in A, F_y in P
 // and F_z in B.
@@ -179,8 +180,8 @@ struct MeshOps {
   int tile;
   // deposit[v]: workgroups of 256 (v = 0) or 512 (v = 1) threads
   void (*deposit[2])(dim3, dim3, hipStream_t, const float4*, const unsigned*, CellMap, TileMap, const SolveScalars*,
-                     long long, float*);
-  void (*x_fwd)(dim3, dim3, hipStream_t, const float*, float2*, const float2*);
+                     long long, float*, long long);
+  void (*x_fwd)(dim3, dim3, hipStream_t, const float*, float2*, const float2*, long long);
   void (*y_fwd)(dim3, dim3, hipStream_t, float2*, const float2*);
   void (*z_green)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
   void (*z_green_col0)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
@@ -205,16 +206,16 @@ MeshOps make_ops() {
   o.yz_threads = TY;
   o.tile = TD;
   o.deposit[0] = [](dim3 g, dim3 b, hipStream_t s, const float4* srt, const unsigned* off, CellMap cm, TileMap tm,
-                    const SolveScalars* sc, long long n, float* rho) {
-    hipLaunchKernelGGL((k_deposit_tiles<TD, 256>), g, b, 0, s, srt, off, cm, tm, sc, n, rho);
+                    const SolveScalars* sc, long long n, float* rho, long long rho_plane) {
+    hipLaunchKernelGGL((k_deposit_tiles<TD, 256>), g, b, 0, s, srt, off, cm, tm, sc, n, rho, rho_plane);
   };
   o.deposit[1] = [](dim3 g, dim3 b, hipStream_t s, const float4* srt, const unsigned* off, CellMap cm, TileMap tm,
-                    const SolveScalars* sc, long long n, float* rho) {
-    hipLaunchKernelGGL((k_deposit_tiles<TD, 512>), g, b, 0, s, srt, off, cm, tm, sc, n, rho);
+                    const SolveScalars* sc, long long n, float* rho, long long rho_plane) {
+    hipLaunchKernelGGL((k_deposit_tiles<TD, 512>), g, b, 0, s, srt, off, cm, tm, sc, n, rho, rho_plane);
   };
 
-  o.x_fwd = [](dim3 g, dim3 b, hipStream_t s, const float* rho, float2* P, const float2* tw) {
-    hipLaunchKernelGGL((k_fft_x_fwd<N, R>), g, b, 0, s, rho, P, tw);
+  o.x_fwd = [](dim3 g, dim3 b, hipStream_t s, const float* rho, float2* P, const float2* tw, long long rho_plane) {
+    hipLaunchKernelGGL((k_fft_x_fwd<N, R>), g, b, 0, s, rho, P, tw, rho_plane);
   };
   o.y_fwd = [](dim3 g, dim3 b, hipStream_t s, float2* P, const float2* tw) {
     hipLaunchKernelGGL((k_fft_y_fwd<N, C, TY>), g, b, 0, s, P, tw);
@@ -396,6 +397,10 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   const MeshOps& o = *p->ops;
   const CellMap cm = cell_map(p);
   float* rho = p->keep_density ? p->d_rho : reinterpret_cast<float*>(p->d_P);
+  // z-plane strides: half-spectrum meshes (complex values), the same buffers
+  // holding real fields (floats), and the mass mesh (floats)
+  const long long cplane = mesh_plane(nm), fplane = 2 * cplane;
+  const long long rho_plane = p->keep_density ? (long long)nm * nm : fplane;
 
   mark(p, 0, s);
   hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, p->d_sc);
@@ -419,10 +424,10 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   mark(p, 2, s);
   const unsigned ntiles = (unsigned)p->tm.nt * p->tm.nt * p->tm.nt;
   o.deposit[p->tune.deposit_wide](dim3(ntiles), dim3(p->tune.deposit_wide ? 512 : 256), s, p->d_sorted, p->d_offset,
-                                   cm, p->tm, p->d_sc, n, rho);
+                                   cm, p->tm, p->d_sc, n, rho, rho_plane);
   mark(p, 3, s);
   const long long rows = (long long)nm * nm;
-  o.x_fwd(dim3((unsigned)(rows / o.x_rows)), dim3(256), s, rho, p->d_P, p->d_tw);
+  o.x_fwd(dim3((unsigned)(rows / o.x_rows)), dim3(256), s, rho, p->d_P, p->d_tw, rho_plane / 2);
   mark(p, 4, s);
   const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
   const dim3 yz_block((unsigned)o.yz_threads);
@@ -453,7 +458,7 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   o.x_inv(dim3((unsigned)(rows / o.x_rows), 3), dim3(256), s, p->d_A, p->d_P, p->d_B, nullptr, p->d_tw, 0, 1.0f);
   mark(p, 8, s);
   if (k > 0)
-    hipLaunchKernelGGL(k_interp, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, x, y, z, idx, k, cm,
+    hipLaunchKernelGGL(k_interp, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, x, y, z, idx, k, cm, fplane,
                        reinterpret_cast<const float*>(p->d_A), reinterpret_cast<const float*>(p->d_P),
                        reinterpret_cast<const float*>(p->d_B), ax, ay, az);
   mark(p, 9, s);
@@ -636,7 +641,7 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
   p->tm.nt = nm / ops->tile;
   p->nb = (long long)p->tm.nt * p->tm.nt * p->tm.nt * 8;
   p->scan_blocks = (int)((p->nb + kScanBlock - 1) / kScanBlock);
-  const size_t half = sizeof(float2) * (size_t)(p->nm3 / 2);
+  const size_t half = sizeof(float2) * (size_t)nm * (size_t)mesh_plane(nm);
 #define GNG_ALLOC(ptr, bytes)                                                                             \
   do {                                                                                                    \
     hipError_t e_ = hipMalloc(&(ptr), (bytes));                                                           \
@@ -916,10 +921,12 @@ int gng_pm_force_meshes(gng_pm_plan* p, float* out) {
   int rc = set_device(p);
   if (rc) return rc;
   GNG_HIP(hipDeviceSynchronize());
-  const size_t bytes = sizeof(float) * (size_t)p->nm3;
-  GNG_HIP(hipMemcpy(out, p->d_A, bytes, hipMemcpyDeviceToHost));
-  GNG_HIP(hipMemcpy(out + p->nm3, p->d_P, bytes, hipMemcpyDeviceToHost));
-  GNG_HIP(hipMemcpy(out + 2 * p->nm3, p->d_B, bytes, hipMemcpyDeviceToHost));
+  // the meshes' z planes sit mesh_plane(nm) complex values apart
+  const size_t nm = (size_t)p->cfg.grid_size, row = sizeof(float) * nm * nm;
+  const size_t pitch = sizeof(float2) * (size_t)mesh_plane((int)nm);
+  const float2* src[3] = {p->d_A, p->d_P, p->d_B};
+  for (int f = 0; f < 3; ++f)
+    GNG_HIP(hipMemcpy2D(out + f * p->nm3, row, src[f], pitch, row, nm, hipMemcpyDeviceToHost));
   return GNG_OK;
 }
 

@@ -50,6 +50,25 @@ struct CellMap {
   int nm;
 };
 
+// Complex values between consecutive z planes of the half-spectrum meshes:
+// N*(N/2) plus a pad of 32 (512 <= N < 1024) or 64 (N >= 1024) values.  The
+// z pass reads one 64- or 128-byte segment from each of N planes; at a
+// power-of-two plane stride (1 MiB at N = 512, 4 MiB at N = 1024) these reads
+// crowd onto few HBM channels.  On MI355X the pad raised a plain column copy
+// with that pattern from 4.5 to 5.0 TB/s (N = 512, 128-byte segments) and
+// from 2.3 to 3.3 TB/s (N = 1024, 64-byte segments).  A real field kept in the
+// same buffer has the same bytes row by row: row (z, y) of N floats starts at
+// float 2*z*plane + y*N, so the x transforms still work in place.
+__host__ __device__ constexpr long long mesh_plane(int n) {
+  return (long long)n * (n / 2) + (n >= 1024 ? 64 : (n >= 512 ? 32 : 0));
+}
+
+// Complex offset of row r = z*N + y of a half-spectrum mesh.
+template <int N>
+__device__ __forceinline__ long long mesh_row(long long r) {
+  return (r / N) * mesh_plane(N) + (r % N) * (N / 2);
+}
+
 // Cell index and offset of one coordinate: c = (x * nm / L) mod nm,
 // i0 = floor(c), d = c - i0 (cic.rs:50-60), with the product carried as
 // p + e (error-free FMA split) so d is accurate to one FP32 ulp of 1.
@@ -112,8 +131,11 @@ struct XCfg {
 // workgroup reads its rows completely before writing them).  The rows are
 // read straight into the registers of the first FFT stage; the pairwise
 // post-processing (X_k needs Z_k and Z_{H-k}) exchanges through LDS.
+// rho_plane: complex-sized (float-pair) stride between z planes of rho, which
+// is mesh_plane(N) when rho shares P's buffer and N*N/2 for a separate one.
 template <int N, int ROWS>
-__global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, const float2* __restrict__ tw_g) {
+__global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, const float2* __restrict__ tw_g,
+                                                   long long rho_plane) {
   constexpr int H = N / 2, T = 256, LS = XCfg<N>::LS;
   using F = RegFft<H, false, false, ROWS, T, LS, 1, false, N>;
   __shared__ float2 buf[ROWS * LS];
@@ -124,10 +146,11 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, 
   for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
   const float2* tw = twl;
   const long long row0 = (long long)blockIdx.x * ROWS;
-  const float2* src = reinterpret_cast<const float2*>(rho + row0 * N);
+  const float2* src = reinterpret_cast<const float2*>(rho);
   float2 vin[F::PER0][F::R0], vout[F::PERL][F::RL];
   for_edge<H, F::R0, ROWS, T, false>(tid, [&](int p, int r, int l, int i, bool ok) {
-    if (ok) vin[p][r] = src[l * H + i];
+    const long long row = row0 + l;
+    if (ok) vin[p][r] = src[(row / N) * rho_plane + (row % N) * H + i];
   });
   F::run(vin, vout, buf, tw, tid);
   __syncthreads();
@@ -135,7 +158,6 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, 
     if (ok) buf[l * LS + i] = vout[p][r];
   });
   __syncthreads();
-  float2* dst = P + row0 * H;
   constexpr int TOT = ROWS * H;
 #pragma unroll
   for (int p = 0; p < (TOT + T - 1) / T; ++p) {
@@ -153,7 +175,7 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, 
         const float2 O = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
         out = cadd(E, cmul(tw[k], O));
       }
-      dst[e] = out;
+      P[mesh_row<N>(row0 + r) + k] = out;
     }
   }
 }
@@ -179,14 +201,12 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float
   const int field = blockIdx.y + field0;
   float2* data = field == 0 ? A : (field == 1 ? P : (field == 2 ? B : A));
   const long long row0 = (long long)blockIdx.x * ROWS;
-  float2* row = data + row0 * H;
-  float2* orow = (field == 3 ? pot : data) + row0 * H;
   constexpr int TOT = ROWS * H;
 #pragma unroll
   for (int p = 0; p < (TOT + T - 1) / T; ++p) {
     const int e = tid + p * T;
     if ((TOT % T == 0) || e < TOT) {
-      float2 v = row[e];
+      float2 v = data[mesh_row<N>(row0 + e / H) + e % H];
       if (field == 0) {
         // -i n_x; slot 0 holds DC (n=0) and Nyquist (n_eff=0): both vanish
         const float nx = (float)(e % H);
@@ -212,8 +232,10 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float
   });
   F::run(vin, vout, buf, tw, tid);
   const float sc = field == 3 ? pot_scale : 1.0f;
+  // the potential goes to its own unpadded mesh
   for_edge<H, F::RL, ROWS, T, false>(tid, [&](int p, int r, int l, int i, bool ok) {
-    if (ok) orow[l * H + i] = make_float2(vout[p][r].x * sc, vout[p][r].y * sc);
+    float2* orow = field == 3 ? pot + (row0 + l) * H : data + mesh_row<N>(row0 + l);
+    if (ok) orow[i] = make_float2(vout[p][r].x * sc, vout[p][r].y * sc);
   });
 }
 
@@ -273,7 +295,7 @@ __global__ __launch_bounds__(T) void k_fft_y_fwd(float2* __restrict__ P, const f
   __shared__ float2 twl[N];
   for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
   const float2* tw = twl;
-  float2* base = P + (size_t)blockIdx.y * N * H + (size_t)line_tile<C, H / C>(blockIdx.x) * C;
+  float2* base = P + (size_t)blockIdx.y * mesh_plane(N) + (size_t)line_tile<C, H / C>(blockIdx.x) * C;
   float2 vin[F::PER0][F::R0], vout[F::PERL][F::RL];
   for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
     if (ok) vin[p][r] = base[(unsigned)i * H + l];
@@ -314,7 +336,7 @@ __global__ __launch_bounds__(T) void k_fft_z_green(const float2* __restrict__ P,
   for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
   const float2* tw = twl;
   const int tile = line_tile<C, H / C>(blockIdx.x), ky = blockIdx.y;
-  const unsigned plane = (unsigned)N * H;
+  const unsigned plane = (unsigned)mesh_plane(N);
   const size_t col0 = (size_t)ky * H + (size_t)tile * C;
   float2 vin[FF::PER0][FF::R0], ph[FF::PERL][FF::RL];
   for_edge<N, FF::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
@@ -383,7 +405,7 @@ __global__ __launch_bounds__(T) void k_fft_z_green_persist(const float2* __restr
   __shared__ float2 twl[N];
   for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
   const float2* tw = twl;
-  const unsigned plane = (unsigned)N * H;
+  const unsigned plane = (unsigned)mesh_plane(N);
   const int items = TILES * N;
   float2 vin[FF::PER0][FF::R0];
   int item = blockIdx.x;
@@ -457,7 +479,7 @@ __global__ __launch_bounds__(256) void k_fft_z_green_col0(const float2* __restri
   __shared__ float2 buf[4 * LS];
   const int tid = threadIdx.x;
   const int ky = blockIdx.x, kym = (N - ky) % N;
-  const size_t plane = (size_t)N * H;
+  const size_t plane = (size_t)mesh_plane(N);
   for (int e = tid; e < 2 * N; e += T) {
     const int l = e / N, z = e % N;
     buf[l * LS + z] = P[(size_t)z * plane + (size_t)(l ? kym : ky) * H];
@@ -524,7 +546,7 @@ __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2*
   __shared__ float2 twl[N];
   for (int e = tid; e < N; e += (int)blockDim.x) twl[e] = tw_g[e];
   const float2* tw = twl;
-  const size_t off = (size_t)blockIdx.y * N * H + (size_t)line_tile<C, H / C>(blockIdx.x) * C;
+  const size_t off = (size_t)blockIdx.y * mesh_plane(N) + (size_t)line_tile<C, H / C>(blockIdx.x) * C;
   float2* a = A + off;
   float2* b = B + off;
   float2* pp = P + off;
@@ -576,18 +598,18 @@ __global__ __launch_bounds__(T) void k_fft_y_fwd_persist(float2* __restrict__ P,
   float2 vin[F::PER0][F::R0];
   int item = blockIdx.x;
   if (item < items) {
-    const float2* b0 = P + (size_t)(item / TILES) * N * H + (size_t)line_tile<C, TILES>(item % TILES) * C;
+    const float2* b0 = P + (size_t)(item / TILES) * mesh_plane(N) + (size_t)line_tile<C, TILES>(item % TILES) * C;
     for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
       if (ok) vin[p][r] = b0[(unsigned)i * H + l];
     });
   }
   for (; item < items; item += gridDim.x) {
-    float2* base = P + (size_t)(item / TILES) * N * H + (size_t)line_tile<C, TILES>(item % TILES) * C;
+    float2* base = P + (size_t)(item / TILES) * mesh_plane(N) + (size_t)line_tile<C, TILES>(item % TILES) * C;
     float2 vout[F::PERL][F::RL];
     F::run(vin, vout, buf, tw, tid);
     const int next = item + (int)gridDim.x;
     if (next < items) {
-      const float2* b1 = P + (size_t)(next / TILES) * N * H + (size_t)line_tile<C, TILES>(next % TILES) * C;
+      const float2* b1 = P + (size_t)(next / TILES) * mesh_plane(N) + (size_t)line_tile<C, TILES>(next % TILES) * C;
       for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
         if (ok) vin[p][r] = b1[(unsigned)i * H + l];
       });
@@ -614,13 +636,13 @@ __global__ __launch_bounds__(T) void k_fft_y_inv_persist(float2* __restrict__ A,
   float2 av[F::PER0][F::R0];
   int item = blockIdx.x;
   if (item < items) {
-    const float2* a0 = A + (size_t)(item / TILES) * N * H + (size_t)line_tile<C, TILES>(item % TILES) * C;
+    const float2* a0 = A + (size_t)(item / TILES) * mesh_plane(N) + (size_t)line_tile<C, TILES>(item % TILES) * C;
     for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
       av[p][r] = ok ? a0[(unsigned)i * H + l] : make_float2(0.f, 0.f);
     });
   }
   for (; item < items; item += gridDim.x) {
-    const size_t off = (size_t)(item / TILES) * N * H + (size_t)line_tile<C, TILES>(item % TILES) * C;
+    const size_t off = (size_t)(item / TILES) * mesh_plane(N) + (size_t)line_tile<C, TILES>(item % TILES) * C;
     float2* a = A + off;
     float2* b = B + off;
     float2* pp = P + off;
@@ -637,7 +659,7 @@ __global__ __launch_bounds__(T) void k_fft_y_inv_persist(float2* __restrict__ A,
       F::run(av, out, buf, tw, tid);
       const int next = item + (int)gridDim.x;
       if (next < items) {
-        const float2* a1 = A + (size_t)(next / TILES) * N * H + (size_t)line_tile<C, TILES>(next % TILES) * C;
+        const float2* a1 = A + (size_t)(next / TILES) * mesh_plane(N) + (size_t)line_tile<C, TILES>(next % TILES) * C;
         for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
           av[p][r] = ok ? a1[(unsigned)i * H + l] : make_float2(0.f, 0.f);
         });
@@ -668,10 +690,12 @@ __global__ __launch_bounds__(T) void k_fft_y_inv_persist(float2* __restrict__ A,
   }
 }
 
-// CIC gather of (Fx, Fy, Fz) at pos[idx[j]] (idx null: j itself).
+// CIC gather of (Fx, Fy, Fz) at pos[idx[j]] (idx null: j itself); fplane is
+// the float stride between z planes of the force meshes.
 __global__ __launch_bounds__(256) void k_interp(const float* __restrict__ x, const float* __restrict__ y,
                                                 const float* __restrict__ z, const long long* __restrict__ idx,
-                                                long long k, const CellMap cm, const float* __restrict__ fx,
+                                                long long k, const CellMap cm, long long fplane,
+                                                const float* __restrict__ fx,
                                                 const float* __restrict__ fy, const float* __restrict__ fz,
                                                 float* __restrict__ ax, float* __restrict__ ay,
                                                 float* __restrict__ az) {
@@ -688,7 +712,6 @@ __global__ __launch_bounds__(256) void k_interp(const float* __restrict__ x, con
   const int xs[2] = {ix, ix + 1 == nm ? 0 : ix + 1};
   const int ys[2] = {iy, iy + 1 == nm ? 0 : iy + 1};
   const int zs[2] = {iz, iz + 1 == nm ? 0 : iz + 1};
-  const size_t nm2 = (size_t)nm * nm;
   float sx = 0.f, sy = 0.f, sz = 0.f;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -697,7 +720,7 @@ __global__ __launch_bounds__(256) void k_interp(const float* __restrict__ x, con
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const float w = wx[c] * wy[b] * wz[a];
-        const size_t q = zs[a] * nm2 + (size_t)ys[b] * nm + xs[c];
+        const size_t q = zs[a] * fplane + (size_t)ys[b] * nm + xs[c];
         sx = fmaf(fx[q], w, sx);
         sy = fmaf(fy[q], w, sy);
         sz = fmaf(fz[q], w, sz);
