@@ -41,7 +41,7 @@ def stage_bytes(n: int, nm: int) -> dict:
     m = nm ** 3
     return {
         "bucket": 16 * n + 8 * n,           # x, y, z, m in; bucket, rank out
-        "sort": 24 * n + 20 * n,            # x, y, z, m, bucket, rank in; float4 + index out
+        "sort": 24 * n + 16 * n,            # x, y, z, m, bucket, rank in; float4 out
         "deposit": 16 * n + 4 * m,          # sorted particles in; f32 density out
         "fft_x_fwd": 4 * m + 4 * m,         # density in, spectrum out
         "fft_y_fwd": 8 * m,                 # spectrum in and out

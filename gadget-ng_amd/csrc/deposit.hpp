@@ -196,13 +196,11 @@ __global__ __launch_bounds__(256) void k_scan_add(unsigned* __restrict__ offset,
 __global__ __launch_bounds__(256) void k_scatter(const float* __restrict__ x, const float* __restrict__ y,
                                                  const float* __restrict__ z, const float* __restrict__ m, long long n,
                                                  const unsigned* __restrict__ bucket, const unsigned* __restrict__ rank,
-                                                 const unsigned* __restrict__ offset, float4* __restrict__ sorted,
-                                                 unsigned* __restrict__ perm) {
+                                                 const unsigned* __restrict__ offset, float4* __restrict__ sorted) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const unsigned slot = offset[bucket[i]] + rank[i];
   sorted[slot] = make_float4(x[i], y[i], z[i], m[i]);
-  perm[slot] = (unsigned)i;
 }
 
 // LDS fixed-point add with wavefront aggregation: the lanes whose cell equals
@@ -231,7 +229,14 @@ __device__ __forceinline__ void lds_add_aggregated(unsigned long long* acc, int 
 // The 14 bucket ranges a tile reads: (d, first class, end class).  d is the
 // neighbour offset (bit 0 x, bit 1 y, bit 2 z: the source tile is u - d) and
 // the classes are those containing every bit of d, merged where adjacent.
-__constant__ unsigned char kRangeD[14] = {0, 1, 1, 1, 1, 2, 2, 3, 3, 4, 5, 5, 6, 7};
+// d of range r is (kRangeD >> 3r) & 7 (packed, so a per-lane r needs no load)
+constexpr unsigned char kRangeDList[14] = {0, 1, 1, 1, 1, 2, 2, 3, 3, 4, 5, 5, 6, 7};
+constexpr unsigned long long pack_range_d() {
+  unsigned long long v = 0;
+  for (int i = 0; i < 14; ++i) v |= (unsigned long long)kRangeDList[i] << (3 * i);
+  return v;
+}
+constexpr unsigned long long kRangeD = pack_range_d();
 __constant__ unsigned char kRangeLo[14] = {0, 1, 3, 5, 7, 2, 6, 3, 7, 4, 5, 7, 6, 7};
 __constant__ unsigned char kRangeHi[14] = {8, 2, 4, 6, 8, 4, 8, 4, 8, 8, 6, 8, 8, 8};
 
@@ -252,7 +257,7 @@ __global__ __launch_bounds__(NT) void k_deposit_tiles(const float4* __restrict__
   constexpr int T3 = T * T * T;
   __shared__ unsigned long long acc[T3];
   __shared__ unsigned toff[8][9];
-  __shared__ unsigned rbeg[14], rpre[15];
+  __shared__ unsigned rpre[15], rdelta[14];
   const int tid = threadIdx.x;
   for (int c = tid; c < T3; c += NT) acc[c] = 0ull;
   const int nt = tm.nt;
@@ -267,35 +272,56 @@ __global__ __launch_bounds__(NT) void k_deposit_tiles(const float4* __restrict__
   }
   __syncthreads();
   if (tid < 64) {
-    unsigned len = 0;
+    unsigned len = 0, beg = 0;
     if (tid < 14) {
-      const int d = kRangeD[tid];
-      rbeg[tid] = toff[d][kRangeLo[tid]];
-      len = toff[d][kRangeHi[tid]] - toff[d][kRangeLo[tid]];
+      const int d = (int)((kRangeD >> (3 * tid)) & 7ull);
+      beg = toff[d][kRangeLo[tid]];
+      len = toff[d][kRangeHi[tid]] - beg;
     }
     unsigned incl = len;
     for (int off = 1; off < 16; off <<= 1) {
       const unsigned t = __shfl_up(incl, off);
       if (tid >= off) incl += t;
     }
-    if (tid < 14) rpre[tid + 1] = incl;
+    if (tid < 14) {
+      rpre[tid + 1] = incl;
+      // slot of particle q of range tid in the sorted array: q + rdelta
+      rdelta[tid] = beg - (incl - len);
+    }
     if (tid == 0) rpre[0] = 0;
   }
   __syncthreads();
+  // range ends, uniform over the workgroup, in scalar registers
+  unsigned rend[13];
+#pragma unroll
+  for (int k = 0; k < 13; ++k) rend[k] = __builtin_amdgcn_readfirstlane(rpre[k + 1]);
   const int s = fixed_shift(sc, ntot);
   const unsigned total = rpre[14];
   // every lane of a wavefront runs the same number of iterations (lanes past
   // the end carry no particle), so the aggregation ballots see whole waves
   const unsigned rounds = (total + NT - 1u) / NT;
-  for (unsigned it = 0; it < rounds; ++it) {
-    const unsigned q = it * NT + (unsigned)tid;
-    const bool live = q < total;
+  // range of q: the number of range ends at or below it (13 past the end)
+  auto range_of = [&](unsigned q) {
     int r = 0;
-    if (live)
-      while (q >= rpre[r + 1]) ++r;
-    const int d = kRangeD[r];
+#pragma unroll
+    for (int k = 0; k < 13; ++k) r += q >= rend[k] ? 1 : 0;
+    return r;
+  };
+  auto fetch = [&](unsigned q, int r) {
+    return q < total ? sorted[q + rdelta[r]] : make_float4(0.f, 0.f, 0.f, 0.f);
+  };
+  unsigned q = (unsigned)tid;
+  int r = range_of(q);
+  float4 pnext = fetch(q, r);
+  for (unsigned it = 0; it < rounds; ++it) {
+    const bool live = q < total;
+    const int d = (int)((kRangeD >> (3 * r)) & 7ull);
     const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-    const float4 p = live ? sorted[rbeg[r] + (q - rpre[r])] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 p = pnext;
+    // the next round's particle is loaded while this one is deposited
+    q += NT;
+    r = range_of(q);
+    if (it + 1 < rounds) pnext = fetch(q, r);
     int ix, iy, iz;
     float fx, fy, fz;
     cell_coord(p.x, cm, ix, fx);

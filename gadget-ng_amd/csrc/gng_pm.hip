@@ -295,7 +295,6 @@ struct gng_pm_plan {
   // per-particle work buffers (grown on demand)
   long long work_cap = 0;
   float4* d_sorted = nullptr;
-  unsigned* d_perm = nullptr;
   unsigned* d_bucket = nullptr;
   unsigned* d_rank = nullptr;
 
@@ -324,11 +323,10 @@ int set_device(gng_pm_plan* p) {
 
 void free_work(gng_pm_plan* p) {
   (void)hipFree(p->d_sorted);
-  (void)hipFree(p->d_perm);
   (void)hipFree(p->d_bucket);
   (void)hipFree(p->d_rank);
   p->d_sorted = nullptr;
-  p->d_perm = p->d_bucket = p->d_rank = nullptr;
+  p->d_bucket = p->d_rank = nullptr;
   p->work_cap = 0;
 }
 
@@ -380,7 +378,6 @@ int ensure_work(gng_pm_plan* p, long long n) {
   free_work(p);
   const size_t m = (size_t)n;
   GNG_HIP(hipMalloc(&p->d_sorted, sizeof(float4) * m));
-  GNG_HIP(hipMalloc(&p->d_perm, sizeof(unsigned) * m));
   GNG_HIP(hipMalloc(&p->d_bucket, sizeof(unsigned) * m));
   GNG_HIP(hipMalloc(&p->d_rank, sizeof(unsigned) * m));
   p->work_cap = n;
@@ -420,7 +417,7 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_offset, p->d_partial, p->nb);
   if (n > 0)
     hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z, m, n, p->d_bucket,
-                       p->d_rank, p->d_offset, p->d_sorted, p->d_perm);
+                       p->d_rank, p->d_offset, p->d_sorted);
   mark(p, 2, s);
   const unsigned ntiles = (unsigned)p->tm.nt * p->tm.nt * p->tm.nt;
   o.deposit[p->tune.deposit_wide](dim3(ntiles), dim3(p->tune.deposit_wide ? 512 : 256), s, p->d_sorted, p->d_offset,

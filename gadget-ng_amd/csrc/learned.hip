@@ -77,39 +77,61 @@ __host__ __device__ constexpr int tap_off(int t, int hx, int hy) {
 }
 
 // Stage input cells (x0-1 .. x0+HXv-2, ...) mod nm, CIN bf16 channels each,
-// into lds[((hz*HYv + hy)*HXv + hx)*CIN + c], moving 32-bit words.
+// into lds[((hz*HYv + hy)*HXv + hx)*CIN + c], moving 32-bit words.  With
+// PLANAR (16 channels only) channels 0-7 and 8-15 go to two separate planes
+// of 16-byte words, cell h at word h and HXv*HYv*HZv + h: sixteen lanes that
+// read one channel half of sixteen consecutive cells then read 256
+// consecutive bytes, which no two of them share a bank for.
 template <int CIN>
 struct HaloWord {  // 16-byte words for 16 channels, one word per cell otherwise
   using T = typename std::conditional<CIN == 16, u32x4, typename std::conditional<CIN == 4, u32x2, unsigned>::type>::type;
   static constexpr int per_cell = CIN == 16 ? 2 : 1;
 };
 
-template <int CIN, int HXv, int HYv, int HZv>
+// All of a thread's global loads are issued before any is written to LDS,
+// so the halo costs one memory latency rather than one per word.  NT is the
+// workgroup size.
+template <int CIN, int HXv, int HYv, int HZv, int NT, bool PLANAR = false>
 __device__ __forceinline__ void load_halo(unsigned short* lds, const unsigned short* __restrict__ in, int x0,
                                           int y0, int z0, int nm) {
   using W = typename HaloWord<CIN>::T;
   constexpr int WPC = HaloWord<CIN>::per_cell;
-  constexpr int NW = HXv * HYv * HZv * WPC;
+  constexpr int NCELL = HXv * HYv * HZv, NW = NCELL * WPC, PER = (NW + NT - 1) / NT;
+  static_assert(!PLANAR || CIN == 16, "the planar layout is for 16 channels");
   const W* src = reinterpret_cast<const W*>(in);
   W* dst = reinterpret_cast<W*>(lds);
-  for (int e = threadIdx.x; e < NW; e += blockDim.x) {
-    const int c = e / WPC, w = e - c * WPC;
-    const int hx = c % HXv, hy = (c / HXv) % HYv, hz = c / (HXv * HYv);
-    int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-    gx += gx < 0 ? nm : 0;
-    gx -= gx >= nm ? nm : 0;
-    gy += gy < 0 ? nm : 0;
-    gy -= gy >= nm ? nm : 0;
-    gz += gz < 0 ? nm : 0;
-    gz -= gz >= nm ? nm : 0;
-    const long long gc = ((long long)gz * nm + gy) * nm + gx;
-    dst[e] = src[gc * WPC + w];
+  W v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = (int)threadIdx.x + k * NT;
+    if (NW % NT == 0 || e < NW) {
+      const int c = e / WPC, w = e - c * WPC;
+      const int hx = c % HXv, hy = (c / HXv) % HYv, hz = c / (HXv * HYv);
+      int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+      gx += gx < 0 ? nm : 0;
+      gx -= gx >= nm ? nm : 0;
+      gy += gy < 0 ? nm : 0;
+      gy -= gy >= nm ? nm : 0;
+      gz += gz < 0 ? nm : 0;
+      gz -= gz >= nm ? nm : 0;
+      const long long gc = ((long long)gz * nm + gy) * nm + gx;
+      v[k] = src[gc * WPC + w];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = (int)threadIdx.x + k * NT;
+    if (NW % NT == 0 || e < NW) {
+      const int c = e / WPC, w = e - c * WPC;
+      dst[PLANAR ? w * NCELL + c : e] = v[k];
+    }
   }
 }
 
-// A fragment of k-step s for the 16-cell row whose lane-m cell sits at halo
+// Fragment of k-step s for the 16-cell row whose lane-m cell sits at halo
 // index `base`: lane (m, g) holds x[cell m + off_t][c] for k = 32s + 8g + j,
-// t = k / CIN, c = k % CIN, j = 0..7 (zero for t >= 27).
+// t = k / CIN, c = k % CIN, j = 0..7 (zero for t >= 27).  For CIN = 16 the
+// halo is in the planar layout of load_halo.
 template <int CIN>
 __device__ __forceinline__ bf16x8 a_frag(const unsigned short* tile, int base, int s, int g) {
   u32x4 v = {0u, 0u, 0u, 0u};
@@ -117,7 +139,7 @@ __device__ __forceinline__ bf16x8 a_frag(const unsigned short* tile, int base, i
     // taps 2s and 2s + 1, channel halves by g & 1
     const int t = 2 * s + (g >> 1);
     const int off = (g >> 1) ? tap_off(2 * s + 1, HX, HY) : tap_off(2 * s, HX, HY);
-    if (t < 27) v = *reinterpret_cast<const u32x4*>(tile + (base + off) * 16 + 8 * (g & 1));
+    if (t < 27) v = reinterpret_cast<const u32x4*>(tile)[(g & 1) * HC + base + off];
   } else if constexpr (CIN == 4) {
     const int t0 = 8 * s + 2 * g;
     if (t0 < 27) {
@@ -142,18 +164,25 @@ __device__ __forceinline__ bf16x8 a_frag(const unsigned short* tile, int base, i
 
 enum { MODE_HIDDEN = 0, MODE_OUT = 1, MODE_BWD = 2, MODE_INFER = 3 };
 
+
 // One periodic 3^3 convolution over an nm^3 mesh (nm % 16 == 0), grid
 // (nm/16, (nm/8)^2, boxes), kConvThreads threads (8 waves, 8 of the tile's 64
-// rows each).  Output channels n = lane % 16:
+// rows each).  The MFMA computes the transposed product D^T[o][cell] = W^T X^T
+// (the weight fragments as the first operand), so lane (m, g) ends with
+// output channels 4g..4g+3 of cell m of the row and stores them as one
+// vector: a wave writes one contiguous run of 16 cells.
 //   MODE_HIDDEN  zout = conv + bias (f32, 16 ch), xout = bf16(silu(zout));
 //   MODE_INFER   xout = bf16(silu(conv + bias)) only;
 //   MODE_OUT     zout = conv + bias (f32, 4 ch: the 3 outputs and a zero);
 //   MODE_BWD     xout = bf16(conv * silu'(zprev)) (16 ch).
 constexpr int kConvThreads = 512;
 constexpr int kRowsPerWave = TY * TZ / (kConvThreads / 64);
+// rows of a wave that share one pipelined k-step loop (one accumulator each)
+constexpr int kRowGroup = 2;
+static_assert(kRowsPerWave % kRowGroup == 0, "row groups must tile a wave's rows");
 
 template <int CIN, int MODE>
-__global__ __launch_bounds__(kConvThreads) void k_conv(const unsigned short* __restrict__ in,
+__global__ __launch_bounds__(kConvThreads, 4) void k_conv(const unsigned short* __restrict__ in,
                                               const u32x4* __restrict__ wfrag, const float* __restrict__ bias,
                                               float* __restrict__ zout, unsigned short* __restrict__ xout,
                                               const float* __restrict__ zprev, int nm) {
@@ -169,52 +198,71 @@ __global__ __launch_bounds__(kConvThreads) void k_conv(const unsigned short* __r
   bf16x8 b[KS];
 #pragma unroll
   for (int s = 0; s < KS; ++s) b[s] = __builtin_bit_cast(bf16x8, wfrag[s * 64 + lane]);
-  float bn = 0.0f;
-  if constexpr (MODE != MODE_BWD) bn = bias[m];
+  f32x4 bn = {0.0f, 0.0f, 0.0f, 0.0f};
+  if constexpr (MODE != MODE_BWD) bn = *reinterpret_cast<const f32x4*>(bias + 4 * g);
 
-  load_halo<CIN, HX, HY, HZ>(tile, in + boff * CIN, x0, y0, z0, nm);
+  load_halo<CIN, HX, HY, HZ, kConvThreads, CIN == 16>(tile, in + boff * CIN, x0, y0, z0, nm);
   __syncthreads();
 
-  // each wave: kRowsPerWave of the tile's rows, four independent accumulators at a time
+  // each wave: kRowsPerWave of the tile's rows, kRowGroup accumulators at a time
+  constexpr int RG = kRowGroup;
 #pragma unroll 1
-  for (int q = 0; q < kRowsPerWave / 4; ++q) {
-    f32x4 acc[4];
-    int base[4];
+  for (int q = 0; q < kRowsPerWave / RG; ++q) {
+    f32x4 acc[RG];
+    int base[RG];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int r = wave * kRowsPerWave + q * 4 + a;
+    for (int a = 0; a < RG; ++a) {
+      const int r = wave * kRowsPerWave + q * RG + a;
       const int ry = r % TY, rz = r / TY;
       base[a] = ((rz + 1) * HY + (ry + 1)) * HX + (m + 1);
       acc[a] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     }
+    // the fragments of k-step s + 1 are read while the MFMAs of step s run
+    bf16x8 cur[RG];
+#pragma unroll
+    for (int a = 0; a < RG; ++a) cur[a] = a_frag<CIN>(tile, base[a], 0, g);
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
+      bf16x8 nxt[RG];
+      if (s + 1 < KS) {
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const bf16x8 av = a_frag<CIN>(tile, base[a], s, g);
-        acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[s], acc[a], 0, 0, 0);
+        for (int a = 0; a < RG; ++a) nxt[a] = a_frag<CIN>(tile, base[a], s + 1, g);
+      }
+      // keep the scheduler from sinking those reads down to their uses
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int a = 0; a < RG; ++a) acc[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[s], cur[a], acc[a], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (s + 1 < KS) {
+#pragma unroll
+        for (int a = 0; a < RG; ++a) cur[a] = nxt[a];
       }
     }
-    // lane holds D[cell 4g + j][channel m]
+    // lane (m, g) holds D^T[channel 4g + j][cell m]
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int r = wave * kRowsPerWave + q * 4 + a;
+    for (int a = 0; a < RG; ++a) {
+      const int r = wave * kRowsPerWave + q * RG + a;
       const int ry = r % TY, rz = r / TY;
-      const long long rowc = boff + ((long long)(z0 + rz) * nm + (y0 + ry)) * nm + x0 + 4 * g;
+      const long long c = boff + ((long long)(z0 + rz) * nm + (y0 + ry)) * nm + x0 + m;
+      if constexpr (MODE == MODE_OUT) {
+        // channels 0-3: the 3 outputs and the zero pad column
+        if (g == 0) *reinterpret_cast<f32x4*>(zout + c * 4) = acc[a] + bn;
+      } else {
+        f32x4 v;
+        if constexpr (MODE == MODE_BWD) {
+          const f32x4 zp = *reinterpret_cast<const f32x4*>(zprev + c * 16 + 4 * g);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const long long c = rowc + j;
-        if constexpr (MODE == MODE_HIDDEN) {
-          const float z = acc[a][j] + bn;
-          zout[c * 16 + m] = z;
-          xout[c * 16 + m] = f2bf(silu(z));
-        } else if constexpr (MODE == MODE_INFER) {
-          xout[c * 16 + m] = f2bf(silu(acc[a][j] + bn));
-        } else if constexpr (MODE == MODE_OUT) {
-          if (m < 4) zout[c * 4 + m] = acc[a][j] + bn;
+          for (int j = 0; j < 4; ++j) v[j] = acc[a][j] * silu_grad(zp[j]);
         } else {
-          xout[c * 16 + m] = f2bf(acc[a][j] * silu_grad(zprev[c * 16 + m]));
+          const f32x4 z = acc[a] + bn;
+          if constexpr (MODE == MODE_HIDDEN) *reinterpret_cast<f32x4*>(zout + c * 16 + 4 * g) = z;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) v[j] = silu(z[j]);
         }
+        u32x2 packed;
+        packed.x = (unsigned)f2bf(v[0]) | ((unsigned)f2bf(v[1]) << 16);
+        packed.y = (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16);
+        *reinterpret_cast<u32x2*>(xout + c * 16 + 4 * g) = packed;
       }
     }
   }
@@ -262,17 +310,23 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const unsigned short* __rest
     const int bx = (int)(rem % ntx), by = (int)((rem / ntx) % nty), bz = (int)(rem / ((long long)ntx * nty));
     const int x0 = bx * WX, y0 = by * WY, z0 = bz * WZ;
     __syncthreads();
-    load_halo<CIN, WHX, WHY, WHZ>(xt, X + box * M * CIN, x0, y0, z0, nm);
+    load_halo<CIN, WHX, WHY, WHZ, 256>(xt, X + box * M * CIN, x0, y0, z0, nm);
     {
-      constexpr int WPC = COS / 2, NW = WX * WY * WZ * WPC;
+      constexpr int WPC = COS / 2, NW = WX * WY * WZ * WPC, PER = NW / 256;
+      static_assert(NW % 256 == 0, "gradient tile words must fill the workgroup");
       const unsigned* src = reinterpret_cast<const unsigned*>(dZ + box * M * COS);
       unsigned* dst = reinterpret_cast<unsigned*>(dt);
-      for (int e = threadIdx.x; e < NW; e += 256) {
+      unsigned v[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int e = (int)threadIdx.x + k * 256;
         const int c = e / WPC, w = e - c * WPC;
         const int cx = c % WX, cy = (c / WX) % WY, cz = c / (WX * WY);
         const long long gc = ((long long)(z0 + cz) * nm + (y0 + cy)) * nm + x0 + cx;
-        dst[e] = src[gc * WPC + w];
+        v[k] = src[gc * WPC + w];
       }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) dst[threadIdx.x + k * 256] = v[k];
     }
     __syncthreads();
 #pragma unroll 1
