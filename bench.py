@@ -87,7 +87,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("zeldovich", "clustered", "ensemble", "learned"), default="zeldovich",
+    ap.add_argument("--workload", choices=("zeldovich", "clustered", "ensemble", "learned", "kdk"),
+                    default="zeldovich",
                     help="zeldovich: 256^3 lattice on a 512^3 mesh (the headline); clustered: 512^3 particles, "
                          "70%% lattice + 30%% in 2000 Plummer spheres, 1024^3 mesh, TreePM long range r_s = 1.25 cells")
     ap.add_argument("--n-side", type=int, default=256)
@@ -117,6 +118,8 @@ def main() -> int:
         return bench_learned(args)
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "kdk":
+        return bench_kdk(args)
 
     import numpy as np
 
@@ -434,6 +437,156 @@ def bench_ensemble(args) -> int:
         "graph_speedup": round(wall_serial / wall, 3),
         "worst_box_rms_force_err": worst,
     }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_kdk(args) -> int:
+    """Device-resident KDK leapfrog: the headline particles (256^3 Zel'dovich
+    lattice on a 512^3 mesh) with growing-mode momenta, LambdaCDM background,
+    K steps of 2 PM forces each with the state in HBM.  Beside it: the same
+    step driven from the host as the reference's GPU path does (positions to
+    f32 and over PCIe, accelerations back, f64 kick/drift on the host), the
+    FP64 CPU oracle on one step, and both forms at the AP-08 mesh sizes."""
+    import numpy as np
+
+    import gadget_ng_pm_hip as pm
+    from gadget_ng_pm_hip import integrate as gi
+    from gadget_ng_pm_hip import workloads
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import integrate_oracle as io  # the host-driven step and the CPU baseline only
+
+    nm, n_side = args.mesh, args.n_side
+    box = float(n_side)
+    om, ol = 0.3, 0.7
+    h0 = (8.0 * 3.141592653589793 / (3.0 * om)) ** 0.5  # G = 1, mean density 1
+    cosmo = gi.CosmologyParams(om, ol, h0)
+    c_tuple = (om, ol, h0, -1.0, 0.0, 0.0)
+    a0 = 0.02
+    hubble = h0 * (om / a0 ** 3 + ol) ** 0.5
+    dt = 0.01 / hubble  # about 1 % growth of a per step
+    parts = workloads.zeldovich_lattice(n_side=n_side, nm=nm, box=box, spectral_index=-2.0, rms_disp_cells=0.5,
+                                        seed=workloads.SEED_ZELDOVICH, workers=-1)
+    n = parts.n
+    pos = parts.pos64()
+    q = np.stack(np.meshgrid(*(np.arange(n_side) * (box / n_side),) * 3, indexing="ij"), -1)[..., ::-1]
+    psi = (pos - q.reshape(-1, 3) + box / 2) % box - box / 2
+    vel = a0 * a0 * hubble * psi  # growing mode, f = 1: p = a^2 H psi
+    mass = parts.mass64()
+    plan = pm.Plan(nm, box, device=0)
+    tuning = plan.autotune(reps=3) if not args.no_autotune else plan.tuning()
+    lf = gi.Leapfrog(plan, n)
+    lf.set_state(pos, vel, mass)
+    k = min(args.steps, 32)  # two solves per step; the stage-timing ring holds 64
+    a = lf.run(max(args.warmup, 1), dt, 1.0, cosmology=cosmo, a=a0, wrap=True)
+    pm.device_synchronize()
+    plan.set_timing(True)
+    t0 = time.perf_counter()
+    a = lf.run(k, dt, 1.0, cosmology=cosmo, a=a, wrap=True)
+    pm.device_synchronize()
+    wall = time.perf_counter() - t0
+    stages = plan.stage_times()
+    plan.set_timing(False)
+    pm_ms = stages.pop("total")
+    ms_step = wall * 1e3 / k
+    log(f"[bench] {k} KDK steps: {ms_step:.3f} ms/step, {2 * pm_ms:.3f} ms of it in the two PM solves")
+    sb = stage_bytes(n, nm)
+    dom = max(stages, key=stages.get)
+    achieved = sb[dom] / (stages[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                "traffic": measured_traffic(dom, nm, "zeldovich"), "algorithmic_bytes": sb[dom],
+                "mean_ms": round(stages[dom], 5)}
+
+    def host_step(p_, v_, a_, plan_, m32):
+        """The reference's GPU path: f64 state on the host, f32 over PCIe per force call."""
+        g = io.gravity_coupling_qksl(1.0, a_)
+        f = io.drift_kick_factors(c_tuple, a_, dt)
+
+        def force(pp):
+            p32 = pp.astype(np.float32)
+            ax, ay, az = plan_.accelerations_f32(p32[:, 0].copy(), p32[:, 1].copy(), p32[:, 2].copy(), m32, g)
+            return np.stack([ax, ay, az], axis=1).astype(np.float64)
+        p_, v_, _ = io.kdk_step(p_, v_, f, force)
+        return io.wrap_position(p_, box), v_, io.advance_a(c_tuple, a_, dt)
+
+    p_h, v_h, _ = lf.state()
+    m32 = mass.astype(np.float32)
+    host_step(p_h, v_h, a, plan, m32)
+    t1 = time.perf_counter()
+    host_step(p_h, v_h, a, plan, m32)
+    host_ms = (time.perf_counter() - t1) * 1e3
+    log(f"[bench] host-driven step (PCIe per force call, host kick/drift): {host_ms:.1f} ms")
+
+    small = {}
+    for grid in (32, 64):
+        qd = workloads.ap08_curve(grid)
+        gp = pm.Plan(grid, qd.box, device=0)
+        pv = qd.pos64() % qd.box
+        vv = np.zeros_like(pv)
+        mv = qd.mass64()
+        sl = gi.Leapfrog(gp, qd.n)
+        sl.set_state(pv, vv, mv)
+        sl.run(3, 0.01, 1.0, wrap=True)
+        pm.device_synchronize()
+        t1 = time.perf_counter()
+        sl.run(50, 0.01, 1.0, wrap=True)
+        pm.device_synchronize()
+        dev_ms = (time.perf_counter() - t1) * 1e3 / 50
+        m32s = mv.astype(np.float32)
+
+        def hforce(pp, g=1.0):
+            p32 = pp.astype(np.float32)
+            ax, ay, az = gp.accelerations_f32(p32[:, 0].copy(), p32[:, 1].copy(), p32[:, 2].copy(), m32s, g)
+            return np.stack([ax, ay, az], axis=1).astype(np.float64)
+        io.run(pv, vv, 2, 0.01, 1.0, hforce, box=qd.box)
+        t1 = time.perf_counter()
+        io.run(pv, vv, 20, 0.01, 1.0, hforce, box=qd.box)
+        h_ms = (time.perf_counter() - t1) * 1e3 / 20
+        small[f"grid{grid}"] = {"n": qd.n, "device_resident_ms_per_step": round(dev_ms, 4),
+                                "host_driven_ms_per_step": round(h_ms, 4)}
+        sl.close()
+        gp.close()
+    log(f"[bench] AP-08 sizes: {small}")
+
+    result = {
+        "metric": "M particle-steps/s, device-resident KDK leapfrog (2 PM forces per step), 256^3 Zel'dovich "
+                  "particles on a 512^3 mesh, LambdaCDM",
+        "value": round(n * k / wall / 1e6, 2), "unit": "M particle-steps/s", "n_gpus": 1, "steps": k,
+        "warmup": max(args.warmup, 1), "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 state, f32 force", "data": "synthetic",
+        "config": {"workload": f"Zel'dovich lattice {n_side}^3 on {nm}^3 mesh, L={box:g}, growing-mode momenta "
+                               f"p = a^2 H psi, Omega_m=0.3, Omega_L=0.7, h0={h0:.4f} (G=1, mean density 1), "
+                               f"a0={a0}, dt={dt:.4e} (1% growth of a), wrap on",
+                   "particles": n, "mesh": nm, "launch_choices": tuning, "final_a": a},
+        "roofline": roofline,
+        "pm_ms_per_solve": round(pm_ms, 4),
+        "integrator_ms_per_step": round(ms_step - 2 * pm_ms, 4),
+        # kick+drift and kick+wrap move 120 bytes per particle each
+        "integrator_gbps": round(240.0 * n / ((ms_step - 2 * pm_ms) * 1e-3) / 1e9, 1),
+        "stage_ms": {kk: round(v, 5) for kk, v in stages.items()},
+        "host_driven_ms_per_step": round(host_ms, 2),
+        "device_resident_speedup": round(host_ms / ms_step, 2),
+        "ap08_sizes": small,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        import oracle  # CPU baseline only
+
+        oracle.build()
+        p_c, v_c, _ = lf.state()
+        t1 = time.perf_counter()
+        io.run(p_c, v_c, 1, dt, 1.0, lambda pp, g: oracle.pm_accelerations(pp, mass, box, nm, g,
+                                                                           mode=oracle.MODE_LEAN),
+               cosmology=c_tuple, a=a, box=box)
+        tc = time.perf_counter() - t1
+        result["cpu_baseline"] = {"value": round(n / tc / 1e6, 4), "unit": "M particle-steps/s",
+                                  "cores": oracle.get_threads(), "kind": "port",
+                                  "sample": f"one KDK step of the full state (2 FP64 oracle PM solves, lean R2C "
+                                            f"mode, plus the f64 kick/drift in numpy): {tc:.2f} s"}
+    lf.close()
+    plan.close()
     print(json.dumps(result), flush=True)
     return 0
 

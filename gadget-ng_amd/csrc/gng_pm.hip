@@ -742,6 +742,13 @@ int gng_pm_accelerations_f64(gng_pm_plan* p, const double* pos, const double* ma
   return GNG_OK;
 }
 
+int gng_pm_get_config(const gng_pm_plan* p, gng_pm_config* out) {
+  if (!p || !out) return fail(GNG_ERR_INVALID, "null argument");
+  *out = p->cfg;
+  out->device = p->device;
+  return GNG_OK;
+}
+
 int gng_pm_solve_device(gng_pm_plan* p, const float* d_x, const float* d_y, const float* d_z, const float* d_mass,
                         int64_t n, const int64_t* d_idx, int64_t k, double g, float* d_ax, float* d_ay,
                         float* d_az, void* stream) {

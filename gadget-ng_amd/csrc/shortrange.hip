@@ -8,8 +8,11 @@
 //
 // Method:
 //   1. particles are counted into a chaining mesh of nc^3 cells of side
-//      L/nc >= r_cut, scanned, and scattered cell by cell (float4 x,y,z,m and
-//      the original index);
+//      L/nc >= r_cut and scanned; a stable radix sort by cell (rocPRIM) puts
+//      them in cell order with ascending particle index inside each cell, and
+//      they are gathered as float4 x,y,z,m plus the original index.  The
+//      order of every sum is then fixed, so the result is bit-identical from
+//      call to call;
 //   2. the work list is (cell, chunk of 256 particles); its length is read
 //      back once to size the grid, so every workgroup has one chunk and the
 //      hardware dispatcher balances cells of very different occupancy;
@@ -24,6 +27,9 @@
 #include <cmath>
 #include <new>
 #include <string>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "../../include/gng_learned.h"
 #include "status.hpp"
@@ -92,16 +98,15 @@ __global__ __launch_bounds__(1024) void k_scan(const unsigned* __restrict__ cnt,
   }
 }
 
-__global__ void k_scatter(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-                          const float* __restrict__ m, long long n, const unsigned* __restrict__ cell_of,
-                          const unsigned* __restrict__ off, unsigned* __restrict__ cursor,
-                          float4* __restrict__ sorted, unsigned* __restrict__ sidx) {
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const unsigned c = cell_of[p];
-  const unsigned at = off[c] + atomicAdd(&cursor[c], 1u);
-  sorted[at] = make_float4(x[p], y[p], z[p], m[p]);
-  sidx[at] = (unsigned)p;
+// sorted[i] = particle order[i] (the cell-sorted permutation)
+__global__ void k_gather(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+                         const float* __restrict__ m, long long n, const unsigned* __restrict__ order,
+                         float4* __restrict__ sorted, unsigned* __restrict__ sidx) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned p = order[i];
+  sorted[i] = make_float4(x[p], y[p], z[p], m[p]);
+  sidx[i] = p;
 }
 
 struct Params {
@@ -203,11 +208,14 @@ struct gng_sr {
   float4* d_sorted = nullptr;
   unsigned* d_sidx = nullptr;
   unsigned* d_cell_of = nullptr;
+  unsigned* d_cell_sorted = nullptr;  // radix-sort outputs
+  unsigned* d_order = nullptr;
+  void* d_temp = nullptr;  // radix-sort work space
+  size_t temp_bytes = 0;
   long long cap_cells = 0;
   unsigned* d_cnt = nullptr;
   unsigned* d_off = nullptr;
   unsigned* d_choff = nullptr;
-  unsigned* d_cursor = nullptr;
   unsigned long long* d_pairs = nullptr;
   long long last_pairs = -1;
   bool pairs_pending = false;
@@ -219,10 +227,12 @@ void free_sr(gng_sr* s) {
   (void)hipFree(s->d_sorted);
   (void)hipFree(s->d_sidx);
   (void)hipFree(s->d_cell_of);
+  (void)hipFree(s->d_cell_sorted);
+  (void)hipFree(s->d_order);
+  (void)hipFree(s->d_temp);
   (void)hipFree(s->d_cnt);
   (void)hipFree(s->d_off);
   (void)hipFree(s->d_choff);
-  (void)hipFree(s->d_cursor);
   (void)hipFree(s->d_pairs);
 }
 
@@ -278,37 +288,56 @@ int gng_sr_accelerations(gng_sr* sr, const float* d_x, const float* d_y, const f
     (void)hipFree(sr->d_sorted);
     (void)hipFree(sr->d_sidx);
     (void)hipFree(sr->d_cell_of);
+    (void)hipFree(sr->d_cell_sorted);
+    (void)hipFree(sr->d_order);
     sr->d_sorted = nullptr;
-    sr->d_sidx = sr->d_cell_of = nullptr;
+    sr->d_sidx = sr->d_cell_of = sr->d_cell_sorted = sr->d_order = nullptr;
     sr->cap_n = 0;
     GNG_HIP(hipMalloc(&sr->d_sorted, sizeof(float4) * (size_t)n));
     GNG_HIP(hipMalloc(&sr->d_sidx, sizeof(unsigned) * (size_t)n));
     GNG_HIP(hipMalloc(&sr->d_cell_of, sizeof(unsigned) * (size_t)n));
+    GNG_HIP(hipMalloc(&sr->d_cell_sorted, sizeof(unsigned) * (size_t)n));
+    GNG_HIP(hipMalloc(&sr->d_order, sizeof(unsigned) * (size_t)n));
     sr->cap_n = n;
+  }
+  // stable LSD radix sort over the bits a cell index needs
+  unsigned end_bit = 1;
+  while ((1LL << end_bit) < ncell) ++end_bit;
+  const rocprim::counting_iterator<unsigned> iota(0u);
+  size_t need = 0;
+  GNG_HIP(rocprim::radix_sort_pairs(nullptr, need, sr->d_cell_of, sr->d_cell_sorted, iota, sr->d_order,
+                                    (unsigned)n, 0u, end_bit, s));
+  if (need > sr->temp_bytes) {
+    (void)hipFree(sr->d_temp);
+    sr->d_temp = nullptr;
+    sr->temp_bytes = 0;
+    GNG_HIP(hipMalloc(&sr->d_temp, need));
+    sr->temp_bytes = need;
   }
   if (ncell > sr->cap_cells) {
     (void)hipFree(sr->d_cnt);
     (void)hipFree(sr->d_off);
     (void)hipFree(sr->d_choff);
-    (void)hipFree(sr->d_cursor);
-    sr->d_cnt = sr->d_off = sr->d_choff = sr->d_cursor = nullptr;
+    sr->d_cnt = sr->d_off = sr->d_choff = nullptr;
     sr->cap_cells = 0;
     GNG_HIP(hipMalloc(&sr->d_cnt, sizeof(unsigned) * (size_t)ncell));
     GNG_HIP(hipMalloc(&sr->d_off, sizeof(unsigned) * (size_t)(ncell + 1)));
     GNG_HIP(hipMalloc(&sr->d_choff, sizeof(unsigned) * (size_t)(ncell + 1)));
-    GNG_HIP(hipMalloc(&sr->d_cursor, sizeof(unsigned) * (size_t)ncell));
     sr->cap_cells = ncell;
   }
   const unsigned cb = (unsigned)((ncell + 255) / 256), pb = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_fill_u32, dim3(cb), dim3(256), 0, s, sr->d_cnt, ncell, 0u);
-  hipLaunchKernelGGL(k_fill_u32, dim3(cb), dim3(256), 0, s, sr->d_cursor, ncell, 0u);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s, sr->d_pairs);
   const float cs = (float)((double)nc / box_size);
   hipLaunchKernelGGL(k_count, dim3(pb), dim3(256), 0, s, d_x, d_y, d_z, (long long)n, nc, cs, sr->d_cnt,
                      sr->d_cell_of);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, sr->d_cnt, ncell, sr->d_off, sr->d_choff);
-  hipLaunchKernelGGL(k_scatter, dim3(pb), dim3(256), 0, s, d_x, d_y, d_z, d_m, (long long)n, sr->d_cell_of,
-                     sr->d_off, sr->d_cursor, sr->d_sorted, sr->d_sidx);
+  GNG_HIP(hipGetLastError());
+  size_t tb = sr->temp_bytes;
+  GNG_HIP(rocprim::radix_sort_pairs(sr->d_temp, tb, sr->d_cell_of, sr->d_cell_sorted, iota, sr->d_order, (unsigned)n,
+                                    0u, end_bit, s));
+  hipLaunchKernelGGL(k_gather, dim3(pb), dim3(256), 0, s, d_x, d_y, d_z, d_m, (long long)n, sr->d_order,
+                     sr->d_sorted, sr->d_sidx);
   GNG_HIP(hipGetLastError());
   unsigned chunks = 0;
   GNG_HIP(hipMemcpyAsync(&chunks, sr->d_choff + ncell, sizeof(unsigned), hipMemcpyDeviceToHost, s));

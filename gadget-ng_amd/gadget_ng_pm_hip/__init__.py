@@ -47,22 +47,39 @@ GNG_ERR_UNSUPPORTED = 5
 
 STAGES = ("bucket", "sort", "deposit", "fft_x_fwd", "fft_y_fwd", "fft_z_green", "fft_y_inv", "fft_x_inv", "interp")
 
-SOURCES = ("gng_pm.hip", "learned.hip", "shortrange.hip", "pm64.hip")
+SOURCES = ("gng_pm.hip", "learned.hip", "shortrange.hip", "pm64.hip", "integrate.hip")
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def build(verbose: bool = False) -> str:
-    """Compile libgng_pm.so for gfx950 in-tree (always recompiles)."""
+    """Compile libgng_pm.so for gfx950 in-tree (always recompiles): one hipcc
+    per source in parallel, then one link."""
+    import tempfile
     os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp] + [os.path.join(CSRC, f) for f in SOURCES]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
-    if verbose and r.stderr:
-        print(r.stderr)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="gng_build_") as work:
+        objs, procs = [], []
+        for f in SOURCES:
+            obj = os.path.join(work, f + ".o")
+            objs.append(obj)
+            procs.append(subprocess.Popen([hipcc, *compile_flags, "-c", os.path.join(CSRC, f), "-o", obj],
+                                          stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
+        errors = []
+        for f, pr in zip(SOURCES, procs):
+            out, err = pr.communicate()
+            if pr.returncode != 0:
+                errors.append(f"{f}:\n{out}{err}")
+            elif verbose and err:
+                print(err)
+        if errors:
+            raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
+        r = subprocess.run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("hipcc link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 
@@ -191,6 +208,23 @@ def lib():
         L.gng_pm64_accelerations.argtypes = [P, P, P, i64, P, i64, d, P]
         L.gng_pm64_solve_device.argtypes = [P, P, P, i64, P, i64, d, P, P]
         for f in (L.gng_pm64_create, L.gng_pm64_accelerations, L.gng_pm64_solve_device):
+            f.restype = i32
+        L.gng_pm_get_config.argtypes = [P, ctypes.POINTER(GngConfig)]
+        L.gng_pm_get_config.restype = i32
+        # device-resident KDK leapfrog (include/gng_integrate.h)
+        L.gng_cosmo_advance_a.argtypes = [P, d, d]
+        L.gng_cosmo_advance_a.restype = d
+        L.gng_cosmo_drift_kick_factors.argtypes = [P, d, d, P]
+        L.gng_kdk_create.argtypes = [P, i64, ctypes.POINTER(P)]
+        L.gng_kdk_destroy.argtypes = [P]
+        L.gng_kdk_destroy.restype = None
+        L.gng_kdk_set_short_range.argtypes = [P, P, d, d, d, i32]
+        L.gng_kdk_set_state.argtypes = [P, P, P, P]
+        L.gng_kdk_get_state.argtypes = [P, P, P, P]
+        L.gng_kdk_step.argtypes = [P, d, d, d, d, i32, P]
+        L.gng_kdk_run.argtypes = [P, P, P, d, d, i32, i32, P]
+        for f in (L.gng_cosmo_drift_kick_factors, L.gng_kdk_create, L.gng_kdk_set_short_range, L.gng_kdk_set_state,
+                  L.gng_kdk_get_state, L.gng_kdk_step, L.gng_kdk_run):
             f.restype = i32
         L.hip_pm_create.argtypes = [i32, f32]
         L.hip_pm_create.restype = P

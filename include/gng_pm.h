@@ -99,6 +99,10 @@ int gng_pm_create(const gng_pm_config *cfg, gng_pm_plan **out);
 
 void gng_pm_destroy(gng_pm_plan *p);
 
+/* The configuration the plan was created with, device resolved to the
+ * ordinal in use. */
+int gng_pm_get_config(const gng_pm_plan *p, gng_pm_config *out);
+
 /* Mirrors GravitySolver::accelerations_for_indices
  * (crates/gadget-ng-core/src/gravity.rs:16-24, PmSolver at
  * crates/gadget-ng-pm/src/solver.rs:74-140): pos is n x 3 f64 interleaved

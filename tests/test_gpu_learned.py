@@ -93,6 +93,19 @@ def test_short_range_matches_direct_sum(learned, kernel, L):
     assert sr.last_pair_count() > len(pos)
 
 
+def test_short_range_is_bit_identical_from_call_to_call(learned):
+    """The cell lists come from a stable sort, so every sum runs in one fixed
+    order; a clustered set (many particles per cell) and a second object
+    (fresh buffers) give the same bits."""
+    print("case short range determinism", flush=True)
+    pos, m = _clustered(3000, 5000, 20.0, 11)
+    a = learned.ShortRange(device=0).accelerations(pos, m, 20.0, 0.5, 2.5, eps2=0.01)
+    sr = learned.ShortRange(device=0)
+    b = sr.accelerations(pos, m, 20.0, 0.5, 2.5, eps2=0.01)
+    c = sr.accelerations(pos, m, 20.0, 0.5, 2.5, eps2=0.01)
+    assert np.array_equal(a, b) and np.array_equal(b, c)
+
+
 def test_treepm_long_plus_short_is_newton(pm, learned):
     g = np.random.default_rng(5)
     L, n, nm = 1.0, 12, 64
