@@ -28,6 +28,7 @@
 
 #include "../../include/gng_pm.h"
 #include "deposit.hpp"
+#include "pk_kernels.hpp"
 #include "pm_kernels.hpp"
 #include "status.hpp"
 
@@ -190,6 +191,11 @@ struct MeshOps {
   void (*y_inv_persist)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
   void (*y_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
   void (*x_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, float2*, const float2*, int, float);
+  // power spectrum back end
+  void (*z_pk)(dim3, dim3, hipStream_t, const float2*, const float2*, const double*, double*, long long,
+               unsigned long long*);
+  void (*pk_col0)(dim3, dim3, hipStream_t, const float2*, const float2*, const double*, double*, long long, long long,
+                  unsigned long long*);
 };
 
 template <int N>
@@ -240,6 +246,14 @@ MeshOps make_ops() {
   o.x_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* P, float2* B, float2* pot, const float2* tw,
                 int field0, float pot_scale) {
     hipLaunchKernelGGL((k_fft_x_inv<N, R>), g, b, 0, s, A, P, B, pot, tw, field0, pot_scale);
+  };
+  o.z_pk = [](dim3 g, dim3 b, hipStream_t s, const float2* P, const float2* tw, const double* w, double* part,
+              long long nblk, unsigned long long* modes) {
+    hipLaunchKernelGGL((k_fft_z_pk<N, C, TY>), g, b, 0, s, P, tw, w, part, nblk, modes);
+  };
+  o.pk_col0 = [](dim3 g, dim3 b, hipStream_t s, const float2* P, const float2* tw, const double* w, double* part,
+                 long long nblk, long long base, unsigned long long* modes) {
+    hipLaunchKernelGGL((k_pk_col0<N>), g, b, 0, s, P, tw, w, part, nblk, base, modes);
   };
   return o;
 }
@@ -307,6 +321,14 @@ struct gng_pm_plan {
 
   long long last_n = 0;
   bool density_valid = false;
+
+  // power spectrum work space (allocated on first use)
+  double* d_pk_w = nullptr;     // 1 / sinc(freq(i) / nm)^2, nm values
+  double* d_pk_part = nullptr;  // per-workgroup shell sums
+  long long pk_part_cap = 0;
+  unsigned long long* d_pk_modes = nullptr;
+  double* d_pk_msum = nullptr;  // mass partials
+  double* d_pk_out = nullptr;   // shell sums times (V / M_tot)^2
   // Stage timing: every solve records kStages+1 events into the next slot of
   // a ring; nothing synchronises until gng_pm_stage_times() reads them.
   bool timing = false;
@@ -335,7 +357,8 @@ void free_plan(gng_pm_plan* p) {
   (void)hipSetDevice(p->device);
   for (void* q : {(void*)p->d_P, (void*)p->d_A, (void*)p->d_B, (void*)p->d_rho, (void*)p->d_pot, (void*)p->d_tw, (void*)p->d_deconv,
                   (void*)p->d_sc, (void*)p->d_count, (void*)p->d_offset, (void*)p->d_partial, (void*)p->d_part,
-                  (void*)p->d_idx, (void*)p->d_tile_rank, (void*)p->d_tile_order})
+                  (void*)p->d_idx, (void*)p->d_tile_rank, (void*)p->d_tile_order, (void*)p->d_pk_w,
+                  (void*)p->d_pk_part, (void*)p->d_pk_modes, (void*)p->d_pk_msum, (void*)p->d_pk_out})
     (void)hipFree(q);
   free_work(p);
   for (auto& slot : p->ev)
@@ -384,9 +407,10 @@ int ensure_work(gng_pm_plan* p, long long n) {
   return GNG_OK;
 }
 
-// The whole pipeline on device pointers.
-int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z, const float* m, long long n,
-                 const long long* idx, long long k, double g, float* ax, float* ay, float* az, hipStream_t s) {
+// Deposit and the forward x and y transforms (shared by the force solve and
+// the power spectrum); the mass mesh is in P as the half spectrum after y.
+int run_front(gng_pm_plan* p, const float* x, const float* y, const float* z, const float* m, long long n,
+              hipStream_t s) {
   int rc = ensure_work(p, n > 0 ? n : 1);
   if (rc) return rc;
   const int nm = p->cfg.grid_size;
@@ -433,6 +457,22 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   else
     o.y_fwd(yz_grid, yz_block, s, p->d_P, p->d_tw);
   mark(p, 5, s);
+  return GNG_OK;
+}
+
+// The whole pipeline on device pointers.
+int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z, const float* m, long long n,
+                 const long long* idx, long long k, double g, float* ax, float* ay, float* az, hipStream_t s) {
+  int rc = run_front(p, x, y, z, m, n, s);
+  if (rc) return rc;
+  const int nm = p->cfg.grid_size;
+  const int H = nm / 2;
+  const MeshOps& o = *p->ops;
+  const CellMap cm = cell_map(p);
+  const long long rows = (long long)nm * nm;
+  const long long fplane = 2 * mesh_plane(nm);
+  const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
+  const dim3 yz_block((unsigned)o.yz_threads);
   const GreenParams gp = green_params(p, g);
   {
     // persistent z pass (one workgroup per CU) or one workgroup per item
@@ -547,6 +587,87 @@ int solve_host_soa(gng_pm_plan* p, const float* x, const float* y, const float* 
     GNG_HIP(hipMemcpyAsync(az, daz, kb, hipMemcpyDeviceToHost, s));
   }
   GNG_HIP(hipStreamSynchronize(s));
+  return GNG_OK;
+}
+
+constexpr int kPkMassBlocks = 1024;
+
+// Power spectrum of n particles on device pointers: shell sums (already times
+// (V / M_tot)^2) and mode counts, nm/2 shells, copied to the host; the call
+// synchronises s.
+int run_pk(gng_pm_plan* p, const float* x, const float* y, const float* z, const float* m, long long n,
+           hipStream_t s, double* sums, unsigned long long* modes) {
+  const int nm = p->cfg.grid_size, H = nm / 2;
+  const MeshOps& o = *p->ops;
+  if (!p->d_pk_w) {
+    // 1 / sinc(f / nm)^2, sinc(x) = sin(pi x) / (pi x), at the signed
+    // frequency f of each index (power_spectrum.rs:130-145)
+    std::vector<double> w((size_t)nm);
+    for (int i = 0; i < nm; ++i) {
+      const int f = i <= H ? i : i - nm;
+      const double xv = (double)f / (double)nm;
+      double sc = 1.0;
+      if (std::fabs(xv) >= 1e-12) {
+        const double px = M_PI * xv;
+        sc = std::sin(px) / px;
+      }
+      w[i] = 1.0 / (sc * sc);
+    }
+    GNG_HIP(hipMalloc(&p->d_pk_modes, sizeof(unsigned long long) * (size_t)H));
+    GNG_HIP(hipMalloc(&p->d_pk_msum, sizeof(double) * (kPkMassBlocks + 1)));
+    GNG_HIP(hipMalloc(&p->d_pk_out, sizeof(double) * (size_t)H));
+    GNG_HIP(hipMalloc(&p->d_pk_w, sizeof(double) * (size_t)nm));
+    GNG_HIP(hipMemcpy(p->d_pk_w, w.data(), sizeof(double) * (size_t)nm, hipMemcpyHostToDevice));
+  }
+  const long long zblocks = (long long)(H / o.yz_cols) * nm, nblk = zblocks + H + 1;
+  if (nblk > p->pk_part_cap) {
+    (void)hipFree(p->d_pk_part);
+    p->d_pk_part = nullptr;
+    p->pk_part_cap = 0;
+    GNG_HIP(hipMalloc(&p->d_pk_part, sizeof(double) * (size_t)nblk * (size_t)H));
+    p->pk_part_cap = nblk;
+  }
+  int rc = run_front(p, x, y, z, m, n, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((H + 255) / 256)), dim3(256), 0, s, p->d_pk_modes, H);
+  hipLaunchKernelGGL(k_mass_partials, dim3(kPkMassBlocks), dim3(256), 0, s, m, n, p->d_pk_msum);
+  hipLaunchKernelGGL(k_mass_total, dim3(1), dim3(1024), 0, s, p->d_pk_msum, kPkMassBlocks);
+  o.z_pk(dim3((unsigned)(H / o.yz_cols), (unsigned)nm), dim3((unsigned)o.yz_threads), s, p->d_P, p->d_tw, p->d_pk_w,
+         p->d_pk_part, nblk, p->d_pk_modes);
+  o.pk_col0(dim3((unsigned)(H + 1)), dim3(256), s, p->d_P, p->d_tw, p->d_pk_w, p->d_pk_part, nblk, zblocks,
+            p->d_pk_modes);
+  // in the stage timings the z transform and binning take the z stage's
+  // slot; the inverse stages and the interpolation are empty
+  for (int i = 6; i < kStages + 1; ++i) mark(p, i, s);
+  if (p->timing) ++p->timed_solves;
+  const double L = p->cfg.box_size;
+  hipLaunchKernelGGL(k_pk_reduce, dim3((unsigned)H), dim3(256), 0, s, p->d_pk_part, nblk,
+                     p->d_pk_msum + kPkMassBlocks, L * L * L, p->d_pk_out);
+  GNG_HIP(hipGetLastError());
+  GNG_HIP(hipMemcpyAsync(sums, p->d_pk_out, sizeof(double) * (size_t)H, hipMemcpyDeviceToHost, s));
+  GNG_HIP(hipMemcpyAsync(modes, p->d_pk_modes, sizeof(unsigned long long) * (size_t)H, hipMemcpyDeviceToHost, s));
+  GNG_HIP(hipStreamSynchronize(s));
+  p->density_valid = p->keep_density;
+  p->potential_valid = false;
+  return GNG_OK;
+}
+
+// Non-empty shells in increasing k: k = (s + 1) 2 pi / L, P = sum / modes
+// (power_spectrum.rs:115-126).
+int pk_bins(const gng_pm_plan* p, const double* sums, const unsigned long long* modes, int32_t capacity, double* k,
+            double* pk, uint64_t* n_modes, int32_t* n_bins) {
+  const int H = p->cfg.grid_size / 2;
+  const double k_fund = 2.0 * M_PI / p->cfg.box_size;
+  int nb = 0;
+  for (int b = 0; b < H; ++b) {
+    if (modes[b] == 0) continue;
+    if (nb >= capacity) return fail(GNG_ERR_INVALID, "capacity below the number of shells (grid_size / 2)");
+    if (k) k[nb] = ((double)b + 1.0) * k_fund;
+    if (pk) pk[nb] = sums[b] / (double)modes[b];
+    if (n_modes) n_modes[nb] = modes[b];
+    ++nb;
+  }
+  *n_bins = nb;
   return GNG_OK;
 }
 
@@ -740,6 +861,53 @@ int gng_pm_accelerations_f64(gng_pm_plan* p, const double* pos, const double* ma
     out[3 * j + 2] = az[j];
   }
   return GNG_OK;
+}
+
+int gng_pm_power_spectrum(gng_pm_plan* p, const float* d_x, const float* d_y, const float* d_z, const float* d_mass,
+                          int64_t n, int32_t capacity, double* k, double* pk, uint64_t* n_modes, int32_t* n_bins,
+                          void* stream) {
+  if (!p || !n_bins) return fail(GNG_ERR_INVALID, "null argument");
+  if (n <= 0) return fail(GNG_ERR_INVALID, "the power spectrum needs at least one particle");
+  if (n >= (1LL << 31)) return fail(GNG_ERR_INVALID, "n of 2^31 or more is not supported");
+  if (!d_x || !d_y || !d_z || !d_mass) return fail(GNG_ERR_INVALID, "null particle array");
+  std::lock_guard<std::mutex> lk(p->mu);
+  int rc = set_device(p);
+  if (rc) return rc;
+  const int H = p->cfg.grid_size / 2;
+  std::vector<double> sums((size_t)H);
+  std::vector<unsigned long long> modes((size_t)H);
+  if ((rc = run_pk(p, d_x, d_y, d_z, d_mass, n, reinterpret_cast<hipStream_t>(stream), sums.data(), modes.data())))
+    return rc;
+  return pk_bins(p, sums.data(), modes.data(), capacity, k, pk, n_modes, n_bins);
+}
+
+int gng_pm_power_spectrum_f64(gng_pm_plan* p, const double* pos, const double* mass, int64_t n, int32_t capacity,
+                              double* k, double* pk, uint64_t* n_modes, int32_t* n_bins) {
+  if (!p || !n_bins) return fail(GNG_ERR_INVALID, "null argument");
+  if (n <= 0) return fail(GNG_ERR_INVALID, "the power spectrum needs at least one particle");
+  if (n >= (1LL << 31)) return fail(GNG_ERR_INVALID, "n of 2^31 or more is not supported");
+  if (!pos || !mass) return fail(GNG_ERR_INVALID, "null input array");
+  std::vector<float> h(4 * (size_t)n);
+  float *x = h.data(), *y = x + n, *z = y + n, *m = z + n;
+  for (long long i = 0; i < n; ++i) {
+    x[i] = (float)pos[3 * i];
+    y[i] = (float)pos[3 * i + 1];
+    z[i] = (float)pos[3 * i + 2];
+    m[i] = (float)mass[i];
+  }
+  std::lock_guard<std::mutex> lk(p->mu);
+  int rc = set_device(p);
+  if (rc) return rc;
+  if ((rc = ensure_particles(p, n, 0, false))) return rc;
+  const size_t cap = (size_t)p->cap, nb = sizeof(float) * (size_t)n;
+  float* d = p->d_part;
+  hipStream_t s = p->stream;
+  for (int c = 0; c < 4; ++c) GNG_HIP(hipMemcpyAsync(d + c * cap, h.data() + c * (size_t)n, nb, hipMemcpyHostToDevice, s));
+  const int H = p->cfg.grid_size / 2;
+  std::vector<double> sums((size_t)H);
+  std::vector<unsigned long long> modes((size_t)H);
+  if ((rc = run_pk(p, d, d + cap, d + 2 * cap, d + 3 * cap, n, s, sums.data(), modes.data()))) return rc;
+  return pk_bins(p, sums.data(), modes.data(), capacity, k, pk, n_modes, n_bins);
 }
 
 int gng_pm_get_config(const gng_pm_plan* p, gng_pm_config* out) {

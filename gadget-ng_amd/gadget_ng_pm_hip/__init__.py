@@ -209,6 +209,10 @@ def lib():
         L.gng_pm64_solve_device.argtypes = [P, P, P, i64, P, i64, d, P, P]
         for f in (L.gng_pm64_create, L.gng_pm64_accelerations, L.gng_pm64_solve_device):
             f.restype = i32
+        L.gng_pm_power_spectrum.argtypes = [P, P, P, P, P, i64, i32, P, P, P, ctypes.POINTER(i32), P]
+        L.gng_pm_power_spectrum_f64.argtypes = [P, P, P, i64, i32, P, P, P, ctypes.POINTER(i32)]
+        L.gng_pm_power_spectrum.restype = i32
+        L.gng_pm_power_spectrum_f64.restype = i32
         L.gng_pm_get_config.argtypes = [P, ctypes.POINTER(GngConfig)]
         L.gng_pm_get_config.restype = i32
         # device-resident KDK leapfrog (include/gng_integrate.h)

@@ -125,6 +125,26 @@ int gng_pm_solve_device(gng_pm_plan *p, const float *d_x, const float *d_y, cons
                         const float *d_mass, int64_t n, const int64_t *d_idx, int64_t k, double g,
                         float *d_ax, float *d_ay, float *d_az, void *stream);
 
+/* Power spectrum of the particles' CIC density on the plan's mesh: mirrors
+ * power_spectrum (crates/gadget-ng-analysis/src/power_spectrum.rs:35-127).
+ * delta = rho / rho_bar - 1; P(k) = <|delta_k|^2 / W(k)^2> (V / nm^3)^2 over
+ * the shell floor(|n| - 1/2) of integer wavenumber |n| in [1/2, nm/2), with
+ * W = prod_alpha sinc(n_alpha / nm); k = (shell + 1) 2 pi / L.  Writes the
+ * non-empty shells in increasing k (n_bins of them, at most nm/2; capacity
+ * must allow nm/2) to k, pk and n_modes (any may be NULL).  The plan's
+ * filters and G play no part; the mass mesh is the force solve's (64-bit
+ * fixed point, f32 per cell) and the transform FP32.  Device pointers on
+ * `stream`; the call synchronises it.  n must be positive; zero total mass
+ * gives non-finite P as in the reference. */
+int gng_pm_power_spectrum(gng_pm_plan *p, const float *d_x, const float *d_y, const float *d_z,
+                          const float *d_mass, int64_t n, int32_t capacity, double *k, double *pk,
+                          uint64_t *n_modes, int32_t *n_bins, void *stream);
+
+/* Same from host f64 arrays: pos n x 3 interleaved (Vec3), mass n. */
+int gng_pm_power_spectrum_f64(gng_pm_plan *p, const double *pos, const double *mass, int64_t n,
+                              int32_t capacity, double *k, double *pk, uint64_t *n_modes,
+                              int32_t *n_bins);
+
 /* Keep the density of the following solves for gng_pm_density (costs one
  * extra nm^3 f32 buffer; off by default, when the density is written straight
  * into the spectrum buffer). */
