@@ -58,16 +58,22 @@ struct KSteps {
   static constexpr int v = (27 * CIN + 31) / 32;
 };
 
-__device__ __forceinline__ unsigned short f2bf(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// FP32 -> bfloat16, round to nearest even (gfx950's v_cvt_pk_bf16_f32)
+__device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
+// two values in one conversion: a in the low half, b in the high half
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// sigma(z) = 1 / (1 + e^-z) with the hardware reciprocal (1 ulp); the
+// activations are stored in bfloat16, 16 bits coarser
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -101,21 +107,39 @@ __device__ __forceinline__ void load_halo(unsigned short* lds, const unsigned sh
   const W* src = reinterpret_cast<const W*>(in);
   W* dst = reinterpret_cast<W*>(lds);
   W v[PER];
+  // Tiles away from the box faces need no periodic wrap: their halo is one
+  // affine block of cells (cell indices inside a box stay below nm^3 <= 2^30).
+  const bool interior = x0 >= 1 && x0 + HXv - 1 <= nm && y0 >= 1 && y0 + HYv - 1 <= nm && z0 >= 1 &&
+                        z0 + HZv - 1 <= nm;
+  if (interior) {
+    const unsigned base = ((unsigned)(z0 - 1) * (unsigned)nm + (unsigned)(y0 - 1)) * (unsigned)nm + (unsigned)(x0 - 1);
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int e = (int)threadIdx.x + k * NT;
-    if (NW % NT == 0 || e < NW) {
-      const int c = e / WPC, w = e - c * WPC;
-      const int hx = c % HXv, hy = (c / HXv) % HYv, hz = c / (HXv * HYv);
-      int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-      gx += gx < 0 ? nm : 0;
-      gx -= gx >= nm ? nm : 0;
-      gy += gy < 0 ? nm : 0;
-      gy -= gy >= nm ? nm : 0;
-      gz += gz < 0 ? nm : 0;
-      gz -= gz >= nm ? nm : 0;
-      const long long gc = ((long long)gz * nm + gy) * nm + gx;
-      v[k] = src[gc * WPC + w];
+    for (int k = 0; k < PER; ++k) {
+      const int e = (int)threadIdx.x + k * NT;
+      if (NW % NT == 0 || e < NW) {
+        const int c = e / WPC, w = e - c * WPC;
+        const int hx = c % HXv, hy = (c / HXv) % HYv, hz = c / (HXv * HYv);
+        const unsigned gc = base + ((unsigned)hz * (unsigned)nm + (unsigned)hy) * (unsigned)nm + (unsigned)hx;
+        v[k] = src[(size_t)gc * WPC + w];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = (int)threadIdx.x + k * NT;
+      if (NW % NT == 0 || e < NW) {
+        const int c = e / WPC, w = e - c * WPC;
+        const int hx = c % HXv, hy = (c / HXv) % HYv, hz = c / (HXv * HYv);
+        int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+        gx += gx < 0 ? nm : 0;
+        gx -= gx >= nm ? nm : 0;
+        gy += gy < 0 ? nm : 0;
+        gy -= gy >= nm ? nm : 0;
+        gz += gz < 0 ? nm : 0;
+        gz -= gz >= nm ? nm : 0;
+        const unsigned gc = ((unsigned)gz * (unsigned)nm + (unsigned)gy) * (unsigned)nm + (unsigned)gx;
+        v[k] = src[(size_t)gc * WPC + w];
+      }
     }
   }
 #pragma unroll
@@ -260,8 +284,8 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_conv(const unsigned short* 
           for (int j = 0; j < 4; ++j) v[j] = silu(z[j]);
         }
         u32x2 packed;
-        packed.x = (unsigned)f2bf(v[0]) | ((unsigned)f2bf(v[1]) << 16);
-        packed.y = (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16);
+        packed.x = pack_bf16(v[0], v[1]);
+        packed.y = pack_bf16(v[2], v[3]);
         *reinterpret_cast<u32x2*>(xout + c * 16 + 4 * g) = packed;
       }
     }
