@@ -673,6 +673,11 @@ def bench_pk(args) -> int:
 
 BF16_DENSE_PEAK_TFLOPS = 2500.0
 NET_FLOP_PER_CELL = 2 * 27 * (2 * 16 + 16 * 16 + 16 * 3)  # 18 144
+# compulsory HBM bytes per cell of one inference: x0 in (2 bf16), x1 out and
+# in (16 bf16 each way), x2 out and in, y out (4 f32)
+NET_BYTES_PER_CELL = 4 + 32 + 32 + 32 + 32 + 16  # 148
+# 18 144 / 148 = 123 flop/byte, below the bf16 ridge of 2500 TFLOP/s / 8 TB/s = 312 flop/byte:
+# the roofline that bounds the network is HBM bandwidth
 
 
 def bench_learned(args) -> int:
@@ -739,10 +744,12 @@ def bench_learned(args) -> int:
         "cost_s": {"network_forward_and_apply": round(t_net, 5), "short_range_sum": round(t_sr, 5),
                    "pm": round(float(np.mean([e["time_pm_s"] for e in evals])), 5),
                    "treepm_long_range": round(float(np.mean([e["time_treepm_long_s"] for e in evals])), 5)},
-        "roofline": {"bound": "mfma", "kernel": "network forward (3 convolutions)",
-                     "achieved": round(achieved, 2), "peak": BF16_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / BF16_DENSE_PEAK_TFLOPS, 5), "traffic": None,
-                     "flop_per_cell": NET_FLOP_PER_CELL, "mean_ms": round(t_fwd * 1e3, 4)},
+        "roofline": {"bound": "hbm", "kernel": "network forward (3 convolutions)",
+                     "achieved": round(NET_BYTES_PER_CELL * cells / t_fwd / 1e9, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(NET_BYTES_PER_CELL * cells / t_fwd / 1e9 / HBM_PEAK_GBPS, 4),
+                     "traffic": None, "bytes_per_cell": NET_BYTES_PER_CELL, "mean_ms": round(t_fwd * 1e3, 4),
+                     "mfma_tflops": round(achieved, 2), "mfma_frac": round(achieved / BF16_DENSE_PEAK_TFLOPS, 5),
+                     "flop_per_cell": NET_FLOP_PER_CELL},
         "training": {"data_s": round(t_data, 2), "train_s": round(t_train, 2), "loss_first": hist[0],
                      "loss_last": hist[-1], "f0": lc.f0},
     }

@@ -196,11 +196,13 @@ __global__ __launch_bounds__(256) void k_scan_add(unsigned* __restrict__ offset,
 __global__ __launch_bounds__(256) void k_scatter(const float* __restrict__ x, const float* __restrict__ y,
                                                  const float* __restrict__ z, const float* __restrict__ m, long long n,
                                                  const unsigned* __restrict__ bucket, const unsigned* __restrict__ rank,
-                                                 const unsigned* __restrict__ offset, float4* __restrict__ sorted) {
+                                                 const unsigned* __restrict__ offset, float4* __restrict__ sorted,
+                                                 unsigned* __restrict__ perm) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const unsigned slot = offset[bucket[i]] + rank[i];
   sorted[slot] = make_float4(x[i], y[i], z[i], m[i]);
+  if (perm) perm[slot] = (unsigned)i;  // only for the sorted-order interpolation
 }
 
 // LDS fixed-point add with wavefront aggregation: the lanes whose cell equals

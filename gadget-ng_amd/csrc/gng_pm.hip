@@ -83,6 +83,7 @@ struct Tuning {
   int y_fwd_persistent = 0;
   int y_inv_persistent = 0;
   int tuned = 0;  // 1 when measured (or read from the tuning cache)
+  int interp_sorted = 0;  // full-set gather in the deposit's sorted order
 };
 
 Tuning default_tuning(int nm) {
@@ -309,6 +310,7 @@ struct gng_pm_plan {
   // per-particle work buffers (grown on demand)
   long long work_cap = 0;
   float4* d_sorted = nullptr;
+  unsigned* d_perm = nullptr;  // original index of each sorted slot
   unsigned* d_bucket = nullptr;
   unsigned* d_rank = nullptr;
 
@@ -345,9 +347,11 @@ int set_device(gng_pm_plan* p) {
 
 void free_work(gng_pm_plan* p) {
   (void)hipFree(p->d_sorted);
+  (void)hipFree(p->d_perm);
   (void)hipFree(p->d_bucket);
   (void)hipFree(p->d_rank);
   p->d_sorted = nullptr;
+  p->d_perm = nullptr;
   p->d_bucket = p->d_rank = nullptr;
   p->work_cap = 0;
 }
@@ -401,6 +405,7 @@ int ensure_work(gng_pm_plan* p, long long n) {
   free_work(p);
   const size_t m = (size_t)n;
   GNG_HIP(hipMalloc(&p->d_sorted, sizeof(float4) * m));
+  GNG_HIP(hipMalloc(&p->d_perm, sizeof(unsigned) * m));
   GNG_HIP(hipMalloc(&p->d_bucket, sizeof(unsigned) * m));
   GNG_HIP(hipMalloc(&p->d_rank, sizeof(unsigned) * m));
   p->work_cap = n;
@@ -441,7 +446,7 @@ int run_front(gng_pm_plan* p, const float* x, const float* y, const float* z, co
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_offset, p->d_partial, p->nb);
   if (n > 0)
     hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z, m, n, p->d_bucket,
-                       p->d_rank, p->d_offset, p->d_sorted);
+                       p->d_rank, p->d_offset, p->d_sorted, p->tune.interp_sorted ? p->d_perm : nullptr);
   mark(p, 2, s);
   const unsigned ntiles = (unsigned)p->tm.nt * p->tm.nt * p->tm.nt;
   o.deposit[p->tune.deposit_wide](dim3(ntiles), dim3(p->tune.deposit_wide ? 512 : 256), s, p->d_sorted, p->d_offset,
@@ -494,7 +499,12 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
             reinterpret_cast<float2*>(p->d_pot), p->d_tw, 3, (float)(p->cfg.box_size / (2.0 * M_PI)));
   o.x_inv(dim3((unsigned)(rows / o.x_rows), 3), dim3(256), s, p->d_A, p->d_P, p->d_B, nullptr, p->d_tw, 0, 1.0f);
   mark(p, 8, s);
-  if (k > 0)
+  if (k > 0 && !idx && p->tune.interp_sorted)
+    // every particle: walk the deposit's sorted copy (local mesh reads whatever the caller's order)
+    hipLaunchKernelGGL(k_interp_sorted, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, p->d_sorted, p->d_perm, k,
+                       cm, fplane, reinterpret_cast<const float*>(p->d_A), reinterpret_cast<const float*>(p->d_P),
+                       reinterpret_cast<const float*>(p->d_B), ax, ay, az);
+  else if (k > 0)
     hipLaunchKernelGGL(k_interp, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, x, y, z, idx, k, cm, fplane,
                        reinterpret_cast<const float*>(p->d_A), reinterpret_cast<const float*>(p->d_P),
                        reinterpret_cast<const float*>(p->d_B), ax, ay, az);
@@ -1033,6 +1043,7 @@ int gng_pm_set_tuning(gng_pm_plan* p, const gng_pm_tuning_t* t) {
   p->tune.deposit_wide = t->deposit_wide != 0;
   p->tune.y_fwd_persistent = t->y_fwd_persistent != 0;
   p->tune.y_inv_persistent = t->y_inv_persistent != 0;
+  p->tune.interp_sorted = t->interp_sorted != 0;
   p->tune.tuned = 0;
   return GNG_OK;
 }
@@ -1045,6 +1056,7 @@ int gng_pm_get_tuning(const gng_pm_plan* p, gng_pm_tuning_t* t) {
   t->y_fwd_persistent = p->tune.y_fwd_persistent;
   t->y_inv_persistent = p->tune.y_inv_persistent;
   t->tuned = p->tune.tuned;
+  t->interp_sorted = p->tune.interp_sorted;
   return GNG_OK;
 }
 

@@ -244,7 +244,7 @@ int gng_kdk_create(gng_pm_plan* plan, int64_t n, gng_kdk** out) {
   *out = nullptr;
   if (!plan) return fail(GNG_ERR_INVALID, "null plan");
   if (n < 0) return fail(GNG_ERR_INVALID, "negative particle count");
-  if (n > (int64_t)0xFFFFFFFF) return fail(GNG_ERR_INVALID, "more than 2^32 - 1 particles");
+  if (n >= (1LL << 31)) return fail(GNG_ERR_INVALID, "n of 2^31 or more is not supported");
   gng_pm_config cfg;
   int rc = gng_pm_get_config(plan, &cfg);
   if (rc) return rc;

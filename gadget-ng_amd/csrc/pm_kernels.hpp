@@ -690,23 +690,16 @@ __global__ __launch_bounds__(T) void k_fft_y_inv_persist(float2* __restrict__ A,
   }
 }
 
-// CIC gather of (Fx, Fy, Fz) at pos[idx[j]] (idx null: j itself); fplane is
-// the float stride between z planes of the force meshes.
-__global__ __launch_bounds__(256) void k_interp(const float* __restrict__ x, const float* __restrict__ y,
-                                                const float* __restrict__ z, const long long* __restrict__ idx,
-                                                long long k, const CellMap cm, long long fplane,
-                                                const float* __restrict__ fx,
-                                                const float* __restrict__ fy, const float* __restrict__ fz,
-                                                float* __restrict__ ax, float* __restrict__ ay,
-                                                float* __restrict__ az) {
-  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= k) return;
-  const long long i = idx ? idx[j] : j;
+// CIC gather of (Fx, Fy, Fz) at the position p (fplane: the float stride
+// between z planes of the force meshes).
+__device__ __forceinline__ float3 cic_gather(float px, float py, float pz, const CellMap cm, long long fplane,
+                                             const float* __restrict__ fx, const float* __restrict__ fy,
+                                             const float* __restrict__ fz) {
   int ix, iy, iz;
   float dx, dy, dz;
-  cell_coord(x[i], cm, ix, dx);
-  cell_coord(y[i], cm, iy, dy);
-  cell_coord(z[i], cm, iz, dz);
+  cell_coord(px, cm, ix, dx);
+  cell_coord(py, cm, iy, dy);
+  cell_coord(pz, cm, iz, dz);
   const int nm = cm.nm;
   const float wx[2] = {1.0f - dx, dx}, wy[2] = {1.0f - dy, dy}, wz[2] = {1.0f - dz, dz};
   const int xs[2] = {ix, ix + 1 == nm ? 0 : ix + 1};
@@ -725,9 +718,47 @@ __global__ __launch_bounds__(256) void k_interp(const float* __restrict__ x, con
         sy = fmaf(fy[q], w, sy);
         sz = fmaf(fz[q], w, sz);
       }
-  ax[j] = sx;
-  ay[j] = sy;
-  az[j] = sz;
+  return make_float3(sx, sy, sz);
+}
+
+// The gather for every particle, walked in the deposit's sorted order (tile
+// by tile along the Hilbert curve), so the 8 corners of consecutive
+// particles come from the same few cache lines whatever order the caller's
+// arrays are in; results go to the particles' own slots (perm).  Each
+// particle's value is the same as the unsorted gather's.
+__global__ __launch_bounds__(256) void k_interp_sorted(const float4* __restrict__ sorted,
+                                                       const unsigned* __restrict__ perm, long long n,
+                                                       const CellMap cm, long long fplane,
+                                                       const float* __restrict__ fx, const float* __restrict__ fy,
+                                                       const float* __restrict__ fz, float* __restrict__ ax,
+                                                       float* __restrict__ ay, float* __restrict__ az) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const float4 p = sorted[j];
+  const float3 a = cic_gather(p.x, p.y, p.z, cm, fplane, fx, fy, fz);
+  const unsigned o = perm[j];
+  ax[o] = a.x;
+  ay[o] = a.y;
+  az[o] = a.z;
+}
+
+// CIC gather of (Fx, Fy, Fz) at pos[idx[j]] (idx null: j itself) in the
+// caller's order; fplane is the float stride between z planes of the force
+// meshes.
+__global__ __launch_bounds__(256) void k_interp(const float* __restrict__ x, const float* __restrict__ y,
+                                                const float* __restrict__ z, const long long* __restrict__ idx,
+                                                long long k, const CellMap cm, long long fplane,
+                                                const float* __restrict__ fx,
+                                                const float* __restrict__ fy, const float* __restrict__ fz,
+                                                float* __restrict__ ax, float* __restrict__ ay,
+                                                float* __restrict__ az) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  const long long i = idx ? idx[j] : j;
+  const float3 a = cic_gather(x[i], y[i], z[i], cm, fplane, fx, fy, fz);
+  ax[j] = a.x;
+  ay[j] = a.y;
+  az[j] = a.z;
 }
 
 }  // namespace gng

@@ -93,12 +93,13 @@ class GngConfig(ctypes.Structure):
 class GngTuning(ctypes.Structure):
     _fields_ = [("z_persistent", ctypes.c_int32), ("deposit_wide", ctypes.c_int32),
                 ("y_fwd_persistent", ctypes.c_int32), ("y_inv_persistent", ctypes.c_int32),
-                ("tuned", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("tuned", ctypes.c_int32), ("interp_sorted", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
     def as_dict(self) -> dict:
         return {"z_persistent": bool(self.z_persistent), "deposit_threads": 512 if self.deposit_wide else 256,
                 "y_fwd_persistent": bool(self.y_fwd_persistent), "y_inv_persistent": bool(self.y_inv_persistent),
-                "tuned": bool(self.tuned)}
+                "interp_sorted": bool(self.interp_sorted), "tuned": bool(self.tuned)}
 
 
 _lib = None
@@ -378,7 +379,7 @@ class Plan:
         return self.tuning()
 
     def set_tuning(self, z_persistent: bool = False, deposit_threads: int = 512, y_fwd_persistent: bool = False,
-                   y_inv_persistent: bool = False) -> None:
+                   y_inv_persistent: bool = False, interp_sorted: bool = False) -> None:
         if deposit_threads not in (256, 512):
             raise ValueError("deposit_threads must be 256 or 512")
         t = GngTuning()
@@ -386,6 +387,7 @@ class Plan:
         t.deposit_wide = int(deposit_threads == 512)
         t.y_fwd_persistent = int(bool(y_fwd_persistent))
         t.y_inv_persistent = int(bool(y_inv_persistent))
+        t.interp_sorted = int(bool(interp_sorted))
         self._check(lib().gng_pm_set_tuning(self._h, ctypes.byref(t)), "gng_pm_set_tuning")
 
     def tuning(self) -> dict:

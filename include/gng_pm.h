@@ -181,7 +181,15 @@ typedef struct gng_pm_tuning_t {
   int32_t y_fwd_persistent;  /* y forward pass: persistent workgroups          */
   int32_t y_inv_persistent;  /* y inverse pass: persistent workgroups          */
   int32_t tuned;             /* 1 when measured or read from the cache (get)  */
-  int32_t reserved[3];
+  int32_t interp_sorted;     /* full-set interpolation in the deposit's sorted
+                                order (1) instead of the caller's (0, default).
+                                Same values either way.  Sorted order keeps the
+                                mesh reads local whatever the caller's order but
+                                scatters the writes: at 256^3 on 512^3 it takes
+                                2.6 instead of 4.7 ms for shuffled particles and
+                                1.1 instead of 0.44 ms for lattice-ordered ones.
+                                Not autotuned: it depends on the caller's order */
+  int32_t reserved[2];
 } gng_pm_tuning_t;
 
 int gng_pm_autotune(gng_pm_plan *p, int reps);

@@ -26,7 +26,29 @@ def test_every_variant_matches_oracle(pm, oracle_mod, nm):
         got = plan.accelerations_f64(pos, m, 1.0)
         assert rms_rel(got, ref) < 1e-5, (zp, dt, yf, yi)
         assert plan.tuning() == {"z_persistent": zp, "deposit_threads": dt, "y_fwd_persistent": yf,
-                                 "y_inv_persistent": yi, "tuned": False}
+                                 "y_inv_persistent": yi, "interp_sorted": False, "tuned": False}
+
+
+@pytest.mark.parametrize("nm", [16, 48, 128])
+def test_interpolation_orders_agree_bit_for_bit(pm, oracle_mod, nm):
+    """The gather in the deposit's sorted order writes the same values as the
+    gather in the caller's order; an index subset takes the caller's order."""
+    print(f"case interpolation order nm={nm}", flush=True)
+    g = np.random.default_rng(nm + 1)
+    L = 7.0
+    n = min(nm ** 3 // 2, 300000)
+    pos = g.uniform(-L, 2 * L, (n, 3)).astype(np.float32).astype(np.float64)  # wrapped by the solver
+    m = g.uniform(0.5, 1.5, n)
+    plan = pm.Plan(nm, L, device=0)
+    plan.set_tuning(interp_sorted=True)
+    a = plan.accelerations_f64(pos, m, 1.0)
+    plan.set_tuning(interp_sorted=False)
+    b = plan.accelerations_f64(pos, m, 1.0)
+    assert np.array_equal(a, b)
+    idx = g.permutation(n)[: n // 3]
+    plan.set_tuning(interp_sorted=True)
+    assert np.array_equal(plan.accelerations_f64(pos, m, 1.0, idx=idx), a[idx])
+    assert rms_rel(a, oracle_mod.pm_accelerations(pos, m, L, nm)) < 1e-5
 
 
 def test_autotune_picks_a_variant_and_new_plans_reuse_it(pm, oracle_mod, tmp_path):
