@@ -179,6 +179,8 @@ struct MeshOps {
   int x_rows;
   int yz_cols;
   int yz_threads;
+  int z_cols;  // the per-item z pass's tiles
+  int z_threads;
   int tile;
   // deposit[v]: workgroups of 256 (v = 0) or 512 (v = 1) threads
   void (*deposit[2])(dim3, dim3, hipStream_t, const float4*, const unsigned*, CellMap, TileMap, const SolveScalars*,
@@ -211,6 +213,8 @@ MeshOps make_ops() {
   o.x_rows = R;
   o.yz_cols = C;
   o.yz_threads = TY;
+  o.z_cols = YZCfg<N>::CZ;
+  o.z_threads = YZCfg<N>::TZ;
   o.tile = TD;
   o.deposit[0] = [](dim3 g, dim3 b, hipStream_t s, const float4* srt, const unsigned* off, CellMap cm, TileMap tm,
                     const SolveScalars* sc, long long n, float* rho, long long rho_plane) {
@@ -228,7 +232,9 @@ MeshOps make_ops() {
     hipLaunchKernelGGL((k_fft_y_fwd<N, C, TY>), g, b, 0, s, P, tw);
   };
   o.z_green = [](dim3 g, dim3 b, hipStream_t s, const float2* P, float2* A, float2* B, const float2* tw,
-                 GreenParams gp) { hipLaunchKernelGGL((k_fft_z_green<N, C, TY>), g, b, 0, s, P, A, B, tw, gp); };
+                 GreenParams gp) {
+    hipLaunchKernelGGL((k_fft_z_green<N, YZCfg<N>::CZ, YZCfg<N>::TZ>), g, b, 0, s, P, A, B, tw, gp);
+  };
   o.z_green_col0 = [](dim3 g, dim3 b, hipStream_t s, const float2* P, float2* A, float2* B, const float2* tw,
                       GreenParams gp) { hipLaunchKernelGGL((k_fft_z_green_col0<N>), g, b, 0, s, P, A, B, tw, gp); };
   o.z_green_persist = [](dim3 g, dim3 b, hipStream_t s, const float2* P, float2* A, float2* B, const float2* tw,
@@ -484,7 +490,8 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
     if (p->tune.z_persistent && p->cus > 0)
       o.z_green_persist(dim3((unsigned)p->cus), yz_block, s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
     else
-      o.z_green(yz_grid, yz_block, s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
+      o.z_green(dim3((unsigned)(H / o.z_cols), (unsigned)nm), dim3((unsigned)o.z_threads), s, p->d_P, p->d_A, p->d_B,
+                p->d_tw, gp);
   }
   o.z_green_col0(dim3((unsigned)(H + 1)), dim3(256), s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
   mark(p, 6, s);

@@ -264,6 +264,10 @@ struct YZCfg {
   static constexpr int T_RAW = C * N / 8;
   static constexpr int T = T_RAW >= 1024 ? 1024 : (T_RAW <= 64 ? 64 : (T_RAW + 63) / 64 * 64);
   static_assert(H % C == 0, "column tiles must cover the half spectrum");
+  // the per-item z pass at N = 1024: full 128-byte tiles (16 values per thread)
+  static constexpr int CZ = N >= 1024 ? yz_cols(H) : C;
+  static constexpr int TZ_RAW = CZ * N / 8;
+  static constexpr int TZ = TZ_RAW >= 1024 ? 1024 : (TZ_RAW <= 64 ? 64 : (TZ_RAW + 63) / 64 * 64);
 };
 
 // Column tiles narrower than a 128-byte line (C = 8 at nm = 1024) share each
