@@ -87,7 +87,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("zeldovich", "clustered", "ensemble", "learned", "kdk", "pk"),
+    ap.add_argument("--workload", choices=("zeldovich", "clustered", "ensemble", "learned", "kdk", "pk", "ic"),
                     default="zeldovich",
                     help="zeldovich: 256^3 lattice on a 512^3 mesh (the headline); clustered: 512^3 particles, "
                          "70%% lattice + 30%% in 2000 Plummer spheres, 1024^3 mesh, TreePM long range r_s = 1.25 cells")
@@ -122,6 +122,8 @@ def main() -> int:
         return bench_kdk(args)
     if args.workload == "pk":
         return bench_pk(args)
+    if args.workload == "ic":
+        return bench_ic(args)
 
     import numpy as np
 
@@ -666,6 +668,93 @@ def bench_pk(args) -> int:
         result["modes_identical"] = bool(np.array_equal(got[2], ref[2]))
         result["max_shell_rel_err"] = float(f"{err.max():.3e}")
         log(f"[bench] FP64 oracle {tc:.1f} s; max shell error {err.max():.2e}")
+    plan.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_ic(args) -> int:
+    """Zel'dovich initial conditions of --mesh^3 particles from a --mesh^3
+    Gaussian field (the reference's generator uses one side for both), into
+    device memory: per-mode ChaCha12 Gaussians in the z pass, the inverse y and
+    x passes of the force pipeline, the lattice kernel.  Timed beside the
+    FP64 numpy restatement of the reference generator on a 64^3 sample."""
+    import numpy as np
+
+    import gadget_ng_pm_hip as pm
+    from gadget_ng_pm_hip import ic
+    from gadget_ng_pm_hip import integrate as gi
+
+    n = args.mesh
+    box = float(n)
+    seed = 0x6E670006
+    cosmo = gi.CosmologyParams(0.3, 0.7, 0.1)
+    a_init = 0.02
+    plan = pm.Plan(n, box, device=0)
+    fn = ic.power_law_sigma(n, 1.0e3, -2.0)
+    sig = ic.sigma_table(n, fn)
+    vf, sc, _ = ic.momentum_factors(cosmo, a_init, 0.1, True)
+    npart = n ** 3
+    dpos = pm.DeviceArray((npart, 3), np.float64)
+    dvel = pm.DeviceArray((npart, 3), np.float64)
+
+    def gen():
+        ic.zeldovich_ics_device(plan, seed, sig, vf, sc, dpos.ptr, dvel.ptr)
+
+    for _ in range(max(args.warmup, 1)):
+        gen()
+    pm.device_synchronize()
+    k = args.steps
+    t0 = time.perf_counter()
+    for _ in range(k):
+        gen()
+    pm.device_synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall * 1e3 / k
+    mh = n * n * (n // 2)
+    # compulsory HBM bytes: the z pass writes A and B (8 B each per half-spectrum
+    # mode); the y pass reads them and writes three fields; the x pass reads
+    # three complex and writes three real fields; the lattice kernel reads the
+    # three real fields and writes 48 B per particle
+    total_bytes = 16 * mh + 40 * mh + 24 * mh + 12 * n ** 3 + 12 * n ** 3 + 48 * npart
+    log(f"[bench] ICs: {ms:.3f} ms for {npart} particles ({npart / (wall / k) / 1e6:.1f} M particles/s)")
+    result = {
+        "metric": f"M particles/s of Zel'dovich initial conditions ({n}^3 lattice from a {n}^3 Gaussian field: "
+                  "per-mode ChaCha12 Gaussians, three inverse transforms, lattice displacement), device-resident",
+        "value": round(npart * k / wall / 1e6, 2), "unit": "M particles/s", "n_gpus": 1, "steps": k,
+        "warmup": max(args.warmup, 1), "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 modes and particles, f32 transforms", "data": "synthetic",
+        "config": {"workload": f"Zel'dovich ICs {n}^3 particles, L={box:g} Mpc/h, power law A=1e3 n_s=-2, "
+                               f"LambdaCDM a_init={a_init}, rescaled to D(1), seed {seed:#x}",
+                   "particles": npart, "mesh": n},
+        "roofline": {"bound": "hbm", "kernel": "whole generator (the z pass is integer-ALU bound)",
+                     "achieved": round(total_bytes / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(total_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "algorithmic_bytes": total_bytes},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ic_oracle as ico  # checker and CPU baseline only
+
+        ns = 64
+        sig_s = ic.sigma_table(ns, ic.power_law_sigma(ns, 1.0e3, -2.0))
+        t1 = time.perf_counter()
+        d = ico.delta_kspace(ns, seed, lambda q: sig_s[np.asarray(q)])
+        psi = ico.displacement(d, float(ns))
+        rp, rv = ico.zeldovich(ns, float(ns), psi, vf, sc)
+        tc = time.perf_counter() - t1
+        result["cpu_baseline"] = {"value": round(ns ** 3 / tc / 1e6, 4), "unit": "M particles/s", "cores": 1,
+                                  "kind": "port",
+                                  "sample": f"64^3 particles through the FP64 numpy restatement of the reference "
+                                            f"generator (vectorised ChaCha12, complex ifftn), one run: {tc:.2f} s"}
+        sp = pm.Plan(ns, float(ns), device=0)
+        gp, gv, _ = ic.zeldovich_ics(sp, seed, ic.power_law_sigma(ns, 1.0e3, -2.0), cosmology=cosmo, a_init=a_init,
+                                     rescale_to_a_init=True)
+        err = float(np.sqrt(((gv - rv) ** 2).sum() / (rv ** 2).sum()))
+        result["sample_rms_rel_err"] = float(f"{err:.3e}")
+        log(f"[bench] FP64 oracle sample {tc:.2f} s; momenta rms rel err {err:.2e}")
+        sp.close()
     plan.close()
     print(json.dumps(result), flush=True)
     return 0

@@ -26,8 +26,10 @@
 #include <string>
 #include <vector>
 
+#include "../../include/gng_ic.h"
 #include "../../include/gng_pm.h"
 #include "deposit.hpp"
+#include "ic_kernels.hpp"
 #include "pk_kernels.hpp"
 #include "pm_kernels.hpp"
 #include "status.hpp"
@@ -84,18 +86,44 @@ struct Tuning {
   int y_inv_persistent = 0;
   int tuned = 0;  // 1 when measured (or read from the tuning cache)
   int interp_sorted = 0;  // full-set gather in the deposit's sorted order
+  int chunk_fwd = 0;      // z planes per chunk of the forward x/y pass pair (0: whole mesh)
+  int chunk_inv = 0;      // same for the inverse y/x pass pair
 };
+
+// Planes per chunk when a pass pair runs chunk by chunk, so that the second
+// pass reads what the first wrote from the 256 MiB last-level cache: the
+// largest power of two whose planes of one complex field take at most
+// budget_mib (the forward pair moves one field, the inverse pair three).
+// 0 where the whole mesh is that small.
+int chunk_rule(int nm, long long budget_mib) {
+  const long long plane_bytes = 8LL * nm * (nm / 2);
+  int c = nm;
+  while (c > 1 && (long long)c * plane_bytes > (budget_mib << 20)) c /= 2;
+  return c < nm && nm % c == 0 ? c : 0;
+}
+// measured best on MI355X: 256 planes at 512^3 and 64 at 1024^3 for the
+// forward pair, 64 and 16 for the inverse pair
+constexpr long long kChunkFwdMiB = 256, kChunkInvMiB = 64;
 
 Tuning default_tuning(int nm) {
   Tuning t;
   t.z_persistent = nm == 512;
   t.deposit_wide = 1;
+  t.chunk_fwd = chunk_rule(nm, kChunkFwdMiB);
+  t.chunk_inv = chunk_rule(nm, kChunkInvMiB);
   return t;
 }
 
-constexpr int kKnobs = 4;
+constexpr int kKnobs = 6;
 int& knob(Tuning& t, int k) {
-  return k == 0 ? t.z_persistent : k == 1 ? t.deposit_wide : k == 2 ? t.y_fwd_persistent : t.y_inv_persistent;
+  switch (k) {
+    case 0: return t.z_persistent;
+    case 1: return t.deposit_wide;
+    case 2: return t.y_fwd_persistent;
+    case 3: return t.y_inv_persistent;
+    case 4: return t.chunk_fwd;
+    default: return t.chunk_inv;
+  }
 }
 
 // Process-wide cache of measured choices, keyed by (device architecture, nm).
@@ -122,9 +150,12 @@ void load_tune_file_locked() {
   if (!fp) return;
   char arch[64];
   int nm, v[kKnobs];
-  while (std::fscanf(fp, "%63s %d %d %d %d %d", arch, &nm, &v[0], &v[1], &v[2], &v[3]) == 2 + kKnobs) {
+  while (std::fscanf(fp, "%63s %d %d %d %d %d %d %d", arch, &nm, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]) ==
+         2 + kKnobs) {
     Tuning t;
-    for (int k = 0; k < kKnobs; ++k) knob(t, k) = v[k] != 0;
+    for (int k = 0; k < 4; ++k) knob(t, k) = v[k] != 0;
+    t.chunk_fwd = v[4] == chunk_rule(nm, kChunkFwdMiB) ? v[4] : 0;
+    t.chunk_inv = v[5] == chunk_rule(nm, kChunkInvMiB) ? v[5] : 0;
     t.tuned = 1;
     g_tune[{std::string(arch), nm}] = t;
   }
@@ -147,8 +178,9 @@ void store_tuning(int dev, int nm, const Tuning& t) {
   FILE* fp = std::fopen(f, "w");
   if (!fp) return;
   for (const auto& kv : g_tune)
-    std::fprintf(fp, "%s %d %d %d %d %d\n", kv.first.first.c_str(), kv.first.second, kv.second.z_persistent,
-                 kv.second.deposit_wide, kv.second.y_fwd_persistent, kv.second.y_inv_persistent);
+    std::fprintf(fp, "%s %d %d %d %d %d %d %d\n", kv.first.first.c_str(), kv.first.second, kv.second.z_persistent,
+                 kv.second.deposit_wide, kv.second.y_fwd_persistent, kv.second.y_inv_persistent, kv.second.chunk_fwd,
+                 kv.second.chunk_inv);
   std::fclose(fp);
 }
 
@@ -199,6 +231,8 @@ struct MeshOps {
                unsigned long long*);
   void (*pk_col0)(dim3, dim3, hipStream_t, const float2*, const float2*, const double*, double*, long long, long long,
                   unsigned long long*);
+  // Zel'dovich initial conditions: generated modes through the inverse z pass
+  void (*ic_z)(dim3, dim3, hipStream_t, IcParams, float2*, float2*, const float2*);
 };
 
 template <int N>
@@ -261,6 +295,9 @@ MeshOps make_ops() {
   o.pk_col0 = [](dim3 g, dim3 b, hipStream_t s, const float2* P, const float2* tw, const double* w, double* part,
                  long long nblk, long long base, unsigned long long* modes) {
     hipLaunchKernelGGL((k_pk_col0<N>), g, b, 0, s, P, tw, w, part, nblk, base, modes);
+  };
+  o.ic_z = [](dim3 g, dim3 b, hipStream_t s, IcParams ip, float2* A, float2* B, const float2* tw) {
+    hipLaunchKernelGGL((k_ic_z<N, C, TY>), g, b, 0, s, ip, A, B, tw);
   };
   return o;
 }
@@ -337,6 +374,9 @@ struct gng_pm_plan {
   unsigned long long* d_pk_modes = nullptr;
   double* d_pk_msum = nullptr;  // mass partials
   double* d_pk_out = nullptr;   // shell sums times (V / M_tot)^2
+  // initial conditions: the per-|n|^2 amplitude table (allocated on first use)
+  double* d_ic_sigma = nullptr;
+  long long ic_sigma_cap = 0;
   // Stage timing: every solve records kStages+1 events into the next slot of
   // a ring; nothing synchronises until gng_pm_stage_times() reads them.
   bool timing = false;
@@ -368,7 +408,8 @@ void free_plan(gng_pm_plan* p) {
   for (void* q : {(void*)p->d_P, (void*)p->d_A, (void*)p->d_B, (void*)p->d_rho, (void*)p->d_pot, (void*)p->d_tw, (void*)p->d_deconv,
                   (void*)p->d_sc, (void*)p->d_count, (void*)p->d_offset, (void*)p->d_partial, (void*)p->d_part,
                   (void*)p->d_idx, (void*)p->d_tile_rank, (void*)p->d_tile_order, (void*)p->d_pk_w,
-                  (void*)p->d_pk_part, (void*)p->d_pk_modes, (void*)p->d_pk_msum, (void*)p->d_pk_out})
+                  (void*)p->d_pk_part, (void*)p->d_pk_modes, (void*)p->d_pk_msum, (void*)p->d_pk_out,
+                  (void*)p->d_ic_sigma})
     (void)hipFree(q);
   free_work(p);
   for (auto& slot : p->ev)
@@ -459,10 +500,25 @@ int run_front(gng_pm_plan* p, const float* x, const float* y, const float* z, co
                                    cm, p->tm, p->d_sc, n, rho, rho_plane);
   mark(p, 3, s);
   const long long rows = (long long)nm * nm;
-  o.x_fwd(dim3((unsigned)(rows / o.x_rows)), dim3(256), s, rho, p->d_P, p->d_tw, rho_plane / 2);
-  mark(p, 4, s);
   const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
   const dim3 yz_block((unsigned)o.yz_threads);
+  const int cz = p->tune.chunk_fwd;
+  if (cz > 0 && !p->tune.y_fwd_persistent) {
+    // chunks of cz planes: each y pass reads the chunk its x pass just wrote
+    // (both passes touch only their own planes; the pointers are offset)
+    const long long pl = mesh_plane(nm);
+    for (int z0 = 0; z0 < nm; z0 += cz) {
+      o.x_fwd(dim3((unsigned)((long long)cz * nm / o.x_rows)), dim3(256), s, rho + rho_plane * z0, p->d_P + pl * z0,
+              p->d_tw, rho_plane / 2);
+      o.y_fwd(dim3((unsigned)(H / o.yz_cols), (unsigned)cz), yz_block, s, p->d_P + pl * z0, p->d_tw);
+    }
+    // the two stages share one timing slot (the x stage's)
+    mark(p, 4, s);
+    mark(p, 5, s);
+    return GNG_OK;
+  }
+  o.x_fwd(dim3((unsigned)(rows / o.x_rows)), dim3(256), s, rho, p->d_P, p->d_tw, rho_plane / 2);
+  mark(p, 4, s);
   if (p->tune.y_fwd_persistent && p->cus > 0)
     o.y_fwd_persist(dim3((unsigned)p->cus), yz_block, s, p->d_P, p->d_tw);
   else
@@ -495,6 +551,20 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   }
   o.z_green_col0(dim3((unsigned)(H + 1)), dim3(256), s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
   mark(p, 6, s);
+  const int cz = p->tune.chunk_inv;
+  if (cz > 0 && !p->tune.y_inv_persistent && !p->keep_potential) {
+    // chunks of cz planes: each x pass reads the three fields its y pass just wrote
+    const long long pl = mesh_plane(nm);
+    for (int z0 = 0; z0 < nm; z0 += cz) {
+      o.y_inv(dim3((unsigned)(H / o.yz_cols), (unsigned)cz), yz_block, s, p->d_A + pl * z0, p->d_B + pl * z0,
+              p->d_P + pl * z0, p->d_tw);
+      o.x_inv(dim3((unsigned)((long long)cz * nm / o.x_rows), 3), dim3(256), s, p->d_A + pl * z0, p->d_P + pl * z0,
+              p->d_B + pl * z0, nullptr, p->d_tw, 0, 1.0f);
+    }
+    // the two stages share one timing slot (the y stage's)
+    mark(p, 7, s);
+    mark(p, 8, s);
+  } else {
   if (p->tune.y_inv_persistent && p->cus > 0)
     o.y_inv_persist(dim3((unsigned)p->cus), yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
   else
@@ -506,6 +576,7 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
             reinterpret_cast<float2*>(p->d_pot), p->d_tw, 3, (float)(p->cfg.box_size / (2.0 * M_PI)));
   o.x_inv(dim3((unsigned)(rows / o.x_rows), 3), dim3(256), s, p->d_A, p->d_P, p->d_B, nullptr, p->d_tw, 0, 1.0f);
   mark(p, 8, s);
+  }
   if (k > 0 && !idx && p->tune.interp_sorted)
     // every particle: walk the deposit's sorted copy (local mesh reads whatever the caller's order)
     hipLaunchKernelGGL(k_interp_sorted, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, p->d_sorted, p->d_perm, k,
@@ -666,6 +737,66 @@ int run_pk(gng_pm_plan* p, const float* x, const float* y, const float* z, const
   GNG_HIP(hipStreamSynchronize(s));
   p->density_valid = p->keep_density;
   p->potential_valid = false;
+  return GNG_OK;
+}
+
+// Initial conditions: the amplitude table to the device.  sigma[n2] is the
+// spectrum at |n| = sqrt(n2) for every |n|^2 of a mode without a Nyquist
+// component, n2 = 0 .. 3 (nm/2 - 1)^2 (the reference evaluates its spectrum
+// closure at sqrt(n2) per mode, ic_zeldovich.rs:315-316).
+int ic_upload_sigma(gng_pm_plan* p, const double* sigma, long long sigma_len, hipStream_t s, IcParams& ip,
+                    unsigned long long seed) {
+  const int nm = p->cfg.grid_size, H = nm / 2;
+  if (nm < 4 || nm % 2) return fail(GNG_ERR_INVALID, "initial conditions need an even grid_size of 4 or more");
+  const long long need = 3LL * (H - 1) * (H - 1) + 1;
+  if (!sigma || sigma_len < need) return fail(GNG_ERR_INVALID, "sigma table shorter than 3 (grid_size/2 - 1)^2 + 1");
+  if (need > p->ic_sigma_cap) {
+    (void)hipFree(p->d_ic_sigma);
+    p->d_ic_sigma = nullptr;
+    p->ic_sigma_cap = 0;
+    GNG_HIP(hipMalloc(&p->d_ic_sigma, sizeof(double) * (size_t)need));
+    p->ic_sigma_cap = need;
+  }
+  GNG_HIP(hipMemcpyAsync(p->d_ic_sigma, sigma, sizeof(double) * (size_t)need, hipMemcpyHostToDevice, s));
+  // the caller's table may go away once this returns
+  GNG_HIP(hipStreamSynchronize(s));
+  ip.seed = seed;
+  ip.sigma = p->d_ic_sigma;
+  return GNG_OK;
+}
+
+// Zel'dovich particles lo .. lo + cnt into device pos / vel (n x 3 f64):
+// the z pass generates the modes, the force pipeline's inverse y and x passes
+// finish the three displacement transforms, then the lattice kernel.
+int run_ic(gng_pm_plan* p, const IcParams& ip, double vel_factor, double scale, long long lo, long long cnt,
+           double* pos, double* vel, hipStream_t s) {
+  const int nm = p->cfg.grid_size, H = nm / 2;
+  const MeshOps& o = *p->ops;
+  const long long rows = (long long)nm * nm;
+  const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
+  const dim3 yz_block((unsigned)o.yz_threads);
+  o.ic_z(yz_grid, yz_block, s, ip, p->d_A, p->d_B, p->d_tw);
+  o.y_inv(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
+  o.x_inv(dim3((unsigned)(rows / o.x_rows), 3), dim3(256), s, p->d_A, p->d_P, p->d_B, nullptr, p->d_tw, 0, 1.0f);
+  p->density_valid = false;
+  p->potential_valid = false;
+  if (cnt > 0) {
+    const double L = p->cfg.box_size;
+    const double d = L / (double)nm;
+    const double ifft_norm = 1.0 / (double)((long long)nm * nm * nm);
+    hipLaunchKernelGGL(k_ic_particles, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const float*>(p->d_B), reinterpret_cast<const float*>(p->d_P),
+                       reinterpret_cast<const float*>(p->d_A), 2 * mesh_plane(nm), nm, d, L, ifft_norm * d, scale,
+                       vel_factor, lo, cnt, pos, vel);
+  }
+  GNG_HIP(hipGetLastError());
+  return GNG_OK;
+}
+
+int ic_check_range(const gng_pm_plan* p, int64_t lo, int64_t hi) {
+  const long long n3 = (long long)p->cfg.grid_size * p->cfg.grid_size * p->cfg.grid_size;
+  if (lo < 0 || hi < lo || hi > n3) return fail(GNG_ERR_INVALID, "need 0 <= lo <= hi <= grid_size^3");
+  if (hi - lo >= (1LL << 31)) return fail(GNG_ERR_INVALID, "2^31 or more particles per call are not supported");
   return GNG_OK;
 }
 
@@ -927,6 +1058,73 @@ int gng_pm_power_spectrum_f64(gng_pm_plan* p, const double* pos, const double* m
   return pk_bins(p, sums.data(), modes.data(), capacity, k, pk, n_modes, n_bins);
 }
 
+int gng_ic_delta_kspace(gng_pm_plan* p, uint64_t seed, const double* sigma, int64_t sigma_len, double* out) {
+  if (!p || !out) return fail(GNG_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  int rc = set_device(p);
+  if (rc) return rc;
+  hipStream_t s = p->stream;
+  IcParams ip{};
+  if ((rc = ic_upload_sigma(p, sigma, sigma_len, s, ip, seed))) return rc;
+  const int nm = p->cfg.grid_size;
+  const long long total = (long long)nm * nm * (nm / 2 + 1);
+  double2* d_out = nullptr;
+  GNG_HIP(hipMalloc(&d_out, sizeof(double2) * (size_t)total));
+  hipLaunchKernelGGL(k_ic_delta, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, ip, nm, d_out);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(double2) * (size_t)total, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d_out);
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP,
+                std::string("initial-condition modes: ") + hipGetErrorString(e));
+  return GNG_OK;
+}
+
+int gng_ic_zeldovich_device(gng_pm_plan* p, uint64_t seed, const double* sigma, int64_t sigma_len,
+                            double vel_factor, double scale, int64_t lo, int64_t hi, double* d_pos, double* d_vel,
+                            void* stream) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  int rc = ic_check_range(p, lo, hi);
+  if (rc) return rc;
+  if (hi > lo && (!d_pos || !d_vel)) return fail(GNG_ERR_INVALID, "null output array");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if ((rc = set_device(p))) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  IcParams ip{};
+  if ((rc = ic_upload_sigma(p, sigma, sigma_len, s, ip, seed))) return rc;
+  return run_ic(p, ip, vel_factor, scale, lo, hi - lo, d_pos, d_vel, s);
+}
+
+int gng_ic_zeldovich(gng_pm_plan* p, uint64_t seed, const double* sigma, int64_t sigma_len, double vel_factor,
+                     double scale, int64_t lo, int64_t hi, double* pos, double* vel) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  int rc = ic_check_range(p, lo, hi);
+  if (rc) return rc;
+  if (hi > lo && (!pos || !vel)) return fail(GNG_ERR_INVALID, "null output array");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if ((rc = set_device(p))) return rc;
+  hipStream_t s = p->stream;
+  IcParams ip{};
+  if ((rc = ic_upload_sigma(p, sigma, sigma_len, s, ip, seed))) return rc;
+  const long long cnt = hi - lo;
+  double* d = nullptr;
+  if (cnt > 0) GNG_HIP(hipMalloc(&d, sizeof(double) * 6 * (size_t)cnt));
+  rc = run_ic(p, ip, vel_factor, scale, lo, cnt, d, d + 3 * cnt, s);
+  hipError_t e = hipSuccess;
+  if (!rc && cnt > 0) {
+    e = hipMemcpyAsync(pos, d, sizeof(double) * 3 * (size_t)cnt, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(vel, d + 3 * cnt, sizeof(double) * 3 * (size_t)cnt, hipMemcpyDeviceToHost, s);
+  }
+  if (!rc && e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d);
+  if (rc) return rc;
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP,
+                std::string("initial conditions: ") + hipGetErrorString(e));
+  return GNG_OK;
+}
+
 int gng_pm_get_config(const gng_pm_plan* p, gng_pm_config* out) {
   if (!p || !out) return fail(GNG_ERR_INVALID, "null argument");
   *out = p->cfg;
@@ -1019,7 +1217,14 @@ int gng_pm_autotune(gng_pm_plan* p, int reps) {
   rc = measure(best_ms);
   for (int k = 0; k < kKnobs && rc == GNG_OK; ++k) {
     Tuning trial = best;
-    knob(trial, k) = !knob(trial, k);
+    if (k >= 4) {
+      // chunked or whole-mesh pass pairs
+      const int rule = chunk_rule(nm, k == 4 ? kChunkFwdMiB : kChunkInvMiB);
+      knob(trial, k) = knob(best, k) ? 0 : rule;
+      if (knob(trial, k) == knob(best, k)) continue;
+    } else {
+      knob(trial, k) = !knob(trial, k);
+    }
     p->tune = trial;
     float ms = 1e30f;
     rc = measure(ms);
@@ -1045,12 +1250,19 @@ int gng_pm_autotune(gng_pm_plan* p, int reps) {
 
 int gng_pm_set_tuning(gng_pm_plan* p, const gng_pm_tuning_t* t) {
   if (!p || !t) return fail(GNG_ERR_INVALID, "null argument");
+  const int nm = p->cfg.grid_size;
+  for (const int c : {t->plane_chunk_fwd, t->plane_chunk_inv})
+    if (c < 0 || (c > 0 && (c >= nm || nm % c != 0 || ((long long)c * nm) % p->ops->x_rows != 0)))
+      return fail(GNG_ERR_INVALID, "a plane chunk must be 0 or a proper divisor of grid_size whose planes hold whole "
+                                   "x-pass workgroups");
   std::lock_guard<std::mutex> lk(p->mu);
   p->tune.z_persistent = t->z_persistent != 0;
   p->tune.deposit_wide = t->deposit_wide != 0;
   p->tune.y_fwd_persistent = t->y_fwd_persistent != 0;
   p->tune.y_inv_persistent = t->y_inv_persistent != 0;
   p->tune.interp_sorted = t->interp_sorted != 0;
+  p->tune.chunk_fwd = t->plane_chunk_fwd;
+  p->tune.chunk_inv = t->plane_chunk_inv;
   p->tune.tuned = 0;
   return GNG_OK;
 }
@@ -1064,6 +1276,8 @@ int gng_pm_get_tuning(const gng_pm_plan* p, gng_pm_tuning_t* t) {
   t->y_inv_persistent = p->tune.y_inv_persistent;
   t->tuned = p->tune.tuned;
   t->interp_sorted = p->tune.interp_sorted;
+  t->plane_chunk_fwd = p->tune.chunk_fwd;
+  t->plane_chunk_inv = p->tune.chunk_inv;
   return GNG_OK;
 }
 

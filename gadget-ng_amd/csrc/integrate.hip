@@ -12,6 +12,7 @@
 // kernel, PM force, kick(+wrap) kernel -- all on one stream, no host sync.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <new>
@@ -125,6 +126,98 @@ double advance_a(const gng_cosmology& c, double a, double dt) {
   return a + dt / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
 }
 
+// H(a) (cosmology.rs:509-520)
+double hubble_param(const gng_cosmology& c, double a) {
+  double omega_de;
+  if (std::fabs(c.w0 + 1.0) < 1e-10 && std::fabs(c.wa) < 1e-10) {
+    omega_de = c.omega_lambda;
+  } else {
+    const double exp_factor = std::exp(3.0 * c.wa * (a - 1.0));
+    const double power = -3.0 * (1.0 + c.w0 + c.wa);
+    omega_de = c.omega_lambda * std::pow(a, power) * exp_factor;
+  }
+  const double h_sq = (c.omega_m + c.omega_nu) / (a * a * a) + omega_de;
+  return c.h0 * std::sqrt(h_sq > 0.0 ? h_sq : 0.0);
+}
+
+// f(a) = Omega_m(a)^0.55 (cosmology.rs:598-610)
+double growth_rate_f(const gng_cosmology& c, double a) {
+  if (a <= 0.0) return 1.0;
+  const double h = hubble_param(c, a);
+  if (h <= 0.0 || c.h0 <= 0.0) return 1.0;
+  const double h_ratio_sq = (h / c.h0) * (h / c.h0);
+  const double omega_m_a = c.omega_m / (a * a * a) / h_ratio_sq;
+  return std::pow(omega_m_a > 0.0 ? omega_m_a : 0.0, 0.55);
+}
+
+// Carroll-Press-Turner 1992 growing mode (cosmology.rs:638-654)
+double growth_cpt92(const gng_cosmology& c, double a) {
+  const double a3 = a * a * a;
+  const double denom = c.omega_m + c.omega_lambda * a3;
+  if (denom <= 0.0) return a;
+  const double om_a = c.omega_m / denom;
+  const double ol_a = c.omega_lambda * a3 / denom;
+  const double bracket =
+      std::pow(om_a > 0.0 ? om_a : 0.0, 4.0 / 7.0) - ol_a + (1.0 + om_a / 2.0) * (1.0 + ol_a / 70.0);
+  if (bracket <= 0.0) return a;
+  return a * (2.5 * om_a / bracket);
+}
+
+// (cosmology.rs:656-666)
+double omega_matter_fraction(const gng_cosmology& c, double a) {
+  const double h = hubble_param(c, a);
+  if (h <= 0.0 || c.h0 <= 0.0) return 0.0;
+  const double h_ratio_sq = (h / c.h0) * (h / c.h0);
+  const double rho_m = (c.omega_m + c.omega_nu) / (a * a * a);
+  return rho_m / (h_ratio_sq > 1e-300 ? h_ratio_sq : 1e-300);
+}
+
+// (cosmology.rs:668-678)
+double d_ln_h_d_ln_a(const gng_cosmology& c, double a) {
+  const double eps = std::max(1e-7, a * 1e-9);
+  const double hp = hubble_param(c, std::min(a + eps, 1e9));
+  const double hm = hubble_param(c, std::max(a - eps, 1e-18));
+  const double h = hubble_param(c, a);
+  if (h <= 0.0) return 0.0;
+  return (a / h) * (hp - hm) / (2.0 * eps);
+}
+
+// RK4 on D'' + (2 + d ln H / d ln a) D' - 3/2 Omega_m(a) D = 0 in ln a, 768
+// steps from a = 1e-5 with D = D' = 1e-5 (cosmology.rs:680-711)
+double growth_ode(const gng_cosmology& c, double a) {
+  constexpr double kStart = 1e-5;
+  constexpr int kSteps = 768;
+  if (a <= kStart) return a > 0.0 ? a : 0.0;
+  const double ln_end = std::log(a), ln_start = std::log(kStart);
+  const double h = (ln_end - ln_start) / (double)kSteps;
+  double ln_a = ln_start, d = kStart, dp = kStart;
+  auto deriv = [&](double la, double y, double yp, double& o0, double& o1) {
+    const double aa = std::exp(la);
+    const double om = omega_matter_fraction(c, aa);
+    const double q = d_ln_h_d_ln_a(c, aa);
+    o0 = yp;
+    o1 = -(2.0 + q) * yp + 1.5 * om * y;
+  };
+  for (int i = 0; i < kSteps; ++i) {
+    double k10, k11, k20, k21, k30, k31, k40, k41;
+    deriv(ln_a, d, dp, k10, k11);
+    deriv(ln_a + 0.5 * h, d + 0.5 * h * k10, dp + 0.5 * h * k11, k20, k21);
+    deriv(ln_a + 0.5 * h, d + 0.5 * h * k20, dp + 0.5 * h * k21, k30, k31);
+    deriv(ln_a + h, d + h * k30, dp + h * k31, k40, k41);
+    d += h / 6.0 * (k10 + 2.0 * k20 + 2.0 * k30 + k40);
+    dp += h / 6.0 * (k11 + 2.0 * k21 + 2.0 * k31 + k41);
+    ln_a += h;
+  }
+  return d;
+}
+
+// D(a), not normalised (cosmology.rs:715-724)
+double growth_factor_d(const gng_cosmology& c, double a) {
+  if (a <= 0.0) return 0.0;
+  if (std::fabs(c.w0 + 1.0) < 1e-10 && std::fabs(c.wa) < 1e-10) return growth_cpt92(c, a);
+  return growth_ode(c, a);
+}
+
 void drift_kick_factors(const gng_cosmology& c, double a0, double dt, double out[3]) {
   constexpr int kSub = 16;
   const double sub_dt = dt / (double)kSub;
@@ -231,6 +324,21 @@ extern "C" {
 double gng_cosmo_advance_a(const gng_cosmology* c, double a, double dt) {
   if (!c) return std::nan("");
   return advance_a(*c, a, dt);
+}
+
+double gng_cosmo_hubble(const gng_cosmology* c, double a) {
+  if (!c) return std::nan("");
+  return hubble_param(*c, a);
+}
+
+double gng_cosmo_growth_rate(const gng_cosmology* c, double a) {
+  if (!c) return std::nan("");
+  return growth_rate_f(*c, a);
+}
+
+double gng_cosmo_growth_factor(const gng_cosmology* c, double a) {
+  if (!c) return std::nan("");
+  return growth_factor_d(*c, a);
 }
 
 int gng_cosmo_drift_kick_factors(const gng_cosmology* c, double a0, double dt, double* out) {

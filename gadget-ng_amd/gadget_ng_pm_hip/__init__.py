@@ -94,12 +94,14 @@ class GngTuning(ctypes.Structure):
     _fields_ = [("z_persistent", ctypes.c_int32), ("deposit_wide", ctypes.c_int32),
                 ("y_fwd_persistent", ctypes.c_int32), ("y_inv_persistent", ctypes.c_int32),
                 ("tuned", ctypes.c_int32), ("interp_sorted", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("plane_chunk_fwd", ctypes.c_int32), ("plane_chunk_inv", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {"z_persistent": bool(self.z_persistent), "deposit_threads": 512 if self.deposit_wide else 256,
                 "y_fwd_persistent": bool(self.y_fwd_persistent), "y_inv_persistent": bool(self.y_inv_persistent),
-                "interp_sorted": bool(self.interp_sorted), "tuned": bool(self.tuned)}
+                "interp_sorted": bool(self.interp_sorted), "chunk_fwd": int(self.plane_chunk_fwd),
+                "chunk_inv": int(self.plane_chunk_inv),
+                "tuned": bool(self.tuned)}
 
 
 _lib = None
@@ -230,6 +232,16 @@ def lib():
         L.gng_kdk_run.argtypes = [P, P, P, d, d, i32, i32, P]
         for f in (L.gng_cosmo_drift_kick_factors, L.gng_kdk_create, L.gng_kdk_set_short_range, L.gng_kdk_set_state,
                   L.gng_kdk_get_state, L.gng_kdk_step, L.gng_kdk_run):
+            f.restype = i32
+        for f in (L.gng_cosmo_hubble, L.gng_cosmo_growth_rate, L.gng_cosmo_growth_factor):
+            f.argtypes = [P, d]
+            f.restype = d
+        # Zel'dovich initial conditions (include/gng_ic.h)
+        u64 = ctypes.c_uint64
+        L.gng_ic_delta_kspace.argtypes = [P, u64, P, i64, P]
+        L.gng_ic_zeldovich.argtypes = [P, u64, P, i64, d, d, i64, i64, P, P]
+        L.gng_ic_zeldovich_device.argtypes = [P, u64, P, i64, d, d, i64, i64, P, P, P]
+        for f in (L.gng_ic_delta_kspace, L.gng_ic_zeldovich, L.gng_ic_zeldovich_device):
             f.restype = i32
         L.hip_pm_create.argtypes = [i32, f32]
         L.hip_pm_create.restype = P
@@ -379,7 +391,8 @@ class Plan:
         return self.tuning()
 
     def set_tuning(self, z_persistent: bool = False, deposit_threads: int = 512, y_fwd_persistent: bool = False,
-                   y_inv_persistent: bool = False, interp_sorted: bool = False) -> None:
+                   y_inv_persistent: bool = False, interp_sorted: bool = False, chunk_fwd: int = 0,
+                   chunk_inv: int = 0) -> None:
         if deposit_threads not in (256, 512):
             raise ValueError("deposit_threads must be 256 or 512")
         t = GngTuning()
@@ -388,6 +401,8 @@ class Plan:
         t.y_fwd_persistent = int(bool(y_fwd_persistent))
         t.y_inv_persistent = int(bool(y_inv_persistent))
         t.interp_sorted = int(bool(interp_sorted))
+        t.plane_chunk_fwd = int(chunk_fwd)
+        t.plane_chunk_inv = int(chunk_inv)
         self._check(lib().gng_pm_set_tuning(self._h, ctypes.byref(t)), "gng_pm_set_tuning")
 
     def tuning(self) -> dict:

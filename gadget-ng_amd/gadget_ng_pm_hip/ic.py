@@ -1,0 +1,139 @@
+"""Zel'dovich initial conditions on the GPU (include/gng_ic.h).
+
+Mirrors crates/gadget-ng-core/src/ic_zeldovich.rs on top of libgng_pm.so:
+
+* :func:`power_law_sigma`  -- ``build_spectrum_fn`` with ``TransferKind::PowerLaw``
+  (:404-415), tabulated at sqrt(n2) for the integer |n|^2 of every mode;
+* :func:`delta_kspace`     -- ``generate_delta_kspace`` (:287-369), FP64 modes
+  iz = 0 .. n/2;
+* :func:`zeldovich_ics`    -- ``zeldovich_ics`` (:621-766): Gaussian field,
+  displacement, lattice particles, canonical momenta a^2 f(a) H(a) Psi, the
+  optional rescaling D(a_init) / D(1), the neutrino suppression of the
+  spectrum (:679-684, :697-709);
+* :func:`zeldovich_ics_power_law` -- (:953-980).
+
+The plan's grid_size is the lattice side and its box_size the box; the
+particles are the global ids lo .. hi - 1 of the lattice.  The spectrum is
+any callable of |n| (the reference's spectrum closure); it is evaluated on
+the host once per integer |n|^2 and the table goes to the device.  Everything
+else runs in the HIP library; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Callable, Optional, Tuple
+
+import numpy as np
+
+from . import GNG_OK, HipExecutionError, Plan, _ptr, last_error, lib
+from .integrate import CosmologyParams
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != GNG_OK:
+        raise HipExecutionError(what, rc, last_error())
+
+
+def table_len(n: int) -> int:
+    """Entries the library needs: every |n|^2 of a mode without a Nyquist
+    component, 0 .. 3 (n/2 - 1)^2."""
+    return 3 * (n // 2 - 1) ** 2 + 1
+
+
+def sigma_table(n: int, spectrum_fn: Callable[[float], float]) -> np.ndarray:
+    """sigma[n2] = spectrum_fn(sqrt(n2)), as the reference evaluates its
+    closure per mode at (n2 as f64).sqrt() (ic_zeldovich.rs:315-316)."""
+    return np.array([spectrum_fn(math.sqrt(float(v))) for v in range(table_len(n))], dtype=np.float64)
+
+
+def power_law_sigma(n: int, amplitude: float, spectral_index: float) -> Callable[[float], float]:
+    """The PowerLaw spectrum closure: A |n|^(n_s/4) / sqrt(n^3), 0 at |n| = 0."""
+    inv_sqrt_n3 = 1.0 / math.sqrt(float(n * n * n))
+
+    def f(n_abs: float) -> float:
+        if n_abs <= 0.0:
+            return 0.0
+        return amplitude * math.pow(n_abs, spectral_index / 4.0) * inv_sqrt_n3
+    return f
+
+
+def delta_kspace(plan: Plan, seed: int, sigma: np.ndarray) -> np.ndarray:
+    """delta-hat[ix, iy, iz] for iz = 0 .. n/2 (complex128, shape (n, n, n/2 + 1))."""
+    n = plan.grid_size
+    sig = np.ascontiguousarray(sigma, dtype=np.float64)
+    out = np.empty((n, n, n // 2 + 1, 2), dtype=np.float64)
+    _check(lib().gng_ic_delta_kspace(plan._h, ctypes.c_uint64(int(seed) & (2 ** 64 - 1)), _ptr(sig), sig.size,
+                                     _ptr(out)), "gng_ic_delta_kspace")
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def momentum_factors(cosmology: Optional[CosmologyParams], a_init: float, h0: float,
+                     rescale_to_a_init: bool, m_nu_ev: float = 0.0) -> Tuple[float, float, float]:
+    """(vel_factor, scale, sqrt of the neutrino suppression) of zeldovich_ics
+    (ic_zeldovich.rs:639-709): without cosmology a = 1 and an Einstein-de
+    Sitter background with h0, so vel_factor = h0 and scale = 1."""
+    if cosmology is None:
+        a = 1.0
+        cp = CosmologyParams(1.0, 0.0, h0)
+        enabled = False
+    else:
+        a = a_init
+        om_nu = m_nu_ev / (93.14 * (cosmology.h0 * 10.0) ** 2) if m_nu_ev > 0.0 else 0.0
+        cp = CosmologyParams(cosmology.omega_m, cosmology.omega_lambda, cosmology.h0, cosmology.w0, cosmology.wa,
+                             om_nu)
+        enabled = True
+    sup = 1.0
+    if enabled and m_nu_ev > 0.0 and cosmology.omega_m > 0.0:
+        f_nu = cp.omega_nu / cosmology.omega_m
+        sup = min(max(1.0 - 8.0 * f_nu, 0.0), 1.0)
+    vel_factor = a * a * cp.growth_rate(a) * cp.hubble(a)
+    scale = 1.0
+    if rescale_to_a_init and enabled:
+        d_den = cp.growth_factor(1.0)
+        scale = 1.0 if d_den == 0.0 else cp.growth_factor(a) / d_den
+    return vel_factor, scale, math.sqrt(sup) if sup < 1.0 else 1.0
+
+
+def zeldovich_ics(plan: Plan, seed: int, spectrum_fn: Callable[[float], float], *,
+                  cosmology: Optional[CosmologyParams] = None, a_init: float = 1.0, h0: float = 0.1,
+                  rescale_to_a_init: bool = False, m_nu_ev: float = 0.0, particle_count: Optional[int] = None,
+                  lo: int = 0, hi: Optional[int] = None):
+    """Particles lo .. hi - 1 as (pos, vel, mass): (k, 3) f64 each and the
+    uniform mass 1 / particle_count (particle_count defaults to n^3; the
+    reference stops at it, ic_zeldovich.rs:735-737)."""
+    n = plan.grid_size
+    n_part = n ** 3 if particle_count is None else int(particle_count)
+    if n_part > n ** 3:
+        raise ValueError("particle_count exceeds the lattice (grid_size^3)")
+    hi = n_part if hi is None else min(int(hi), n_part)
+    lo = min(int(lo), hi)
+    if m_nu_ev > 0.0 and cosmology is None:
+        raise ValueError("m_nu_ev needs the cosmology (its omega_m and h0)")
+    vel_factor, scale, sqrt_sup = momentum_factors(cosmology, a_init, h0, rescale_to_a_init, m_nu_ev)
+    fn = spectrum_fn if sqrt_sup == 1.0 else (lambda x: spectrum_fn(x) * sqrt_sup)
+    sig = sigma_table(n, fn)
+    pos = np.empty((hi - lo, 3), dtype=np.float64)
+    vel = np.empty((hi - lo, 3), dtype=np.float64)
+    _check(lib().gng_ic_zeldovich(plan._h, ctypes.c_uint64(int(seed) & (2 ** 64 - 1)), _ptr(sig), sig.size,
+                                  float(vel_factor), float(scale), lo, hi, _ptr(pos), _ptr(vel)), "gng_ic_zeldovich")
+    return pos, vel, np.full(hi - lo, 1.0 / n_part)
+
+
+def zeldovich_ics_device(plan: Plan, seed: int, sigma: np.ndarray, vel_factor: float, scale: float,
+                         d_pos: int, d_vel: int, lo: int = 0, hi: Optional[int] = None, stream: int = 0) -> None:
+    """Same into device buffers (raw pointers of (hi - lo) x 3 f64 each),
+    enqueued on `stream`; sigma is the table of :func:`sigma_table`."""
+    n = plan.grid_size
+    hi = n ** 3 if hi is None else int(hi)
+    sig = np.ascontiguousarray(sigma, dtype=np.float64)
+    _check(lib().gng_ic_zeldovich_device(plan._h, ctypes.c_uint64(int(seed) & (2 ** 64 - 1)), _ptr(sig), sig.size,
+                                         float(vel_factor), float(scale), int(lo), hi, ctypes.c_void_p(d_pos),
+                                         ctypes.c_void_p(d_vel), ctypes.c_void_p(stream)),
+           "gng_ic_zeldovich_device")
+
+
+def zeldovich_ics_power_law(plan: Plan, seed: int, amplitude: float, spectral_index: float, **kw):
+    """zeldovich_ics with the PowerLaw spectrum and no rescaling (:953-980)."""
+    kw.setdefault("rescale_to_a_init", False)
+    return zeldovich_ics(plan, seed, power_law_sigma(plan.grid_size, amplitude, spectral_index), **kw)

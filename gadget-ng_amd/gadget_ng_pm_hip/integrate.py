@@ -51,6 +51,21 @@ class CosmologyParams:
         c = self._c()
         return float(lib().gng_cosmo_advance_a(ctypes.byref(c), float(a), float(dt)))
 
+    def hubble(self, a: float) -> float:
+        """H(a) (cosmology.rs:509-520)."""
+        c = self._c()
+        return float(lib().gng_cosmo_hubble(ctypes.byref(c), float(a)))
+
+    def growth_rate(self, a: float) -> float:
+        """f(a) = Omega_m(a)^0.55 (cosmology.rs:598-610)."""
+        c = self._c()
+        return float(lib().gng_cosmo_growth_rate(ctypes.byref(c), float(a)))
+
+    def growth_factor(self, a: float) -> float:
+        """D(a), not normalised (cosmology.rs:715-724)."""
+        c = self._c()
+        return float(lib().gng_cosmo_growth_factor(ctypes.byref(c), float(a)))
+
     def drift_kick_factors(self, a0: float, dt: float) -> Tuple[float, float, float]:
         c = self._c()
         out = np.zeros(3, dtype=np.float64)

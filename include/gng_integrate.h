@@ -54,6 +54,21 @@ double gng_cosmo_advance_a(const gng_cosmology *c, double a, double dt);
  * drift_kick_factors).  Host-only arithmetic. */
 int gng_cosmo_drift_kick_factors(const gng_cosmology *c, double a0, double dt, double *out);
 
+/* H(a) = h0 sqrt(max((omega_m + omega_nu)/a^3 + Omega_DE(a), 0))
+ * (cosmology.rs:509-520).  Host-only; NaN for a NULL c. */
+double gng_cosmo_hubble(const gng_cosmology *c, double a);
+
+/* Linear growth rate f(a) = Omega_m(a)^0.55 (Linder 2005), 1 for a <= 0
+ * (cosmology.rs:598-610).  Host-only; NaN for a NULL c. */
+double gng_cosmo_growth_rate(const gng_cosmology *c, double a);
+
+/* Linear growth factor D(a), not normalised: Carroll-Press-Turner 1992 for
+ * w0 = -1, wa = 0, otherwise RK4 on the growth equation in ln a, 768 steps
+ * from a = 1e-5 (cosmology.rs:638-724).  0 for a <= 0.  Host-only; NaN for a
+ * NULL c.  Zel'dovich momenta use a^2 f(a) H(a) and the optional rescaling
+ * D(a_init) / D(1) (ic_zeldovich.rs:685-703). */
+double gng_cosmo_growth_factor(const gng_cosmology *c, double a);
+
 /* Particle state of one simulation on the plan's device. */
 typedef struct gng_kdk gng_kdk;
 

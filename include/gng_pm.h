@@ -189,7 +189,16 @@ typedef struct gng_pm_tuning_t {
                                 2.6 instead of 4.7 ms for shuffled particles and
                                 1.1 instead of 0.44 ms for lattice-ordered ones.
                                 Not autotuned: it depends on the caller's order */
-  int32_t reserved[2];
+  int32_t plane_chunk_fwd;   /* the forward x/y pass pair runs in chunks of
+                                this many z planes (0: over the whole mesh), so
+                                the y pass reads the x pass's output from the
+                                last-level cache */
+  int32_t plane_chunk_inv;   /* same for the inverse y/x pair (three fields).
+                                set: 0 or a proper divisor of grid_size; the
+                                autotuner chooses between 0 and a size-based
+                                value (fwd: 256 MiB of planes, 256 at 512^3 and
+                                64 at 1024^3; inv: 64 MiB, 64 at 512^3 and 16
+                                at 1024^3); none at 256^3 and below */
 } gng_pm_tuning_t;
 
 int gng_pm_autotune(gng_pm_plan *p, int reps);

@@ -15,7 +15,7 @@ import gadget_ng_pm_hip as pm
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-HEADERS = ("gng_pm.h", "gng_learned.h", "gng_integrate.h")
+HEADERS = ("gng_pm.h", "gng_learned.h", "gng_integrate.h", "gng_ic.h")
 
 
 def declared_functions():

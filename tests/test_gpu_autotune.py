@@ -26,7 +26,41 @@ def test_every_variant_matches_oracle(pm, oracle_mod, nm):
         got = plan.accelerations_f64(pos, m, 1.0)
         assert rms_rel(got, ref) < 1e-5, (zp, dt, yf, yi)
         assert plan.tuning() == {"z_persistent": zp, "deposit_threads": dt, "y_fwd_persistent": yf,
-                                 "y_inv_persistent": yi, "interp_sorted": False, "tuned": False}
+                                 "y_inv_persistent": yi, "interp_sorted": False, "chunk_fwd": 0, "chunk_inv": 0,
+                                 "tuned": False}
+
+
+@pytest.mark.parametrize("nm,chunks", [(32, (4, 16)), (64, (8, 32)), (96, (12, 48)), (128, (16, 64))])
+def test_plane_chunks_give_identical_forces(pm, oracle_mod, nm, chunks):
+    """The x/y pass pairs run chunk by chunk over z planes touch the same
+    values with the same kernels: forces are bit-identical to the whole-mesh
+    passes, with and without the potential kept (which turns chunking off for
+    the inverse pair)."""
+    print(f"case plane chunks nm={nm} chunks={chunks}", flush=True)
+    g = np.random.default_rng(nm + 7)
+    L = 12.0
+    n = min(nm ** 3 // 4, 300000)
+    pos = g.uniform(0, L, (n, 3)).astype(np.float32).astype(np.float64)
+    m = g.uniform(0.5, 1.5, n)
+    plan = pm.Plan(nm, L, r_split=0.4, device=0)
+    plan.set_tuning()
+    ref = plan.accelerations_f64(pos, m, 1.0)
+    assert rms_rel(ref, oracle_mod.pm_accelerations(pos, m, L, nm, r_split=0.4)) < 1e-5
+    for c in chunks:
+        for cf, ci in ((c, c), (c, 0), (0, c), (chunks[0], chunks[1])):
+            plan.set_tuning(chunk_fwd=cf, chunk_inv=ci)
+            assert (plan.tuning()["chunk_fwd"], plan.tuning()["chunk_inv"]) == (cf, ci)
+            assert np.array_equal(plan.accelerations_f64(pos, m, 1.0), ref), (cf, ci)
+        plan.set_tuning(chunk_fwd=c, chunk_inv=c, y_fwd_persistent=True, y_inv_persistent=True)
+        assert np.array_equal(plan.accelerations_f64(pos, m, 1.0), ref), (c, "persistent")
+    kept = pm.Plan(nm, L, r_split=0.4, device=0, keep_potential=True)
+    kept.set_tuning(chunk_fwd=chunks[1], chunk_inv=chunks[0])
+    assert np.array_equal(kept.accelerations_f64(pos, m, 1.0), ref)
+    for bad in (-1, nm, 2 * nm, 5 if nm % 5 else 7):
+        with pytest.raises(Exception):
+            plan.set_tuning(chunk_fwd=bad)
+        with pytest.raises(Exception):
+            plan.set_tuning(chunk_inv=bad)
 
 
 @pytest.mark.parametrize("nm", [16, 48, 128])
