@@ -711,6 +711,16 @@ def bench_ic(args) -> int:
     pm.device_synchronize()
     wall = time.perf_counter() - t0
     ms = wall * 1e3 / k
+    # second order (2LPT) into the same buffers, timed the same way
+    v1, v2, d2, sc2 = ic.factors_2lpt(cosmo, a_init, 0.1, True)
+    ic.zeldovich_2lpt_ics_device(plan, seed, sig, v1, v2, d2, sc2, dpos.ptr, dvel.ptr)
+    k2 = max(1, min(k, 5))
+    t2 = time.perf_counter()
+    for _ in range(k2):
+        ic.zeldovich_2lpt_ics_device(plan, seed, sig, v1, v2, d2, sc2, dpos.ptr, dvel.ptr)
+    pm.device_synchronize()
+    ms2 = (time.perf_counter() - t2) * 1e3 / k2
+    log(f"[bench] 2LPT ICs: {ms2:.3f} ms for {npart} particles")
     mh = n * n * (n // 2)
     # compulsory HBM bytes: the z pass writes A and B (8 B each per half-spectrum
     # mode); the y pass reads them and writes three fields; the x pass reads
@@ -731,6 +741,7 @@ def bench_ic(args) -> int:
                      "achieved": round(total_bytes / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(total_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4), "traffic": None,
                      "algorithmic_bytes": total_bytes},
+        "second_order_ms_per_step": round(ms2, 4),
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

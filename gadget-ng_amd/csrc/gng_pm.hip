@@ -231,8 +231,11 @@ struct MeshOps {
                unsigned long long*);
   void (*pk_col0)(dim3, dim3, hipStream_t, const float2*, const float2*, const double*, double*, long long, long long,
                   unsigned long long*);
-  // Zel'dovich initial conditions: generated modes through the inverse z pass
-  void (*ic_z)(dim3, dim3, hipStream_t, IcParams, float2*, float2*, const float2*);
+  // initial conditions: generated modes through the inverse z pass, and the
+  // inverse y and x passes with mode multipliers (second derivatives)
+  void (*ic_z)(dim3, dim3, hipStream_t, IcParams, float2*, float2*, const float2*, int2);
+  void (*y_inv_mul)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*, int3);
+  void (*x_inv_mul)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*, int3);
 };
 
 template <int N>
@@ -282,11 +285,11 @@ MeshOps make_ops() {
     hipLaunchKernelGGL((k_fft_y_inv_persist<N, C, TY>), g, b, 0, s, A, B, P, tw);
   };
   o.y_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* B, float2* P, const float2* tw) {
-    hipLaunchKernelGGL((k_fft_y_inv<N, C, TY>), g, b, 0, s, A, B, P, tw);
+    hipLaunchKernelGGL((k_fft_y_inv<N, C, TY>), g, b, 0, s, A, B, P, tw, make_int3(0, 0, 0));
   };
   o.x_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* P, float2* B, float2* pot, const float2* tw,
                 int field0, float pot_scale) {
-    hipLaunchKernelGGL((k_fft_x_inv<N, R>), g, b, 0, s, A, P, B, pot, tw, field0, pot_scale);
+    hipLaunchKernelGGL((k_fft_x_inv<N, R>), g, b, 0, s, A, P, B, pot, tw, field0, pot_scale, make_int3(0, 0, 0));
   };
   o.z_pk = [](dim3 g, dim3 b, hipStream_t s, const float2* P, const float2* tw, const double* w, double* part,
               long long nblk, unsigned long long* modes) {
@@ -296,8 +299,14 @@ MeshOps make_ops() {
                  long long nblk, long long base, unsigned long long* modes) {
     hipLaunchKernelGGL((k_pk_col0<N>), g, b, 0, s, P, tw, w, part, nblk, base, modes);
   };
-  o.ic_z = [](dim3 g, dim3 b, hipStream_t s, IcParams ip, float2* A, float2* B, const float2* tw) {
-    hipLaunchKernelGGL((k_ic_z<N, C, TY>), g, b, 0, s, ip, A, B, tw);
+  o.ic_z = [](dim3 g, dim3 b, hipStream_t s, IcParams ip, float2* A, float2* B, const float2* tw, int2 zc) {
+    hipLaunchKernelGGL((k_ic_z<N, C, TY>), g, b, 0, s, ip, A, B, tw, zc);
+  };
+  o.y_inv_mul = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* B, float2* P, const float2* tw, int3 codes) {
+    hipLaunchKernelGGL((k_fft_y_inv<N, C, TY, true>), g, b, 0, s, A, B, P, tw, codes);
+  };
+  o.x_inv_mul = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* P, float2* B, const float2* tw, int3 codes) {
+    hipLaunchKernelGGL((k_fft_x_inv<N, R, true>), g, b, 0, s, A, P, B, nullptr, tw, 0, 1.0f, codes);
   };
   return o;
 }
@@ -377,6 +386,7 @@ struct gng_pm_plan {
   // initial conditions: the per-|n|^2 amplitude table (allocated on first use)
   double* d_ic_sigma = nullptr;
   long long ic_sigma_cap = 0;
+  float* d_ic_mask = nullptr;  // 1 per axis index, 0 at the Nyquist index (second order)
   // Stage timing: every solve records kStages+1 events into the next slot of
   // a ring; nothing synchronises until gng_pm_stage_times() reads them.
   bool timing = false;
@@ -409,7 +419,7 @@ void free_plan(gng_pm_plan* p) {
                   (void*)p->d_sc, (void*)p->d_count, (void*)p->d_offset, (void*)p->d_partial, (void*)p->d_part,
                   (void*)p->d_idx, (void*)p->d_tile_rank, (void*)p->d_tile_order, (void*)p->d_pk_w,
                   (void*)p->d_pk_part, (void*)p->d_pk_modes, (void*)p->d_pk_msum, (void*)p->d_pk_out,
-                  (void*)p->d_ic_sigma})
+                  (void*)p->d_ic_sigma, (void*)p->d_ic_mask})
     (void)hipFree(q);
   free_work(p);
   for (auto& slot : p->ev)
@@ -769,13 +779,13 @@ int ic_upload_sigma(gng_pm_plan* p, const double* sigma, long long sigma_len, hi
 // the z pass generates the modes, the force pipeline's inverse y and x passes
 // finish the three displacement transforms, then the lattice kernel.
 int run_ic(gng_pm_plan* p, const IcParams& ip, double vel_factor, double scale, long long lo, long long cnt,
-           double* pos, double* vel, hipStream_t s) {
+           double* pos, double* vel, hipStream_t s, int wrap = 1) {
   const int nm = p->cfg.grid_size, H = nm / 2;
   const MeshOps& o = *p->ops;
   const long long rows = (long long)nm * nm;
   const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
   const dim3 yz_block((unsigned)o.yz_threads);
-  o.ic_z(yz_grid, yz_block, s, ip, p->d_A, p->d_B, p->d_tw);
+  o.ic_z(yz_grid, yz_block, s, ip, p->d_A, p->d_B, p->d_tw, make_int2(0, 1));  // a, -i n_z a
   o.y_inv(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
   o.x_inv(dim3((unsigned)(rows / o.x_rows), 3), dim3(256), s, p->d_A, p->d_P, p->d_B, nullptr, p->d_tw, 0, 1.0f);
   p->density_valid = false;
@@ -787,9 +797,81 @@ int run_ic(gng_pm_plan* p, const IcParams& ip, double vel_factor, double scale, 
     hipLaunchKernelGGL(k_ic_particles, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
                        reinterpret_cast<const float*>(p->d_B), reinterpret_cast<const float*>(p->d_P),
                        reinterpret_cast<const float*>(p->d_A), 2 * mesh_plane(nm), nm, d, L, ifft_norm * d, scale,
-                       vel_factor, lo, cnt, pos, vel);
+                       vel_factor, lo, cnt, pos, vel, wrap);
   }
   GNG_HIP(hipGetLastError());
+  return GNG_OK;
+}
+
+// Second-order (2LPT) particles (ic_2lpt.rs:348-553): the first-order part as
+// above without the wrap; then the six second derivatives of phi in two
+// rounds of three fields through the z, y and x passes with mode multipliers
+// (round 0: xx yy zz, round 1: xz yz xy), accumulated into the source S; then
+// S through the force pipeline with G = 1/|n|^2 and every mode with a Nyquist
+// component masked, which leaves Psi2 = Re IFFT(-i n S-hat / |n|^2) in the
+// force meshes; then the second-order particle kernel.
+int run_ic_2lpt(gng_pm_plan* p, const IcParams& ip, double vel1, double vel2, double d2, double scale, long long lo,
+                long long cnt, double* pos, double* vel, hipStream_t s) {
+  const int nm = p->cfg.grid_size, H = nm / 2;
+  const MeshOps& o = *p->ops;
+  const long long rows = (long long)nm * nm, nm3 = rows * nm;
+  const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
+  const dim3 yz_block((unsigned)o.yz_threads);
+  const long long fplane = 2 * mesh_plane(nm);
+  if (!p->d_ic_mask) {
+    std::vector<float> mask((size_t)nm, 1.0f);
+    mask[(size_t)H] = 0.0f;
+    GNG_HIP(hipMalloc(&p->d_ic_mask, sizeof(float) * (size_t)nm));
+    GNG_HIP(hipMemcpy(p->d_ic_mask, mask.data(), sizeof(float) * (size_t)nm, hipMemcpyHostToDevice));
+  }
+  int rc = run_ic(p, ip, vel1, scale, lo, cnt, pos, vel, s, 0);
+  if (rc) return rc;
+  float* S = nullptr;  // the source, an unpadded real mesh
+  GNG_HIP(hipMalloc(&S, sizeof(float) * (size_t)nm3));
+  const float* fa = reinterpret_cast<const float*>(p->d_A);
+  const float* fp = reinterpret_cast<const float*>(p->d_P);
+  const float* fb = reinterpret_cast<const float*>(p->d_B);
+  const double inv_n6 = 1.0 / ((double)nm3 * (double)nm3);  // two unnormalised inverse transforms per factor
+  const dim3 xg((unsigned)(rows / o.x_rows), 3);
+  // round 0: (a, n_z^2 a) -> y (1, n_y^2, 1) -> x (n_x^2, 1, 1) = xx, yy, zz (mesh axes)
+  o.ic_z(yz_grid, yz_block, s, ip, p->d_A, p->d_B, p->d_tw, make_int2(0, 3));
+  o.y_inv_mul(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw, make_int3(0, 3, 0));
+  o.x_inv_mul(xg, dim3(256), s, p->d_A, p->d_P, p->d_B, p->d_tw, make_int3(3, 0, 0));
+  hipLaunchKernelGGL(k_ic_source, dim3((unsigned)((nm3 + 255) / 256)), dim3(256), 0, s, fa, fp, fb, S, fplane,
+                     (long long)rows, nm, inv_n6, 0);
+  // round 1: (n_z a, a) -> y (1, n_y, n_y) -> x (n_x, 1, n_x) = xz, yz, xy
+  o.ic_z(yz_grid, yz_block, s, ip, p->d_A, p->d_B, p->d_tw, make_int2(2, 0));
+  o.y_inv_mul(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw, make_int3(0, 2, 2));
+  o.x_inv_mul(xg, dim3(256), s, p->d_A, p->d_P, p->d_B, p->d_tw, make_int3(2, 0, 2));
+  hipLaunchKernelGGL(k_ic_source, dim3((unsigned)((nm3 + 255) / 256)), dim3(256), 0, s, fa, fp, fb, S, fplane,
+                     (long long)rows, nm, inv_n6, 1);
+  // S -> Psi2 through the force pipeline's passes
+  GreenParams gp;
+  gp.c64 = -1.0;  // G = -c / |n|^2 = 1 / |n|^2
+  gp.a64 = 0.0;
+  gp.c = -1.0f;
+  gp.a = 0.0f;
+  gp.deconv = p->d_ic_mask;
+  o.x_fwd(dim3((unsigned)(rows / o.x_rows)), dim3(256), s, S, p->d_P, p->d_tw, rows / 2);
+  o.y_fwd(yz_grid, yz_block, s, p->d_P, p->d_tw);
+  o.z_green(dim3((unsigned)(H / o.z_cols), (unsigned)nm), dim3((unsigned)o.z_threads), s, p->d_P, p->d_A, p->d_B,
+            p->d_tw, gp);
+  o.z_green_col0(dim3((unsigned)(H + 1)), dim3(256), s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
+  o.y_inv(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
+  o.x_inv(xg, dim3(256), s, p->d_A, p->d_P, p->d_B, nullptr, p->d_tw, 0, 1.0f);
+  if (cnt > 0) {
+    const double L = p->cfg.box_size;
+    const double d = L / (double)nm;
+    const double ifft_norm = 1.0 / (double)nm3;
+    hipLaunchKernelGGL(k_ic_particles_2lpt, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, fb, fp, fa, fplane,
+                       nm, L, ifft_norm * d, scale * scale, d2, vel2, lo, cnt, pos, vel);
+  }
+  hipError_t e = hipGetLastError();
+  // S must outlive the kernels that read it
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(S);
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP, std::string("2LPT: ") + hipGetErrorString(e));
   return GNG_OK;
 }
 
@@ -1122,6 +1204,50 @@ int gng_ic_zeldovich(gng_pm_plan* p, uint64_t seed, const double* sigma, int64_t
   if (e != hipSuccess)
     return fail(e == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP,
                 std::string("initial conditions: ") + hipGetErrorString(e));
+  return GNG_OK;
+}
+
+int gng_ic_2lpt_device(gng_pm_plan* p, uint64_t seed, const double* sigma, int64_t sigma_len, double vel1,
+                       double vel2, double d2_over_d1sq, double scale, int64_t lo, int64_t hi, double* d_pos,
+                       double* d_vel, void* stream) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  int rc = ic_check_range(p, lo, hi);
+  if (rc) return rc;
+  if (hi > lo && (!d_pos || !d_vel)) return fail(GNG_ERR_INVALID, "null output array");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if ((rc = set_device(p))) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  IcParams ip{};
+  if ((rc = ic_upload_sigma(p, sigma, sigma_len, s, ip, seed))) return rc;
+  return run_ic_2lpt(p, ip, vel1, vel2, d2_over_d1sq, scale, lo, hi - lo, d_pos, d_vel, s);
+}
+
+int gng_ic_2lpt(gng_pm_plan* p, uint64_t seed, const double* sigma, int64_t sigma_len, double vel1, double vel2,
+                double d2_over_d1sq, double scale, int64_t lo, int64_t hi, double* pos, double* vel) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  int rc = ic_check_range(p, lo, hi);
+  if (rc) return rc;
+  if (hi > lo && (!pos || !vel)) return fail(GNG_ERR_INVALID, "null output array");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if ((rc = set_device(p))) return rc;
+  hipStream_t s = p->stream;
+  IcParams ip{};
+  if ((rc = ic_upload_sigma(p, sigma, sigma_len, s, ip, seed))) return rc;
+  const long long cnt = hi - lo;
+  double* d = nullptr;
+  if (cnt > 0) GNG_HIP(hipMalloc(&d, sizeof(double) * 6 * (size_t)cnt));
+  rc = run_ic_2lpt(p, ip, vel1, vel2, d2_over_d1sq, scale, lo, cnt, d, d + 3 * cnt, s);
+  hipError_t e = hipSuccess;
+  if (!rc && cnt > 0) {
+    e = hipMemcpyAsync(pos, d, sizeof(double) * 3 * (size_t)cnt, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(vel, d + 3 * cnt, sizeof(double) * 3 * (size_t)cnt, hipMemcpyDeviceToHost, s);
+  }
+  if (!rc && e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d);
+  if (rc) return rc;
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP,
+                std::string("2LPT initial conditions: ") + hipGetErrorString(e));
   return GNG_OK;
 }
 

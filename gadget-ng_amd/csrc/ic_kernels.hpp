@@ -140,13 +140,14 @@ __global__ __launch_bounds__(256) void k_ic_delta(IcParams ip, int n, double2* _
 }
 
 // One (column tile, ky) item: the modes of C columns along all kz generated
-// into LDS as A = -delta-hat / |n|^2, then the inverse z transforms of A and
-// of B = -i n_z A, written to the plan's A and B meshes in the layout the
-// inverse y pass reads.  Mode (column kx, row ky, plane kz) is the
-// reference's (ix, iy, iz) = (kz, ky, kx).
+// into LDS as a = -delta-hat / |n|^2, then the inverse z transforms of
+// m_A(n_z) a and m_B(n_z) a (multiplier codes of ic_mul: zc.x, zc.y; the
+// displacement takes 1 and -i n_z), written to the plan's A and B meshes in
+// the layout the inverse y pass reads.  Mode (column kx, row ky, plane kz) is
+// the reference's (ix, iy, iz) = (kz, ky, kx).
 template <int N, int C, int T>
 __global__ __launch_bounds__(T) void k_ic_z(const IcParams ip, float2* __restrict__ A, float2* __restrict__ B,
-                                            const float2* __restrict__ tw_g) {
+                                            const float2* __restrict__ tw_g, const int2 zc) {
   constexpr int H = N / 2, IS = C + 1;
   using F = RegFft<N, true, false, C, T, 1, IS, true, N>;
   __shared__ float2 buf[N * IS];
@@ -169,10 +170,10 @@ __global__ __launch_bounds__(T) void k_ic_z(const IcParams ip, float2* __restric
   for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
     a[p][r] = ok ? buf[i * IS + l] : make_float2(0.f, 0.f);
   });
-#pragma unroll
-  for (int p = 0; p < F::PER0; ++p)
-#pragma unroll
-    for (int r = 0; r < F::R0; ++r) w[p][r] = a[p][r];
+  for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+    const float nze = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
+    w[p][r] = ic_mul(zc.x, nze, a[p][r]);
+  });
   // begins with a barrier: every thread has read its values from LDS
   F::run(w, out, buf, twl, tid);
   for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(T) void k_ic_z(const IcParams ip, float2* __restric
   });
   for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
     const float nze = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
-    w[p][r] = make_float2(nze * a[p][r].y, -nze * a[p][r].x);
+    w[p][r] = ic_mul(zc.y, nze, a[p][r]);
   });
   F::run(w, out, buf, twl, tid);
   for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(256) void k_ic_particles(const float* __restrict__ 
                                                       const float* __restrict__ psi_z, long long fplane, int n,
                                                       double d, double box, double norm, double scale,
                                                       double vel_factor, long long lo, long long cnt,
-                                                      double* __restrict__ pos, double* __restrict__ vel) {
+                                                      double* __restrict__ pos, double* __restrict__ vel, int wrap) {
 #pragma clang fp contract(off)
   const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
   if (t >= cnt) return;
@@ -222,12 +223,68 @@ __global__ __launch_bounds__(256) void k_ic_particles(const float* __restrict__ 
   const double q0 = ((double)ix + 0.5) * d, q1 = ((double)iy + 0.5) * d, q2 = ((double)iz + 0.5) * d;
   double* P = pos + 3 * t;
   double* V = vel + 3 * t;
-  P[0] = ic_rem_euclid(q0 + p0, box);
-  P[1] = ic_rem_euclid(q1 + p1, box);
-  P[2] = ic_rem_euclid(q2 + p2, box);
+  // without wrap: q + Psi, for the second-order kernel to complete
+  P[0] = wrap ? ic_rem_euclid(q0 + p0, box) : q0 + p0;
+  P[1] = wrap ? ic_rem_euclid(q1 + p1, box) : q1 + p1;
+  P[2] = wrap ? ic_rem_euclid(q2 + p2, box) : q2 + p2;
   V[0] = p0 * vel_factor;
   V[1] = p1 * vel_factor;
   V[2] = p2 * vel_factor;
+}
+
+// Second order: the source S of the 2LPT Poisson equation from the six
+// second derivatives of phi (ic_2lpt.rs:146-159), accumulated over two
+// rounds of three fields: round 0 writes xx yy + yy zz + zz xx (f0..f2 =
+// xx, yy, zz), round 1 subtracts xy^2 + yz^2 + xz^2 (f0..f2 in any order).
+// The fields are real meshes in the plan's row layout (row (z, y) at float
+// z fplane + y n), S an unpadded mesh (plane stride s_plane); FP64
+// arithmetic, scaled by `scale` (1/n^6 undoes the unnormalised inverse
+// transforms of both factors), S stored as f32.
+__global__ __launch_bounds__(256) void k_ic_source(const float* __restrict__ f0, const float* __restrict__ f1,
+                                                   const float* __restrict__ f2, float* __restrict__ S,
+                                                   long long fplane, long long s_plane, int n, double scale,
+                                                   int round) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long rows = (long long)n * n;
+  if (e >= rows * n) return;
+  const long long r = e / n, x = e % n;
+  const long long off = (r / n) * fplane + (r % n) * n + x;
+  const long long so = (r / n) * s_plane + (r % n) * n + x;
+  const double a = f0[off], b = f1[off], c = f2[off];
+  if (round == 0)
+    S[so] = (float)((a * b + b * c + c * a) * scale);
+  else
+    S[so] = (float)((double)S[so] - (a * a + b * b + c * c) * scale);
+}
+
+// Second-order particles: adds D2/D1^2 s^2 Psi2 to the first-order positions
+// q + s Psi1 that k_ic_particles left unwrapped (wrap = 0), wraps them, and
+// adds s^2 Psi2 v2 to the first-order momenta (ic_2lpt.rs:536-550; the sums
+// run in the reference's order).  psi2_* as in k_ic_particles.
+__global__ __launch_bounds__(256) void k_ic_particles_2lpt(const float* __restrict__ psi_x,
+                                                           const float* __restrict__ psi_y,
+                                                           const float* __restrict__ psi_z, long long fplane, int n,
+                                                           double box, double norm, double scale2, double d2,
+                                                           double vel2, long long lo, long long cnt,
+                                                           double* __restrict__ pos, double* __restrict__ vel) {
+#pragma clang fp contract(off)
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= cnt) return;
+  const long long gid = lo + t;
+  const long long nn = (long long)n * n;
+  const int ix = (int)(gid / nn);
+  const long long rem = gid % nn;
+  const int iy = (int)(rem / n), iz = (int)(rem % n);
+  const long long off = (long long)ix * fplane + (long long)iy * n + iz;
+  double p[3] = {(double)psi_x[off] * norm, (double)psi_y[off] * norm, (double)psi_z[off] * norm};
+  double* P = pos + 3 * t;
+  double* V = vel + 3 * t;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (scale2 != 1.0) p[c] *= scale2;
+    P[c] = ic_rem_euclid(P[c] + d2 * p[c], box);
+    V[c] = V[c] + p[c] * vel2;
+  }
 }
 
 }  // namespace gng

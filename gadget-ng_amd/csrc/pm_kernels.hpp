@@ -180,13 +180,27 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, 
   }
 }
 
+// Mode multipliers of the initial-condition passes (codes): 0 -> 1,
+// 1 -> -i n, 2 -> n, 3 -> n^2.
+__device__ __forceinline__ float2 ic_mul(int code, float n, float2 v) {
+  switch (code) {
+    case 1: return make_float2(n * v.y, -n * v.x);
+    case 2: return make_float2(n * v.x, n * v.y);
+    case 3: return make_float2(n * n * v.x, n * n * v.y);
+    default: return v;
+  }
+}
+
 // C2R along x, in place, for field blockIdx.y (0: A with -i n_x, 1: P, 2: B).
+// With MUL the three fields take the multipliers codes.x, .y, .z instead (the
+// initial-condition passes); the force solve instantiates MUL = false.
 // The pairwise pre-processing reads the row from LDS and feeds the first FFT
 // stage in registers; the last stage stores the real row directly.
-template <int N, int ROWS>
+template <int N, int ROWS, bool MUL = false>
 __global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float2* __restrict__ P,
                                                    float2* __restrict__ B, float2* __restrict__ pot,
-                                                   const float2* __restrict__ tw_g, int field0, float pot_scale) {
+                                                   const float2* __restrict__ tw_g, int field0, float pot_scale,
+                                                   int3 codes) {
   constexpr int H = N / 2, T = 256, LS = XCfg<N>::LS;
   using F = RegFft<H, true, false, ROWS, T, LS, 1, false, N>;
   __shared__ float2 buf[ROWS * LS];
@@ -207,7 +221,10 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float
     const int e = tid + p * T;
     if ((TOT % T == 0) || e < TOT) {
       float2 v = data[mesh_row<N>(row0 + e / H) + e % H];
-      if (field == 0) {
+      if constexpr (MUL) {
+        // slot 0 (DC and the packed Nyquist plane) has n = 0
+        v = ic_mul(field == 0 ? codes.x : (field == 1 ? codes.y : codes.z), (float)(e % H), v);
+      } else if (field == 0) {
         // -i n_x; slot 0 holds DC (n=0) and Nyquist (n_eff=0): both vanish
         const float nx = (float)(e % H);
         v = make_float2(nx * v.y, -nx * v.x);
@@ -537,10 +554,13 @@ __global__ __launch_bounds__(256) void k_fft_z_green_col0(const float2* __restri
 }
 
 // Inverse y transforms: A -> A, -i n_y A -> P, B -> B, one after the other,
-// each starting from registers and storing from registers.
-template <int N, int C, int T>
+// each starting from registers and storing from registers.  With MUL the
+// three take the multipliers codes.x (A), .y (P, from A) and .z (B) instead
+// (the initial-condition passes); the force solve instantiates MUL = false.
+template <int N, int C, int T, bool MUL = false>
 __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2* __restrict__ B,
-                                                 float2* __restrict__ P, const float2* __restrict__ tw_g) {
+                                                 float2* __restrict__ P, const float2* __restrict__ tw_g,
+                                                 int3 codes) {
   constexpr int H = N / 2, IS = C + 1;
   using F = RegFft<N, true, false, C, T, 1, IS, true, N>;
   __shared__ float2 buf[N * IS];
@@ -562,17 +582,28 @@ __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2*
   for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
     bv[p][r] = ok ? b[(unsigned)i * H + l] : make_float2(0.f, 0.f);
   });
+  if constexpr (MUL) {
+    for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      const float nye = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
+      w[p][r] = ic_mul(codes.x, nye, av[p][r]);
+      bv[p][r] = ic_mul(codes.z, nye, bv[p][r]);
+    });
+  } else {
 #pragma unroll
-  for (int p = 0; p < F::PER0; ++p)
+    for (int p = 0; p < F::PER0; ++p)
 #pragma unroll
-    for (int r = 0; r < F::R0; ++r) w[p][r] = av[p][r];
+      for (int r = 0; r < F::R0; ++r) w[p][r] = av[p][r];
+  }
   F::run(w, out, buf, tw, tid);
   for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
     if (ok) a[(unsigned)i * H + l] = out[p][r];
   });
   for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
     const float nye = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
-    w[p][r] = make_float2(nye * av[p][r].y, -nye * av[p][r].x);
+    if constexpr (MUL)
+      w[p][r] = ic_mul(codes.y, nye, av[p][r]);
+    else
+      w[p][r] = make_float2(nye * av[p][r].y, -nye * av[p][r].x);
   });
   F::run(w, out, buf, tw, tid);
   for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {

@@ -241,7 +241,10 @@ def lib():
         L.gng_ic_delta_kspace.argtypes = [P, u64, P, i64, P]
         L.gng_ic_zeldovich.argtypes = [P, u64, P, i64, d, d, i64, i64, P, P]
         L.gng_ic_zeldovich_device.argtypes = [P, u64, P, i64, d, d, i64, i64, P, P, P]
-        for f in (L.gng_ic_delta_kspace, L.gng_ic_zeldovich, L.gng_ic_zeldovich_device):
+        L.gng_ic_2lpt.argtypes = [P, u64, P, i64, d, d, d, d, i64, i64, P, P]
+        L.gng_ic_2lpt_device.argtypes = [P, u64, P, i64, d, d, d, d, i64, i64, P, P, P]
+        for f in (L.gng_ic_delta_kspace, L.gng_ic_zeldovich, L.gng_ic_zeldovich_device, L.gng_ic_2lpt,
+                  L.gng_ic_2lpt_device):
             f.restype = i32
         L.hip_pm_create.argtypes = [i32, f32]
         L.hip_pm_create.restype = P

@@ -10,7 +10,10 @@ Mirrors crates/gadget-ng-core/src/ic_zeldovich.rs on top of libgng_pm.so:
   displacement, lattice particles, canonical momenta a^2 f(a) H(a) Psi, the
   optional rescaling D(a_init) / D(1), the neutrino suppression of the
   spectrum (:679-684, :697-709);
-* :func:`zeldovich_ics_power_law` -- (:953-980).
+* :func:`zeldovich_ics_power_law` -- (:953-980);
+* :func:`zeldovich_2lpt_ics`  -- ``zeldovich_2lpt_ics`` of
+  crates/gadget-ng-core/src/ic_2lpt.rs:348-553 (second-order displacement
+  from the six second derivatives of the first-order potential).
 
 The plan's grid_size is the lattice side and its box_size the box; the
 particles are the global ids lo .. hi - 1 of the lattice.  The spectrum is
@@ -137,3 +140,63 @@ def zeldovich_ics_power_law(plan: Plan, seed: int, amplitude: float, spectral_in
     """zeldovich_ics with the PowerLaw spectrum and no rescaling (:953-980)."""
     kw.setdefault("rescale_to_a_init", False)
     return zeldovich_ics(plan, seed, power_law_sigma(plan.grid_size, amplitude, spectral_index), **kw)
+
+
+def factors_2lpt(cosmology: Optional[CosmologyParams], a_init: float, h0: float,
+                 rescale_to_a_init: bool) -> Tuple[float, float, float, float]:
+    """(vel1, vel2, d2_over_d1sq, scale) of zeldovich_2lpt_ics
+    (ic_2lpt.rs:414-453): its background is (omega_m, omega_lambda, h0) only;
+    Omega_m(a) from H(a), f2 = 2 Omega_m(a)^(6/11), D2/D1^2 =
+    -3/7 Omega_m(a)^(-1/143); without cosmology a = 1 and Omega_m(a) = 1."""
+    enabled = cosmology is not None
+    cp = CosmologyParams(cosmology.omega_m, cosmology.omega_lambda, cosmology.h0) if enabled else \
+        CosmologyParams(1.0, 0.0, h0)
+    a = a_init if enabled else 1.0
+    h_a = cp.hubble(a)
+    f1 = cp.growth_rate(a)
+    if enabled and cp.h0 > 0.0:
+        hr = (h_a / cp.h0) * (h_a / cp.h0)
+        om_a = max(cp.omega_m / (a * a * a) / hr, 0.0)
+    else:
+        om_a = 1.0
+    f2 = 2.0 * math.pow(om_a, 6.0 / 11.0)
+    d2 = -3.0 / 7.0 * math.pow(om_a, -1.0 / 143.0)
+    scale = 1.0
+    if rescale_to_a_init and enabled:
+        d_den = cp.growth_factor(1.0)
+        scale = 1.0 if d_den == 0.0 else cp.growth_factor(a) / d_den
+    return a * a * f1 * h_a, a * a * f2 * h_a * d2, d2, scale
+
+
+def zeldovich_2lpt_ics(plan: Plan, seed: int, spectrum_fn: Callable[[float], float], *,
+                       cosmology: Optional[CosmologyParams] = None, a_init: float = 1.0, h0: float = 0.1,
+                       rescale_to_a_init: bool = False, particle_count: Optional[int] = None,
+                       lo: int = 0, hi: Optional[int] = None):
+    """Second-order particles lo .. hi - 1 as (pos, vel, mass), like
+    :func:`zeldovich_ics`."""
+    n = plan.grid_size
+    n_part = n ** 3 if particle_count is None else int(particle_count)
+    if n_part > n ** 3:
+        raise ValueError("particle_count exceeds the lattice (grid_size^3)")
+    hi = n_part if hi is None else min(int(hi), n_part)
+    lo = min(int(lo), hi)
+    vel1, vel2, d2, scale = factors_2lpt(cosmology, a_init, h0, rescale_to_a_init)
+    sig = sigma_table(n, spectrum_fn)
+    pos = np.empty((hi - lo, 3), dtype=np.float64)
+    vel = np.empty((hi - lo, 3), dtype=np.float64)
+    _check(lib().gng_ic_2lpt(plan._h, ctypes.c_uint64(int(seed) & (2 ** 64 - 1)), _ptr(sig), sig.size, float(vel1),
+                             float(vel2), float(d2), float(scale), lo, hi, _ptr(pos), _ptr(vel)), "gng_ic_2lpt")
+    return pos, vel, np.full(hi - lo, 1.0 / n_part)
+
+
+def zeldovich_2lpt_ics_device(plan: Plan, seed: int, sigma: np.ndarray, vel1: float, vel2: float, d2: float,
+                              scale: float, d_pos: int, d_vel: int, lo: int = 0, hi: Optional[int] = None,
+                              stream: int = 0) -> None:
+    """Same into device buffers ((hi - lo) x 3 f64 each); returns when done."""
+    n = plan.grid_size
+    hi = n ** 3 if hi is None else int(hi)
+    sig = np.ascontiguousarray(sigma, dtype=np.float64)
+    _check(lib().gng_ic_2lpt_device(plan._h, ctypes.c_uint64(int(seed) & (2 ** 64 - 1)), _ptr(sig), sig.size,
+                                    float(vel1), float(vel2), float(d2), float(scale), int(lo), hi,
+                                    ctypes.c_void_p(d_pos), ctypes.c_void_p(d_vel), ctypes.c_void_p(stream)),
+           "gng_ic_2lpt_device")

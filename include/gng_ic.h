@@ -2,7 +2,8 @@
  *
  * Replaces the CPU generator of crates/gadget-ng-core/src/ic_zeldovich.rs:
  * generate_delta_kspace (:287-369), delta_to_displacement (:541-603) and the
- * particle loop of zeldovich_ics (:621-766).  The plan's grid_size n is both
+ * particle loop of zeldovich_ics (:621-766), and the second-order generator
+ * of crates/gadget-ng-core/src/ic_2lpt.rs.  The plan's grid_size n is both
  * the lattice side (n^3 particles, global id ix n^2 + iy n + iz) and the
  * side of the Gaussian field, as in the reference; its box_size is L.  The
  * plan's meshes are used as work space (a later gng_pm_density or
@@ -53,6 +54,26 @@ int gng_ic_zeldovich(gng_pm_plan *p, uint64_t seed, const double *sigma, int64_t
 int gng_ic_zeldovich_device(gng_pm_plan *p, uint64_t seed, const double *sigma, int64_t sigma_len,
                             double vel_factor, double scale, int64_t lo, int64_t hi, double *d_pos, double *d_vel,
                             void *stream);
+
+/* Second-order (2LPT) particles lo .. hi - 1 of zeldovich_2lpt_ics
+ * (crates/gadget-ng-core/src/ic_2lpt.rs:348-553):
+ *   Psi1 = s * (the first-order displacement), Psi2 = s^2 * Re IFFT(-i n
+ *   S-hat / |n|^2) d with S = sum over axis pairs of (phi,aa phi,bb -
+ *   phi,ab^2) and phi,ab-hat = -n_a n_b delta-hat / |n|^2 (:86-244),
+ *   pos = rem_euclid(q + Psi1 + d2_over_d1sq Psi2, L),
+ *   vel = Psi1 vel1 + Psi2 vel2,
+ * as (hi - lo) x 3 interleaved f64 in host memory.  vel1 = a^2 f1 H,
+ * vel2 = a^2 f2 H d2_over_d1sq, d2_over_d1sq = -3/7 Omega_m(a)^(-1/143)
+ * (:414-443), s the optional rescaling D(a_init) / D(1).  Needs one extra
+ * n^3 f32 buffer while it runs. */
+int gng_ic_2lpt(gng_pm_plan *p, uint64_t seed, const double *sigma, int64_t sigma_len, double vel1, double vel2,
+                double d2_over_d1sq, double scale, int64_t lo, int64_t hi, double *pos, double *vel);
+
+/* Same into device memory, enqueued on `stream`; returns after the work
+ * completes (the source buffer is freed). */
+int gng_ic_2lpt_device(gng_pm_plan *p, uint64_t seed, const double *sigma, int64_t sigma_len, double vel1,
+                       double vel2, double d2_over_d1sq, double scale, int64_t lo, int64_t hi, double *d_pos,
+                       double *d_vel, void *stream);
 
 #ifdef __cplusplus
 }

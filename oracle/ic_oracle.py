@@ -23,7 +23,11 @@ gadget-ng tree):
                             p = s Psi * vel_factor;
 * :func:`hubble_param`, :func:`growth_rate_f`, :func:`growth_factor_d`
                             crates/gadget-ng-core/src/cosmology.rs:509-520,
-                            :598-610, :638-733.
+                            :598-610, :638-733;
+* second order, crates/gadget-ng-core/src/ic_2lpt.rs:
+  :func:`phi_second_derivatives` (:86-136), :func:`source_2lpt` (:146-159),
+  :func:`source_to_psi2` (:181-244), :func:`factors_2lpt` (:414-443),
+  :func:`zeldovich_2lpt` (the particle loop, :517-553).
 
 The per-mode generator is rand's StdRng (rand 0.10.1: Cargo.lock:1940, the
 workspace asks for rand = "0.10" at Cargo.toml:62), which is ChaCha12 (the
@@ -251,6 +255,88 @@ def zeldovich(n: int, box: float, psi, vel_factor: float, scale: float = 1.0, lo
         p = p * scale
     q = np.stack([(ix + 0.5) * d, (iy + 0.5) * d, (iz + 0.5) * d], axis=1)
     return rem_euclid(q + p, box), p * vel_factor
+
+
+# -- second order (crates/gadget-ng-core/src/ic_2lpt.rs)
+
+PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+
+
+def phi_second_derivatives(delta: np.ndarray):
+    """phi_second_derivatives (ic_2lpt.rs:86-136): [xx, yy, zz, xy, xz, yz]
+    = Re IFFT(-n_a n_b / |n|^2 delta-hat)."""
+    n = delta.shape[0]
+    f = mode_int(np.arange(n), n).astype(np.float64)
+    ns = np.meshgrid(f, f, f, indexing="ij")
+    n2 = ns[0] ** 2 + ns[1] ** 2 + ns[2] ** 2
+    n2s = np.where(n2 == 0.0, 1.0, n2)
+    out = []
+    for a, b in PAIRS:
+        c = np.where(n2 == 0.0, 0.0, -ns[a] * ns[b] / n2s)
+        out.append(np.fft.ifftn(c * delta.real + 1j * (c * delta.imag)).real)
+    return out
+
+
+def source_2lpt(phi):
+    """build_2lpt_source (ic_2lpt.rs:146-159)."""
+    xx, yy, zz, xy, xz, yz = phi
+    return (xx * yy - xy * xy) + (yy * zz - yz * yz) + (zz * xx - xz * xz)
+
+
+def source_to_psi2(src: np.ndarray, box: float):
+    """source_to_psi2 (ic_2lpt.rs:181-244): Psi2_a = Re IFFT(-i n_a S-hat /
+    |n|^2) d, every mode with DC or a Nyquist component zero."""
+    n = src.shape[0]
+    sk = np.fft.fftn(src)
+    fi = mode_int(np.arange(n), n)
+    f = fi.astype(np.float64)
+    ns = np.meshgrid(f, f, f, indexing="ij")
+    n2 = ns[0] ** 2 + ns[1] ** 2 + ns[2] ** 2
+    nyq = np.abs(fi) == n // 2
+    bad = (n2 == 0.0) | nyq[:, None, None] | nyq[None, :, None] | nyq[None, None, :]
+    n2s = np.where(bad, 1.0, n2)
+    d = box / n
+    out = []
+    for a in range(3):
+        ph = np.where(bad, 0.0, (ns[a] * sk.imag / n2s) + 1j * (-ns[a] * sk.real / n2s))
+        out.append(np.fft.ifftn(ph).real * d)
+    return out
+
+
+def factors_2lpt(c, a_init: float, enabled: bool = True):
+    """(vel1, vel2, d2_over_d1sq) of zeldovich_2lpt_ics (ic_2lpt.rs:414-443):
+    the cosmology is (omega_m, omega_lambda, h0) only; Omega_m(a) from H(a);
+    f2 = 2 Omega_m(a)^(6/11); D2/D1^2 = -3/7 Omega_m(a)^(-1/143)."""
+    cp = (c[0], c[1], c[2], -1.0, 0.0, 0.0) if enabled else (1.0, 0.0, c[2], -1.0, 0.0, 0.0)
+    a = a_init if enabled else 1.0
+    h_a = hubble_param(cp, a)
+    f1 = growth_rate_f(cp, a)
+    if enabled and cp[2] > 0.0:
+        hr = (h_a / cp[2]) * (h_a / cp[2])
+        om_a = max(cp[0] / (a * a * a) / hr, 0.0)
+    else:
+        om_a = 1.0
+    f2 = 2.0 * math.pow(om_a, 6.0 / 11.0)
+    d2 = -3.0 / 7.0 * math.pow(om_a, -1.0 / 143.0)
+    return a * a * f1 * h_a, a * a * f2 * h_a * d2, d2
+
+
+def zeldovich_2lpt(n: int, box: float, psi1, psi2, vel1: float, vel2: float, d2: float, scale: float = 1.0,
+                   lo: int = 0, hi=None):
+    """The particle loop of zeldovich_2lpt_ics (ic_2lpt.rs:517-553):
+    x = rem_euclid(q + s Psi1 + D2/D1^2 s^2 Psi2, L), p = s Psi1 v1 + s^2 Psi2 v2."""
+    hi = n ** 3 if hi is None else min(hi, n ** 3)
+    gid = np.arange(lo, hi)
+    ix, rem = gid // (n * n), gid % (n * n)
+    iy, iz = rem // n, rem % n
+    d = box / n
+    p1 = np.stack([psi1[0][ix, iy, iz], psi1[1][ix, iy, iz], psi1[2][ix, iy, iz]], axis=1)
+    p2 = np.stack([psi2[0][ix, iy, iz], psi2[1][ix, iy, iz], psi2[2][ix, iy, iz]], axis=1)
+    if scale != 1.0:
+        p1 = p1 * scale
+        p2 = p2 * (scale * scale)
+    q = np.stack([(ix + 0.5) * d, (iy + 0.5) * d, (iz + 0.5) * d], axis=1)
+    return rem_euclid(q + p1 + d2 * p2, box), p1 * vel1 + p2 * vel2
 
 
 # -- background cosmology (cosmology.rs), c = (omega_m, omega_lambda, h0, w0, wa, omega_nu)

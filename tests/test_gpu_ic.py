@@ -178,6 +178,48 @@ def test_full_size_invariants(pm):
     assert 0.9 < mean < 1.1
 
 
+@pytest.mark.parametrize("n,box,seed,ns,resc", [(8, 8.0, 3, -2.0, False), (16, 100.0, 11, -1.5, True),
+                                                (24, 48.0, 5, -2.5, True), (32, 64.0, 2, -2.0, False)])
+def test_2lpt_particles_match_the_oracle(pm, n, box, seed, ns, resc):
+    """Second order: the first-order part as before; the second-order term
+    (the difference from the first-order particles) within 1e-4 of its own
+    size -- its source is a difference of f32 products."""
+    from gadget_ng_pm_hip import ic
+    from gadget_ng_pm_hip import integrate as gi
+    print(f"case ic 2lpt n={n} box={box} seed={seed} n_s={ns} rescale={resc}", flush=True)
+    amp = _amplitude_for(n, box, ns, target_cells=0.8)
+    plan = pm.Plan(n, box)
+    cosmo = gi.CosmologyParams(*LCDM[:3])
+    fn = ic.power_law_sigma(n, amp, ns)
+    pos, vel, _ = ic.zeldovich_2lpt_ics(plan, seed, fn, cosmology=cosmo, a_init=0.02, rescale_to_a_init=resc)
+    v1, v2, d2, sc = ic.factors_2lpt(cosmo, 0.02, 0.1, resc)
+    assert (v1, v2, d2) == ico.factors_2lpt(LCDM, 0.02, True)
+    sig = ic.sigma_table(n, fn)
+    delta = _oracle_field(n, seed, sig)
+    psi1 = ico.displacement(delta, box)
+    psi2 = ico.source_to_psi2(ico.source_2lpt(ico.phi_second_derivatives(delta)), box)
+    rp, rv = ico.zeldovich_2lpt(n, box, psi1, psi2, v1, v2, d2, sc)
+    # split the momenta into the two orders with the first-order oracle
+    _, rv1 = ico.zeldovich(n, box, psi1, v1, sc)
+    sec_ref = rv - rv1
+    sec_got = vel - rv1
+    print(f"second-order share of the momenta {np.abs(sec_ref).max() / np.abs(rv).max():.3e}", flush=True)
+    assert np.abs(sec_ref).max() > 1e-3 * np.abs(rv).max()
+    assert rms_rel(sec_got, sec_ref) < 1e-4
+    assert rms_rel(vel, rv) < 2e-5
+    dp = (pos - rp + box / 2) % box - box / 2
+    assert np.abs(dp).max() < 1e-4 * (np.abs(rv1).max() / v1 + abs(d2) * np.abs(sec_ref).max() / abs(v2))
+    assert pos.min() >= 0.0 and pos.max() < box
+    # sub-range and device entry
+    p2, v2b, _ = ic.zeldovich_2lpt_ics(plan, seed, fn, cosmology=cosmo, a_init=0.02, rescale_to_a_init=resc,
+                                       lo=7, hi=n ** 3 - 3)
+    assert np.array_equal(p2, pos[7:n ** 3 - 3]) and np.array_equal(v2b, vel[7:n ** 3 - 3])
+    dpos = pm.DeviceArray((n ** 3, 3), np.float64)
+    dvel = pm.DeviceArray((n ** 3, 3), np.float64)
+    ic.zeldovich_2lpt_ics_device(plan, seed, sig, v1, v2, d2, sc, dpos.ptr, dvel.ptr)
+    assert np.array_equal(dpos.numpy(), pos) and np.array_equal(dvel.numpy(), vel)
+
+
 def test_bad_arguments(pm):
     from gadget_ng_pm_hip import ic
     plan = pm.Plan(16, 10.0)

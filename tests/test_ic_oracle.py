@@ -216,3 +216,58 @@ def test_library_growth_equals_the_oracle_bit_for_bit(c):
         assert cp.hubble(a) == ico.hubble_param(c, a)
         assert cp.growth_rate(a) == ico.growth_rate_f(c, a)
         assert cp.growth_factor(a) == ico.growth_factor_d(c, a)
+
+
+def test_2lpt_planted_modes_closed_form():
+    """delta-hat with one cosine mode along axis 0 (m1) and one along axis 1
+    (m2): phi,xx = -delta1(x), phi,yy = -delta2(y), every other derivative 0,
+    so S = delta1 delta2 = A cos(t1) cos(t2), A = 4 c1 c2 / n^6, and Psi2_a =
+    d (A/2) sum over m = (m1, +-m2, 0) of m_a / |m|^2 sin(t1 +- t2)."""
+    n, box, m1, m2, c1, c2 = 16, 2.0, 2, 3, 0.8, -0.5
+    d = np.zeros((n, n, n), dtype=np.complex128)
+    d[m1, 0, 0] = d[n - m1, 0, 0] = c1
+    d[0, m2, 0] = d[0, n - m2, 0] = c2
+    phi = ico.phi_second_derivatives(d)
+    j = np.arange(n)
+    t1 = (2 * np.pi * m1 * j / n)[:, None, None]
+    t2 = (2 * np.pi * m2 * j / n)[None, :, None]
+    amp = 4 * c1 * c2 / n ** 6
+    s = ico.source_2lpt(phi)
+    np.testing.assert_allclose(s, np.broadcast_to(amp * np.cos(t1) * np.cos(t2), s.shape), rtol=0,
+                               atol=1e-12 * abs(amp))
+    psi2 = ico.source_to_psi2(s, box)
+    want = [np.zeros((n, n, n)) for _ in range(3)]
+    for sgn in (1, -1):
+        m = np.array([m1, sgn * m2, 0.0])
+        mm = (m * m).sum()
+        for a in range(3):
+            want[a] = want[a] + (box / n) * 0.5 * amp * m[a] / mm * np.sin(t1 + sgn * t2)
+    for a in range(3):
+        np.testing.assert_allclose(psi2[a], np.broadcast_to(want[a], (n, n, n)), rtol=0,
+                                   atol=1e-12 * abs(amp) * box)
+
+
+def test_2lpt_source_is_quadratic_and_psi2_real():
+    n = 16
+    d = ico.delta_kspace(n, 42, ico.power_law_sigma(n, 1e-2, -4.0))
+    s1 = ico.source_2lpt(ico.phi_second_derivatives(d))
+    s2 = ico.source_2lpt(ico.phi_second_derivatives(2.0 * d))
+    np.testing.assert_allclose(s2, 4.0 * s1, rtol=1e-12, atol=1e-30)
+    p1 = ico.source_to_psi2(s1, 1.0)
+    p2 = ico.source_to_psi2(s2, 1.0)
+    for a in range(3):
+        assert np.all(np.isfinite(p1[a]))
+        np.testing.assert_allclose(p2[a], 4.0 * p1[a], rtol=1e-10, atol=1e-12 * np.abs(p2[a]).max())
+
+
+def test_2lpt_factors():
+    v1, v2, d2 = ico.factors_2lpt(EDS, 0.5, True)
+    assert d2 == -3.0 / 7.0
+    assert v1 == pytest.approx(0.25 * 0.7 * 0.5 ** -1.5, rel=1e-14)
+    assert v2 == pytest.approx(0.25 * 2.0 * 0.7 * 0.5 ** -1.5 * (-3.0 / 7.0), rel=1e-14)
+    v1, v2, d2 = ico.factors_2lpt(LCDM, 0.02, True)
+    assert -0.44 < d2 < -0.42  # Bouchet et al.: about -0.428 at Omega_m(a) near 1
+    from gadget_ng_pm_hip import ic
+    cp = gi.CosmologyParams(*LCDM[:3])
+    assert ic.factors_2lpt(cp, 0.02, 0.1, False)[:3] == (v1, v2, d2)
+    assert ic.factors_2lpt(None, 0.3, 0.7, False)[:3] == ico.factors_2lpt((1.0, 0.0, 0.7), 0.3, False)
