@@ -57,7 +57,21 @@ def stage_bytes(n: int, nm: int) -> dict:
 STAGE_KERNELS = {"bucket": ("k_bucket_count",), "sort": ("k_scan_local", "k_scan_partials", "k_scan_add", "k_scatter"),
                  "deposit": ("k_deposit_tiles",), "fft_x_fwd": ("k_fft_x_fwd",), "fft_y_fwd": ("k_fft_y_fwd",),
                  "fft_z_green": ("k_fft_z_green",), "fft_y_inv": ("k_fft_y_inv",), "fft_x_inv": ("k_fft_x_inv",),
-                 "interp": ("k_interp",)}
+                 "interp": ("k_interp",),
+                 "fft_xy_fwd": ("k_fft_x_fwd", "k_fft_y_fwd"), "fft_yx_inv": ("k_fft_y_inv", "k_fft_x_inv")}
+
+
+def merge_chunked(stages: dict, sb: dict, tuning: dict) -> None:
+    """With plane chunks a pass pair runs interleaved chunk by chunk and its
+    two stages share one timing slot: report the pair as one stage, with the
+    sum of both passes' algorithmic bytes."""
+    for on, a, b, name in ((tuning.get("chunk_fwd", 0) and not tuning.get("y_fwd_persistent"), "fft_x_fwd",
+                            "fft_y_fwd", "fft_xy_fwd"),
+                           (tuning.get("chunk_inv", 0) and not tuning.get("y_inv_persistent"), "fft_y_inv",
+                            "fft_x_inv", "fft_yx_inv")):
+        if on and a in stages and b in stages:
+            stages[name] = stages.pop(a) + stages.pop(b)
+            sb[name] = sb.pop(a) + sb.pop(b)
 
 
 def measured_traffic(stage: str, nm: int, workload: str):
@@ -77,7 +91,8 @@ def measured_traffic(stage: str, nm: int, workload: str):
     for k, v in kern.items():
         short = k.split("::")[-1]
         if short.startswith(STAGE_KERNELS[stage]) and (f"<{nm}" in short or "<" not in short):
-            total += v["read_bytes"] + v["write_bytes"]
+            # bytes per solve: a chunked pass runs several launches per solve
+            total += v.get("per_solve_bytes", v["read_bytes"] + v["write_bytes"])
             found = True
     return int(total) if found else None
 
@@ -187,6 +202,7 @@ def main() -> int:
     log("[bench] stage ms: " + ", ".join(f"{k}={v:.4f}" for k, v in stages.items()))
 
     sb = stage_bytes(n, nm)
+    merge_chunked(stages, sb, tuning)
     dom = max(stages, key=stages.get)
     dom_ms = stages[dom]
     achieved = sb[dom] / (dom_ms * 1e-3) / 1e9
@@ -496,6 +512,7 @@ def bench_kdk(args) -> int:
     ms_step = wall * 1e3 / k
     log(f"[bench] {k} KDK steps: {ms_step:.3f} ms/step, {2 * pm_ms:.3f} ms of it in the two PM solves")
     sb = stage_bytes(n, nm)
+    merge_chunked(stages, sb, tuning)
     dom = max(stages, key=stages.get)
     achieved = sb[dom] / (stages[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
@@ -635,6 +652,8 @@ def bench_pk(args) -> int:
         + ", ".join(f"{kk}={v:.4f}" for kk, v in stages.items()))
     sb = stage_bytes(n, nm)
     sb["fft_z_shells"] = 4 * nm ** 3  # the half spectrum read once
+    merge_chunked(stages, sb, {"chunk_fwd": tuning.get("chunk_fwd", 0),
+                               "y_fwd_persistent": tuning.get("y_fwd_persistent", False)})
     dom = max(stages, key=stages.get)
     achieved = sb[dom] / (stages[dom] * 1e-3) / 1e9
     result = {
