@@ -4,6 +4,9 @@ Mirrors crates/gadget-ng-core/src/ic_zeldovich.rs on top of libgng_pm.so:
 
 * :func:`power_law_sigma`  -- ``build_spectrum_fn`` with ``TransferKind::PowerLaw``
   (:404-415), tabulated at sqrt(n2) for the integer |n|^2 of every mode;
+* :func:`eisenstein_hu_sigma` -- the ``EisensteinHu`` branch (:417-449) with
+  crates/gadget-ng-core/src/transfer_fn.rs (no-wiggle transfer function,
+  sigma_8 normalisation);
 * :func:`delta_kspace`     -- ``generate_delta_kspace`` (:287-369), FP64 modes
   iz = 0 .. n/2;
 * :func:`zeldovich_ics`    -- ``zeldovich_ics`` (:621-766): Gaussian field,
@@ -25,6 +28,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+from dataclasses import dataclass
 from typing import Callable, Optional, Tuple
 
 import numpy as np
@@ -58,6 +62,88 @@ def power_law_sigma(n: int, amplitude: float, spectral_index: float) -> Callable
         if n_abs <= 0.0:
             return 0.0
         return amplitude * math.pow(n_abs, spectral_index / 4.0) * inv_sqrt_n3
+    return f
+
+
+@dataclass(frozen=True)
+class EisensteinHuParams:
+    """crates/gadget-ng-core/src/transfer_fn.rs:39-60 (Planck 2018 defaults)."""
+    omega_m: float = 0.315
+    omega_b: float = 0.049
+    h: float = 0.674
+    t_cmb: float = 2.7255
+
+
+def transfer_eh_nowiggle(k_hmpc: float, p: EisensteinHuParams) -> float:
+    """The reference's no-wiggle Eisenstein-Hu transfer function
+    (transfer_fn.rs:89-114), term by term: sound horizon s, alpha_Gamma,
+    Gamma_eff with k s in Mpc, q = k / Gamma_eff (k in h/Mpc; the reference
+    leaves out the CMB-temperature factor of the paper), T = L0 / (L0 + C0 q^2),
+    clamped to [0, 1]."""
+    if k_hmpc <= 0.0:
+        return 1.0
+    om_h2 = p.omega_m * p.h * p.h
+    ob_h2 = p.omega_b * p.h * p.h
+    fb = p.omega_b / p.omega_m
+    s = 44.5 * math.log(9.83 / om_h2) / math.sqrt(1.0 + 10.0 * math.pow(ob_h2, 0.75))
+    alpha = 1.0 - 0.328 * math.log(431.0 * om_h2) * fb + 0.38 * math.log(22.3 * om_h2) * fb * fb
+    ks = 0.43 * (k_hmpc * p.h) * s
+    gamma_eff = p.omega_m * p.h * (alpha + (1.0 - alpha) / (1.0 + ks ** 4))
+    q = k_hmpc / max(gamma_eff, 1e-30)
+    l0 = math.log(math.e * 2.0 + 1.8 * q)
+    c0 = 14.2 + 731.0 / (1.0 + 62.5 * q)
+    t = l0 / (l0 + c0 * q * q)
+    return min(max(t, 0.0), 1.0)
+
+
+def tophat_window(x: float) -> float:
+    """3 (sin x - x cos x) / x^3, 1 - x^2/10 below 1e-4 (transfer_fn.rs)."""
+    if abs(x) < 1e-4:
+        return 1.0 - x * x / 10.0
+    return 3.0 * (math.sin(x) - x * math.cos(x)) / (x * x * x)
+
+
+def sigma_sq_unit(r_mpc_h: float, n_s: float, p: EisensteinHuParams) -> float:
+    """sigma^2(R) for unit amplitude: trapezoid rule in ln k over 8192 steps
+    from 1e-5 to 500 of k^(n_s+3) T^2 W^2 / (2 pi^2) (transfer_fn.rs:150-183)."""
+    lo, hi, steps = math.log(1e-5), math.log(5e2), 8192
+    dl = (hi - lo) / steps
+
+    def f(k):
+        t = transfer_eh_nowiggle(k, p)
+        w = tophat_window(k * r_mpc_h)
+        return math.pow(k, n_s + 3.0) * t * t * w * w
+    total = 0.0
+    fp = f(1e-5)
+    for i in range(1, steps + 1):
+        fk = f(math.exp(lo + i * dl))
+        total += 0.5 * (fp + fk) * dl
+        fp = fk
+    return total / (2.0 * math.pi * math.pi)
+
+
+def amplitude_for_sigma8(sigma8: float, n_s: float, p: EisensteinHuParams) -> float:
+    """A with sigma(8 Mpc/h) = sigma8 (transfer_fn.rs:222-228)."""
+    s2 = sigma_sq_unit(8.0, n_s, p)
+    return sigma8 if s2 <= 0.0 else sigma8 / math.sqrt(s2)
+
+
+def eisenstein_hu_sigma(n: int, spectral_index: float, amplitude: float, p: EisensteinHuParams,
+                        sigma8: Optional[float] = None,
+                        box_size_mpc_h: Optional[float] = None) -> Callable[[float], float]:
+    """build_spectrum_fn, EisensteinHu branch (ic_zeldovich.rs:417-449):
+    A k^(n_s/2) T(k) / sqrt(n^3) with k = 2 pi h |n| / L (L defaults to
+    100 Mpc/h) and A from sigma8 when given."""
+    inv_sqrt_n3 = 1.0 / math.sqrt(float(n * n * n))
+    bsm = 100.0 if box_size_mpc_h is None else box_size_mpc_h
+    amp = amplitude_for_sigma8(sigma8, spectral_index, p) if sigma8 is not None else amplitude
+    c = 2.0 * math.pi * p.h / bsm
+
+    def f(n_abs: float) -> float:
+        if n_abs <= 0.0:
+            return 0.0
+        k = c * n_abs
+        return amp * math.pow(k, spectral_index / 2.0) * transfer_eh_nowiggle(k, p) * inv_sqrt_n3
     return f
 
 

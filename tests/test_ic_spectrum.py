@@ -1,0 +1,71 @@
+"""Host-side spectrum builders of the initial conditions
+(gadget_ng_pm_hip.ic): the Eisenstein-Hu branch of build_spectrum_fn and the
+sigma_8 normalisation (crates/gadget-ng-core/src/transfer_fn.rs).  No GPU.
+
+Pinned by the reference's own tests (transfer_fn.rs:288-440: T -> 1 at low
+k, its range at k = 0.1 h/Mpc, monotonic decrease, the top-hat window, the
+amplitude reproducing sigma_8) and by scipy's adaptive quadrature of the
+sigma^2 integral, independent of the restated trapezoid rule."""
+import math
+
+import numpy as np
+import pytest
+from scipy.integrate import quad
+
+from gadget_ng_pm_hip import ic
+
+P18 = ic.EisensteinHuParams()
+
+
+def test_transfer_reference_checks():
+    assert abs(ic.transfer_eh_nowiggle(1e-4, P18) - 1.0) < 0.01
+    assert ic.transfer_eh_nowiggle(1e-3, P18) > 0.90
+    t = ic.transfer_eh_nowiggle(0.1, P18)
+    assert 0.05 < t < 0.30
+    assert ic.transfer_eh_nowiggle(0.01, P18) > t > ic.transfer_eh_nowiggle(1.0, P18)
+    for e in range(-4, 4):
+        assert 0.0 <= ic.transfer_eh_nowiggle(10.0 ** e, P18) <= 1.0
+    assert ic.transfer_eh_nowiggle(0.0, P18) == 1.0
+
+
+def test_tophat_window():
+    assert abs(ic.tophat_window(0.0) - 1.0) < 1e-12
+    assert ic.tophat_window(1.0) < ic.tophat_window(0.1)
+    assert ic.tophat_window(10.0) < ic.tophat_window(1.0)
+    # the series branch joins the closed form
+    assert ic.tophat_window(0.99e-4) == pytest.approx(3 * (math.sin(1.01e-4) - 1.01e-4 * math.cos(1.01e-4))
+                                                      / 1.01e-4 ** 3, abs=1e-8)
+
+
+@pytest.mark.parametrize("r,ns", [(8.0, 0.965), (4.0, 1.0), (16.0, 0.9)])
+def test_sigma_integral_against_quadrature(r, ns):
+    def f(lk):
+        k = math.exp(lk)
+        t = ic.transfer_eh_nowiggle(k, P18)
+        w = ic.tophat_window(k * r)
+        return k ** (ns + 3.0) * t * t * w * w
+    ref = quad(f, math.log(1e-5), math.log(5e2), limit=500, epsrel=1e-10)[0] / (2 * math.pi ** 2)
+    # the restated trapezoid rule (8192 steps in ln k) is good to ~1e-4 once the
+    # window oscillates (larger R); a wrong power or factor is off by far more
+    assert ic.sigma_sq_unit(r, ns, P18) == pytest.approx(ref, rel=2e-4)
+
+
+def test_amplitude_reproduces_sigma8():
+    a = ic.amplitude_for_sigma8(0.8, 0.965, P18)
+    assert a * math.sqrt(ic.sigma_sq_unit(8.0, 0.965, P18)) == pytest.approx(0.8, rel=1e-12)
+
+
+def test_eisenstein_hu_spectrum_closure():
+    n, box = 64, 200.0
+    f = ic.eisenstein_hu_sigma(n, 0.965, 1.0, P18, sigma8=0.8, box_size_mpc_h=box)
+    a = ic.amplitude_for_sigma8(0.8, 0.965, P18)
+    assert f(0.0) == 0.0
+    k = 2 * math.pi * P18.h / box * 5.0
+    assert f(5.0) == pytest.approx(a * k ** (0.965 / 2) * ic.transfer_eh_nowiggle(k, P18) / math.sqrt(n ** 3),
+                                   rel=1e-14)
+    tab = ic.sigma_table(n, f)
+    assert tab.shape == (ic.table_len(n),) and tab[0] == 0.0 and np.all(tab[1:] > 0.0)
+    # without sigma8 the amplitude is used as given; the box defaults to 100 Mpc/h
+    g = ic.eisenstein_hu_sigma(n, 1.0, 2.0, P18)
+    k1 = 2 * math.pi * P18.h / 100.0
+    assert g(1.0) == pytest.approx(2.0 * k1 ** 0.5 * ic.transfer_eh_nowiggle(k1, P18) / math.sqrt(n ** 3), rel=1e-14)
