@@ -333,6 +333,14 @@ __global__ __launch_bounds__(NT) void k_deposit_tiles(const float4* __restrict__
     // local base index in the particle's own tile; its class follows from it
     const int lx0 = ix % T, ly0 = iy % T, lz0 = iz % T;
     const int cx = lx0 == T - 1, cy = ly0 == T - 1, cz = lz0 == T - 1;
+    // Crowded wavefront: some lane shares its base cell (and source range)
+    // with the lane before it, as in dense clumps, where the sorted order puts
+    // a cell's particles next to each other.  Only then do the targets go
+    // through the wave-level aggregation; otherwise each lane adds directly.
+    // Integer sums are exact, so both paths give the same totals.
+    const int key = (((lz0 * T + ly0) * T + lx0) << 3) | d;
+    const int prev_key = __shfl_up(key, 1);
+    const bool crowded = __ballot(live && (tid & 63) != 0 && prev_key == key) != 0ull;
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
       const int lz = dz ? 0 : lz0 + a;
@@ -345,7 +353,10 @@ __global__ __launch_bounds__(NT) void k_deposit_tiles(const float4* __restrict__
           const bool valid = live && (a & cz) == dz && (b & cy) == dy && (c & cx) == dx;
           const float w = p.w * wx[c] * wy[b] * wz[a];
           const long long qv = valid ? __float2ll_rn(ldexpf(w, s)) : 0ll;
-          lds_add_aggregated(acc, valid ? (lz * T + ly) * T + lx : 0, qv, valid);
+          if (crowded)
+            lds_add_aggregated(acc, valid ? (lz * T + ly) * T + lx : 0, qv, valid);
+          else if (valid)
+            atomicAdd(acc + (lz * T + ly) * T + lx, (unsigned long long)qv);
         }
       }
     }
