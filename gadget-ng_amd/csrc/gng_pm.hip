@@ -285,7 +285,15 @@ MeshOps make_ops() {
     hipLaunchKernelGGL((k_fft_y_inv_persist<N, C, TY>), g, b, 0, s, A, B, P, tw);
   };
   o.y_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* B, float2* P, const float2* tw) {
-    hipLaunchKernelGGL((k_fft_y_inv<N, C, TY>), g, b, 0, s, A, B, P, tw, make_int3(0, 0, 0));
+    if constexpr (YZCfg<N>::YSPLIT) {
+      // full-line tiles of the z pass's width; the caller's grid counts C-wide tiles
+      constexpr int CS = YZCfg<N>::CZ, TS = YZCfg<N>::TZ;
+      const dim3 gs((unsigned)(N / 2 / CS), g.y);
+      hipLaunchKernelGGL((k_fft_y_inv<N, CS, TS, false, 1>), gs, dim3(TS), 0, s, A, B, P, tw, make_int3(0, 0, 0));
+      hipLaunchKernelGGL((k_fft_y_inv<N, CS, TS, false, 2>), gs, dim3(TS), 0, s, A, B, P, tw, make_int3(0, 0, 0));
+    } else {
+      hipLaunchKernelGGL((k_fft_y_inv<N, C, TY>), g, b, 0, s, A, B, P, tw, make_int3(0, 0, 0));
+    }
   };
   o.x_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* P, float2* B, float2* pot, const float2* tw,
                 int field0, float pot_scale) {

@@ -285,6 +285,9 @@ struct YZCfg {
   static constexpr int CZ = N >= 1024 ? yz_cols(H) : C;
   static constexpr int TZ_RAW = CZ * N / 8;
   static constexpr int TZ = TZ_RAW >= 1024 ? 1024 : (TZ_RAW <= 64 ? 64 : (TZ_RAW + 63) / 64 * 64);
+  // the inverse y pass at N = 1024 as two launches (fields from A, then B) on
+  // full 128-byte tiles, 16 values per thread
+  static constexpr bool YSPLIT = N >= 1024;
 };
 
 // Column tiles narrower than a 128-byte line (C = 8 at nm = 1024) share each
@@ -557,11 +560,15 @@ __global__ __launch_bounds__(256) void k_fft_z_green_col0(const float2* __restri
 // each starting from registers and storing from registers.  With MUL the
 // three take the multipliers codes.x (A), .y (P, from A) and .z (B) instead
 // (the initial-condition passes); the force solve instantiates MUL = false.
-template <int N, int C, int T, bool MUL = false>
+template <int N, int C, int T, bool MUL = false, int PART = 0>
 __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2* __restrict__ B,
                                                  float2* __restrict__ P, const float2* __restrict__ tw_g,
                                                  int3 codes) {
+  // PART 1: only the two fields from A (A and P); PART 2: only B.  Two such
+  // launches read each input once, like the combined kernel, with fewer
+  // live registers (for 16 values per thread at N = 1024).
   constexpr int H = N / 2, IS = C + 1;
+  constexpr bool DO_A = PART != 2, DO_B = PART != 1;
   using F = RegFft<N, true, false, C, T, 1, IS, true, N>;
   __shared__ float2 buf[N * IS];
   const int tid = threadIdx.x;
@@ -574,45 +581,69 @@ __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2*
   float2* a = A + off;
   float2* b = B + off;
   float2* pp = P + off;
-  float2 av[F::PER0][F::R0], bv[F::PER0][F::R0], w[F::PER0][F::R0], out[F::PERL][F::RL];
-  for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
-    av[p][r] = ok ? a[(unsigned)i * H + l] : make_float2(0.f, 0.f);
-  });
-  // field B is loaded now, so its load is in flight during both A transforms
-  for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
-    bv[p][r] = ok ? b[(unsigned)i * H + l] : make_float2(0.f, 0.f);
-  });
-  if constexpr (MUL) {
+  float2 out[F::PERL][F::RL];
+  if constexpr (DO_A) {
+    float2 av[F::PER0][F::R0], w[F::PER0][F::R0];
+    for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      av[p][r] = ok ? a[(unsigned)i * H + l] : make_float2(0.f, 0.f);
+    });
+    float2 bv[F::PER0][F::R0];
+    if constexpr (DO_B) {
+      // field B is loaded now, so its load is in flight during both A transforms
+      for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        bv[p][r] = ok ? b[(unsigned)i * H + l] : make_float2(0.f, 0.f);
+      });
+    }
+    if constexpr (MUL) {
+      for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        const float nye = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
+        w[p][r] = ic_mul(codes.x, nye, av[p][r]);
+        if constexpr (DO_B) bv[p][r] = ic_mul(codes.z, nye, bv[p][r]);
+      });
+    } else {
+#pragma unroll
+      for (int p = 0; p < F::PER0; ++p)
+#pragma unroll
+        for (int r = 0; r < F::R0; ++r) w[p][r] = av[p][r];
+    }
+    F::run(w, out, buf, tw, tid);
+    for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      if (ok) a[(unsigned)i * H + l] = out[p][r];
+    });
     for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
       const float nye = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
-      w[p][r] = ic_mul(codes.x, nye, av[p][r]);
-      bv[p][r] = ic_mul(codes.z, nye, bv[p][r]);
+      if constexpr (MUL)
+        w[p][r] = ic_mul(codes.y, nye, av[p][r]);
+      else
+        w[p][r] = make_float2(nye * av[p][r].y, -nye * av[p][r].x);
     });
+    F::run(w, out, buf, tw, tid);
+    for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      if (ok) pp[(unsigned)i * H + l] = out[p][r];
+    });
+    if constexpr (DO_B) {
+      F::run(bv, out, buf, tw, tid);
+      for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        if (ok) b[(unsigned)i * H + l] = out[p][r];
+      });
+    }
   } else {
-#pragma unroll
-    for (int p = 0; p < F::PER0; ++p)
-#pragma unroll
-      for (int r = 0; r < F::R0; ++r) w[p][r] = av[p][r];
+    float2 bv[F::PER0][F::R0];
+    for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      bv[p][r] = ok ? b[(unsigned)i * H + l] : make_float2(0.f, 0.f);
+    });
+    if constexpr (MUL) {
+      for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+        const float nye = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
+        bv[p][r] = ic_mul(codes.z, nye, bv[p][r]);
+      });
+    }
+    // no barrier before the first stage's LDS writes is needed: run begins with one
+    F::run(bv, out, buf, tw, tid);
+    for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
+      if (ok) b[(unsigned)i * H + l] = out[p][r];
+    });
   }
-  F::run(w, out, buf, tw, tid);
-  for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
-    if (ok) a[(unsigned)i * H + l] = out[p][r];
-  });
-  for_edge<N, F::R0, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
-    const float nye = (i == N / 2) ? 0.0f : (float)freq_of(i, N);
-    if constexpr (MUL)
-      w[p][r] = ic_mul(codes.y, nye, av[p][r]);
-    else
-      w[p][r] = make_float2(nye * av[p][r].y, -nye * av[p][r].x);
-  });
-  F::run(w, out, buf, tw, tid);
-  for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
-    if (ok) pp[(unsigned)i * H + l] = out[p][r];
-  });
-  F::run(bv, out, buf, tw, tid);
-  for_edge<N, F::RL, C, T, true>(tid, [&](int p, int r, int l, int i, bool ok) {
-    if (ok) b[(unsigned)i * H + l] = out[p][r];
-  });
 }
 
 // Persistent forms of the y passes: gridDim.x workgroups walk the (column
