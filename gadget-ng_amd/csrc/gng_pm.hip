@@ -719,11 +719,23 @@ int run_pk(gng_pm_plan* p, const float* x, const float* y, const float* z, const
       }
       w[i] = 1.0 / (sc * sc);
     }
-    GNG_HIP(hipMalloc(&p->d_pk_modes, sizeof(unsigned long long) * (size_t)H));
-    GNG_HIP(hipMalloc(&p->d_pk_msum, sizeof(double) * (kPkMassBlocks + 1)));
-    GNG_HIP(hipMalloc(&p->d_pk_out, sizeof(double) * (size_t)H));
-    GNG_HIP(hipMalloc(&p->d_pk_w, sizeof(double) * (size_t)nm));
-    GNG_HIP(hipMemcpy(p->d_pk_w, w.data(), sizeof(double) * (size_t)nm, hipMemcpyHostToDevice));
+    // all four buffers or none: d_pk_w, set last, marks the set as ready
+    unsigned long long* modes_b = nullptr;
+    double *msum_b = nullptr, *out_b = nullptr, *w_b = nullptr;
+    hipError_t e = hipMalloc(&modes_b, sizeof(unsigned long long) * (size_t)H);
+    if (e == hipSuccess) e = hipMalloc(&msum_b, sizeof(double) * (kPkMassBlocks + 1));
+    if (e == hipSuccess) e = hipMalloc(&out_b, sizeof(double) * (size_t)H);
+    if (e == hipSuccess) e = hipMalloc(&w_b, sizeof(double) * (size_t)nm);
+    if (e == hipSuccess) e = hipMemcpy(w_b, w.data(), sizeof(double) * (size_t)nm, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      for (void* q : {(void*)modes_b, (void*)msum_b, (void*)out_b, (void*)w_b}) (void)hipFree(q);
+      return fail(e == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP,
+                  std::string("power-spectrum tables: ") + hipGetErrorString(e));
+    }
+    p->d_pk_modes = modes_b;
+    p->d_pk_msum = msum_b;
+    p->d_pk_out = out_b;
+    p->d_pk_w = w_b;
   }
   const long long zblocks = (long long)(H / o.yz_cols) * nm, nblk = zblocks + H + 1;
   if (nblk > p->pk_part_cap) {
@@ -830,7 +842,13 @@ int run_ic_2lpt(gng_pm_plan* p, const IcParams& ip, double vel1, double vel2, do
     std::vector<float> mask((size_t)nm, 1.0f);
     mask[(size_t)H] = 0.0f;
     GNG_HIP(hipMalloc(&p->d_ic_mask, sizeof(float) * (size_t)nm));
-    GNG_HIP(hipMemcpy(p->d_ic_mask, mask.data(), sizeof(float) * (size_t)nm, hipMemcpyHostToDevice));
+    const hipError_t e = hipMemcpy(p->d_ic_mask, mask.data(), sizeof(float) * (size_t)nm, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      // leave no uninitialised mask behind for the next call
+      (void)hipFree(p->d_ic_mask);
+      p->d_ic_mask = nullptr;
+      return fail(GNG_ERR_HIP, std::string("2LPT mask upload: ") + hipGetErrorString(e));
+    }
   }
   int rc = run_ic(p, ip, vel1, scale, lo, cnt, pos, vel, s, 0);
   if (rc) return rc;
