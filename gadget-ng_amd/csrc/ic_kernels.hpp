@@ -206,30 +206,41 @@ __global__ __launch_bounds__(256) void k_ic_particles(const float* __restrict__ 
                                                       double vel_factor, long long lo, long long cnt,
                                                       double* __restrict__ pos, double* __restrict__ vel, int wrap) {
 #pragma clang fp contract(off)
-  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= cnt) return;
-  const long long gid = lo + t;
-  const long long nn = (long long)n * n;
-  const int ix = (int)(gid / nn);
-  const long long rem = gid % nn;
-  const int iy = (int)(rem / n), iz = (int)(rem % n);
-  const long long off = (long long)ix * fplane + (long long)iy * n + iz;
-  double p0 = (double)psi_x[off] * norm, p1 = (double)psi_y[off] * norm, p2 = (double)psi_z[off] * norm;
-  if (scale != 1.0) {
-    p0 *= scale;
-    p1 *= scale;
-    p2 *= scale;
+  // the workgroup's 256 particles are staged in LDS so that the interleaved
+  // f64 outputs go out as contiguous, coalesced stores
+  __shared__ double sp[3 * 256], sv[3 * 256];
+  const long long base = (long long)blockIdx.x * 256;
+  const long long t = base + threadIdx.x;
+  const int nloc = (int)(cnt - base < 256 ? cnt - base : 256);
+  if (t < cnt) {
+    const long long gid = lo + t;
+    const long long nn = (long long)n * n;
+    const int ix = (int)(gid / nn);
+    const long long rem = gid % nn;
+    const int iy = (int)(rem / n), iz = (int)(rem % n);
+    const long long off = (long long)ix * fplane + (long long)iy * n + iz;
+    double p0 = (double)psi_x[off] * norm, p1 = (double)psi_y[off] * norm, p2 = (double)psi_z[off] * norm;
+    if (scale != 1.0) {
+      p0 *= scale;
+      p1 *= scale;
+      p2 *= scale;
+    }
+    const double q0 = ((double)ix + 0.5) * d, q1 = ((double)iy + 0.5) * d, q2 = ((double)iz + 0.5) * d;
+    double* P = sp + 3 * threadIdx.x;
+    double* V = sv + 3 * threadIdx.x;
+    // without wrap: q + Psi, for the second-order kernel to complete
+    P[0] = wrap ? ic_rem_euclid(q0 + p0, box) : q0 + p0;
+    P[1] = wrap ? ic_rem_euclid(q1 + p1, box) : q1 + p1;
+    P[2] = wrap ? ic_rem_euclid(q2 + p2, box) : q2 + p2;
+    V[0] = p0 * vel_factor;
+    V[1] = p1 * vel_factor;
+    V[2] = p2 * vel_factor;
   }
-  const double q0 = ((double)ix + 0.5) * d, q1 = ((double)iy + 0.5) * d, q2 = ((double)iz + 0.5) * d;
-  double* P = pos + 3 * t;
-  double* V = vel + 3 * t;
-  // without wrap: q + Psi, for the second-order kernel to complete
-  P[0] = wrap ? ic_rem_euclid(q0 + p0, box) : q0 + p0;
-  P[1] = wrap ? ic_rem_euclid(q1 + p1, box) : q1 + p1;
-  P[2] = wrap ? ic_rem_euclid(q2 + p2, box) : q2 + p2;
-  V[0] = p0 * vel_factor;
-  V[1] = p1 * vel_factor;
-  V[2] = p2 * vel_factor;
+  __syncthreads();
+  for (int e = threadIdx.x; e < 3 * nloc; e += 256) {
+    pos[3 * base + e] = sp[e];
+    vel[3 * base + e] = sv[e];
+  }
 }
 
 // Second order: the source S of the 2LPT Poisson equation from the six
@@ -268,22 +279,37 @@ __global__ __launch_bounds__(256) void k_ic_particles_2lpt(const float* __restri
                                                            double vel2, long long lo, long long cnt,
                                                            double* __restrict__ pos, double* __restrict__ vel) {
 #pragma clang fp contract(off)
-  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= cnt) return;
-  const long long gid = lo + t;
-  const long long nn = (long long)n * n;
-  const int ix = (int)(gid / nn);
-  const long long rem = gid % nn;
-  const int iy = (int)(rem / n), iz = (int)(rem % n);
-  const long long off = (long long)ix * fplane + (long long)iy * n + iz;
-  double p[3] = {(double)psi_x[off] * norm, (double)psi_y[off] * norm, (double)psi_z[off] * norm};
-  double* P = pos + 3 * t;
-  double* V = vel + 3 * t;
+  // first-order values in and results out through LDS, coalesced
+  __shared__ double sp[3 * 256], sv[3 * 256];
+  const long long base = (long long)blockIdx.x * 256;
+  const long long t = base + threadIdx.x;
+  const int nloc = (int)(cnt - base < 256 ? cnt - base : 256);
+  for (int e = threadIdx.x; e < 3 * nloc; e += 256) {
+    sp[e] = pos[3 * base + e];
+    sv[e] = vel[3 * base + e];
+  }
+  __syncthreads();
+  if (t < cnt) {
+    const long long gid = lo + t;
+    const long long nn = (long long)n * n;
+    const int ix = (int)(gid / nn);
+    const long long rem = gid % nn;
+    const int iy = (int)(rem / n), iz = (int)(rem % n);
+    const long long off = (long long)ix * fplane + (long long)iy * n + iz;
+    double p[3] = {(double)psi_x[off] * norm, (double)psi_y[off] * norm, (double)psi_z[off] * norm};
+    double* P = sp + 3 * threadIdx.x;
+    double* V = sv + 3 * threadIdx.x;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    if (scale2 != 1.0) p[c] *= scale2;
-    P[c] = ic_rem_euclid(P[c] + d2 * p[c], box);
-    V[c] = V[c] + p[c] * vel2;
+    for (int c = 0; c < 3; ++c) {
+      if (scale2 != 1.0) p[c] *= scale2;
+      P[c] = ic_rem_euclid(P[c] + d2 * p[c], box);
+      V[c] = V[c] + p[c] * vel2;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 3 * nloc; e += 256) {
+    pos[3 * base + e] = sp[e];
+    vel[3 * base + e] = sv[e];
   }
 }
 

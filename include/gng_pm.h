@@ -198,7 +198,8 @@ typedef struct gng_pm_tuning_t {
                                 autotuner chooses between 0 and a size-based
                                 value (fwd: 256 MiB of planes, 256 at 512^3 and
                                 64 at 1024^3; inv: 64 MiB, 64 at 512^3 and 16
-                                at 1024^3); none at 256^3 and below */
+                                at 1024^3, where the default is 0); none at
+                                256^3 and below */
 } gng_pm_tuning_t;
 
 int gng_pm_autotune(gng_pm_plan *p, int reps);
