@@ -59,26 +59,39 @@ __device__ __forceinline__ unsigned bucket_of_cell(int ix, int iy, int iz, const
 }
 
 // Each thread classifies kBucketPer particles (block stride 256, so every
-// wavefront still sees 64 consecutive particles) and issues all their counter
-// atomics before it uses any returned value: the atomics' round trips overlap
-// instead of each one stalling its wavefront.
+// wavefront still sees 64 consecutive particles).  Counting is two-level:
+// runs of equal buckets within a wavefront take one slot of a block-local
+// hash table in LDS and one LDS atomic; then each distinct bucket of the
+// block takes one global atomic, which returns the block's base rank in that
+// bucket.  A block of clustered particles thus costs a few global atomics
+// instead of one per run.  Ranks within a bucket stay unique; their order
+// differs from call to call, which no result depends on (the deposit sums
+// integers).
 constexpr int kBucketPer = 4;
+constexpr int kBucketSlots = 2048;  // > 256 * kBucketPer keys: probing always ends
+constexpr unsigned kEmptyKey = 0xffffffffu;
 
 __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ x, const float* __restrict__ y,
                                                       const float* __restrict__ z, const float* __restrict__ m,
                                                       long long n, const CellMap cm, const TileMap tm,
                                                       unsigned* __restrict__ count, unsigned* __restrict__ bucket,
                                                       unsigned* __restrict__ rank, SolveScalars* sc) {
+  __shared__ unsigned hkey[kBucketSlots], hcnt[kBucketSlots];
+  for (int e = threadIdx.x; e < kBucketSlots; e += 256) {
+    hkey[e] = kEmptyKey;
+    hcnt[e] = 0u;
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const long long i0 = (long long)blockIdx.x * (256 * kBucketPer) + threadIdx.x;
-  unsigned b[kBucketPer], base[kBucketPer];
-  int my_head[kBucketPer];
+  unsigned b[kBucketPer], local[kBucketPer];
+  int slot[kBucketPer];
   float am = 0.0f;
 #pragma unroll
   for (int k = 0; k < kBucketPer; ++k) {
     const long long i = i0 + 256LL * k;
     const bool live = i < n;
-    b[k] = 0xffffffffu;
+    b[k] = kEmptyKey;
     if (live) {
       int ix, iy, iz;
       float d;
@@ -93,20 +106,40 @@ __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ 
     const bool head = live && (lane == 0 || prev != b[k]);
     const unsigned long long heads = __ballot(head);
     const unsigned long long below = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
-    my_head[k] = 63 - __clzll(heads & below);  // highest head at or below this lane
+    const int my_head = 63 - __clzll(heads & below);  // highest head at or below this lane
     const unsigned long long above = heads & ~below;
     const unsigned long long live_mask = __ballot(live);
     const int next_head = above ? __ffsll((long long)above) - 1 : 64 - __clzll(live_mask);
-    base[k] = 0;
-    if (head) base[k] = atomicAdd(count + b[k], (unsigned)(next_head - lane));
+    int hs = 0;
+    unsigned hb = 0;
+    if (head) {
+      // insert or find the bucket in the block's table (linear probing)
+      hs = (int)((b[k] * 2654435761u) >> 21);  // 11 bits: kBucketSlots = 2048
+      for (;;) {
+        const unsigned old = atomicCAS(&hkey[hs], kEmptyKey, b[k]);
+        if (old == kEmptyKey || old == b[k]) break;
+        hs = (hs + 1) & (kBucketSlots - 1);
+      }
+      hb = atomicAdd(&hcnt[hs], (unsigned)(next_head - lane));
+    }
+    const int src = my_head < 0 ? 0 : my_head;
+    slot[k] = __shfl(hs, src);
+    local[k] = __shfl(hb, src) + (unsigned)(lane - my_head);
   }
+  __syncthreads();
+  // one global atomic per distinct bucket of the block; hcnt becomes the base
+#pragma unroll
+  for (int e = threadIdx.x; e < kBucketSlots; e += 256) {
+    const unsigned key = hkey[e];
+    if (key != kEmptyKey) hcnt[e] = atomicAdd(count + key, hcnt[e]);
+  }
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < kBucketPer; ++k) {
     const long long i = i0 + 256LL * k;
-    const unsigned bb = __shfl(base[k], my_head[k] < 0 ? 0 : my_head[k]);
     if (i < n) {
       bucket[i] = b[k];
-      rank[i] = bb + (unsigned)(lane - my_head[k]);
+      rank[i] = hcnt[slot[k]] + local[k];
     }
   }
   // max |m|: wave, then block, then one atomic per block only when it raises
