@@ -7,6 +7,8 @@ Mirrors crates/gadget-ng-core/src/ic_zeldovich.rs on top of libgng_pm.so:
 * :func:`eisenstein_hu_sigma` -- the ``EisensteinHu`` branch (:417-449) with
   crates/gadget-ng-core/src/transfer_fn.rs (no-wiggle transfer function,
   sigma_8 normalisation);
+* :func:`tabulated_sigma`, :class:`TabulatedTransfer` -- the ``Tabulated``
+  branch (:450-460) and its PCHIP table (:118-260);
 * :func:`delta_kspace`     -- ``generate_delta_kspace`` (:287-369), FP64 modes
   iz = 0 .. n/2;
 * :func:`zeldovich_ics`    -- ``zeldovich_ics`` (:621-766): Gaussian field,
@@ -144,6 +146,112 @@ def eisenstein_hu_sigma(n: int, spectral_index: float, amplitude: float, p: Eise
             return 0.0
         k = c * n_abs
         return amp * math.pow(k, spectral_index / 2.0) * transfer_eh_nowiggle(k, p) * inv_sqrt_n3
+    return f
+
+
+class TabulatedTransfer:
+    """A transfer function read from a text table of (k [h/Mpc], T) rows,
+    interpolated by monotone piecewise-cubic Hermite (PCHIP, Fritsch-Carlson
+    slopes) in (ln k, ln T), extrapolated linearly in log space with the end
+    slopes (ic_zeldovich.rs:118-260).  Rows: the first two numbers of a line,
+    separated by spaces, commas or semicolons; '#' lines and rows without
+    k > 0 and T > 0 are skipped; duplicate k keep the first row."""
+
+    def __init__(self, rows):
+        rows = sorted(((float(k), float(t)) for k, t in rows), key=lambda r: r[0])
+        kept = []
+        for k, t in rows:
+            if not kept or abs(k - kept[-1][0]) > np.finfo(np.float64).eps:
+                kept.append((k, t))
+        if len(kept) < 2:
+            raise ValueError("a tabulated transfer function needs at least 2 valid rows (k > 0, T > 0)")
+        self.x = [math.log(k) for k, _ in kept]
+        self.y = [math.log(t) for _, t in kept]
+        self.m = self._slopes(self.x, self.y)
+
+    @classmethod
+    def from_file(cls, path: str) -> "TabulatedTransfer":
+        rows = []
+        with open(path) as f:
+            for raw in f:
+                line = raw.strip()
+                if not line or line.startswith("#"):
+                    continue
+                vals = []
+                for tok in line.replace(",", " ").replace(";", " ").split():
+                    try:
+                        vals.append(float(tok))
+                    except ValueError:
+                        continue
+                    if len(vals) >= 2:
+                        break
+                if len(vals) >= 2 and all(math.isfinite(v) and v > 0.0 for v in vals[:2]):
+                    rows.append((vals[0], vals[1]))
+        return cls(rows)
+
+    @staticmethod
+    def _slopes(x, y):
+        n = len(x)
+        h = [x[i + 1] - x[i] for i in range(n - 1)]
+        d = [(y[i + 1] - y[i]) / h[i] for i in range(n - 1)]
+        if n == 2:
+            return [d[0], d[0]]
+        sgn = lambda v: math.copysign(1.0, v)
+        m = [0.0] * n
+        m[0] = ((2.0 * h[0] + h[1]) * d[0] - h[0] * d[1]) / (h[0] + h[1])
+        if sgn(m[0]) != sgn(d[0]):
+            m[0] = 0.0
+        elif sgn(d[0]) != sgn(d[1]) and abs(m[0]) > 3.0 * abs(d[0]):
+            m[0] = 3.0 * d[0]
+        m[-1] = ((2.0 * h[-1] + h[-2]) * d[-1] - h[-1] * d[-2]) / (h[-1] + h[-2])
+        if sgn(m[-1]) != sgn(d[-1]):
+            m[-1] = 0.0
+        elif sgn(d[-1]) != sgn(d[-2]) and abs(m[-1]) > 3.0 * abs(d[-1]):
+            m[-1] = 3.0 * d[-1]
+        for i in range(1, n - 1):
+            if d[i - 1] == 0.0 or d[i] == 0.0 or sgn(d[i - 1]) != sgn(d[i]):
+                m[i] = 0.0
+            else:
+                w1, w2 = 2.0 * h[i] + h[i - 1], h[i] + 2.0 * h[i - 1]
+                m[i] = (w1 + w2) / (w1 / d[i - 1] + w2 / d[i])
+        return m
+
+    def __call__(self, k: float) -> float:
+        if k <= 0.0 or not math.isfinite(k):
+            return 0.0
+        x, y, m = self.x, self.y, self.m
+        xq = math.log(k)
+        if xq <= x[0]:
+            return math.exp(y[0] + m[0] * (xq - x[0]))
+        if xq >= x[-1]:
+            return math.exp(y[-1] + m[-1] * (xq - x[-1]))
+        lo, hi = 0, len(x) - 1
+        while hi - lo > 1:
+            mid = (lo + hi) // 2
+            if x[mid] <= xq:
+                lo = mid
+            else:
+                hi = mid
+        h = x[lo + 1] - x[lo]
+        t = (xq - x[lo]) / h
+        t2, t3 = t * t, t * t * t
+        yq = ((2.0 * t3 - 3.0 * t2 + 1.0) * y[lo] + (t3 - 2.0 * t2 + t) * h * m[lo]
+              + (-2.0 * t3 + 3.0 * t2) * y[lo + 1] + (t3 - t2) * h * m[lo + 1])
+        return math.exp(yq)
+
+
+def tabulated_sigma(n: int, spectral_index: float, amplitude: float, transfer: TabulatedTransfer, h: float,
+                    box_size_mpc_h: Optional[float] = None) -> Callable[[float], float]:
+    """build_spectrum_fn, Tabulated branch (ic_zeldovich.rs:450-460):
+    A k^(n_s/2) T(k) / sqrt(n^3) with k = 2 pi h |n| / L (L defaults to 100)."""
+    inv_sqrt_n3 = 1.0 / math.sqrt(float(n * n * n))
+
+    def f(n_abs: float) -> float:
+        if n_abs <= 0.0:
+            return 0.0
+        bsm = 100.0 if box_size_mpc_h is None else box_size_mpc_h
+        k = 2.0 * math.pi * h / bsm * n_abs
+        return amplitude * math.pow(k, spectral_index / 2.0) * transfer(k) * inv_sqrt_n3
     return f
 
 

@@ -69,3 +69,47 @@ def test_eisenstein_hu_spectrum_closure():
     g = ic.eisenstein_hu_sigma(n, 1.0, 2.0, P18)
     k1 = 2 * math.pi * P18.h / 100.0
     assert g(1.0) == pytest.approx(2.0 * k1 ** 0.5 * ic.transfer_eh_nowiggle(k1, P18) / math.sqrt(n ** 3), rel=1e-14)
+
+
+def _table_rows():
+    ks = [1e-3, 1.5e-3, 2e-3, 3e-3, 5e-3, 7e-3, 1e-2, 1.5e-2, 2e-2, 3e-2, 5e-2, 7e-2, 0.1, 0.15, 0.2, 0.3, 0.5,
+          0.7, 1.0]
+    tk = lambda k: (1.0 + 0.02 * math.sin(1.5 * math.log(k))) * (1.0 + 20.0 * k) ** -0.8
+    return ks, tk
+
+
+def test_tabulated_transfer_knots_and_midpoints(tmp_path):
+    """The reference's own test (ic_zeldovich.rs tabulated_transfer_reconstructs_knots_and_midpoints):
+    knots to 1e-7 through the spectrum closure, midpoints to 0.1 %."""
+    ks, tk = _table_rows()
+    path = tmp_path / "tf.dat"
+    path.write_text("# k[h/Mpc] T(k)\n" + "".join(f"{k:.8e} {tk(k):.8e}\n" for k in ks))
+    tab = ic.TabulatedTransfer.from_file(str(path))
+    n, h = 32, 0.674
+    spec = ic.tabulated_sigma(n, 0.0, 1.0, tab, h, box_size_mpc_h=1.0)
+    inv = 1.0 / math.sqrt(n ** 3)
+    for k in ks:
+        est = spec(k / (2 * math.pi * h)) / inv
+        assert abs(est - tk(k)) / tk(k) < 1e-7
+    for a, b in zip(ks[:-1], ks[1:]):
+        km = math.sqrt(a * b)
+        assert abs(tab(km) - tk(km)) / tk(km) < 1e-3
+
+
+def test_tabulated_transfer_against_scipy_pchip():
+    """scipy's PchipInterpolator uses the same Fritsch-Carlson interior slopes
+    and shape-preserving end slopes: an independent implementation inside the
+    table; outside, the extrapolation is linear in log space."""
+    from scipy.interpolate import PchipInterpolator
+    g = np.random.default_rng(5)
+    k = np.sort(g.uniform(-3, 1, 25))
+    t = np.cumsum(g.normal(0, 0.3, 25))  # non-monotone ln T: exercises the zero-slope rule
+    tab = ic.TabulatedTransfer(zip(np.exp(k), np.exp(t)))
+    ref = PchipInterpolator(k, t)
+    for xq in np.linspace(k[0], k[-1], 400):
+        assert math.log(tab(math.exp(xq))) == pytest.approx(float(ref(xq)), abs=1e-9)
+    lo = k[0] - 0.5
+    assert math.log(tab(math.exp(lo))) == pytest.approx(t[0] + tab.m[0] * (lo - k[0]), abs=1e-12)
+    assert tab(0.0) == 0.0 and tab(-1.0) == 0.0
+    with pytest.raises(ValueError):
+        ic.TabulatedTransfer([(1.0, 1.0)])
