@@ -9,6 +9,8 @@ Mirrors crates/gadget-ng-core/src/ic_zeldovich.rs on top of libgng_pm.so:
   sigma_8 normalisation);
 * :func:`tabulated_sigma`, :class:`TabulatedTransfer` -- the ``Tabulated``
   branch (:450-460) and its PCHIP table (:118-260);
+* :func:`with_dark_matter_cutoff` -- (:468-489) with the warm and fuzzy
+  suppressions of crates/gadget-ng-core/src/dark_matter.rs;
 * :func:`delta_kspace`     -- ``generate_delta_kspace`` (:287-369), FP64 modes
   iz = 0 .. n/2;
 * :func:`zeldovich_ics`    -- ``zeldovich_ics`` (:621-766): Gaussian field,
@@ -252,6 +254,62 @@ def tabulated_sigma(n: int, spectral_index: float, amplitude: float, transfer: T
         bsm = 100.0 if box_size_mpc_h is None else box_size_mpc_h
         k = 2.0 * math.pi * h / bsm * n_abs
         return amplitude * math.pow(k, spectral_index / 2.0) * transfer(k) * inv_sqrt_n3
+    return f
+
+
+def wdm_transfer_suppression(k_hmpc: float, m_wdm_kev: float, omega_m: float, h: float) -> float:
+    """Warm dark matter: (1 + (alpha k)^(2 nu))^(-5/nu), nu = 1.12 (Viel et al.
+    2005 fit as in crates/gadget-ng-core/src/dark_matter.rs)."""
+    if k_hmpc <= 0.0 or m_wdm_kev <= 0.0:
+        return 1.0
+    nu = 1.12
+    alpha = (0.049 * math.pow(m_wdm_kev, -1.11) * math.pow(max(omega_m, 1e-30) / 0.25, 0.11)
+             * math.pow(max(h, 1e-30) / 0.7, 1.22))
+    return math.pow(1.0 + math.pow(alpha * k_hmpc, 2.0 * nu), -5.0 / nu)
+
+
+def wdm_half_mode_k(m_wdm_kev: float, omega_m: float, h: float) -> float:
+    """k where the WDM suppression of the power is 1/2."""
+    if m_wdm_kev <= 0.0:
+        return math.inf
+    nu = 1.12
+    alpha = (0.049 * math.pow(m_wdm_kev, -1.11) * math.pow(max(omega_m, 1e-30) / 0.25, 0.11)
+             * math.pow(max(h, 1e-30) / 0.7, 1.22))
+    return math.pow(math.pow(2.0, nu / 10.0) - 1.0, 1.0 / (2.0 * nu)) / alpha
+
+
+def fdm_half_mode_k(m_fdm_22: float, omega_m: float, h: float) -> float:
+    """Fuzzy dark matter half-mode wavenumber (dark_matter.rs)."""
+    if m_fdm_22 <= 0.0:
+        return math.inf
+    return 9.1 * math.pow(m_fdm_22, 4.0 / 9.0) * math.pow(max(omega_m, 1e-30) / 0.27, 0.11) * math.pow(h / 0.7, 1.22)
+
+
+def fdm_transfer_suppression(k_hmpc: float, m_fdm_22: float, omega_m: float, h: float) -> float:
+    """Fuzzy dark matter: (1 + (k / k_half)^8)^(-1/2)."""
+    if k_hmpc <= 0.0 or m_fdm_22 <= 0.0:
+        return 1.0
+    r = k_hmpc / fdm_half_mode_k(m_fdm_22, omega_m, h)
+    return math.pow(1.0 + r ** 8, -0.5)
+
+
+def with_dark_matter_cutoff(spectrum_fn: Callable[[float], float], model: str, mass: float, omega_m: float,
+                            h: float, box_size_mpc_h: Optional[float] = None) -> Callable[[float], float]:
+    """with_dark_matter_cutoff (ic_zeldovich.rs:468-489): the spectrum times
+    the suppression at k = 2 pi h |n| / L; model 'cold' (no change), 'warm'
+    (mass = m_wdm in keV) or 'fuzzy' (mass = m_fdm in 1e-22 eV)."""
+    if model == "cold":
+        return spectrum_fn
+    if model not in ("warm", "fuzzy"):
+        raise ValueError("model must be 'cold', 'warm' or 'fuzzy'")
+    bsm = 100.0 if box_size_mpc_h is None else box_size_mpc_h
+    sup = wdm_transfer_suppression if model == "warm" else fdm_transfer_suppression
+
+    def f(n_abs: float) -> float:
+        if n_abs <= 0.0:
+            return 0.0
+        k = 2.0 * math.pi * h / bsm * n_abs
+        return spectrum_fn(n_abs) * sup(k, mass, omega_m, h)
     return f
 
 

@@ -113,3 +113,30 @@ def test_tabulated_transfer_against_scipy_pchip():
     assert tab(0.0) == 0.0 and tab(-1.0) == 0.0
     with pytest.raises(ValueError):
         ic.TabulatedTransfer([(1.0, 1.0)])
+
+
+@pytest.mark.parametrize("m", [0.5, 2.0, 7.0])
+def test_dark_matter_half_modes(m):
+    """At its half-mode wavenumber each suppression halves the power (T^2 = 1/2),
+    it is 1 at k -> 0 and falls monotonically."""
+    om, h = 0.3, 0.7
+    kw = ic.wdm_half_mode_k(m, om, h)
+    assert ic.wdm_transfer_suppression(kw, m, om, h) ** 2 == pytest.approx(0.5, rel=1e-12)
+    kf = ic.fdm_half_mode_k(m, om, h)
+    assert ic.fdm_transfer_suppression(kf, m, om, h) ** 2 == pytest.approx(0.5, rel=1e-12)
+    for fn, k0 in ((ic.wdm_transfer_suppression, kw), (ic.fdm_transfer_suppression, kf)):
+        assert fn(1e-6 * k0, m, om, h) == pytest.approx(1.0, abs=1e-6)
+        vals = [fn(k0 * f, m, om, h) for f in (0.1, 0.5, 1.0, 2.0, 10.0)]
+        assert all(a > b for a, b in zip(vals, vals[1:]))
+    assert ic.wdm_transfer_suppression(1.0, 0.0, om, h) == 1.0 and ic.fdm_half_mode_k(0.0, om, h) == math.inf
+
+
+def test_dark_matter_cutoff_wraps_the_spectrum():
+    n, box = 64, 50.0
+    base = ic.power_law_sigma(n, 1.0, -2.0)
+    f = ic.with_dark_matter_cutoff(base, "warm", 1.0, 0.3, 0.7, box_size_mpc_h=box)
+    k = 2 * math.pi * 0.7 / box * 7.0
+    assert f(7.0) == pytest.approx(base(7.0) * ic.wdm_transfer_suppression(k, 1.0, 0.3, 0.7), rel=1e-15)
+    assert ic.with_dark_matter_cutoff(base, "cold", 1.0, 0.3, 0.7) is base
+    with pytest.raises(ValueError):
+        ic.with_dark_matter_cutoff(base, "hot", 1.0, 0.3, 0.7)
