@@ -123,7 +123,7 @@ def main() -> int:
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this PM solver has no multi-GPU path "
-              "(every GPU box here has one GPU); run with --gpus 1", file=sys.stderr)
+              "(the slab decomposition is a design, not built); run with --gpus 1", file=sys.stderr)
         return 2
     if not 1 <= args.steps <= 64:
         print("--steps must be in [1, 64] (the stage-timing ring holds 64 solves)", file=sys.stderr)
