@@ -90,6 +90,25 @@ __global__ __launch_bounds__(256) void k_kick_wrap(double* __restrict__ x, doubl
   }
 }
 
+// Device-side state load: n x 3 interleaved f64 (Vec3) positions and
+// velocities into the SoA state, the f32 copies the force reads, masses
+// (per particle, or one value when m is null) and zeroed accelerations.
+__global__ __launch_bounds__(256) void k_load_state(const double* __restrict__ pos, const double* __restrict__ vel,
+                                                    const double* __restrict__ m, double m_all, long long n,
+                                                    double* __restrict__ soa, float* __restrict__ f) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double p = pos[3 * i + c];
+    soa[c * n + i] = p;
+    soa[(3 + c) * n + i] = vel[3 * i + c];
+    f[(1 + c) * n + i] = (float)p;
+    f[(4 + c) * n + i] = 0.0f;
+  }
+  f[i] = (float)(m ? m[i] : m_all);
+}
+
 __global__ __launch_bounds__(256) void k_add3(float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az,
                                               const float* __restrict__ bx, const float* __restrict__ by,
                                               const float* __restrict__ bz, long long n) {
@@ -432,6 +451,22 @@ int gng_kdk_set_state(gng_kdk* s, const double* pos, const double* vel, const do
   GNG_HIP(hipMemcpy(s->d_pos, soa.data(), sizeof(double) * 6 * n, hipMemcpyHostToDevice));
   GNG_HIP(hipMemcpy(s->d_f, f.data(), sizeof(float) * 4 * n, hipMemcpyHostToDevice));
   GNG_HIP(hipMemset(s->ax(), 0, sizeof(float) * 3 * n));
+  s->have_state = true;
+  return GNG_OK;
+}
+
+int gng_kdk_set_state_device(gng_kdk* s, const double* d_pos, const double* d_vel, const double* d_mass,
+                             double uniform_mass, void* stream) {
+  if (!s) return fail(GNG_ERR_INVALID, "null integrator");
+  if (s->n > 0 && (!d_pos || !d_vel)) return fail(GNG_ERR_INVALID, "null state array");
+  std::lock_guard<std::mutex> lk(s->mu);
+  GNG_HIP(hipSetDevice(s->device));
+  const long long n = s->n;
+  if (n > 0)
+    hipLaunchKernelGGL(k_load_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), d_pos, d_vel, d_mass, uniform_mass, n, s->d_pos,
+                       s->d_f);
+  GNG_HIP(hipGetLastError());
   s->have_state = true;
   return GNG_OK;
 }

@@ -227,6 +227,8 @@ def lib():
         L.gng_kdk_destroy.restype = None
         L.gng_kdk_set_short_range.argtypes = [P, P, d, d, d, i32]
         L.gng_kdk_set_state.argtypes = [P, P, P, P]
+        L.gng_kdk_set_state_device.argtypes = [P, P, P, P, d, P]
+        L.gng_kdk_set_state_device.restype = i32
         L.gng_kdk_get_state.argtypes = [P, P, P, P]
         L.gng_kdk_step.argtypes = [P, d, d, d, d, i32, P]
         L.gng_kdk_run.argtypes = [P, P, P, d, d, i32, i32, P]

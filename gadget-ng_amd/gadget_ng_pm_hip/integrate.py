@@ -127,6 +127,15 @@ class Leapfrog:
         mass = np.ascontiguousarray(mass, dtype=np.float64).reshape(self.n)
         _check(lib().gng_kdk_set_state(self._h, _ptr(pos), _ptr(vel), _ptr(mass)), "gng_kdk_set_state")
 
+    def set_state_device(self, d_pos: int, d_vel: int, d_mass: Optional[int] = None, uniform_mass: float = 0.0,
+                         stream: int = 0) -> None:
+        """The state from device buffers (n x 3 f64 positions and velocities;
+        masses from d_mass, or uniform_mass for all), without a host copy."""
+        _check(lib().gng_kdk_set_state_device(self._h, ctypes.c_void_p(d_pos), ctypes.c_void_p(d_vel),
+                                              ctypes.c_void_p(d_mass) if d_mass else None, float(uniform_mass),
+                                              ctypes.c_void_p(stream) if stream else None),
+               "gng_kdk_set_state_device")
+
     def state(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
         """(positions, velocities, last accelerations), each (n, 3) float64."""
         pos = np.zeros((self.n, 3), dtype=np.float64)

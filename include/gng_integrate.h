@@ -89,6 +89,15 @@ int gng_kdk_set_short_range(gng_kdk *s, gng_sr *sr, double r_split, double r_cut
  * Synchronous.  The masses enter the force as f32. */
 int gng_kdk_set_state(gng_kdk *s, const double *pos, const double *vel, const double *mass);
 
+/* Same from device memory (d_pos, d_vel: n x 3 interleaved f64; d_mass: n
+ * f64, or NULL for uniform_mass on every particle), enqueued on `stream`
+ * (a hipStream_t; NULL = default).  No host copy: initial conditions made
+ * with gng_ic_zeldovich_device / gng_ic_2lpt_device go straight into the
+ * state.  No reference counterpart (the reference's state lives on the
+ * host). */
+int gng_kdk_set_state_device(gng_kdk *s, const double *d_pos, const double *d_vel, const double *d_mass,
+                             double uniform_mass, void *stream);
+
 /* Any output may be NULL.  acc is the last force evaluated (the
  * Particle::acceleration the reference stores after a step); it is zero
  * before the first step.  Synchronises with the state's work. */

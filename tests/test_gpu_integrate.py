@@ -166,3 +166,39 @@ def test_empty_state_and_bad_arguments(pm):
     other.set_state(np.zeros((8, 3)), np.zeros((8, 3)), np.ones(8))
     with pytest.raises(Exception):
         other.step(gi.CosmoFactors(float("nan"), 0.1, 0.1), 1.0)
+
+
+def test_device_state_load_from_gpu_ics_equals_the_host_load(pm):
+    """GPU Zel'dovich particles handed to the integrator in device memory
+    give the state, and the step after it, of the host load, bit for bit."""
+    from gadget_ng_pm_hip import integrate as gi
+    from gadget_ng_pm_hip import ic
+    n, box = 16, 32.0
+    plan = pm.Plan(n, box)
+    N = n ** 3
+    sig = ic.sigma_table(n, lambda k: 1e-3 * k ** -1.5 if k > 0 else 0.0)
+    print(f"case kdk device load from ICs N={N} nm={n}", flush=True)
+    d_pos = pm.DeviceArray((N, 3), np.float64)
+    d_vel = pm.DeviceArray((N, 3), np.float64)
+    ic.zeldovich_ics_device(plan, 7, sig, 0.05, 1.0, d_pos.ptr, d_vel.ptr)
+    pm.device_synchronize()
+    pos, vel = d_pos.numpy(), d_vel.numpy()
+    mass = np.full(N, 1.0 / N)
+    a, b, c = gi.Leapfrog(plan, N), gi.Leapfrog(plan, N), gi.Leapfrog(plan, N)
+    a.set_state(pos, vel, mass)
+    b.set_state_device(d_pos.ptr, d_vel.ptr, uniform_mass=1.0 / N)
+    d_mass = pm.DeviceArray.from_host(mass)
+    c.set_state_device(d_pos.ptr, d_vel.ptr, d_mass.ptr)
+    pm.device_synchronize()
+    sa, sb, sc = a.state(), b.state(), c.state()
+    for x, y, z in zip(sa, sb, sc):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
+    assert np.array_equal(sa[0], pos) and not sa[2].any()
+    for lf in (a, b, c):
+        lf.step(gi.CosmoFactors.flat(0.2), 1.5, wrap=True)
+    sa, sb, sc = a.state(), b.state(), c.state()
+    for x, y, z in zip(sa, sb, sc):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
+    assert np.abs(sa[2]).max() > 0.0
+    with pytest.raises(Exception):
+        b.set_state_device(0, d_vel.ptr)
