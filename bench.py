@@ -3,17 +3,22 @@
 
 Workload: 256^3 particles from a Zel'dovich displacement (P(k) ~ k^-2 up to
 the mesh Nyquist, 3-D rms displacement 0.5 cell) of a node-aligned lattice,
-box L = 256 Mpc/h, 512^3 mesh, equal f32 masses, G = 1, no filter.  One step
-is one full force evaluation: bucket sort, fixed-point CIC deposit, R2C FFT,
+box L = 256 Mpc/h, 512^3 mesh, equal masses, G = 1, no filter.  One step is
+one full force evaluation: bucket sort, fixed-point CIC deposit, R2C FFT,
 fused Green's function x gradient, C2R FFTs, CIC interpolation, with the
-particle arrays already resident in HBM.
+particle arrays already resident in HBM.  The timed path takes the
+positions as the reference's Vec3 f64 array (unrounded; cells formed in FP64
+on the device), f64 masses, and writes f64 Vec3 accelerations; the f32 SoA
+entry point is timed beside it.
 
 Printed JSON line (stdout, one line):
   value         whole-job M particle-forces/s (N * steps / timed wall time)
-  roofline      dominant kernel's algorithmic bytes / its mean event-timed
-                duration inside the timed region, against 8 TB/s
+  roofline      the slowest stage group's algorithmic bytes (the compulsory
+                bytes of that part of the solve, as the survey's measurement
+                section defines them) / its mean event-timed duration inside
+                the timed region, against 8 TB/s
   cpu_baseline  the FP64 CPU oracle (lean R2C mode, all host cores) on the
-                same f32 inputs
+                same unrounded f64 inputs
   rms_force_err / max_force_err  vs that FP64 result
 """
 from __future__ import annotations
@@ -35,36 +40,55 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stage_bytes(n: int, nm: int) -> dict:
-    """Algorithmic bytes of each stage of one evaluation (packed half spectrum:
-    one complex field = nm^3/2 complex = 4 nm^3 bytes)."""
+def stage_bytes(n: int, nm: int, f64: bool = False) -> dict:
+    """Bytes each pass of this implementation must move (packed half
+    spectrum: one complex field = nm^3/2 complex = 4 nm^3 bytes); with f64
+    the particles are Vec3 f64 positions, f64 masses and f64 accelerations."""
     m = nm ** 3
+    pin, pos, acc, rec = (32, 24, 24, 24) if f64 else (16, 12, 12, 24)
     return {
-        "bucket": 16 * n + 8 * n,           # x, y, z, m in; bucket, rank out
-        "sort": 24 * n + 16 * n,            # x, y, z, m, bucket, rank in; float4 out
-        "deposit": 16 * n + 4 * m,          # sorted particles in; f32 density out
-        "fft_x_fwd": 4 * m + 4 * m,         # density in, spectrum out
-        "fft_y_fwd": 8 * m,                 # spectrum in and out
-        "fft_z_green": 4 * m + 8 * m,       # spectrum in, A and B out
-        "fft_y_inv": 8 * m + 12 * m,        # A, B in; three fields out
-        "fft_x_inv": 12 * m + 12 * m,       # three fields in, three force meshes out
-        "interp": 12 * n + 12 * m + 12 * n,  # positions, force meshes in; accelerations out
+        "bucket": pin * n + 8 * n,           # positions, masses in; bucket, rank out
+        "sort": pin * n + 8 * n + rec * n,   # positions, masses, bucket, rank in; cell records out
+        "deposit": rec * n + 4 * m,          # cell records in; f32 mass mesh out
+        "fft_x_fwd": 4 * m + 4 * m,          # mass mesh in, spectrum out
+        "fft_y_fwd": 8 * m,                  # spectrum in and out
+        "fft_z_green": 4 * m + 8 * m,        # spectrum in, A and B out
+        "fft_y_inv": 8 * m + 12 * m,         # A, B in; three fields out
+        "fft_x_inv": 12 * m + 12 * m,        # three fields in, three force meshes out
+        "interp": pos * n + 12 * m + acc * n,  # positions, force meshes in; accelerations out
     }
 
 
-# kernel-name prefix of each stage's launch (the z stage also launches the
-# small k_x = 0 column kernel, counted with it)
-STAGE_KERNELS = {"bucket": ("k_bucket_count",), "sort": ("k_scan_local", "k_scan_partials", "k_scan_add", "k_scatter"),
-                 "deposit": ("k_deposit_tiles",), "fft_x_fwd": ("k_fft_x_fwd",), "fft_y_fwd": ("k_fft_y_fwd",),
-                 "fft_z_green": ("k_fft_z_green",), "fft_y_inv": ("k_fft_y_inv",), "fft_x_inv": ("k_fft_x_inv",),
-                 "interp": ("k_interp",),
-                 "fft_xy_fwd": ("k_fft_x_fwd", "k_fft_y_fwd"), "fft_yx_inv": ("k_fft_y_inv", "k_fft_x_inv")}
+# Stage groups of the survey's algorithmic-byte definition and the pipeline
+# stages that make up each: deposit chain (bucket count, sort, deposit), R2C
+# (x and y forward), k-space (z pass with the Green's function), C2R (y and x
+# inverse) and interpolation.
+GROUPS = {"deposit_chain": ("bucket", "sort", "deposit"), "r2c": ("fft_x_fwd", "fft_y_fwd"),
+          "kspace": ("fft_z_green",), "c2r": ("fft_y_inv", "fft_x_inv"), "interp": ("interp",)}
+
+
+F32_PARTICLES = (16, 12, 12)   # f32 x, y, z, m in; x, y, z re-read; f32 accelerations out
+F64_PARTICLES = (32, 24, 24)   # f64 Vec3 positions and f64 masses; f64 Vec3 accelerations
+SOA64_PARTICLES = (28, 24, 12)  # f64 SoA positions and f32 masses; f32 accelerations (the integrator)
+
+
+def group_bytes(n: int, nm: int, particles=F32_PARTICLES) -> dict:
+    """Algorithmic bytes per stage group (M = nm^3, M_h = nm^2 (nm/2 + 1)):
+    deposit write 4M, R2C 4M + 8M_h, k-space 8M_h + 24M_h, C2R 24M_h + 12M,
+    interpolation read 12M; particles (in, re-read, out) bytes each:
+    16, 12, 12 for f32 SoA, so the groups sum to B = 40N + 32M + 64M_h; 32,
+    24, 24 for f64 Vec3 positions, f64 masses and f64 accelerations
+    (80N + 32M + 64M_h)."""
+    m, mh = nm ** 3, nm * nm * (nm // 2 + 1)
+    pin, pre, pout = particles
+    return {"deposit_chain": pin * n + 4 * m, "r2c": 4 * m + 8 * mh, "kspace": 8 * mh + 24 * mh,
+            "c2r": 24 * mh + 12 * m, "interp": 12 * m + pre * n + pout * n}
 
 
 def merge_chunked(stages: dict, sb: dict, tuning: dict) -> None:
     """With plane chunks a pass pair runs interleaved chunk by chunk and its
     two stages share one timing slot: report the pair as one stage, with the
-    sum of both passes' algorithmic bytes."""
+    sum of both passes' bytes."""
     for on, a, b, name in ((tuning.get("chunk_fwd", 0) and not tuning.get("y_fwd_persistent"), "fft_x_fwd",
                             "fft_y_fwd", "fft_xy_fwd"),
                            (tuning.get("chunk_inv", 0) and not tuning.get("y_inv_persistent"), "fft_y_inv",
@@ -74,27 +98,59 @@ def merge_chunked(stages: dict, sb: dict, tuning: dict) -> None:
             sb[name] = sb.pop(a) + sb.pop(b)
 
 
-def measured_traffic(stage: str, nm: int, workload: str):
-    """HBM bytes per launch of a stage's kernels (FETCH_SIZE + WRITE_SIZE,
-    each scaled by what the counter reads for a 1 GiB copy) from the
-    committed rocprofv3 summary of the same workload, or None."""
+def roofline_of(stages: dict, n: int, nm: int, workload: str, particles=F32_PARTICLES) -> tuple:
+    """(roofline object, per-group table) of one solve's stage times."""
+    gb = group_bytes(n, nm, particles)
+    gt = group_times(stages)
+    dom = max(gt, key=gt.get)
+    achieved = gb[dom] / (gt[dom] * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(dom, nm, workload)
+    roof = {"bound": "hbm", "kernel": dom, "kernels": list(GROUP_KERNELS[dom]), "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": traffic, "traffic_source": traffic_src, "algorithmic_bytes": gb[dom],
+            "mean_ms": round(gt[dom], 5)}
+    groups = {g: {"ms": round(gt[g], 5), "algorithmic_bytes": gb[g],
+                  "frac": round(gb[g] / (gt[g] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4) if gt[g] > 0 else None}
+              for g in gb}
+    return roof, groups
+
+
+def group_times(stages: dict) -> dict:
+    return {g: sum(stages.get(s, 0.0) for s in members) for g, members in GROUPS.items()}
+
+
+# kernel-name prefixes of each stage group's launches (the k-space stage also
+# launches the small k_x = 0 column kernel, counted with it)
+GROUP_KERNELS = {"deposit_chain": ("k_bucket_count", "k_scan_local", "k_scan_partials", "k_scan_add", "k_scatter",
+                                   "k_deposit_tiles"),
+                 "r2c": ("k_fft_x_fwd", "k_fft_y_fwd"), "kspace": ("k_fft_z_green",),
+                 "c2r": ("k_fft_y_inv", "k_fft_x_inv"), "interp": ("k_interp",)}
+
+
+def measured_traffic(group: str, nm: int, workload: str):
+    """HBM bytes per solve of a stage group's kernels (FETCH_SIZE +
+    WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy) from
+    the committed rocprofv3 summary of the same workload, with the file and
+    the commit it was measured at; (None, None) when there is none."""
     name = {"zeldovich": "pm_headline_256cubed_512mesh_traffic.json",
             "clustered": "pm_clustered_512cubed_1024mesh_traffic.json"}.get(workload)
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
-        return None
+        return None, None
     try:
-        kern = json.load(open(path))["kernels"]
+        doc = json.load(open(path))
+        kern = doc["kernels"]
     except (OSError, ValueError, KeyError):
-        return None
+        return None, None
     total, found = 0.0, False
     for k, v in kern.items():
         short = k.split("::")[-1]
-        if short.startswith(STAGE_KERNELS[stage]) and (f"<{nm}" in short or "<" not in short):
+        if short.startswith(GROUP_KERNELS[group]) and (f"<{nm}" in short or "<" not in short):
             # bytes per solve: a chunked pass runs several launches per solve
             total += v.get("per_solve_bytes", v["read_bytes"] + v["write_bytes"])
             found = True
-    return int(total) if found else None
+    src = f"profiles/{name}" + (f" (measured at commit {doc['commit']})" if doc.get("commit") else "")
+    return (int(total) if found else None), src
 
 
 def main() -> int:
@@ -109,7 +165,10 @@ def main() -> int:
     ap.add_argument("--n-side", type=int, default=256)
     ap.add_argument("--mesh", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the FP64 CPU oracle (and the error check)")
-    ap.add_argument("--cpu-runs", type=int, default=2, help="timed CPU oracle runs after one warm-up")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU oracle runs after one warm-up")
+    ap.add_argument("--no-cpu-faithful", action="store_true",
+                    help="skip timing the reference-Rayon-faithful CPU mode (per-thread deposit grids, "
+                         "single-threaded Z2Z passes; about 10 s at 512^3, 18 GB of host memory)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch choices")
     ap.add_argument("--no-fp64-form", action="store_true",
                     help="skip timing the FP64 GPU form of the reference algorithm")
@@ -151,28 +210,29 @@ def main() -> int:
         nm, n_side = args.mesh, args.n_side
         box = float(n_side)  # 1 Mpc/h lattice spacing
         parts = workloads.zeldovich_lattice(n_side=n_side, nm=nm, box=box, spectral_index=-2.0, rms_disp_cells=0.5,
-                                            seed=workloads.SEED_ZELDOVICH, workers=-1)
+                                            seed=workloads.SEED_ZELDOVICH, workers=-1, keep_f64=True)
         wl_desc = (f"Zel'dovich lattice {n_side}^3 particles on {nm}^3 mesh, L={box:g} Mpc/h, P(k)~k^-2, "
-                   f"rms displacement 0.5 cell, seed 0x6E670002, G=1, no filter")
+                   f"rms displacement 0.5 cell, seed 0x6E670002, G=1, no filter; f64 Vec3 positions")
     else:
         nm, box = 1024, 512.0
-        parts = workloads.clustered_box(workers=-1)
+        parts = workloads.clustered_box(workers=-1, keep_f64=True)
         r_split = 1.25 * box / nm  # reference convention exp(-k^2 r_s^2 / 2)
         wl_desc = ("512^3 particles: 454^3 Zel'dovich lattice + 40641064 in 2000 Plummer spheres (a=0.2 Mpc/h), "
-                   f"L=512 Mpc/h, 1024^3 mesh, TreePM long range r_s=1.25 cells, seed 0x6E670003, G=1")
+                   f"L=512 Mpc/h, 1024^3 mesh, TreePM long range r_s=1.25 cells, seed 0x6E670003, G=1; "
+                   f"f64 Vec3 positions")
     n = parts.n
+    pos64 = np.ascontiguousarray(parts.p64)
+    mass64 = parts.mass64()
     log(f"[bench] workload generated in {time.perf_counter() - t0:.1f}s: {parts.name}, N={n}")
 
     # inputs resident in HBM before the timed region (library-owned device buffers)
-    x, y, z, m = (pm.DeviceArray.from_host(a) for a in (parts.x, parts.y, parts.z, parts.m))
-    ax, ay, az = (pm.DeviceArray((n,), np.float32) for _ in range(3))
+    d_pos = pm.DeviceArray.from_host(pos64)
+    d_mass = pm.DeviceArray.from_host(mass64)
+    d_out = pm.DeviceArray((n, 3), np.float64)
     plan = pm.Plan(nm, box, r_split=r_split, device=0)
     t1 = time.perf_counter()
     tuning = plan.autotune(reps=3) if not args.no_autotune else plan.tuning()
     log(f"[bench] launch choices ({time.perf_counter() - t1:.2f} s): {tuning}")
-
-    def step():
-        plan.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, ax.ptr, ay.ptr, az.ptr, 1.0)
 
     if args.copy_calibration:
         src = pm.DeviceArray((1 << 28,), np.float32)
@@ -181,48 +241,67 @@ def main() -> int:
         pm.device_synchronize()
         del src, dst
 
-    for _ in range(args.warmup):
-        step()
-    pm.device_synchronize()
+    def timed(step, steps, warmup):
+        for _ in range(warmup):
+            step()
+        pm.device_synchronize()
+        plan.set_timing(True)
+        pm.device_synchronize()
+        t_start = time.perf_counter()
+        for _ in range(steps):
+            step()
+        pm.device_synchronize()
+        wall = time.perf_counter() - t_start
+        st = plan.stage_times()
+        plan.set_timing(False)
+        return wall, st
 
-    plan.set_timing(True)
-    pm.device_synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    pm.device_synchronize()
-    wall = time.perf_counter() - t_start
-    stages = plan.stage_times()
-    plan.set_timing(False)
+    # the headline: f64 Vec3 positions and masses in, f64 Vec3 accelerations out
+    wall, stages = timed(lambda: plan.solve_device_f64(d_pos.ptr, d_mass.ptr, n, d_out.ptr, 1.0), args.steps,
+                         args.warmup)
     gpu_ms = stages.pop("total")
     ms_per_step = wall * 1e3 / args.steps
     value = n * args.steps / wall / 1e6
-    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step wall, {gpu_ms:.3f} ms/step GPU events, "
-        f"{value:.1f} M particle-forces/s")
+    log(f"[bench] {args.steps} steps (f64 Vec3 in/out): {ms_per_step:.3f} ms/step wall, {gpu_ms:.3f} ms/step GPU "
+        f"events, {value:.1f} M particle-forces/s")
     log("[bench] stage ms: " + ", ".join(f"{k}={v:.4f}" for k, v in stages.items()))
+    acc = d_out.numpy()
 
-    sb = stage_bytes(n, nm)
+    # every stage group against its algorithmic bytes, and each implementation
+    # pass against the bytes that pass itself must move
+    roofline, groups = roofline_of(stages, n, nm, args.workload, F64_PARTICLES)
+    achieved = roofline["achieved"]
+    sb = stage_bytes(n, nm, f64=True)
     merge_chunked(stages, sb, tuning)
-    dom = max(stages, key=stages.get)
-    dom_ms = stages[dom]
-    achieved = sb[dom] / (dom_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                "traffic": measured_traffic(dom, nm, args.workload),
-                "algorithmic_bytes": sb[dom], "mean_ms": round(dom_ms, 5)}
-    b_alg = workloads.algorithmic_bytes(n, nm)
+    per_pass = {k: {"ms": round(v, 5), "bytes": sb[k], "gbps": round(sb[k] / (v * 1e-3) / 1e9, 1) if v > 0 else None}
+                for k, v in stages.items()}
+    b_alg = sum(g["algorithmic_bytes"] for g in groups.values())
     pipeline_gbps = b_alg / (gpu_ms * 1e-3) / 1e9
 
-    # drop-in rate through the host C ABI (H2D + D2H of f32 SoA included)
+    # the f32 SoA entry point (the hip_pm_solve data layout) on the rounded positions
+    x, y, z, m = (pm.DeviceArray.from_host(a) for a in (parts.x, parts.y, parts.z, parts.m))
+    ax, ay, az = (pm.DeviceArray((n,), np.float32) for _ in range(3))
+    wall32, stages32 = timed(lambda: plan.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, ax.ptr, ay.ptr, az.ptr, 1.0),
+                             args.steps, args.warmup)
+    ms32 = wall32 * 1e3 / args.steps
+    acc32 = np.stack([ax.numpy(), ay.numpy(), az.numpy()], axis=1).astype(np.float64)
+    for a in (x, y, z, m, ax, ay, az):
+        a.free()
+    log(f"[bench] f32 SoA entry point: {ms32:.3f} ms/step")
+
+    # drop-in rates through the host C ABI (H2D + D2H included)
+    t_e2e, t_e2e32 = [], []
     host_out = None
-    t_e2e = []
     for _ in range(3):
         t1 = time.perf_counter()
-        host_out = plan.accelerations_f32(parts.x, parts.y, parts.z, parts.m, 1.0)
+        host_out = plan.accelerations_f64(pos64, mass64, 1.0)
         t_e2e.append(time.perf_counter() - t1)
+        t1 = time.perf_counter()
+        plan.accelerations_f32(parts.x, parts.y, parts.z, parts.m, 1.0)
+        t_e2e32.append(time.perf_counter() - t1)
     e2e_ms = statistics.median(t_e2e) * 1e3
+    e2e32_ms = statistics.median(t_e2e32) * 1e3
 
-    acc = np.stack([ax.numpy(), ay.numpy(), az.numpy()], axis=1).astype(np.float64)
     result = {
         "metric": "M particle-forces/s + HBM GB/s at 256^3 part/512^3 mesh; rms force err vs FP64"
                   if args.workload == "zeldovich" else "M particle-forces/s, TreePM long range, 512^3 clustered "
@@ -239,14 +318,19 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl_desc, "particles": n, "mesh": nm, "box": box, "r_split": r_split,
-                   "precision": "FP32 mesh/FFT/interp, int64 deposit", "launch_choices": tuning},
+                   "precision": "f64 positions -> FP64 cell index and 43-bit fraction; FP64 weights, int64 "
+                                "deposit; FP32 mesh/FFT/interp", "launch_choices": tuning},
         "roofline": roofline,
+        "stage_groups": groups,
+        "per_pass": per_pass,
         "hbm_gbps": round(pipeline_gbps, 1),
         "hbm_frac": round(pipeline_gbps / HBM_PEAK_GBPS, 4),
         "algorithmic_bytes": b_alg,
         "gpu_ms_per_step": round(gpu_ms, 4),
-        "stage_ms": {k: round(v, 5) for k, v in stages.items()},
+        "f32_entry": {"ms_per_step": round(ms32, 4), "value": round(n / (ms32 * 1e-3) / 1e6, 2),
+                      "gpu_ms_per_step": round(stages32["total"], 4)},
         "pcie_inclusive_ms": round(e2e_ms, 3),
+        "pcie_inclusive_ms_f32_entry": round(e2e32_ms, 3),
         "cpu_baseline": None,
         "rms_force_err": None,
         "max_force_err": None,
@@ -275,26 +359,27 @@ def main() -> int:
         # (kernels only) and through its host entry point (f64 copies in and
         # out on every call, as the reference's GPU crates do)
         plan.close()
+        for a in (d_pos, d_mass, d_out):
+            a.free()
         p64 = pm.Plan64(nm, box, r_split=r_split, device=0)
-        pos64 = np.ascontiguousarray(parts.pos64())
-        d_pos = pm.DeviceArray.from_host(pos64)
-        d_mass = pm.DeviceArray.from_host(parts.mass64())
-        d_out = pm.DeviceArray((n, 3), np.float64)
-        p64.solve_device(d_pos.ptr, d_mass.ptr, n, d_out.ptr, 1.0)
+        e_pos = pm.DeviceArray.from_host(pos64)
+        e_mass = pm.DeviceArray.from_host(mass64)
+        e_out = pm.DeviceArray((n, 3), np.float64)
+        p64.solve_device(e_pos.ptr, e_mass.ptr, n, e_out.ptr, 1.0)
         pm.device_synchronize()
         t_dev = []
         for _ in range(3):
             t1 = time.perf_counter()
-            p64.solve_device(d_pos.ptr, d_mass.ptr, n, d_out.ptr, 1.0)
+            p64.solve_device(e_pos.ptr, e_mass.ptr, n, e_out.ptr, 1.0)
             pm.device_synchronize()
             t_dev.append(time.perf_counter() - t1)
-        a64 = d_out.numpy()
-        for a in (d_pos, d_mass, d_out):
+        a64 = e_out.numpy()
+        for a in (e_pos, e_mass, e_out):
             a.free()
         t_host = []
         for _ in range(2):
             t1 = time.perf_counter()
-            p64.accelerations(pos64, parts.mass64(), 1.0)
+            p64.accelerations(pos64, mass64, 1.0)
             t_host.append(time.perf_counter() - t1)
         p64.close()
         ms_dev = statistics.median(t_dev) * 1e3
@@ -302,6 +387,7 @@ def main() -> int:
         result["fp64_gpu_form"] = {
             "ms_device": round(ms_dev, 3), "ms_host_round_trip": round(ms_host, 3),
             "fp32_speedup_device": round(ms_dev / ms_per_step, 2),
+            "fp32_speedup_host_vs_host": round(ms_host / e2e_ms, 2),
             "fp32_speedup_vs_host_round_trip": round(ms_host / ms_per_step, 2),
             "fp32_rms_vs_fp64_form": float(f"{np.sqrt(((acc - a64) ** 2).sum() / (a64 ** 2).sum()):.4e}")}
         log(f"[bench] FP64 GPU form: {ms_dev:.2f} ms device-resident, {ms_host:.2f} ms with host copies")
@@ -311,33 +397,51 @@ def main() -> int:
         import oracle  # CPU baseline and checker only
 
         oracle.build()
-        pos = parts.pos64()
-        mass = parts.mass64()
         cores = oracle.get_threads()
         t1 = time.perf_counter()
-        ref = oracle.pm_accelerations(pos, mass, box, nm, 1.0, r_split=r_split, mode=oracle.MODE_LEAN)
+        ref = oracle.pm_accelerations(pos64, mass64, box, nm, 1.0, r_split=r_split, mode=oracle.MODE_LEAN)
         warm = time.perf_counter() - t1
         times = []
         for _ in range(args.cpu_runs):
             t1 = time.perf_counter()
-            oracle.pm_accelerations(pos, mass, box, nm, 1.0, r_split=r_split, mode=oracle.MODE_LEAN)
+            oracle.pm_accelerations(pos64, mass64, box, nm, 1.0, r_split=r_split, mode=oracle.MODE_LEAN)
             times.append(time.perf_counter() - t1)
         tc = statistics.median(times) if times else warm
         result["cpu_baseline"] = {
             "value": round(n / tc / 1e6, 3), "unit": "M particle-forces/s", "cores": cores, "kind": "port",
-            "sample": f"the full workload ({n} particles, {nm}^3 mesh), FP64 oracle lean R2C mode, "
-                      f"median of {len(times)} timed runs after 1 warm-up ({tc:.2f} s each)"}
-        d = acc - ref
-        nr = np.linalg.norm(ref, axis=1)
-        floor = 0.05 * nr.mean()
-        rel = np.linalg.norm(d, axis=1) / np.maximum(np.maximum(nr, floor), 1e-300)
-        result["rms_force_err"] = float(f"{np.sqrt((d ** 2).sum() / (ref ** 2).sum()):.4e}")
+            "sample": f"the full workload ({n} particles, {nm}^3 mesh, unrounded f64 positions), FP64 oracle "
+                      f"lean R2C mode, median of {len(times)} timed runs after 1 warm-up ({tc:.2f} s each)"}
+
+        def errs(a, r):
+            d = a - r
+            nr = np.linalg.norm(r, axis=1)
+            floor = 0.05 * nr.mean()
+            rel = np.linalg.norm(d, axis=1) / np.maximum(np.maximum(nr, floor), 1e-300)
+            return float(f"{np.sqrt((d ** 2).sum() / (r ** 2).sum()):.4e}"), rel
+        result["rms_force_err"], rel = errs(acc, ref)
         result["max_force_err"] = float(f"{rel.max():.4e}")
         result["p99_force_err"] = float(f"{np.percentile(rel, 99):.4e}")
-        dh = host_out.T.astype(np.float64) - ref
-        result["rms_force_err_host_path"] = float(f"{np.sqrt((dh ** 2).sum() / (ref ** 2).sum()):.4e}")
+        result["rms_force_err_host_path"] = errs(host_out, ref)[0]
+        # the f32 entry point: against the oracle on its own (rounded) inputs
+        # and against the unrounded f64 truth (the rounding's own share)
+        ref32 = oracle.pm_accelerations(parts.pos64(), mass64, box, nm, 1.0, r_split=r_split,
+                                        mode=oracle.MODE_LEAN)
+        result["f32_entry"]["rms_vs_rounded_inputs"] = errs(acc32, ref32)[0]
+        result["f32_entry"]["rms_vs_unrounded_inputs"] = errs(acc32, ref)[0]
+        result["f32_entry"]["rounding_alone_rms"] = errs(ref32, ref)[0]
         log(f"[bench] CPU oracle {tc:.2f}s on {cores} threads; rms err {result['rms_force_err']:.3e}, "
-            f"max {result['max_force_err']:.3e}")
+            f"max {result['max_force_err']:.3e}; f32 entry rms {result['f32_entry']['rms_vs_rounded_inputs']:.3e} "
+            f"vs rounded, {result['f32_entry']['rms_vs_unrounded_inputs']:.3e} vs unrounded inputs")
+        if not args.no_cpu_faithful and args.workload == "zeldovich":
+            # the reference-Rayon-faithful mode: per-thread deposit grids and
+            # single-threaded Z2Z passes (cic.rs:355-401, fft_poisson.rs:356-373)
+            t1 = time.perf_counter()
+            oracle.pm_accelerations(pos64, mass64, box, nm, 1.0, r_split=r_split, mode=oracle.MODE_FAITHFUL,
+                                    deposit_mode=oracle.DEPOSIT_PRIVATE_GRIDS)
+            tf = time.perf_counter() - t1
+            result["cpu_faithful"] = {"value": round(n / tf / 1e6, 4), "unit": "M particle-forces/s",
+                                      "cores": cores, "seconds": round(tf, 2), "runs": 1}
+            log(f"[bench] CPU oracle faithful mode: {tf:.1f} s")
         # calibration against the reference's only published PM timings (AP-08 input,
         # serial CPU PM: 1.83 ms at grid 32, 22.4 ms at grid 64), with this GPU beside it
         ap = {}
@@ -355,11 +459,11 @@ def main() -> int:
                                         deposit_mode=oracle.DEPOSIT_SERIAL)
                 tt.append(time.perf_counter() - t1)
             gp = pm.Plan(grid, q.box, device=0)
-            gp.accelerations_f32(q.x, q.y, q.z, q.m, 1.0)
+            gp.accelerations_f64(p64, m64, 1.0)
             tg = []
             for _ in range(20):
                 t1 = time.perf_counter()
-                gp.accelerations_f32(q.x, q.y, q.z, q.m, 1.0)
+                gp.accelerations_f64(p64, m64, 1.0)
                 tg.append(time.perf_counter() - t1)
             gp.close()
             ap[f"grid{grid}"] = {"n": q.n, "reference_published_cpu_ms": published,
@@ -487,9 +591,9 @@ def bench_kdk(args) -> int:
     hubble = h0 * (om / a0 ** 3 + ol) ** 0.5
     dt = 0.01 / hubble  # about 1 % growth of a per step
     parts = workloads.zeldovich_lattice(n_side=n_side, nm=nm, box=box, spectral_index=-2.0, rms_disp_cells=0.5,
-                                        seed=workloads.SEED_ZELDOVICH, workers=-1)
+                                        seed=workloads.SEED_ZELDOVICH, workers=-1, keep_f64=True)
     n = parts.n
-    pos = parts.pos64()
+    pos = parts.p64
     q = np.stack(np.meshgrid(*(np.arange(n_side) * (box / n_side),) * 3, indexing="ij"), -1)[..., ::-1]
     psi = (pos - q.reshape(-1, 3) + box / 2) % box - box / 2
     vel = a0 * a0 * hubble * psi  # growing mode, f = 1: p = a^2 H psi
@@ -511,14 +615,7 @@ def bench_kdk(args) -> int:
     pm_ms = stages.pop("total")
     ms_step = wall * 1e3 / k
     log(f"[bench] {k} KDK steps: {ms_step:.3f} ms/step, {2 * pm_ms:.3f} ms of it in the two PM solves")
-    sb = stage_bytes(n, nm)
-    merge_chunked(stages, sb, tuning)
-    dom = max(stages, key=stages.get)
-    achieved = sb[dom] / (stages[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                "traffic": measured_traffic(dom, nm, "zeldovich"), "algorithmic_bytes": sb[dom],
-                "mean_ms": round(stages[dom], 5)}
+    roofline, _ = roofline_of(stages, n, nm, "zeldovich", SOA64_PARTICLES)
 
     def host_step(p_, v_, a_, plan_, m32):
         """The reference's GPU path: f64 state on the host, f32 over PCIe per force call."""

@@ -3,6 +3,10 @@
 // Reference semantics: crates/gadget-ng-pm/src/cic.rs:37-93 (mass per cell,
 // node-centred trilinear weights, periodic wrap of all 8 indices).
 //
+// Positions enter as FP32 or FP64 (any stride); every kernel forms the cell
+// coordinate c = (x * nm / L) rem_euclid nm in FP64, as cic.rs:50-52 does,
+// so FP64 inputs keep their accuracy (cell_fix in pm_kernels.hpp).
+//
 // Method:
 //   1. k_bucket_count  each particle's base cell (i0,j0,k0) gives a tile of
 //      T^3 cells and a class: bit a is set when the base cell is on the tile's
@@ -13,16 +17,21 @@
 //      particle's rank inside its bucket.  The max |m| for the fixed-point
 //      scale is reduced in the same pass.
 //   2. k_scan_*        exclusive scan of the bucket counts.
-//   3. k_scatter       particles to bucket order as float4 (x, y, z, m).
+//   3. k_scatter       particles to bucket order as 24-byte cell records
+//      (PartRec: per axis the cell index and a 43-bit in-cell fraction, plus
+//      the FP32 mass), and the sum of the fixed-point masses.
 //   4. k_deposit_tiles one workgroup per tile owns its T^3 cells outright.  It
 //      reads its own particles and, from the 7 lower neighbour tiles, only the
 //      classes that reach into it, and adds each of the 8 CIC targets exactly
 //      once over all tiles (target offset a lands in tile t + (a & class)).
-//      Weights become 64-bit fixed point (w * m * 2^s, N max|m| 2^s < 2^62)
-//      and accumulate with LDS integer atomics, so the sum is exact and
-//      order-independent; the tile is then converted in FP64 and written as
-//      FP32 mass per cell with plain stores.  No global atomics touch the mesh
-//      and no mesh clearing pass is needed.
+//      Weights are formed in FP64 from the fractions and become 64-bit fixed
+//      point (w * m * 2^s, N max|m| 2^s < 2^62); they accumulate with LDS
+//      integer atomics, so the sum is exact and order-independent.  The tile
+//      is then written as FP32 mass per cell minus the mean mass per cell
+//      (an integer subtracted before the conversion, so a nearly uniform
+//      density keeps its contrast to FP32 precision; the constant only
+//      touches the k = 0 mode, which the Green's function drops).  No global
+//      atomics touch the mesh and no mesh clearing pass is needed.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -71,9 +80,9 @@ constexpr int kBucketPer = 4;
 constexpr int kBucketSlots = 2048;  // > 256 * kBucketPer keys: probing always ends
 constexpr unsigned kEmptyKey = 0xffffffffu;
 
-__global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ x, const float* __restrict__ y,
-                                                      const float* __restrict__ z, const float* __restrict__ m,
-                                                      long long n, const CellMap cm, const TileMap tm,
+template <typename T, typename M>
+__global__ __launch_bounds__(256) void k_bucket_count(const PosSrc<T> src, const M* __restrict__ m, long long n,
+                                                      const CellMap cm, const TileMap tm,
                                                       unsigned* __restrict__ count, unsigned* __restrict__ bucket,
                                                       unsigned* __restrict__ rank, SolveScalars* sc) {
   __shared__ unsigned hkey[kBucketSlots], hcnt[kBucketSlots];
@@ -94,12 +103,13 @@ __global__ __launch_bounds__(256) void k_bucket_count(const float* __restrict__ 
     b[k] = kEmptyKey;
     if (live) {
       int ix, iy, iz;
-      float d;
-      cell_coord(x[i], cm, ix, d);
-      cell_coord(y[i], cm, iy, d);
-      cell_coord(z[i], cm, iz, d);
+      unsigned long long f;
+      const long long o = i * src.stride;
+      cell_fix((double)src.x[o], cm, ix, f);
+      cell_fix((double)src.y[o], cm, iy, f);
+      cell_fix((double)src.z[o], cm, iz, f);
       b[k] = bucket_of_cell(ix, iy, iz, tm);
-      am = fmaxf(am, fabsf(m[i]));
+      am = fmaxf(am, fabsf((float)m[i]));
     }
     // runs of equal buckets: a lane heads a run when its bucket differs from the previous lane's
     const unsigned prev = __shfl_up(b[k], 1);
@@ -226,16 +236,39 @@ __global__ __launch_bounds__(256) void k_scan_add(unsigned* __restrict__ offset,
     if (base + k < nb) offset[base + k] += add;
 }
 
-__global__ __launch_bounds__(256) void k_scatter(const float* __restrict__ x, const float* __restrict__ y,
-                                                 const float* __restrict__ z, const float* __restrict__ m, long long n,
-                                                 const unsigned* __restrict__ bucket, const unsigned* __restrict__ rank,
-                                                 const unsigned* __restrict__ offset, float4* __restrict__ sorted,
-                                                 unsigned* __restrict__ perm) {
+// Particle i to its bucket slot as a cell record, and the sum of the
+// particles' fixed-point masses round(m 2^s) (one integer atomic per block,
+// so the total does not depend on the order): the deposit subtracts the mean
+// per cell before it converts to FP32.
+template <typename T, typename M>
+__global__ __launch_bounds__(256) void k_scatter(const PosSrc<T> src, const M* __restrict__ m, long long n,
+                                                 const CellMap cm, const unsigned* __restrict__ bucket,
+                                                 const unsigned* __restrict__ rank,
+                                                 const unsigned* __restrict__ offset, PartRec* __restrict__ sorted,
+                                                 unsigned* __restrict__ perm, SolveScalars* sc) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const unsigned slot = offset[bucket[i]] + rank[i];
-  sorted[slot] = make_float4(x[i], y[i], z[i], m[i]);
-  if (perm) perm[slot] = (unsigned)i;  // only for the sorted-order interpolation
+  long long q = 0;
+  if (i < n) {
+    int c[3];
+    unsigned long long f[3];
+    const long long o = i * src.stride;
+    cell_fix((double)src.x[o], cm, c[0], f[0]);
+    cell_fix((double)src.y[o], cm, c[1], f[1]);
+    cell_fix((double)src.z[o], cm, c[2], f[2]);
+    const float mf = (float)m[i];
+    const unsigned slot = offset[bucket[i]] + rank[i];
+    sorted[slot] = encode_rec(c, f, mf);
+    if (perm) perm[slot] = (unsigned)i;  // only for the sorted-order interpolation
+    q = __double2ll_rn(ldexp((double)mf, fixed_shift(sc, n)));
+  }
+  for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off);
+  __shared__ long long wsum[4];
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = q;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const long long t = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+    if (t) atomicAdd(&sc->qsum, (unsigned long long)t);
+  }
 }
 
 // LDS fixed-point add with wavefront aggregation: the lanes whose cell equals
@@ -276,7 +309,8 @@ __constant__ unsigned char kRangeLo[14] = {0, 1, 3, 5, 7, 2, 6, 3, 7, 4, 5, 7, 6
 __constant__ unsigned char kRangeHi[14] = {8, 2, 4, 6, 8, 4, 8, 4, 8, 8, 6, 8, 8, 8};
 
 // One workgroup per tile; T is the tile side (T^3 int64 accumulators in LDS).
-// Cell (x, y, z) of the mass mesh is rho[z*rho_plane + y*nm + x].
+// Cell (x, y, z) of the shifted mass mesh is rho[z*rho_plane + y*nm + x];
+// keep (when not null) receives the unshifted mass per cell, unpadded.
 // The 72 bucket offsets of the 8 source tiles are fetched in parallel, the 14
 // ranges are laid end to end, and all their particles are processed in one
 // flat loop, so the only serial latency is one offset load.  Eight wavefronts
@@ -285,10 +319,11 @@ __constant__ unsigned char kRangeHi[14] = {8, 2, 4, 6, 8, 4, 8, 4, 8, 8, 6, 8, 8
 constexpr int kDepositThreads = 512;
 
 template <int T, int NT = kDepositThreads>
-__global__ __launch_bounds__(NT) void k_deposit_tiles(const float4* __restrict__ sorted,
+__global__ __launch_bounds__(NT) void k_deposit_tiles(const PartRec* __restrict__ sorted,
                                                        const unsigned* __restrict__ offset, const CellMap cm,
                                                        const TileMap tm, const SolveScalars* sc, long long ntot,
-                                                       float* __restrict__ rho, long long rho_plane) {
+                                                       float* __restrict__ rho, long long rho_plane,
+                                                       float* __restrict__ keep) {
   constexpr int T3 = T * T * T;
   __shared__ unsigned long long acc[T3];
   __shared__ unsigned toff[8][9];
@@ -343,28 +378,36 @@ __global__ __launch_bounds__(NT) void k_deposit_tiles(const float4* __restrict__
     return r;
   };
   auto fetch = [&](unsigned q, int r) {
-    return q < total ? sorted[q + rdelta[r]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    PartRec v;
+    if (q < total) {
+      v = sorted[q + rdelta[r]];
+    } else {
+      v.a[0] = v.a[1] = v.a[2] = 0ull;
+    }
+    return v;
   };
+  // m 2^s of a record: the FP32 mass scaled exactly
+  const double scale = ldexp(1.0, s);
   unsigned q = (unsigned)tid;
   int r = range_of(q);
-  float4 pnext = fetch(q, r);
+  PartRec pnext = fetch(q, r);
   for (unsigned it = 0; it < rounds; ++it) {
     const bool live = q < total;
     const int d = (int)((kRangeD >> (3 * r)) & 7ull);
     const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-    const float4 p = pnext;
+    const PartRec p = pnext;
     // the next round's particle is loaded while this one is deposited
     q += NT;
     r = range_of(q);
     if (it + 1 < rounds) pnext = fetch(q, r);
-    int ix, iy, iz;
-    float fx, fy, fz;
-    cell_coord(p.x, cm, ix, fx);
-    cell_coord(p.y, cm, iy, fy);
-    cell_coord(p.z, cm, iz, fz);
-    const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+    // FP64 weights: the 43-bit fractions convert exactly, 1 - f is exact, and
+    // the products round far below the fixed-point unit
+    const double fx = rec_frac(p, 0), fy = rec_frac(p, 1), fz = rec_frac(p, 2);
+    const double ms = (double)rec_mass(p) * scale;
+    const double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy};
+    const double wz[2] = {(1.0 - fz) * ms, fz * ms};
     // local base index in the particle's own tile; its class follows from it
-    const int lx0 = ix % T, ly0 = iy % T, lz0 = iz % T;
+    const int lx0 = rec_cell(p, 0) % T, ly0 = rec_cell(p, 1) % T, lz0 = rec_cell(p, 2) % T;
     const int cx = lx0 == T - 1, cy = ly0 == T - 1, cz = lz0 == T - 1;
     // Crowded wavefront: some lane shares its base cell (and source range)
     // with the lane before it, as in dense clumps, where the sorted order puts
@@ -384,8 +427,7 @@ __global__ __launch_bounds__(NT) void k_deposit_tiles(const float4* __restrict__
         for (int c = 0; c < 2; ++c) {
           const int lx = dx ? 0 : lx0 + c;
           const bool valid = live && (a & cz) == dz && (b & cy) == dy && (c & cx) == dx;
-          const float w = p.w * wx[c] * wy[b] * wz[a];
-          const long long qv = valid ? __float2ll_rn(ldexpf(w, s)) : 0ll;
+          const long long qv = valid ? __double2ll_rn(wx[c] * (wy[b] * wz[a])) : 0ll;
           if (crowded)
             lds_add_aggregated(acc, valid ? (lz * T + ly) * T + lx : 0, qv, valid);
           else if (valid)
@@ -396,10 +438,14 @@ __global__ __launch_bounds__(NT) void k_deposit_tiles(const float4* __restrict__
   }
   __syncthreads();
   const double unit = ldexp(1.0, -s);
+  // the integer part of the mean fixed-point mass per cell
+  const long long qmean = (long long)(sc->qsum / (unsigned long long)((long long)cm.nm * cm.nm * cm.nm));
   for (int c = tid; c < T3; c += NT) {
     const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
-    const size_t g = (size_t)(uz * T + lz) * rho_plane + (size_t)(uy * T + ly) * cm.nm + (ux * T + lx);
-    rho[g] = (float)((double)(long long)acc[c] * unit);
+    const size_t row = (size_t)(uy * T + ly) * cm.nm + (ux * T + lx);
+    const long long v = (long long)acc[c];
+    rho[(size_t)(uz * T + lz) * rho_plane + row] = (float)((double)(v - qmean) * unit);
+    if (keep) keep[(size_t)(uz * T + lz) * cm.nm * cm.nm + row] = (float)((double)v * unit);
   }
 }
 

@@ -217,8 +217,8 @@ struct MeshOps {
   int z_threads;
   int tile;
   // deposit[v]: workgroups of 256 (v = 0) or 512 (v = 1) threads
-  void (*deposit[2])(dim3, dim3, hipStream_t, const float4*, const unsigned*, CellMap, TileMap, const SolveScalars*,
-                     long long, float*, long long);
+  void (*deposit[2])(dim3, dim3, hipStream_t, const PartRec*, const unsigned*, CellMap, TileMap, const SolveScalars*,
+                     long long, float*, long long, float*);
   void (*x_fwd)(dim3, dim3, hipStream_t, const float*, float2*, const float2*, long long);
   void (*y_fwd)(dim3, dim3, hipStream_t, float2*, const float2*);
   void (*z_green)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
@@ -255,13 +255,13 @@ MeshOps make_ops() {
   o.z_cols = YZCfg<N>::CZ;
   o.z_threads = YZCfg<N>::TZ;
   o.tile = TD;
-  o.deposit[0] = [](dim3 g, dim3 b, hipStream_t s, const float4* srt, const unsigned* off, CellMap cm, TileMap tm,
-                    const SolveScalars* sc, long long n, float* rho, long long rho_plane) {
-    hipLaunchKernelGGL((k_deposit_tiles<TD, 256>), g, b, 0, s, srt, off, cm, tm, sc, n, rho, rho_plane);
+  o.deposit[0] = [](dim3 g, dim3 b, hipStream_t s, const PartRec* srt, const unsigned* off, CellMap cm, TileMap tm,
+                    const SolveScalars* sc, long long n, float* rho, long long rho_plane, float* keep) {
+    hipLaunchKernelGGL((k_deposit_tiles<TD, 256>), g, b, 0, s, srt, off, cm, tm, sc, n, rho, rho_plane, keep);
   };
-  o.deposit[1] = [](dim3 g, dim3 b, hipStream_t s, const float4* srt, const unsigned* off, CellMap cm, TileMap tm,
-                    const SolveScalars* sc, long long n, float* rho, long long rho_plane) {
-    hipLaunchKernelGGL((k_deposit_tiles<TD, 512>), g, b, 0, s, srt, off, cm, tm, sc, n, rho, rho_plane);
+  o.deposit[1] = [](dim3 g, dim3 b, hipStream_t s, const PartRec* srt, const unsigned* off, CellMap cm, TileMap tm,
+                    const SolveScalars* sc, long long n, float* rho, long long rho_plane, float* keep) {
+    hipLaunchKernelGGL((k_deposit_tiles<TD, 512>), g, b, 0, s, srt, off, cm, tm, sc, n, rho, rho_plane, keep);
   };
 
   o.x_fwd = [](dim3 g, dim3 b, hipStream_t s, const float* rho, float2* P, const float2* tw, long long rho_plane) {
@@ -371,7 +371,7 @@ struct gng_pm_plan {
 
   // per-particle work buffers (grown on demand)
   long long work_cap = 0;
-  float4* d_sorted = nullptr;
+  PartRec* d_sorted = nullptr;
   unsigned* d_perm = nullptr;  // original index of each sorted slot
   unsigned* d_bucket = nullptr;
   unsigned* d_rank = nullptr;
@@ -379,6 +379,13 @@ struct gng_pm_plan {
   // particle staging for the host entry points
   long long cap = 0;
   float* d_part = nullptr;  // x, y, z, m, ax, ay, az (7 * cap floats)
+  long long cap64 = 0;
+  double* d_part64 = nullptr;  // pos (3 n), mass (n), out (3 k) of the f64 entry points: 7 * cap64 doubles
+  // Ordering across streams: every entry point records done_ev on the stream
+  // it used; the next entry point on another stream waits for it first.
+  hipEvent_t done_ev = nullptr;
+  hipStream_t done_stream = nullptr;
+  bool done_valid = false;
   long long idx_cap = 0;
   long long* d_idx = nullptr;
   hipStream_t stream = nullptr;
@@ -411,6 +418,24 @@ int set_device(gng_pm_plan* p) {
   return GNG_OK;
 }
 
+// Stream order for one entry point's work on a plan: construction makes s
+// wait for the previous entry point's work when that ran on another stream;
+// destruction records the point the next entry point waits for.  Construct it
+// after set_device, under the plan's lock.
+struct StreamOrder {
+  gng_pm_plan* p;
+  hipStream_t s;
+  StreamOrder(gng_pm_plan* p_, hipStream_t s_) : p(p_), s(s_) {
+    if (p->done_valid && p->done_stream != s) (void)hipStreamWaitEvent(s, p->done_ev, 0);
+  }
+  ~StreamOrder() {
+    if (hipEventRecord(p->done_ev, s) == hipSuccess) {
+      p->done_valid = true;
+      p->done_stream = s;
+    }
+  }
+};
+
 void free_work(gng_pm_plan* p) {
   (void)hipFree(p->d_sorted);
   (void)hipFree(p->d_perm);
@@ -429,8 +454,9 @@ void free_plan(gng_pm_plan* p) {
                   (void*)p->d_sc, (void*)p->d_count, (void*)p->d_offset, (void*)p->d_partial, (void*)p->d_part,
                   (void*)p->d_idx, (void*)p->d_tile_rank, (void*)p->d_tile_order, (void*)p->d_pk_w,
                   (void*)p->d_pk_part, (void*)p->d_pk_modes, (void*)p->d_pk_msum, (void*)p->d_pk_out,
-                  (void*)p->d_ic_sigma, (void*)p->d_ic_mask})
+                  (void*)p->d_ic_sigma, (void*)p->d_ic_mask, (void*)p->d_part64})
     (void)hipFree(q);
+  if (p->done_ev) (void)hipEventDestroy(p->done_ev);
   free_work(p);
   for (auto& slot : p->ev)
     for (auto& e : slot)
@@ -440,12 +466,117 @@ void free_plan(gng_pm_plan* p) {
 }
 
 CellMap cell_map(const gng_pm_plan* p) {
-  const double inv = (double)p->cfg.grid_size / p->cfg.box_size;
   CellMap cm;
-  cm.hi = (float)inv;
-  cm.lo = (float)(inv - (double)cm.hi);
+  cm.inv_cell = (double)p->cfg.grid_size / p->cfg.box_size;  // cic.rs:262
   cm.nm = p->cfg.grid_size;
   return cm;
+}
+
+// Particle inputs of a solve: FP32 or FP64 positions at any element stride,
+// FP32 or FP64 masses (stride 1); accelerations out as FP32 or FP64.
+struct PartIn {
+  int pos64 = 0;
+  const void* x = nullptr;
+  const void* y = nullptr;
+  const void* z = nullptr;
+  long long stride = 1;
+  int mass64 = 0;
+  const void* m = nullptr;
+};
+struct AccOut {
+  int f64 = 0;
+  void* x = nullptr;
+  void* y = nullptr;
+  void* z = nullptr;
+  long long stride = 1;
+};
+
+PartIn soa32(const float* x, const float* y, const float* z, const float* m) {
+  PartIn in;
+  in.x = x;
+  in.y = y;
+  in.z = z;
+  in.m = m;
+  return in;
+}
+AccOut acc32(float* x, float* y, float* z) {
+  AccOut o;
+  o.x = x;
+  o.y = y;
+  o.z = z;
+  return o;
+}
+PartIn offset_in(PartIn in, long long o) {
+  const long long b = o * in.stride * (in.pos64 ? 8 : 4);
+  in.x = (const char*)in.x + b;
+  in.y = (const char*)in.y + b;
+  in.z = (const char*)in.z + b;
+  in.m = (const char*)in.m + o * (in.mass64 ? 8 : 4);
+  return in;
+}
+AccOut offset_out(AccOut out, long long o) {
+  const long long b = o * out.stride * (out.f64 ? 8 : 4);
+  out.x = (char*)out.x + b;
+  out.y = (char*)out.y + b;
+  out.z = (char*)out.z + b;
+  return out;
+}
+
+template <typename T>
+PosSrc<T> pos_src(const PartIn& in) {
+  return PosSrc<T>{(const T*)in.x, (const T*)in.y, (const T*)in.z, in.stride};
+}
+template <typename T>
+AccDst<T> acc_dst(const AccOut& o) {
+  return AccDst<T>{(T*)o.x, (T*)o.y, (T*)o.z, o.stride};
+}
+
+// The particle kernels for each combination of input types.
+template <typename T, typename M>
+void launch_bucket_t(const PartIn& in, long long n, const CellMap& cm, const TileMap& tm, unsigned* count,
+                     unsigned* bucket, unsigned* rank, SolveScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL((k_bucket_count<T, M>), dim3((unsigned)((n + 256 * kBucketPer - 1) / (256 * kBucketPer))),
+                     dim3(256), 0, s, pos_src<T>(in), (const M*)in.m, n, cm, tm, count, bucket, rank, sc);
+}
+void launch_bucket(const PartIn& in, long long n, const CellMap& cm, const TileMap& tm, unsigned* count,
+                   unsigned* bucket, unsigned* rank, SolveScalars* sc, hipStream_t s) {
+  if (in.pos64)
+    (in.mass64 ? launch_bucket_t<double, double> : launch_bucket_t<double, float>)(in, n, cm, tm, count, bucket,
+                                                                                     rank, sc, s);
+  else
+    (in.mass64 ? launch_bucket_t<float, double> : launch_bucket_t<float, float>)(in, n, cm, tm, count, bucket, rank,
+                                                                                   sc, s);
+}
+template <typename T, typename M>
+void launch_scatter_t(const PartIn& in, long long n, const CellMap& cm, const unsigned* bucket,
+                      const unsigned* rank, const unsigned* offset, PartRec* sorted, unsigned* perm,
+                      SolveScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL((k_scatter<T, M>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos_src<T>(in),
+                     (const M*)in.m, n, cm, bucket, rank, offset, sorted, perm, sc);
+}
+void launch_scatter(const PartIn& in, long long n, const CellMap& cm, const unsigned* bucket, const unsigned* rank,
+                    const unsigned* offset, PartRec* sorted, unsigned* perm, SolveScalars* sc, hipStream_t s) {
+  if (in.pos64)
+    (in.mass64 ? launch_scatter_t<double, double> : launch_scatter_t<double, float>)(in, n, cm, bucket, rank, offset,
+                                                                                       sorted, perm, sc, s);
+  else
+    (in.mass64 ? launch_scatter_t<float, double> : launch_scatter_t<float, float>)(in, n, cm, bucket, rank, offset,
+                                                                                     sorted, perm, sc, s);
+}
+template <typename T, typename O>
+void launch_interp_t(const PartIn& in, const long long* idx, long long k, const CellMap& cm, long long fplane,
+                     const float* fx, const float* fy, const float* fz, const AccOut& out, hipStream_t s) {
+  hipLaunchKernelGGL((k_interp<T, O>), dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, pos_src<T>(in), idx, k,
+                     cm, fplane, fx, fy, fz, acc_dst<O>(out));
+}
+void launch_interp(const PartIn& in, const long long* idx, long long k, const CellMap& cm, long long fplane,
+                   const float* fx, const float* fy, const float* fz, const AccOut& out, hipStream_t s) {
+  if (in.pos64)
+    (out.f64 ? launch_interp_t<double, double> : launch_interp_t<double, float>)(in, idx, k, cm, fplane, fx, fy, fz,
+                                                                                   out, s);
+  else
+    (out.f64 ? launch_interp_t<float, double> : launch_interp_t<float, float>)(in, idx, k, cm, fplane, fx, fy, fz,
+                                                                                 out, s);
 }
 
 GreenParams green_params(const gng_pm_plan* p, double g) {
@@ -471,7 +602,7 @@ int ensure_work(gng_pm_plan* p, long long n) {
   if (n <= p->work_cap) return GNG_OK;
   free_work(p);
   const size_t m = (size_t)n;
-  GNG_HIP(hipMalloc(&p->d_sorted, sizeof(float4) * m));
+  GNG_HIP(hipMalloc(&p->d_sorted, sizeof(PartRec) * m));
   GNG_HIP(hipMalloc(&p->d_perm, sizeof(unsigned) * m));
   GNG_HIP(hipMalloc(&p->d_bucket, sizeof(unsigned) * m));
   GNG_HIP(hipMalloc(&p->d_rank, sizeof(unsigned) * m));
@@ -481,19 +612,18 @@ int ensure_work(gng_pm_plan* p, long long n) {
 
 // Deposit and the forward x and y transforms (shared by the force solve and
 // the power spectrum); the mass mesh is in P as the half spectrum after y.
-int run_front(gng_pm_plan* p, const float* x, const float* y, const float* z, const float* m, long long n,
-              hipStream_t s) {
+int run_front(gng_pm_plan* p, const PartIn& in, long long n, hipStream_t s) {
   int rc = ensure_work(p, n > 0 ? n : 1);
   if (rc) return rc;
   const int nm = p->cfg.grid_size;
   const int H = nm / 2;
   const MeshOps& o = *p->ops;
   const CellMap cm = cell_map(p);
-  float* rho = p->keep_density ? p->d_rho : reinterpret_cast<float*>(p->d_P);
-  // z-plane strides: half-spectrum meshes (complex values), the same buffers
-  // holding real fields (floats), and the mass mesh (floats)
+  // the shifted mass mesh goes into P as real rows (float z-plane stride
+  // fplane); a kept density is also written unshifted and unpadded
+  float* rho = reinterpret_cast<float*>(p->d_P);
   const long long cplane = mesh_plane(nm), fplane = 2 * cplane;
-  const long long rho_plane = p->keep_density ? (long long)nm * nm : fplane;
+  const long long rho_plane = fplane;
 
   mark(p, 0, s);
   hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, p->d_sc);
@@ -502,22 +632,19 @@ int run_front(gng_pm_plan* p, const float* x, const float* y, const float* z, co
     hipLaunchKernelGGL(k_zero, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, s,
                        reinterpret_cast<uint4*>(p->d_count), n16);
   }
-  if (n > 0)
-    hipLaunchKernelGGL(k_bucket_count, dim3((unsigned)((n + 256 * kBucketPer - 1) / (256 * kBucketPer))), dim3(256), 0,
-                       s, x, y, z, m, n, cm, p->tm,
-                       p->d_count, p->d_bucket, p->d_rank, p->d_sc);
+  if (n > 0) launch_bucket(in, n, cm, p->tm, p->d_count, p->d_bucket, p->d_rank, p->d_sc, s);
   mark(p, 1, s);
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_count, p->d_offset,
                      p->d_partial, p->nb);
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(256), 0, s, p->d_partial, p->scan_blocks, p->d_offset, p->nb);
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_offset, p->d_partial, p->nb);
   if (n > 0)
-    hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z, m, n, p->d_bucket,
-                       p->d_rank, p->d_offset, p->d_sorted, p->tune.interp_sorted ? p->d_perm : nullptr);
+    launch_scatter(in, n, cm, p->d_bucket, p->d_rank, p->d_offset, p->d_sorted,
+                   p->tune.interp_sorted ? p->d_perm : nullptr, p->d_sc, s);
   mark(p, 2, s);
   const unsigned ntiles = (unsigned)p->tm.nt * p->tm.nt * p->tm.nt;
   o.deposit[p->tune.deposit_wide](dim3(ntiles), dim3(p->tune.deposit_wide ? 512 : 256), s, p->d_sorted, p->d_offset,
-                                   cm, p->tm, p->d_sc, n, rho, rho_plane);
+                                   cm, p->tm, p->d_sc, n, rho, rho_plane, p->keep_density ? p->d_rho : nullptr);
   mark(p, 3, s);
   const long long rows = (long long)nm * nm;
   const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
@@ -548,9 +675,9 @@ int run_front(gng_pm_plan* p, const float* x, const float* y, const float* z, co
 }
 
 // The whole pipeline on device pointers.
-int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z, const float* m, long long n,
-                 const long long* idx, long long k, double g, float* ax, float* ay, float* az, hipStream_t s) {
-  int rc = run_front(p, x, y, z, m, n, s);
+int run_pipeline(gng_pm_plan* p, const PartIn& in, long long n, const long long* idx, long long k, double g,
+                 const AccOut& out, hipStream_t s) {
+  int rc = run_front(p, in, n, s);
   if (rc) return rc;
   const int nm = p->cfg.grid_size;
   const int H = nm / 2;
@@ -597,15 +724,21 @@ int run_pipeline(gng_pm_plan* p, const float* x, const float* y, const float* z,
   o.x_inv(dim3((unsigned)(rows / o.x_rows), 3), dim3(256), s, p->d_A, p->d_P, p->d_B, nullptr, p->d_tw, 0, 1.0f);
   mark(p, 8, s);
   }
-  if (k > 0 && !idx && p->tune.interp_sorted)
+  const float* fx = reinterpret_cast<const float*>(p->d_A);
+  const float* fy = reinterpret_cast<const float*>(p->d_P);
+  const float* fz = reinterpret_cast<const float*>(p->d_B);
+  if (k > 0 && !idx && p->tune.interp_sorted) {
     // every particle: walk the deposit's sorted copy (local mesh reads whatever the caller's order)
-    hipLaunchKernelGGL(k_interp_sorted, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, p->d_sorted, p->d_perm, k,
-                       cm, fplane, reinterpret_cast<const float*>(p->d_A), reinterpret_cast<const float*>(p->d_P),
-                       reinterpret_cast<const float*>(p->d_B), ax, ay, az);
-  else if (k > 0)
-    hipLaunchKernelGGL(k_interp, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, x, y, z, idx, k, cm, fplane,
-                       reinterpret_cast<const float*>(p->d_A), reinterpret_cast<const float*>(p->d_P),
-                       reinterpret_cast<const float*>(p->d_B), ax, ay, az);
+    const dim3 gi((unsigned)((k + 255) / 256));
+    if (out.f64)
+      hipLaunchKernelGGL(k_interp_sorted<double>, gi, dim3(256), 0, s, p->d_sorted, p->d_perm, k, cm, fplane, fx, fy,
+                         fz, acc_dst<double>(out));
+    else
+      hipLaunchKernelGGL(k_interp_sorted<float>, gi, dim3(256), 0, s, p->d_sorted, p->d_perm, k, cm, fplane, fx, fy,
+                         fz, acc_dst<float>(out));
+  } else if (k > 0) {
+    launch_interp(in, idx, k, cm, fplane, fx, fy, fz, out, s);
+  }
   mark(p, 9, s);
   GNG_HIP(hipGetLastError());
   if (p->timing) ++p->timed_solves;
@@ -637,6 +770,17 @@ int read_timing(gng_pm_plan* p, float* ms) {
   return GNG_OK;
 }
 
+int ensure_idx(gng_pm_plan* p, long long k) {
+  if (k > p->idx_cap) {
+    (void)hipFree(p->d_idx);
+    p->d_idx = nullptr;
+    p->idx_cap = 0;
+    GNG_HIP(hipMalloc(&p->d_idx, sizeof(long long) * (size_t)k));
+    p->idx_cap = k;
+  }
+  return GNG_OK;
+}
+
 int ensure_particles(gng_pm_plan* p, long long n, long long k, bool need_idx) {
   const long long need = n > k ? n : k;
   if (need > p->cap) {
@@ -646,14 +790,42 @@ int ensure_particles(gng_pm_plan* p, long long n, long long k, bool need_idx) {
     GNG_HIP(hipMalloc(&p->d_part, sizeof(float) * 7 * (size_t)need));
     p->cap = need;
   }
-  if (need_idx && k > p->idx_cap) {
-    (void)hipFree(p->d_idx);
-    p->d_idx = nullptr;
-    p->idx_cap = 0;
-    GNG_HIP(hipMalloc(&p->d_idx, sizeof(long long) * (size_t)k));
-    p->idx_cap = k;
-  }
+  return need_idx ? ensure_idx(p, k) : GNG_OK;
+}
+
+// Device staging of the f64 entry points: positions 3 cap64, masses cap64,
+// accelerations 3 cap64 doubles.
+int ensure_part64(gng_pm_plan* p, long long n, long long k) {
+  const long long need = n > k ? n : k;
+  if (need <= p->cap64) return GNG_OK;
+  (void)hipFree(p->d_part64);
+  p->d_part64 = nullptr;
+  p->cap64 = 0;
+  GNG_HIP(hipMalloc(&p->d_part64, sizeof(double) * 7 * (size_t)need));
+  p->cap64 = need;
   return GNG_OK;
+}
+
+// The reference's Vec3 arrays: n x 3 interleaved f64 positions, f64 masses.
+PartIn aos64(const double* pos, const double* mass) {
+  PartIn in;
+  in.pos64 = 1;
+  in.x = pos;
+  in.y = pos + 1;
+  in.z = pos + 2;
+  in.stride = 3;
+  in.mass64 = 1;
+  in.m = mass;
+  return in;
+}
+AccOut aos64_out(double* out) {
+  AccOut o;
+  o.f64 = 1;
+  o.x = out;
+  o.y = out + 1;
+  o.z = out + 2;
+  o.stride = 3;
+  return o;
 }
 
 int validate(gng_pm_plan* p, long long n, const void* idx, long long k) {
@@ -674,6 +846,7 @@ int solve_host_soa(gng_pm_plan* p, const float* x, const float* y, const float* 
   float *dx = d, *dy = d + cap, *dz = d + 2 * cap, *dm = d + 3 * cap;
   float *dax = d + 4 * cap, *day = d + 5 * cap, *daz = d + 6 * cap;
   hipStream_t s = p->stream;
+  StreamOrder order(p, s);
   const size_t nb = sizeof(float) * (size_t)n;
   if (n > 0) {
     GNG_HIP(hipMemcpyAsync(dx, x, nb, hipMemcpyHostToDevice, s));
@@ -686,7 +859,7 @@ int solve_host_soa(gng_pm_plan* p, const float* x, const float* y, const float* 
       if (idx[j] < 0 || idx[j] >= n) return fail(GNG_ERR_INVALID, "index out of range");
     GNG_HIP(hipMemcpyAsync(p->d_idx, idx, sizeof(long long) * (size_t)k, hipMemcpyHostToDevice, s));
   }
-  rc = run_pipeline(p, dx, dy, dz, dm, n, idx ? p->d_idx : nullptr, k, g, dax, day, daz, s);
+  rc = run_pipeline(p, soa32(dx, dy, dz, dm), n, idx ? p->d_idx : nullptr, k, g, acc32(dax, day, daz), s);
   if (rc) return rc;
   const size_t kb = sizeof(float) * (size_t)k;
   if (k > 0) {
@@ -703,8 +876,7 @@ constexpr int kPkMassBlocks = 1024;
 // Power spectrum of n particles on device pointers: shell sums (already times
 // (V / M_tot)^2) and mode counts, nm/2 shells, copied to the host; the call
 // synchronises s.
-int run_pk(gng_pm_plan* p, const float* x, const float* y, const float* z, const float* m, long long n,
-           hipStream_t s, double* sums, unsigned long long* modes) {
+int run_pk(gng_pm_plan* p, const PartIn& in, long long n, hipStream_t s, double* sums, unsigned long long* modes) {
   const int nm = p->cfg.grid_size, H = nm / 2;
   const MeshOps& o = *p->ops;
   if (!p->d_pk_w) {
@@ -747,10 +919,15 @@ int run_pk(gng_pm_plan* p, const float* x, const float* y, const float* z, const
     GNG_HIP(hipMalloc(&p->d_pk_part, sizeof(double) * (size_t)nblk * (size_t)H));
     p->pk_part_cap = nblk;
   }
-  int rc = run_front(p, x, y, z, m, n, s);
+  int rc = run_front(p, in, n, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((H + 255) / 256)), dim3(256), 0, s, p->d_pk_modes, H);
-  hipLaunchKernelGGL(k_mass_partials, dim3(kPkMassBlocks), dim3(256), 0, s, m, n, p->d_pk_msum);
+  if (in.mass64)
+    hipLaunchKernelGGL(k_mass_partials<double>, dim3(kPkMassBlocks), dim3(256), 0, s, (const double*)in.m, n,
+                       p->d_pk_msum);
+  else
+    hipLaunchKernelGGL(k_mass_partials<float>, dim3(kPkMassBlocks), dim3(256), 0, s, (const float*)in.m, n,
+                       p->d_pk_msum);
   hipLaunchKernelGGL(k_mass_total, dim3(1), dim3(1024), 0, s, p->d_pk_msum, kPkMassBlocks);
   o.z_pk(dim3((unsigned)(H / o.yz_cols), (unsigned)nm), dim3((unsigned)o.yz_threads), s, p->d_P, p->d_tw, p->d_pk_w,
          p->d_pk_part, nblk, p->d_pk_modes);
@@ -1074,6 +1251,7 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
   hipError_t e = hipMemcpy(p->d_tw, tw.data(), sizeof(float2) * nm, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_deconv, dc.data(), sizeof(float) * nm, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->done_ev, hipEventDisableTiming);
   for (int j = 0; j < kTimingSlots && e == hipSuccess; ++j)
     for (int i = 0; i <= kStages && e == hipSuccess; ++i) e = hipEventCreate(&p->ev[j][i]);
   if (e != hipSuccess) return bail(fail(GNG_ERR_HIP, std::string("plan setup: ") + hipGetErrorString(e)));
@@ -1101,24 +1279,62 @@ int gng_pm_accelerations_f64(gng_pm_plan* p, const double* pos, const double* ma
   if (rc) return rc;
   if (n > 0 && (!pos || !mass)) return fail(GNG_ERR_INVALID, "null input array");
   if (k > 0 && !out) return fail(GNG_ERR_INVALID, "null output array");
-  std::vector<float> h(4 * (size_t)n + 3 * (size_t)k);
-  float *x = h.data(), *y = x + n, *z = y + n, *m = z + n, *ax = m + n, *ay = ax + k, *az = ay + k;
-  for (long long i = 0; i < n; ++i) {
-    x[i] = (float)pos[3 * i];
-    y[i] = (float)pos[3 * i + 1];
-    z[i] = (float)pos[3 * i + 2];
-    m[i] = (float)mass[i];
-  }
+  if (idx)
+    for (long long j = 0; j < k; ++j)
+      if (idx[j] < 0 || idx[j] >= n) return fail(GNG_ERR_INVALID, "index out of range");
   std::lock_guard<std::mutex> lk(p->mu);
   if ((rc = set_device(p))) return rc;
-  rc = solve_host_soa(p, x, y, z, m, n, idx, k, g, ax, ay, az);
-  if (rc) return rc;
-  for (long long j = 0; j < k; ++j) {
-    out[3 * j] = ax[j];
-    out[3 * j + 1] = ay[j];
-    out[3 * j + 2] = az[j];
+  hipStream_t s = p->stream;
+  StreamOrder order(p, s);
+  if ((rc = ensure_part64(p, n, k))) return rc;
+  if (idx && (rc = ensure_idx(p, k))) return rc;
+  double* dpos = p->d_part64;
+  double* dm = dpos + 3 * p->cap64;
+  double* dout = dm + p->cap64;
+  if (n > 0) {
+    GNG_HIP(hipMemcpyAsync(dpos, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+    GNG_HIP(hipMemcpyAsync(dm, mass, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s));
   }
+  if (idx && k > 0) GNG_HIP(hipMemcpyAsync(p->d_idx, idx, sizeof(long long) * (size_t)k, hipMemcpyHostToDevice, s));
+  rc = run_pipeline(p, aos64(dpos, dm), n, idx ? p->d_idx : nullptr, k, g, aos64_out(dout), s);
+  if (rc) return rc;
+  if (k > 0) GNG_HIP(hipMemcpyAsync(out, dout, sizeof(double) * 3 * (size_t)k, hipMemcpyDeviceToHost, s));
+  GNG_HIP(hipStreamSynchronize(s));
   return GNG_OK;
+}
+
+int gng_pm_solve_device_f64(gng_pm_plan* p, const double* d_pos, const double* d_mass, int64_t n,
+                            const int64_t* d_idx, int64_t k, double g, double* d_out, void* stream) {
+  int rc = validate(p, n, d_idx, k);
+  if (rc) return rc;
+  if (n > 0 && (!d_pos || !d_mass)) return fail(GNG_ERR_INVALID, "null input array");
+  if (k > 0 && !d_out) return fail(GNG_ERR_INVALID, "null output array");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if ((rc = set_device(p))) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  StreamOrder order(p, s);
+  return run_pipeline(p, aos64(d_pos, d_mass), n, reinterpret_cast<const long long*>(d_idx), k, g,
+                      aos64_out(d_out), s);
+}
+
+int gng_pm_solve_device_soa64(gng_pm_plan* p, const double* d_x, const double* d_y, const double* d_z,
+                              const float* d_mass, int64_t n, const int64_t* d_idx, int64_t k, double g, float* d_ax,
+                              float* d_ay, float* d_az, void* stream) {
+  int rc = validate(p, n, d_idx, k);
+  if (rc) return rc;
+  if (n > 0 && (!d_x || !d_y || !d_z || !d_mass)) return fail(GNG_ERR_INVALID, "null input array");
+  if (k > 0 && (!d_ax || !d_ay || !d_az)) return fail(GNG_ERR_INVALID, "null output array");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if ((rc = set_device(p))) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  StreamOrder order(p, s);
+  PartIn in;
+  in.pos64 = 1;
+  in.x = d_x;
+  in.y = d_y;
+  in.z = d_z;
+  in.m = d_mass;
+  return run_pipeline(p, in, n, reinterpret_cast<const long long*>(d_idx), k, g, acc32(d_ax, d_ay, d_az), s);
 }
 
 int gng_pm_power_spectrum(gng_pm_plan* p, const float* d_x, const float* d_y, const float* d_z, const float* d_mass,
@@ -1134,7 +1350,9 @@ int gng_pm_power_spectrum(gng_pm_plan* p, const float* d_x, const float* d_y, co
   const int H = p->cfg.grid_size / 2;
   std::vector<double> sums((size_t)H);
   std::vector<unsigned long long> modes((size_t)H);
-  if ((rc = run_pk(p, d_x, d_y, d_z, d_mass, n, reinterpret_cast<hipStream_t>(stream), sums.data(), modes.data())))
+  StreamOrder order(p, reinterpret_cast<hipStream_t>(stream));
+  if ((rc = run_pk(p, soa32(d_x, d_y, d_z, d_mass), n, reinterpret_cast<hipStream_t>(stream), sums.data(),
+                   modes.data())))
     return rc;
   return pk_bins(p, sums.data(), modes.data(), capacity, k, pk, n_modes, n_bins);
 }
@@ -1145,26 +1363,20 @@ int gng_pm_power_spectrum_f64(gng_pm_plan* p, const double* pos, const double* m
   if (n <= 0) return fail(GNG_ERR_INVALID, "the power spectrum needs at least one particle");
   if (n >= (1LL << 31)) return fail(GNG_ERR_INVALID, "n of 2^31 or more is not supported");
   if (!pos || !mass) return fail(GNG_ERR_INVALID, "null input array");
-  std::vector<float> h(4 * (size_t)n);
-  float *x = h.data(), *y = x + n, *z = y + n, *m = z + n;
-  for (long long i = 0; i < n; ++i) {
-    x[i] = (float)pos[3 * i];
-    y[i] = (float)pos[3 * i + 1];
-    z[i] = (float)pos[3 * i + 2];
-    m[i] = (float)mass[i];
-  }
   std::lock_guard<std::mutex> lk(p->mu);
   int rc = set_device(p);
   if (rc) return rc;
-  if ((rc = ensure_particles(p, n, 0, false))) return rc;
-  const size_t cap = (size_t)p->cap, nb = sizeof(float) * (size_t)n;
-  float* d = p->d_part;
   hipStream_t s = p->stream;
-  for (int c = 0; c < 4; ++c) GNG_HIP(hipMemcpyAsync(d + c * cap, h.data() + c * (size_t)n, nb, hipMemcpyHostToDevice, s));
+  StreamOrder order(p, s);
+  if ((rc = ensure_part64(p, n, 0))) return rc;
+  double* dpos = p->d_part64;
+  double* dm = dpos + 3 * p->cap64;
+  GNG_HIP(hipMemcpyAsync(dpos, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+  GNG_HIP(hipMemcpyAsync(dm, mass, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s));
   const int H = p->cfg.grid_size / 2;
   std::vector<double> sums((size_t)H);
   std::vector<unsigned long long> modes((size_t)H);
-  if ((rc = run_pk(p, d, d + cap, d + 2 * cap, d + 3 * cap, n, s, sums.data(), modes.data()))) return rc;
+  if ((rc = run_pk(p, aos64(dpos, dm), n, s, sums.data(), modes.data()))) return rc;
   return pk_bins(p, sums.data(), modes.data(), capacity, k, pk, n_modes, n_bins);
 }
 
@@ -1174,6 +1386,7 @@ int gng_ic_delta_kspace(gng_pm_plan* p, uint64_t seed, const double* sigma, int6
   int rc = set_device(p);
   if (rc) return rc;
   hipStream_t s = p->stream;
+  StreamOrder order(p, s);
   IcParams ip{};
   if ((rc = ic_upload_sigma(p, sigma, sigma_len, s, ip, seed))) return rc;
   const int nm = p->cfg.grid_size;
@@ -1201,6 +1414,7 @@ int gng_ic_zeldovich_device(gng_pm_plan* p, uint64_t seed, const double* sigma, 
   std::lock_guard<std::mutex> lk(p->mu);
   if ((rc = set_device(p))) return rc;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  StreamOrder order(p, s);
   IcParams ip{};
   if ((rc = ic_upload_sigma(p, sigma, sigma_len, s, ip, seed))) return rc;
   return run_ic(p, ip, vel_factor, scale, lo, hi - lo, d_pos, d_vel, s);
@@ -1215,6 +1429,7 @@ int gng_ic_zeldovich(gng_pm_plan* p, uint64_t seed, const double* sigma, int64_t
   std::lock_guard<std::mutex> lk(p->mu);
   if ((rc = set_device(p))) return rc;
   hipStream_t s = p->stream;
+  StreamOrder order(p, s);
   IcParams ip{};
   if ((rc = ic_upload_sigma(p, sigma, sigma_len, s, ip, seed))) return rc;
   const long long cnt = hi - lo;
@@ -1245,6 +1460,7 @@ int gng_ic_2lpt_device(gng_pm_plan* p, uint64_t seed, const double* sigma, int64
   std::lock_guard<std::mutex> lk(p->mu);
   if ((rc = set_device(p))) return rc;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  StreamOrder order(p, s);
   IcParams ip{};
   if ((rc = ic_upload_sigma(p, sigma, sigma_len, s, ip, seed))) return rc;
   return run_ic_2lpt(p, ip, vel1, vel2, d2_over_d1sq, scale, lo, hi - lo, d_pos, d_vel, s);
@@ -1259,6 +1475,7 @@ int gng_ic_2lpt(gng_pm_plan* p, uint64_t seed, const double* sigma, int64_t sigm
   std::lock_guard<std::mutex> lk(p->mu);
   if ((rc = set_device(p))) return rc;
   hipStream_t s = p->stream;
+  StreamOrder order(p, s);
   IcParams ip{};
   if ((rc = ic_upload_sigma(p, sigma, sigma_len, s, ip, seed))) return rc;
   const long long cnt = hi - lo;
@@ -1293,8 +1510,9 @@ int gng_pm_solve_device(gng_pm_plan* p, const float* d_x, const float* d_y, cons
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(p->mu);
   if ((rc = set_device(p))) return rc;
-  return run_pipeline(p, d_x, d_y, d_z, d_mass, n, reinterpret_cast<const long long*>(d_idx), k, g, d_ax, d_ay,
-                      d_az, reinterpret_cast<hipStream_t>(stream));
+  StreamOrder order(p, reinterpret_cast<hipStream_t>(stream));
+  return run_pipeline(p, soa32(d_x, d_y, d_z, d_mass), n, reinterpret_cast<const long long*>(d_idx), k, g,
+                      acc32(d_ax, d_ay, d_az), reinterpret_cast<hipStream_t>(stream));
 }
 
 int gng_pm_keep_density(gng_pm_plan* p, int enable) {
@@ -1334,6 +1552,7 @@ int gng_pm_autotune(gng_pm_plan* p, int reps) {
   const size_t cap = (size_t)p->cap;
   float* d = p->d_part;
   hipStream_t s = p->stream;
+  StreamOrder order(p, s);
   hipLaunchKernelGGL(k_synth_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, d + cap, d + 2 * cap,
                      d + 3 * cap, n, (float)p->cfg.box_size);
   GNG_HIP(hipGetLastError());
@@ -1347,14 +1566,14 @@ int gng_pm_autotune(gng_pm_plan* p, int reps) {
   }
   // time one configuration: best of `reps` solves after a warm-up
   auto measure = [&](float& out_ms) -> int {
-    int r2 = run_pipeline(p, d, d + cap, d + 2 * cap, d + 3 * cap, n, nullptr, n, 1.0, d + 4 * cap, d + 5 * cap,
-                          d + 6 * cap, s);
+    const PartIn in = soa32(d, d + cap, d + 2 * cap, d + 3 * cap);
+    const AccOut out = acc32(d + 4 * cap, d + 5 * cap, d + 6 * cap);
+    int r2 = run_pipeline(p, in, n, nullptr, n, 1.0, out, s);
     float t_min = 1e30f;
     for (int r = 0; r < reps && r2 == GNG_OK; ++r) {
       if (hipEventRecord(e0, s) != hipSuccess) r2 = fail(GNG_ERR_HIP, "event record failed");
       if (r2 == GNG_OK)
-        r2 = run_pipeline(p, d, d + cap, d + 2 * cap, d + 3 * cap, n, nullptr, n, 1.0, d + 4 * cap, d + 5 * cap,
-                          d + 6 * cap, s);
+        r2 = run_pipeline(p, in, n, nullptr, n, 1.0, out, s);
       float ms = 0.0f;
       if (r2 == GNG_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
@@ -1657,8 +1876,8 @@ int gng_pm_batch_solve_device(gng_pm_batch* b, const float* d_x, const float* d_
     for (int k = 0; k < b->branches && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->side[k], b->root_ev, 0);
     for (int i = 0; i < nb && e == hipSuccess && rc == GNG_OK; ++i) {
       const long long o = offsets[i], n = offsets[i + 1] - offsets[i];
-      rc = run_pipeline(b->plans[i], d_x + o, d_y + o, d_z + o, d_mass + o, n, nullptr, n, g, d_ax + o, d_ay + o,
-                        d_az + o, b->side[i % b->branches]);
+      rc = run_pipeline(b->plans[i], soa32(d_x + o, d_y + o, d_z + o, d_mass + o), n, nullptr, n, g,
+                        acc32(d_ax + o, d_ay + o, d_az + o), b->side[i % b->branches]);
     }
     for (int k = 0; k < b->branches && e == hipSuccess; ++k) {
       e = hipEventRecord(b->join_ev[k], b->side[k]);

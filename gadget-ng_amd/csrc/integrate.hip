@@ -6,9 +6,10 @@
 // :354-356 (G a^3 coupling), :759-778 (periodic wrap), and the periodic PM
 // loop of crates/gadget-ng-cli/src/engine/stepping/mod.rs:1076-1576.
 //
-// The state lives in HBM as SoA f64 (x, y, z, vx, vy, vz), with an f32 copy of
-// the positions that the force solver reads, f32 masses and the f32
-// accelerations of the last force call.  Each step is: PM force, kick+drift
+// The state lives in HBM as SoA f64 (x, y, z, vx, vy, vz), f32 masses and the
+// f32 accelerations of the last force call.  The PM force reads the f64
+// positions (cells formed in FP64); an f32 copy of the positions feeds the
+// optional short-range sum.  Each step is: PM force, kick+drift
 // kernel, PM force, kick(+wrap) kernel -- all on one stream, no host sync.
 #include <hip/hip_runtime.h>
 
@@ -300,10 +301,11 @@ void free_kdk(gng_kdk* s) {
   s->d_f = nullptr;
 }
 
-// a = F(xf): PM (long range when the plan has r_split) plus the optional short range
+// a = F(x): PM on the f64 positions (long range when the plan has r_split)
+// plus the optional short range on their f32 copy
 int force(gng_kdk* s, double g, hipStream_t st) {
-  int rc = gng_pm_solve_device(s->plan, s->xf(), s->yf(), s->zf(), s->m(), s->n, nullptr, s->n, g, s->ax(),
-                               s->ay(), s->az(), st);
+  int rc = gng_pm_solve_device_soa64(s->plan, s->x(), s->y(), s->z(), s->m(), s->n, nullptr, s->n, g, s->ax(),
+                                     s->ay(), s->az(), st);
   if (rc) return rc;
   if (s->sr) {
     rc = gng_sr_accelerations(s->sr, s->xf(), s->yf(), s->zf(), s->m(), s->n, s->box, s->sr_r_split, s->sr_r_cut,
@@ -459,6 +461,8 @@ int gng_kdk_set_state_device(gng_kdk* s, const double* d_pos, const double* d_ve
                              double uniform_mass, void* stream) {
   if (!s) return fail(GNG_ERR_INVALID, "null integrator");
   if (s->n > 0 && (!d_pos || !d_vel)) return fail(GNG_ERR_INVALID, "null state array");
+  if (!d_mass && !(std::isfinite(uniform_mass) && uniform_mass > 0.0))
+    return fail(GNG_ERR_INVALID, "without d_mass, uniform_mass must be finite and > 0");
   std::lock_guard<std::mutex> lk(s->mu);
   GNG_HIP(hipSetDevice(s->device));
   const long long n = s->n;

@@ -170,7 +170,8 @@ __global__ __launch_bounds__(256) void k_pk_col0(const float2* __restrict__ P, c
 }
 
 // Per-block sums of the masses, each block in a fixed order: msum[blockIdx.x].
-__global__ __launch_bounds__(256) void k_mass_partials(const float* __restrict__ m, long long n,
+template <typename M>
+__global__ __launch_bounds__(256) void k_mass_partials(const M* __restrict__ m, long long n,
                                                        double* __restrict__ msum) {
   __shared__ double red[256];
   const int tid = threadIdx.x;

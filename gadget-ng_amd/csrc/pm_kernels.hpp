@@ -43,12 +43,91 @@
 
 namespace gng {
 
-// 1/L of the cell map as a float pair: inv_cell = hi + lo, nm / L exactly
-// enough that x * inv_cell keeps ~1e-12 relative accuracy in FP32 ops.
+// Positions to cells: c = (x * inv_cell) rem_euclid nm with inv_cell = nm / L
+// in FP64, as cic.rs:50-52 and :262 form it.
 struct CellMap {
-  float hi, lo;
+  double inv_cell;
   int nm;
 };
+
+// Particle positions: x[i * stride], y[i * stride], z[i * stride] (stride 1:
+// SoA arrays; stride 3 with y = x + 1, z = x + 2: the reference's Vec3 array).
+template <typename T>
+struct PosSrc {
+  const T* x;
+  const T* y;
+  const T* z;
+  long long stride;
+};
+
+// Accelerations out, with the same addressing.
+template <typename T>
+struct AccDst {
+  T* x;
+  T* y;
+  T* z;
+  long long stride;
+};
+
+constexpr int kFracBits = 43;
+constexpr unsigned long long kFracOne = 1ull << kFracBits;
+
+// Cell index and in-cell fraction of one coordinate, in FP64:
+// c = (x * inv_cell) rem_euclid nm, i = floor(c) (cic.rs:50-56), and the
+// fraction c - i (exact in FP64) rounded to kFracBits bits.  A fraction that
+// rounds up to 1 moves to the next cell; index nm (rem_euclid rounding up
+// for tiny negative x, or that carry) wraps to 0.  A non-finite coordinate
+// lands in cell 0, so no index leaves the mesh.
+__device__ __forceinline__ void cell_fix(double x, const CellMap cm, int& i, unsigned long long& frac) {
+  const double nmf = (double)cm.nm;
+  double c = x * cm.inv_cell;
+  if (!(c >= 0.0 && c < nmf)) {
+    c = fmod(c, nmf);
+    if (c < 0.0) c += nmf;
+  }
+  const double f = floor(c);
+  int ii = (int)f;
+  unsigned long long q = (unsigned long long)__double2ll_rn((c - f) * (double)kFracOne);
+  if (q >= kFracOne) {
+    q = 0ull;
+    ++ii;
+  }
+  if ((unsigned)ii >= (unsigned)cm.nm) ii = 0;
+  i = ii;
+  frac = q;
+}
+
+// The same coordinate for the interpolation: index and FP32 fraction.
+__device__ __forceinline__ void cell_frac32(double x, const CellMap cm, int& i, float& d) {
+  unsigned long long q;
+  cell_fix(x, cm, i, q);
+  d = (float)q * (1.0f / (float)kFracOne);
+}
+
+// A particle in bucket order: 24 bytes.  Word a (axis a) holds the cell index
+// (10 bits, nm <= 1024) at bit 43, the fraction in bits 0..42, and 11 bits of
+// the FP32 mass's bit pattern at bit 53 (bits 0-10, 11-21, 22-31).
+struct alignas(8) PartRec {
+  unsigned long long a[3];
+};
+
+__device__ __forceinline__ PartRec encode_rec(const int (&c)[3], const unsigned long long (&f)[3], float m) {
+  const unsigned mb = __float_as_uint(m);
+  const unsigned mp[3] = {mb & 0x7FFu, (mb >> 11) & 0x7FFu, mb >> 22};
+  PartRec r;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    r.a[k] = ((unsigned long long)mp[k] << 53) | ((unsigned long long)c[k] << kFracBits) | f[k];
+  return r;
+}
+__device__ __forceinline__ int rec_cell(const PartRec& r, int k) { return (int)((r.a[k] >> kFracBits) & 0x3FFu); }
+__device__ __forceinline__ double rec_frac(const PartRec& r, int k) {
+  return (double)(r.a[k] & (kFracOne - 1ull)) * (1.0 / (double)kFracOne);
+}
+__device__ __forceinline__ float rec_mass(const PartRec& r) {
+  return __uint_as_float((unsigned)(r.a[0] >> 53) | ((unsigned)(r.a[1] >> 53) << 11) |
+                         ((unsigned)(r.a[2] >> 53) << 22));
+}
 
 // Complex values between consecutive z planes of the half-spectrum meshes:
 // N*(N/2) plus a pad of 32 (512 <= N < 1024) or 64 (N >= 1024) values.  The
@@ -69,36 +148,12 @@ __device__ __forceinline__ long long mesh_row(long long r) {
   return (r / N) * mesh_plane(N) + (r % N) * (N / 2);
 }
 
-// Cell index and offset of one coordinate: c = (x * nm / L) mod nm,
-// i0 = floor(c), d = c - i0 (cic.rs:50-60), with the product carried as
-// p + e (error-free FMA split) so d is accurate to one FP32 ulp of 1.
-// Contraction is switched off here: fusing p - f into fma(x, hi, -f) would
-// count the product's rounding error twice and leave d wrong by up to
-// ulp(c), an error that grows with the mesh size.
-__device__ __forceinline__ void cell_coord(float x, const CellMap cm, int& i0, float& d) {
-#pragma clang fp contract(off)
-  const float p = x * cm.hi;
-  const float e = fmaf(x, cm.hi, -p) + x * cm.lo;
-  const float f = floorf(p);
-  float t = (p - f) + e;
-  int i = (int)f;
-  if (t < 0.0f) {
-    t += 1.0f;
-    i -= 1;
-  } else if (t >= 1.0f) {
-    t -= 1.0f;
-    i += 1;
-  }
-  i %= cm.nm;
-  if (i < 0) i += cm.nm;
-  i0 = i;
-  d = t;
-}
-
 // Per-solve scalars kept on the device (no host round trip).
 struct SolveScalars {
   unsigned int mmax_bits;  // max |m| as float bits (non-negative floats order as integers)
-  unsigned int pad[63];
+  unsigned int pad0;
+  unsigned long long qsum;  // sum of round(m 2^s) over the particles
+  unsigned int pad[60];
 };
 
 // Fixed-point exponent: contributions are w*m*2^s with N*max|m|*2^s < 2^62,
@@ -111,7 +166,10 @@ __device__ __forceinline__ int fixed_shift(const SolveScalars* sc, long long nto
 }
 
 __global__ void k_reset_scalars(SolveScalars* sc) {
-  if (threadIdx.x == 0) sc->mmax_bits = 0u;
+  if (threadIdx.x == 0) {
+    sc->mmax_bits = 0u;
+    sc->qsum = 0ull;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -756,17 +814,12 @@ __global__ __launch_bounds__(T) void k_fft_y_inv_persist(float2* __restrict__ A,
   }
 }
 
-// CIC gather of (Fx, Fy, Fz) at the position p (fplane: the float stride
-// between z planes of the force meshes).
-__device__ __forceinline__ float3 cic_gather(float px, float py, float pz, const CellMap cm, long long fplane,
-                                             const float* __restrict__ fx, const float* __restrict__ fy,
-                                             const float* __restrict__ fz) {
-  int ix, iy, iz;
-  float dx, dy, dz;
-  cell_coord(px, cm, ix, dx);
-  cell_coord(py, cm, iy, dy);
-  cell_coord(pz, cm, iz, dz);
-  const int nm = cm.nm;
+// CIC gather of (Fx, Fy, Fz) at base cell (ix, iy, iz) with in-cell offsets
+// (dx, dy, dz) (fplane: the float stride between z planes of the force
+// meshes), cic.rs:100-170.
+__device__ __forceinline__ float3 cic_gather(int ix, int iy, int iz, float dx, float dy, float dz, int nm,
+                                             long long fplane, const float* __restrict__ fx,
+                                             const float* __restrict__ fy, const float* __restrict__ fz) {
   const float wx[2] = {1.0f - dx, dx}, wy[2] = {1.0f - dy, dy}, wz[2] = {1.0f - dz, dz};
   const int xs[2] = {ix, ix + 1 == nm ? 0 : ix + 1};
   const int ys[2] = {iy, iy + 1 == nm ? 0 : iy + 1};
@@ -792,39 +845,45 @@ __device__ __forceinline__ float3 cic_gather(float px, float py, float pz, const
 // particles come from the same few cache lines whatever order the caller's
 // arrays are in; results go to the particles' own slots (perm).  Each
 // particle's value is the same as the unsorted gather's.
-__global__ __launch_bounds__(256) void k_interp_sorted(const float4* __restrict__ sorted,
+template <typename O>
+__global__ __launch_bounds__(256) void k_interp_sorted(const PartRec* __restrict__ sorted,
                                                        const unsigned* __restrict__ perm, long long n,
                                                        const CellMap cm, long long fplane,
                                                        const float* __restrict__ fx, const float* __restrict__ fy,
-                                                       const float* __restrict__ fz, float* __restrict__ ax,
-                                                       float* __restrict__ ay, float* __restrict__ az) {
+                                                       const float* __restrict__ fz, const AccDst<O> out) {
   const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const float4 p = sorted[j];
-  const float3 a = cic_gather(p.x, p.y, p.z, cm, fplane, fx, fy, fz);
-  const unsigned o = perm[j];
-  ax[o] = a.x;
-  ay[o] = a.y;
-  az[o] = a.z;
+  const PartRec p = sorted[j];
+  const float k = 1.0f / (float)kFracOne;
+  const float3 a = cic_gather(rec_cell(p, 0), rec_cell(p, 1), rec_cell(p, 2),
+                              (float)(p.a[0] & (kFracOne - 1ull)) * k, (float)(p.a[1] & (kFracOne - 1ull)) * k,
+                              (float)(p.a[2] & (kFracOne - 1ull)) * k, cm.nm, fplane, fx, fy, fz);
+  const long long o = (long long)perm[j] * out.stride;
+  out.x[o] = (O)a.x;
+  out.y[o] = (O)a.y;
+  out.z[o] = (O)a.z;
 }
 
-// CIC gather of (Fx, Fy, Fz) at pos[idx[j]] (idx null: j itself) in the
-// caller's order; fplane is the float stride between z planes of the force
-// meshes.
-__global__ __launch_bounds__(256) void k_interp(const float* __restrict__ x, const float* __restrict__ y,
-                                                const float* __restrict__ z, const long long* __restrict__ idx,
-                                                long long k, const CellMap cm, long long fplane,
-                                                const float* __restrict__ fx,
+// CIC gather of (Fx, Fy, Fz) at particle idx[j] (idx null: j itself) in the
+// caller's order; the cell coordinate is formed in FP64 (cell_frac32).
+template <typename T, typename O>
+__global__ __launch_bounds__(256) void k_interp(const PosSrc<T> src, const long long* __restrict__ idx, long long k,
+                                                const CellMap cm, long long fplane, const float* __restrict__ fx,
                                                 const float* __restrict__ fy, const float* __restrict__ fz,
-                                                float* __restrict__ ax, float* __restrict__ ay,
-                                                float* __restrict__ az) {
+                                                const AccDst<O> out) {
   const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= k) return;
-  const long long i = idx ? idx[j] : j;
-  const float3 a = cic_gather(x[i], y[i], z[i], cm, fplane, fx, fy, fz);
-  ax[j] = a.x;
-  ay[j] = a.y;
-  az[j] = a.z;
+  const long long i = (idx ? idx[j] : j) * src.stride;
+  int ix, iy, iz;
+  float dx, dy, dz;
+  cell_frac32((double)src.x[i], cm, ix, dx);
+  cell_frac32((double)src.y[i], cm, iy, dy);
+  cell_frac32((double)src.z[i], cm, iz, dz);
+  const float3 a = cic_gather(ix, iy, iz, dx, dy, dz, cm.nm, fplane, fx, fy, fz);
+  const long long o = j * out.stride;
+  out.x[o] = (O)a.x;
+  out.y[o] = (O)a.y;
+  out.z[o] = (O)a.z;
 }
 
 }  // namespace gng

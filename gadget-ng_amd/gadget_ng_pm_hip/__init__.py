@@ -132,6 +132,10 @@ def lib():
         L.gng_pm_accelerations_f64.argtypes = [P, P, P, i64, P, i64, d, d, P]
         L.gng_pm_accelerations_f32.argtypes = [P, P, P, P, P, i64, P, i64, d, P, P, P]
         L.gng_pm_solve_device.argtypes = [P, P, P, P, P, i64, P, i64, d, P, P, P, P]
+        L.gng_pm_solve_device_f64.argtypes = [P, P, P, i64, P, i64, d, P, P]
+        L.gng_pm_solve_device_f64.restype = i32
+        L.gng_pm_solve_device_soa64.argtypes = [P, P, P, P, P, i64, P, i64, d, P, P, P, P]
+        L.gng_pm_solve_device_soa64.restype = i32
         L.gng_pm_density.argtypes = [P, P]
         L.gng_pm_keep_density.argtypes = [P, i32]
         L.gng_pm_force_meshes.argtypes = [P, P]
@@ -382,6 +386,23 @@ class Plan:
         k = n if k is None else k
         self._check(lib().gng_pm_solve_device(self._h, x_ptr, y_ptr, z_ptr, m_ptr, n, idx_ptr, k, float(g), ax_ptr,
                                               ay_ptr, az_ptr, stream or None), "gng_pm_solve_device")
+
+    def solve_device_f64(self, pos_ptr: int, m_ptr: int, n: int, out_ptr: int, g: float, stream: int = 0,
+                         idx_ptr: Optional[int] = None, k: Optional[int] = None):
+        """Enqueue a solve on device Vec3 arrays: n x 3 f64 positions, n f64
+        masses, k x 3 f64 accelerations out."""
+        k = n if k is None else k
+        self._check(lib().gng_pm_solve_device_f64(self._h, pos_ptr, m_ptr, n, idx_ptr, k, float(g), out_ptr,
+                                                  stream or None), "gng_pm_solve_device_f64")
+
+    def solve_device_soa64(self, x_ptr: int, y_ptr: int, z_ptr: int, m_ptr: int, n: int, ax_ptr: int, ay_ptr: int,
+                           az_ptr: int, g: float, stream: int = 0, idx_ptr: Optional[int] = None,
+                           k: Optional[int] = None):
+        """Enqueue a solve on device SoA f64 positions, f32 masses, f32 accelerations."""
+        k = n if k is None else k
+        self._check(lib().gng_pm_solve_device_soa64(self._h, x_ptr, y_ptr, z_ptr, m_ptr, n, idx_ptr, k, float(g),
+                                                    ax_ptr, ay_ptr, az_ptr, stream or None),
+                    "gng_pm_solve_device_soa64")
 
     def density(self) -> np.ndarray:
         """Mass per cell of the last solve (needs keep_density=True)."""

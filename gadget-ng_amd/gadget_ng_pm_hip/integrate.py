@@ -127,10 +127,14 @@ class Leapfrog:
         mass = np.ascontiguousarray(mass, dtype=np.float64).reshape(self.n)
         _check(lib().gng_kdk_set_state(self._h, _ptr(pos), _ptr(vel), _ptr(mass)), "gng_kdk_set_state")
 
-    def set_state_device(self, d_pos: int, d_vel: int, d_mass: Optional[int] = None, uniform_mass: float = 0.0,
-                         stream: int = 0) -> None:
+    def set_state_device(self, d_pos: int, d_vel: int, d_mass: Optional[int] = None,
+                         uniform_mass: Optional[float] = None, stream: int = 0) -> None:
         """The state from device buffers (n x 3 f64 positions and velocities;
-        masses from d_mass, or uniform_mass for all), without a host copy."""
+        masses from d_mass, or uniform_mass for all), without a host copy.
+        Enqueued on `stream`: order it after steps running on other streams."""
+        if not d_mass and uniform_mass is None:
+            raise ValueError("give d_mass or uniform_mass")
+        uniform_mass = 0.0 if uniform_mass is None else uniform_mass
         _check(lib().gng_kdk_set_state_device(self._h, ctypes.c_void_p(d_pos), ctypes.c_void_p(d_vel),
                                               ctypes.c_void_p(d_mass) if d_mass else None, float(uniform_mass),
                                               ctypes.c_void_p(stream) if stream else None),

@@ -27,6 +27,7 @@ RNG is reproduced; the shapes follow the workloads the PM path is measured on:
 from __future__ import annotations
 
 from dataclasses import dataclass
+from typing import Optional
 
 import numpy as np
 
@@ -42,6 +43,7 @@ class Particles:
     m: np.ndarray
     box: float
     name: str
+    p64: Optional[np.ndarray] = None  # (N, 3) float64 positions before the float32 rounding, when kept
 
     @property
     def n(self) -> int:
@@ -114,10 +116,11 @@ def zeldovich_displacement_field(nm: int, box: float, spectral_index: float, see
 
 def zeldovich_lattice(n_side: int = 256, nm: int = 512, box: float = 256.0, spectral_index: float = -2.0,
                       rms_disp_cells: float = 0.5, seed: int = SEED_ZELDOVICH, mass: float = 1.0,
-                      workers: int = -1) -> Particles:
+                      workers: int = -1, keep_f64: bool = False) -> Particles:
     """Lattice q = i * box/n_side (node-aligned on the nm mesh) displaced by Psi(q),
     Psi scaled to a 3-D rms |Psi| of rms_disp_cells mesh cells; positions wrapped
-    into [0, box) and rounded to float32; equal masses."""
+    into [0, box) and rounded to float32; equal masses.  With keep_f64 the
+    unrounded float64 positions are kept as well (``p64``)."""
     if nm % n_side != 0:
         raise ValueError("lattice must be node-aligned: nm must be a multiple of n_side")
     step = nm // n_side
@@ -128,12 +131,15 @@ def zeldovich_lattice(n_side: int = 256, nm: int = 512, box: float = 256.0, spec
     scale = rms_disp_cells * cell / rms if rms > 0 else 0.0
     q = np.arange(n_side, dtype=np.float64) * (box / n_side)
     out = []
+    p64 = np.empty((n_side ** 3, 3), dtype=np.float64) if keep_f64 else None
     for a in range(3):
         # axis a of the position: x varies fastest (flat index iz*n^2 + iy*n + ix)
         shape = [1, 1, 1]
         shape[2 - a] = n_side
         qa = np.broadcast_to(q.reshape(shape), (n_side, n_side, n_side))
         xa = np.mod(qa + scale * sub[a], box)
+        if keep_f64:
+            p64[:, a] = xa.reshape(-1)
         out.append(xa.astype(np.float32).reshape(-1))
     del psi, sub
     # float32 rounding can land exactly on box; wrap those to 0
@@ -142,7 +148,7 @@ def zeldovich_lattice(n_side: int = 256, nm: int = 512, box: float = 256.0, spec
     n = n_side ** 3
     m = np.full(n, mass, dtype=np.float32)
     return Particles(out[0], out[1], out[2], m, float(box),
-                     f"zeldovich lattice {n_side}^3 on {nm}^3 L={box} n_s={spectral_index}")
+                     f"zeldovich lattice {n_side}^3 on {nm}^3 L={box} n_s={spectral_index}", p64)
 
 
 SEED_CLUSTERED = 0x6E670003
@@ -194,25 +200,31 @@ def plummer_spheres(n_total: int, n_spheres: int, box: float, a: float, seed: in
 
 def clustered_box(n_total: int = 512 ** 3, lattice_side: int = 454, n_spheres: int = 2000, box: float = 512.0,
                   nm: int = 1024, a: float = 0.2, spectral_index: float = -2.0, rms_disp_cells: float = 0.5,
-                  seed: int = SEED_CLUSTERED, workers: int = -1) -> Particles:
+                  seed: int = SEED_CLUSTERED, workers: int = -1, keep_f64: bool = False) -> Particles:
     """Clustered TreePM long-range workload: a Zel'dovich lattice of lattice_side^3
     particles (rms displacement rms_disp_cells mesh cells) plus the remaining
-    particles in Plummer spheres; equal unit masses, float32 positions."""
+    particles in Plummer spheres; equal unit masses, float32 positions (and,
+    with keep_f64, the unrounded float64 ones as ``p64``)."""
     n_lat = lattice_side ** 3
     if n_lat > n_total:
         raise ValueError("lattice larger than the particle total")
     lat = zeldovich_offgrid_lattice(lattice_side, box, spectral_index, rms_disp_cells * box / nm, seed, workers)
     sph, _ = plummer_spheres(n_total - n_lat, n_spheres, box, a, seed + 1)
     xs = []
+    p64 = np.empty((n_total, 3), dtype=np.float64) if keep_f64 else None
     for c in range(3):
-        v = np.concatenate([lat[c], sph[c]]).astype(np.float32)
+        v64 = np.concatenate([lat[c], sph[c]])
+        if keep_f64:
+            p64[:, c] = v64
+        v = v64.astype(np.float32)
+        del v64
         v[v >= np.float32(box)] = 0.0
         xs.append(v)
     del lat, sph
     m = np.ones(n_total, dtype=np.float32)
     return Particles(xs[0], xs[1], xs[2], m, float(box),
                      f"clustered {lattice_side}^3 lattice + {n_total - n_lat} in {n_spheres} Plummer spheres, "
-                     f"L={box} a={a}")
+                     f"L={box} a={a}", p64)
 
 
 SEED_ENSEMBLE = 0x6E670004

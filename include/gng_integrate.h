@@ -18,11 +18,12 @@
  * and, with wrap != 0, x = rem_euclid(x, L) per component afterwards, as the
  * periodic cosmological engine does after each step
  * (crates/gadget-ng-cli/src/engine/stepping/mod.rs:1572-1576,
- * crates/gadget-ng-core/src/cosmology.rs:759-778).  The force sees positions
- * rounded to f32, as the reference's GPU solvers do; the kick and drift are
- * f64 with separately rounded products and sums (no fused multiply-add), so
- * given the same f32 accelerations a step reproduces the reference's f64
- * update bit for bit.
+ * crates/gadget-ng-core/src/cosmology.rs:759-778).  The PM force reads the
+ * f64 positions (gng_pm_solve_device_soa64: cells formed in FP64, as the
+ * reference's CPU PM does); an attached short-range sum sees them rounded to
+ * f32.  The kick and drift are f64 with separately rounded products and sums
+ * (no fused multiply-add), so given the same f32 accelerations a step
+ * reproduces the reference's f64 update bit for bit.
  */
 #ifndef GNG_INTEGRATE_H
 #define GNG_INTEGRATE_H
@@ -90,11 +91,13 @@ int gng_kdk_set_short_range(gng_kdk *s, gng_sr *sr, double r_split, double r_cut
 int gng_kdk_set_state(gng_kdk *s, const double *pos, const double *vel, const double *mass);
 
 /* Same from device memory (d_pos, d_vel: n x 3 interleaved f64; d_mass: n
- * f64, or NULL for uniform_mass on every particle), enqueued on `stream`
- * (a hipStream_t; NULL = default).  No host copy: initial conditions made
- * with gng_ic_zeldovich_device / gng_ic_2lpt_device go straight into the
- * state.  No reference counterpart (the reference's state lives on the
- * host). */
+ * f64, or NULL for uniform_mass on every particle, which must then be
+ * finite and > 0), enqueued on `stream` (a hipStream_t; NULL = default).  No
+ * host copy: initial conditions made with gng_ic_zeldovich_device /
+ * gng_ic_2lpt_device go straight into the state.  The load is not ordered
+ * against steps in flight on other streams: the caller orders it after them
+ * (as after the initial-condition kernels that wrote d_pos and d_vel).  No
+ * reference counterpart (the reference's state lives on the host). */
 int gng_kdk_set_state_device(gng_kdk *s, const double *d_pos, const double *d_vel, const double *d_mass,
                              double uniform_mass, void *stream);
 

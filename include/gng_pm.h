@@ -29,9 +29,15 @@
  *   - plummer_eps > 0 applies exp(-k^2 eps^2) (fft_poisson.rs:106-122);
  *   - deconvolve != 0 divides the Green's function by W_CIC(k)^2,
  *     W_CIC = prod_alpha sinc^2(pi n_alpha / nm) (an extension; off = reference);
- *   - the CIC density is accumulated in 64-bit fixed point, so it is
- *     bit-identical from run to run.
- * A handle is not reentrant: calls on one handle must not overlap.
+ *   - the cell coordinate c = (x nm / L) rem_euclid nm is formed in FP64 on
+ *     the device (cic.rs:50-52) for FP32 and FP64 positions alike; the
+ *     CIC weights are FP64 and accumulate in 64-bit fixed point, so the
+ *     density is bit-identical from run to run; the FP32 mesh holds the mass
+ *     per cell minus the mean (FFT, Green's function and interpolation FP32).
+ * A handle is not reentrant: calls on one handle must not overlap.  The GPU
+ * work of successive calls on one handle is ordered even across streams: an
+ * entry point that runs on another stream than the previous call's first
+ * waits for that call's work.
  */
 #ifndef GNG_PM_H
 #define GNG_PM_H
@@ -107,8 +113,9 @@ int gng_pm_get_config(const gng_pm_plan *p, gng_pm_config *out);
  * (crates/gadget-ng-core/src/gravity.rs:16-24, PmSolver at
  * crates/gadget-ng-pm/src/solver.rs:74-140): pos is n x 3 f64 interleaved
  * (the layout of Vec3), mass n f64, idx k int64 indices into [0, n) or NULL
- * (then k must equal n), out k x 3 f64.  Host memory; synchronous.  The
- * device computes in FP32 from f32-rounded inputs. */
+ * (then k must equal n), out k x 3 f64.  Host memory; synchronous.  The f64
+ * positions go to the device unrounded; cells and weights are formed in
+ * FP64 there (masses enter the fixed-point deposit as FP32). */
 int gng_pm_accelerations_f64(gng_pm_plan *p, const double *pos, const double *mass, int64_t n,
                              const int64_t *idx, int64_t k, double eps2, double g, double *out);
 
@@ -125,6 +132,20 @@ int gng_pm_solve_device(gng_pm_plan *p, const float *d_x, const float *d_y, cons
                         const float *d_mass, int64_t n, const int64_t *d_idx, int64_t k, double g,
                         float *d_ax, float *d_ay, float *d_az, void *stream);
 
+/* gng_pm_accelerations_f64 on device memory (GravitySolver::accelerations_for_indices,
+ * gravity.rs:16-24, with the Vec3 layouts of solver.rs:74-140): d_pos n x 3
+ * f64 interleaved, d_mass n f64, d_idx k int64 or NULL (k == n), d_out k x 3
+ * f64; enqueued on `stream`, returns without synchronising. */
+int gng_pm_solve_device_f64(gng_pm_plan *p, const double *d_pos, const double *d_mass, int64_t n,
+                            const int64_t *d_idx, int64_t k, double g, double *d_out, void *stream);
+
+/* Device-resident solve on SoA f64 positions (the integrator's particle state,
+ * crates/gadget-ng-integrators/src/leapfrog.rs:5-90 keeps positions in f64),
+ * f32 masses and f32 accelerations; otherwise as gng_pm_solve_device. */
+int gng_pm_solve_device_soa64(gng_pm_plan *p, const double *d_x, const double *d_y, const double *d_z,
+                              const float *d_mass, int64_t n, const int64_t *d_idx, int64_t k, double g,
+                              float *d_ax, float *d_ay, float *d_az, void *stream);
+
 /* Power spectrum of the particles' CIC density on the plan's mesh: mirrors
  * power_spectrum (crates/gadget-ng-analysis/src/power_spectrum.rs:35-127).
  * delta = rho / rho_bar - 1; P(k) = <|delta_k|^2 / W(k)^2> (V / nm^3)^2 over
@@ -140,7 +161,8 @@ int gng_pm_power_spectrum(gng_pm_plan *p, const float *d_x, const float *d_y, co
                           const float *d_mass, int64_t n, int32_t capacity, double *k, double *pk,
                           uint64_t *n_modes, int32_t *n_bins, void *stream);
 
-/* Same from host f64 arrays: pos n x 3 interleaved (Vec3), mass n. */
+/* Same from host f64 arrays: pos n x 3 interleaved (Vec3), mass n; the
+ * positions reach the device unrounded (cells formed in FP64). */
 int gng_pm_power_spectrum_f64(gng_pm_plan *p, const double *pos, const double *mass, int64_t n,
                               int32_t capacity, double *k, double *pk, uint64_t *n_modes,
                               int32_t *n_bins);
@@ -263,11 +285,17 @@ int gng_pm_batch_solve_device(gng_pm_batch *b, const float *d_x, const float *d_
                               const float *d_mass, const int64_t *offsets, double g, float *d_ax,
                               float *d_ay, float *d_az, void *stream);
 
-/* 3-D Hilbert key of (x, y, z), `bits` bits per axis, x most significant in
- * the interleave (Skilling's transform; the curve of
- * crates/gadget-ng-parallel/src/sfc.rs:95-130, used there for domains).  The
- * deposit orders its tiles, and hence the sorted particles, by this key.
- * Pure host function; needs no device. */
+/* 3-D Hilbert key of (x, y, z), `bits` bits per axis: Skilling's
+ * AxesToTranspose (J. Skilling, AIP Conf. Proc. 707, 381 (2004)) with the axes
+ * visited in ascending order and the transposed bits packed from the most
+ * significant level down, x first in each level.  Consecutive keys are face
+ * neighbours at every order.  The deposit orders its tiles, and hence the
+ * sorted particles, by this key.  Pure host function; needs no device.
+ * Analogue of coords_to_hilbert (crates/gadget-ng-parallel/src/sfc.rs:118-183,
+ * used there for MPI domains), which agrees at bits = 1 (the order listed at
+ * sfc.rs:113-116) but not beyond: it visits the axes in reverse (:134) and
+ * packs the least significant level first (:177-181), so its keys at
+ * bits >= 2 are not a face-adjacent curve.  This key does not reproduce it. */
 uint64_t gng_hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int32_t bits);
 
 /* ---- FP64 GPU form --------------------------------------------------------

@@ -165,3 +165,56 @@ def test_hilbert_curve_is_a_face_adjacent_bijection(bits):
     for k in range(n ** 3 - 1):
         a, b = keys[k], keys[k + 1]
         assert sum(abs(u - v) for u, v in zip(a, b)) == 1, (k, a, b)
+
+
+def _reference_coords_to_hilbert(ix, iy, iz, p):
+    """Restatement of the reference's coords_to_hilbert
+    (crates/gadget-ng-parallel/src/sfc.rs:118-183): axes visited in reverse in
+    the AxesToTranspose loop (:134), then Gray encode and the XOR correction,
+    then the transposed bits packed from level k = 0 (least significant)
+    upward (:177-181)."""
+    x = [ix, iy, iz]
+    m = 1 << (p - 1)
+    q = m
+    while q > 1:
+        pm_ = q - 1
+        for i in (2, 1, 0):
+            if x[i] & q:
+                x[0] ^= pm_
+            else:
+                t = (x[0] ^ x[i]) & pm_
+                x[0] ^= t
+                x[i] ^= t
+        q >>= 1
+    for i in (1, 2):
+        x[i] ^= x[i - 1]
+    t, q = 0, m
+    while q > 1:
+        if x[2] & q:
+            t ^= q - 1
+        q >>= 1
+    x = [v ^ t for v in x]
+    h = 0
+    for k in range(p):
+        for v in x:
+            h = (h << 1) | ((v >> k) & 1)
+    return h
+
+
+def test_reference_hilbert_key_diverges_beyond_first_order():
+    """The reference curve equals this key at p = 1 and differs beyond it;
+    its own order at p >= 2 is not face-adjacent (a reference defect, recorded
+    so that nobody takes the two keys as interchangeable)."""
+    order = [(0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), (1, 0, 0)]
+    assert [_reference_coords_to_hilbert(*c, 1) for c in order] == list(range(8))
+    for p, differ in ((2, 59), (3, 493)):
+        n = 1 << p
+        cells = [(x, y, z) for z in range(n) for y in range(n) for x in range(n)]
+        ref = {c: _reference_coords_to_hilbert(*c, p) for c in cells}
+        ours = {c: pm.hilbert_key3(*c, p) for c in cells}
+        assert sorted(ref.values()) == list(range(n ** 3))  # still a bijection
+        assert sum(ref[c] != ours[c] for c in cells) == differ
+        inv = {v: c for c, v in ref.items()}
+        jumps = sum(1 for k in range(n ** 3 - 1)
+                    if sum(abs(a - b) for a, b in zip(inv[k], inv[k + 1])) != 1)
+        assert jumps > 0

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def headline():
     from gadget_ng_pm_hip import workloads
-    return workloads.zeldovich_lattice(256, 512, 256.0, workers=-1)
+    return workloads.zeldovich_lattice(256, 512, 256.0, workers=-1, keep_f64=True)
 
 
 def test_headline_parity_with_fp64(pm, oracle_mod, headline):
@@ -31,6 +31,23 @@ def test_headline_parity_with_fp64(pm, oracle_mod, headline):
     assert s.max() <= 1e-5 * np.abs(got).sum(axis=0).max()
     # bitwise reproducible
     again = plan.accelerations_f32(p.x, p.y, p.z, p.m, 1.0).T.astype(np.float64)
+    assert np.array_equal(got, again)
+
+
+def test_headline_unrounded_f64_parity(pm, oracle_mod, headline):
+    """The same workload on its unrounded f64 positions through the f64 entry
+    point: rounding to f32 alone costs about 2e-5 rms here."""
+    p = headline
+    print(f"case headline f64 N={p.n} nm=512", flush=True)
+    plan = pm.Plan(512, p.box)
+    m = p.mass64()
+    got = plan.accelerations_f64(p.p64, m, 1.0)
+    ref = oracle_mod.pm_accelerations(p.p64, m, p.box, 512, 1.0)
+    r = rms_rel(got, ref)
+    print(f"f64 path rms {r:.3e}, max {max_rel_robust(got, ref):.3e}", flush=True)
+    assert r <= 1e-5
+    assert max_rel_robust(got, ref) <= 1e-3
+    again = plan.accelerations_f64(p.p64, m, 1.0)
     assert np.array_equal(got, again)
 
 

@@ -31,17 +31,14 @@ def test_gpu_ics_and_pk_against_the_reference(pm, fields, name, rescale):  # noq
     r = np.abs(ipk / opk - 1.0)
     print(f"GPU particles vs oracle particles: max rel {r.max():.2e} in bin {int(r.argmax())}", flush=True)
     assert r.max() < 1e-5
-    # the GPU estimator on the same particles.  It takes f32 positions, as the
-    # force does; for the rescaled set (displacements ~1e-4 of the box, the
-    # particles almost on the lattice) the rounding of q + Psi to f32 is a
-    # 4e-4 relative displacement error and swamps the top bins, so there the
-    # GPU estimator is only reported.
+    # the GPU estimator on the same f64 particles: cells formed in FP64 on the
+    # device, so the rescaled set (displacements ~1e-4 of the box, the
+    # particles almost on the lattice) keeps its contrast
     k, pk, modes = analysis.power_spectrum_arrays(plan, pos, mass)
     assert np.array_equal(modes.astype(np.int64), om) and np.allclose(k, ok, rtol=1e-12)
     r = np.abs(pk / ipk - 1.0)
     print(f"GPU estimator vs oracle estimator: max rel {r.max():.2e} in bin {int(r.argmax())}", flush=True)
-    if not rescale:
-        assert r.max() < 2e-3  # 6.4e-4 measured
+    assert r.max() < 1e-4
     _, rp, rm = _ref(name)
     d = ipk / rp - 1.0
     w = rm / 4.0

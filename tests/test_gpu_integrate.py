@@ -4,7 +4,9 @@ integrator oracle.
 Given the accelerations the GPU force produced, a step must reproduce the
 oracle's f64 update bit for bit (the kernels round every product and sum
 separately, as the reference does).  The PM force is a deterministic function
-of the f32 positions, so the oracle can take it from the same plan.  Over
+of the f64 positions (the integrator solves on them through the SoA f64 entry
+point, which gives the host f64 entry point's result bit for bit), so the
+oracle can take it from the same plan.  Over
 several steps the trajectory is also compared with the oracle driven by the
 FP64 CPU PM."""
 import numpy as np
@@ -30,12 +32,10 @@ def _box(n_side, box, seed, vscale=0.05):
 
 
 def _gpu_force(plan, mass):
-    m32 = np.asarray(mass, dtype=np.float32)
+    m = np.asarray(mass, dtype=np.float64)
 
     def force(p, g):
-        p32 = np.asarray(p, dtype=np.float32)
-        ax, ay, az = plan.accelerations_f32(p32[:, 0].copy(), p32[:, 1].copy(), p32[:, 2].copy(), m32, g)
-        return np.stack([ax, ay, az], axis=1).astype(np.float64)
+        return plan.accelerations_f64(np.asarray(p, dtype=np.float64), m, g)
     return force
 
 
@@ -102,8 +102,9 @@ def test_run_equals_steps_with_host_factors(pm):
 
 
 def test_treepm_step_is_bit_identical_to_the_oracle(pm):
-    """TreePM: PM long range plus the short range, added in f32; both halves
-    are deterministic functions of the f32 positions."""
+    """TreePM: PM long range plus the short range, added in f32; the long
+    range is a deterministic function of the f64 positions, the short range
+    of their f32 rounding."""
     from gadget_ng_pm_hip import integrate as gi
     from gadget_ng_pm_hip import learned
     box, nm = 32.0, 32
@@ -142,7 +143,7 @@ def test_trajectory_follows_the_fp64_pm_oracle(pm, oracle_mod):
     p8, v8, _ = lf.state()
 
     def f64_force(p, g):
-        return oracle_mod.pm_accelerations(f32_round(p), f32_round(mass), box, nm, g=g)
+        return oracle_mod.pm_accelerations(p, f32_round(mass), box, nm, g=g)
     rp, rv, _, _ = io.run(pos, vel, 8, 0.05, 40.0, f64_force, cosmology=LCDM, a=0.3, box=box)
     dp = np.abs((p8 - rp + box / 2) % box - box / 2)
     moved = np.abs((rp - pos + box / 2) % box - box / 2).max()
