@@ -175,6 +175,9 @@ def main() -> int:
     ap.add_argument("--train-boxes", type=int, default=200, help="learned: training seeds 1..N (64^3 boxes)")
     ap.add_argument("--train-steps", type=int, default=2000, help="learned: Adam steps")
     ap.add_argument("--eval-boxes", type=int, default=2, help="learned: held-out 256^3 seeds from 1001")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="time the f64 entry point only (no f32 entry, host entry points or copy rate): for "
+                         "profiler runs, whose per-solve figures need one kind of solve")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a kernel of known traffic for scaling "
                          "profiler byte counters)")
@@ -277,6 +280,12 @@ def main() -> int:
                 for k, v in stages.items()}
     b_alg = sum(g["algorithmic_bytes"] for g in groups.values())
     pipeline_gbps = b_alg / (gpu_ms * 1e-3) / 1e9
+
+    if args.only_headline:
+        print(json.dumps({"metric": "profiler run", "value": round(value, 2), "unit": "M particle-forces/s",
+                          "ms_per_step": round(ms_per_step, 4), "roofline": roofline}), flush=True)
+        plan.close()
+        return 0
 
     # the f32 SoA entry point (the hip_pm_solve data layout) on the rounded positions
     x, y, z, m = (pm.DeviceArray.from_host(a) for a in (parts.x, parts.y, parts.z, parts.m))

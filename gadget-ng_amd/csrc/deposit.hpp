@@ -102,14 +102,15 @@ __global__ __launch_bounds__(256) void k_bucket_count(const PosSrc<T> src, const
     const bool live = i < n;
     b[k] = kEmptyKey;
     if (live) {
+      const long long o = i * src.stride;
+      const double px = (double)src.x[o], py = (double)src.y[o], pz = (double)src.z[o];
+      am = fmaxf(am, fabsf((float)m[i]));
       int ix, iy, iz;
       unsigned long long f;
-      const long long o = i * src.stride;
-      cell_fix((double)src.x[o], cm, ix, f);
-      cell_fix((double)src.y[o], cm, iy, f);
-      cell_fix((double)src.z[o], cm, iz, f);
+      cell_fix(px, cm, ix, f);
+      cell_fix(py, cm, iy, f);
+      cell_fix(pz, cm, iz, f);
       b[k] = bucket_of_cell(ix, iy, iz, tm);
-      am = fmaxf(am, fabsf((float)m[i]));
     }
     // runs of equal buckets: a lane heads a run when its bucket differs from the previous lane's
     const unsigned prev = __shfl_up(b[k], 1);
@@ -249,14 +250,17 @@ __global__ __launch_bounds__(256) void k_scatter(const PosSrc<T> src, const M* _
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long q = 0;
   if (i < n) {
-    int c[3];
-    unsigned long long f[3];
+    // every load first (the slot's offset depends on the bucket), so the
+    // memory round trips overlap instead of following each coordinate's math
     const long long o = i * src.stride;
-    cell_fix((double)src.x[o], cm, c[0], f[0]);
-    cell_fix((double)src.y[o], cm, c[1], f[1]);
-    cell_fix((double)src.z[o], cm, c[2], f[2]);
+    const double px = (double)src.x[o], py = (double)src.y[o], pz = (double)src.z[o];
     const float mf = (float)m[i];
     const unsigned slot = offset[bucket[i]] + rank[i];
+    int c[3];
+    unsigned long long f[3];
+    cell_fix(px, cm, c[0], f[0]);
+    cell_fix(py, cm, c[1], f[1]);
+    cell_fix(pz, cm, c[2], f[2]);
     sorted[slot] = encode_rec(c, f, mf);
     if (perm) perm[slot] = (unsigned)i;  // only for the sorted-order interpolation
     q = __double2ll_rn(ldexp((double)mf, fixed_shift(sc, n)));
@@ -267,7 +271,7 @@ __global__ __launch_bounds__(256) void k_scatter(const PosSrc<T> src, const M* _
   __syncthreads();
   if (threadIdx.x == 0) {
     const long long t = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
-    if (t) atomicAdd(&sc->qsum, (unsigned long long)t);
+    if (t) atomicAdd(&sc->qsum[(blockIdx.x % kQSlots) * kQStride], (unsigned long long)t);
   }
 }
 
@@ -439,7 +443,7 @@ __global__ __launch_bounds__(NT) void k_deposit_tiles(const PartRec* __restrict_
   __syncthreads();
   const double unit = ldexp(1.0, -s);
   // the integer part of the mean fixed-point mass per cell
-  const long long qmean = (long long)(sc->qsum / (unsigned long long)((long long)cm.nm * cm.nm * cm.nm));
+  const long long qmean = (long long)(qsum_total(sc) / (unsigned long long)((long long)cm.nm * cm.nm * cm.nm));
   for (int c = tid; c < T3; c += NT) {
     const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
     const size_t row = (size_t)(uy * T + ly) * cm.nm + (ux * T + lx);

@@ -81,7 +81,7 @@ constexpr unsigned long long kFracOne = 1ull << kFracBits;
 __device__ __forceinline__ void cell_fix(double x, const CellMap cm, int& i, unsigned long long& frac) {
   const double nmf = (double)cm.nm;
   double c = x * cm.inv_cell;
-  if (!(c >= 0.0 && c < nmf)) {
+  if (__builtin_expect(!(c >= 0.0 && c < nmf), 0)) {
     c = fmod(c, nmf);
     if (c < 0.0) c += nmf;
   }
@@ -149,12 +149,24 @@ __device__ __forceinline__ long long mesh_row(long long r) {
 }
 
 // Per-solve scalars kept on the device (no host round trip).
+// The sum of round(m 2^s) over the particles is kept in kQSlots partial
+// sums, one 128-byte line apart: blocks add to slot blockIdx % kQSlots, so
+// tens of thousands of blocks do not serialise on one address.
+constexpr int kQSlots = 64;
+constexpr int kQStride = 16;  // 64-bit words between slots
 struct SolveScalars {
   unsigned int mmax_bits;  // max |m| as float bits (non-negative floats order as integers)
-  unsigned int pad0;
-  unsigned long long qsum;  // sum of round(m 2^s) over the particles
-  unsigned int pad[60];
+  unsigned int pad[63];
+  unsigned long long qsum[kQSlots * kQStride];
 };
+
+// The integer sum of the partial mass sums (the order does not matter).
+__device__ __forceinline__ unsigned long long qsum_total(const SolveScalars* sc) {
+  unsigned long long t = 0ull;
+#pragma unroll 8
+  for (int k = 0; k < kQSlots; ++k) t += sc->qsum[k * kQStride];
+  return t;
+}
 
 // Fixed-point exponent: contributions are w*m*2^s with N*max|m|*2^s < 2^62,
 // so no cell sum can overflow int64 whatever the clustering.
@@ -166,10 +178,8 @@ __device__ __forceinline__ int fixed_shift(const SolveScalars* sc, long long nto
 }
 
 __global__ void k_reset_scalars(SolveScalars* sc) {
-  if (threadIdx.x == 0) {
-    sc->mmax_bits = 0u;
-    sc->qsum = 0ull;
-  }
+  if (threadIdx.x == 0) sc->mmax_bits = 0u;
+  if (threadIdx.x < kQSlots) sc->qsum[threadIdx.x * kQStride] = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -874,11 +884,12 @@ __global__ __launch_bounds__(256) void k_interp(const PosSrc<T> src, const long 
   const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= k) return;
   const long long i = (idx ? idx[j] : j) * src.stride;
+  const double px = (double)src.x[i], py = (double)src.y[i], pz = (double)src.z[i];
   int ix, iy, iz;
   float dx, dy, dz;
-  cell_frac32((double)src.x[i], cm, ix, dx);
-  cell_frac32((double)src.y[i], cm, iy, dy);
-  cell_frac32((double)src.z[i], cm, iz, dz);
+  cell_frac32(px, cm, ix, dx);
+  cell_frac32(py, cm, iy, dy);
+  cell_frac32(pz, cm, iz, dz);
   const float3 a = cic_gather(ix, iy, iz, dx, dy, dz, cm.nm, fplane, fx, fy, fz);
   const long long o = j * out.stride;
   out.x[o] = (O)a.x;
