@@ -208,6 +208,69 @@ __global__ void k_synth_particles(float* __restrict__ x, float* __restrict__ y, 
   m[i] = 1.0f;
 }
 
+// 3-D Hilbert key of integer coordinates with `bits` bits each: Skilling's
+// transform of the axes to the "transposed" Hilbert index (J. Skilling,
+// Programming the Hilbert curve, AIP Conf. Proc. 707, 381 (2004)), then the
+// bits interleaved with x most significant.  For bits = 1 the curve visits
+// (0,0,0) (0,0,1) (0,1,1) (0,1,0) (1,1,0) (1,1,1) (1,0,1) (1,0,0).
+__host__ __device__ uint64_t hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int bits) {
+  uint32_t X[3] = {x, y, z};
+  if (bits <= 0) return 0;
+  const uint32_t M = 1u << (bits - 1);
+  for (uint32_t Q = M; Q > 1; Q >>= 1) {
+    const uint32_t P = Q - 1;
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        const uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+  uint32_t t = 0;
+  for (uint32_t Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  for (int i = 0; i < 3; ++i) X[i] ^= t;
+  uint64_t key = 0;
+  for (int b = bits - 1; b >= 0; --b)
+    for (int i = 0; i < 3; ++i) key = (key << 1) | ((X[i] >> b) & 1u);
+  return key;
+}
+
+
+// The plan's tables, computed on the device (plan creation and the
+// allocation-free entry point share this kernel, so both use the same
+// values): tw[m] = exp(-2 pi i m / nm); dc[m] = 1 / sinc^4(pi f / nm) at the
+// signed frequency f of index m (the CIC deconvolution); with rank/order,
+// the Hilbert rank of every deposit tile and its inverse (nt^3 tiles, by
+// counting the smaller keys: for small tile counts only).  One workgroup.
+__global__ __launch_bounds__(256) void k_plan_tables(int nm, int nt, int bits, float2* __restrict__ tw,
+                                                     float* __restrict__ dc, unsigned* __restrict__ rank,
+                                                     unsigned* __restrict__ order) {
+  for (int m = threadIdx.x; m < nm; m += blockDim.x) {
+    double sn, cs;
+    sincospi(2.0 * (double)m / (double)nm, &sn, &cs);
+    tw[m] = make_float2((float)cs, (float)(-sn));
+    const int f = m <= nm / 2 ? m : m - nm;
+    const double xx = M_PI * (double)f / (double)nm;
+    const double sc = f == 0 ? 1.0 : sin(xx) / xx;
+    dc[m] = (float)(1.0 / (sc * sc * sc * sc));
+  }
+  if (!rank) return;
+  const int nt3 = nt * nt * nt;
+  for (int t = threadIdx.x; t < nt3; t += blockDim.x) {
+    const uint64_t kt = hilbert_key3((uint32_t)(t % nt), (uint32_t)((t / nt) % nt), (uint32_t)(t / (nt * nt)), bits);
+    unsigned r = 0;
+    for (int u = 0; u < nt3; ++u)
+      r += hilbert_key3((uint32_t)(u % nt), (uint32_t)((u / nt) % nt), (uint32_t)(u / (nt * nt)), bits) < kt ? 1u : 0u;
+    rank[t] = r;
+    order[r] = (unsigned)t;
+  }
+}
+
 struct MeshOps {
   int nm;
   int x_rows;
@@ -1106,38 +1169,6 @@ int pk_bins(const gng_pm_plan* p, const double* sums, const unsigned long long* 
   return GNG_OK;
 }
 
-// 3-D Hilbert key of integer coordinates with `bits` bits each: Skilling's
-// transform of the axes to the "transposed" Hilbert index (J. Skilling,
-// Programming the Hilbert curve, AIP Conf. Proc. 707, 381 (2004)), then the
-// bits interleaved with x most significant.  For bits = 1 the curve visits
-// (0,0,0) (0,0,1) (0,1,1) (0,1,0) (1,1,0) (1,1,1) (1,0,1) (1,0,0).
-uint64_t hilbert_key3(uint32_t x, uint32_t y, uint32_t z, int bits) {
-  uint32_t X[3] = {x, y, z};
-  if (bits <= 0) return 0;
-  const uint32_t M = 1u << (bits - 1);
-  for (uint32_t Q = M; Q > 1; Q >>= 1) {
-    const uint32_t P = Q - 1;
-    for (int i = 0; i < 3; ++i) {
-      if (X[i] & Q) {
-        X[0] ^= P;
-      } else {
-        const uint32_t t = (X[0] ^ X[i]) & P;
-        X[0] ^= t;
-        X[i] ^= t;
-      }
-    }
-  }
-  for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
-  uint32_t t = 0;
-  for (uint32_t Q = M; Q > 1; Q >>= 1)
-    if (X[2] & Q) t ^= Q - 1;
-  for (int i = 0; i < 3; ++i) X[i] ^= t;
-  uint64_t key = 0;
-  for (int b = bits - 1; b >= 0; --b)
-    for (int i = 0; i < 3; ++i) key = (key << 1) | ((X[i] >> b) & 1u);
-  return key;
-}
-
 }  // namespace
 
 extern "C" {
@@ -1238,19 +1269,13 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
     p->tm.rank = p->d_tile_rank;
     p->tm.order = p->d_tile_order;
   }
-  std::vector<float2> tw(nm);
-  std::vector<float> dc(nm);
-  for (int m = 0; m < nm; ++m) {
-    const long double a = 6.283185307179586476925286766559005768L * (long double)m / (long double)nm;
-    tw[m] = make_float2((float)cosl(a), (float)(-sinl(a)));
-    const int f = m <= nm / 2 ? m : m - nm;
-    const double xx = M_PI * (double)f / (double)nm;
-    const double s = f == 0 ? 1.0 : std::sin(xx) / xx;
-    dc[m] = (float)(1.0 / (s * s * s * s));
+  hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_plan_tables, dim3(1), dim3(256), 0, p->stream, nm, 0, 0, p->d_tw, p->d_deconv, nullptr,
+                       nullptr);
+    e = hipGetLastError();
   }
-  hipError_t e = hipMemcpy(p->d_tw, tw.data(), sizeof(float2) * nm, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(p->d_deconv, dc.data(), sizeof(float) * nm, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&p->done_ev, hipEventDisableTiming);
   for (int j = 0; j < kTimingSlots && e == hipSuccess; ++j)
     for (int i = 0; i <= kStages && e == hipSuccess; ++i) e = hipEventCreate(&p->ev[j][i]);
@@ -1943,6 +1968,126 @@ int gng_dev_axpy(float* d_y, const float* d_x, int64_t n, float a, void* stream)
                      d_y, d_x, (long long)n, a);
   GNG_HIP(hipGetLastError());
   return GNG_OK;
+}
+
+// ---- allocation-free solve ------------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+// Every work buffer of gng_pm_solve_device_noalloc lives in this one static
+// device array (one copy per GPU, held until the process ends).  Host code
+// never passes its name to a kernel or a copy: the array's device address
+// comes from hipGetSymbolAddress, once per device.
+constexpr int kNoAllocMaxMesh = GNG_PM_NOALLOC_MAX_GRID;
+constexpr long long kNoAllocMaxParticles = GNG_PM_NOALLOC_MAX_PARTICLES;
+constexpr size_t kNoAllocBytes = GNG_PM_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_device(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(GNG_ERR_INVALID, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(GNG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return GNG_OK;
+}
+
+// Bump allocation of 256-byte aligned regions inside the scratch.
+struct Carve {
+  unsigned char* base;
+  size_t used = 0;
+  template <typename T>
+  T* take(size_t count) {
+    T* q = reinterpret_cast<T*>(base + used);
+    used += (sizeof(T) * count + 255) / 256 * 256;
+    return q;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int gng_pm_solve_device_noalloc(const gng_pm_config* cfg, const float* d_x, const float* d_y, const float* d_z,
+                                const float* d_mass, int64_t n, const int64_t* d_idx, int64_t k, double g,
+                                float* d_ax, float* d_ay, float* d_az, void* stream) {
+  if (!cfg) return fail(GNG_ERR_INVALID, "null config");
+  if (!(cfg->box_size > 0.0) || !std::isfinite(cfg->box_size)) return fail(GNG_ERR_INVALID, "box_size must be > 0");
+  const int nm = cfg->grid_size;
+  const MeshOps* ops = find_ops(nm);
+  if (!ops || nm > kNoAllocMaxMesh)
+    return fail(GNG_ERR_INVALID, "the allocation-free solve takes a compiled grid_size of at most " +
+                                     std::to_string(kNoAllocMaxMesh));
+  if (n < 0 || k < 0) return fail(GNG_ERR_INVALID, "negative particle count");
+  if (n > kNoAllocMaxParticles)
+    return fail(GNG_ERR_INVALID, "the allocation-free solve takes at most " + std::to_string(kNoAllocMaxParticles) +
+                                     " particles");
+  if (!d_idx && k != n) return fail(GNG_ERR_INVALID, "idx is NULL but k != n");
+  if (k >= (1LL << 31)) return fail(GNG_ERR_INVALID, "k of 2^31 or more is not supported");
+  if (n > 0 && (!d_x || !d_y || !d_z || !d_mass)) return fail(GNG_ERR_INVALID, "null input array");
+  if (k > 0 && (!d_ax || !d_ay || !d_az)) return fail(GNG_ERR_INVALID, "null output array");
+  int dev = cfg->device;
+  if (dev >= 0) {
+    GNG_HIP(hipSetDevice(dev));
+  } else {
+    GNG_HIP(hipGetDevice(&dev));
+  }
+  unsigned char* base = nullptr;
+  int rc = scratch_for_device(dev, &base);
+  if (rc) return rc;
+  // a plan that lives on the host stack and points into the scratch; its
+  // work capacity is the particle limit, so the pipeline allocates nothing
+  gng_pm_plan p;
+  p.cfg = *cfg;
+  p.ops = ops;
+  p.device = dev;
+  p.tune = default_tuning(nm);
+  p.tm.T = ops->tile;
+  p.tm.nt = nm / ops->tile;
+  p.nb = (long long)p.tm.nt * p.tm.nt * p.tm.nt * 8;
+  p.scan_blocks = (int)((p.nb + kScanBlock - 1) / kScanBlock);
+  p.nm3 = (long long)nm * nm * nm;
+  const size_t half = (size_t)nm * (size_t)mesh_plane(nm);
+  Carve c{base};
+  p.d_P = c.take<float2>(half);
+  p.d_A = c.take<float2>(half);
+  p.d_B = c.take<float2>(half);
+  p.d_tw = c.take<float2>((size_t)nm);
+  p.d_deconv = c.take<float>((size_t)nm);
+  p.d_sc = c.take<SolveScalars>(1);
+  p.d_count = c.take<unsigned>((size_t)((p.nb + 3) / 4 * 4));
+  p.d_offset = c.take<unsigned>((size_t)(p.nb + 1));
+  p.d_partial = c.take<unsigned>((size_t)p.scan_blocks);
+  p.d_tile_rank = c.take<unsigned>((size_t)(p.nb / 8));
+  p.d_tile_order = c.take<unsigned>((size_t)(p.nb / 8));
+  p.d_sorted = c.take<PartRec>((size_t)kNoAllocMaxParticles);
+  p.d_bucket = c.take<unsigned>((size_t)kNoAllocMaxParticles);
+  p.d_rank = c.take<unsigned>((size_t)kNoAllocMaxParticles);
+  if (c.used > kNoAllocBytes) return fail(GNG_ERR_INVALID, "allocation-free scratch layout exceeds its size");
+  p.work_cap = kNoAllocMaxParticles;
+  p.tm.rank = p.d_tile_rank;
+  p.tm.order = p.d_tile_order;
+  int bits = 0;
+  while ((1 << bits) < p.tm.nt) ++bits;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // the tables are rebuilt on every call: the scratch keeps whatever the
+  // last call on this GPU (any mesh size) left in it
+  hipLaunchKernelGGL(k_plan_tables, dim3(1), dim3(256), 0, s, nm, p.tm.nt, bits, p.d_tw, p.d_deconv, p.d_tile_rank,
+                     p.d_tile_order);
+  GNG_HIP(hipGetLastError());
+  return run_pipeline(&p, soa32(d_x, d_y, d_z, d_mass), n, reinterpret_cast<const long long*>(d_idx), k, g,
+                      acc32(d_ax, d_ay, d_az), s);
 }
 
 // ---- reference-compatible entry points ----------------------------------

@@ -136,6 +136,8 @@ def lib():
         L.gng_pm_solve_device_f64.restype = i32
         L.gng_pm_solve_device_soa64.argtypes = [P, P, P, P, P, i64, P, i64, d, P, P, P, P]
         L.gng_pm_solve_device_soa64.restype = i32
+        L.gng_pm_solve_device_noalloc.argtypes = [ctypes.POINTER(GngConfig), P, P, P, P, i64, P, i64, d, P, P, P, P]
+        L.gng_pm_solve_device_noalloc.restype = i32
         L.gng_pm_density.argtypes = [P, P]
         L.gng_pm_keep_density.argtypes = [P, i32]
         L.gng_pm_force_meshes.argtypes = [P, P]
@@ -522,6 +524,31 @@ class Plan64:
                                          stream or None)
         if rc != GNG_OK:
             raise HipExecutionError("gng_pm64_solve_device", rc, last_error())
+
+
+NOALLOC_MAX_GRID = 64
+NOALLOC_MAX_PARTICLES = 1 << 19
+
+
+def solve_device_noalloc(grid_size: int, box_size: float, x_ptr: int, y_ptr: int, z_ptr: int, m_ptr: int, n: int,
+                         ax_ptr: int, ay_ptr: int, az_ptr: int, g: float, r_split: float = 0.0,
+                         plummer_eps: float = 0.0, deconvolve: bool = False, device: int = -1, stream: int = 0,
+                         idx_ptr: Optional[int] = None, k: Optional[int] = None) -> None:
+    """gng_pm_solve_device_noalloc: a plan-free solve on device pointers whose
+    work buffers live in the library's static per-GPU scratch (grid_size <=
+    64, n <= 2^19); calls on one GPU must not overlap."""
+    cfg = GngConfig()
+    cfg.grid_size = int(grid_size)
+    cfg.deconvolve = int(bool(deconvolve))
+    cfg.box_size = float(box_size)
+    cfg.r_split = float(r_split)
+    cfg.plummer_eps = float(plummer_eps)
+    cfg.device = int(device)
+    k = n if k is None else k
+    rc = lib().gng_pm_solve_device_noalloc(ctypes.byref(cfg), x_ptr, y_ptr, z_ptr, m_ptr, int(n), idx_ptr, int(k),
+                                           float(g), ax_ptr, ay_ptr, az_ptr, stream or None)
+    if rc != GNG_OK:
+        raise HipExecutionError("gng_pm_solve_device_noalloc", rc, last_error())
 
 
 class DeviceArray:

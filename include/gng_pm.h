@@ -285,6 +285,27 @@ int gng_pm_batch_solve_device(gng_pm_batch *b, const float *d_x, const float *d_
                               const float *d_mass, const int64_t *offsets, double g, float *d_ax,
                               float *d_ay, float *d_az, void *stream);
 
+/* ---- allocation-free solve ------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  gng_pm_solve_device without a plan: the meshes, tables
+ * and sort buffers live in one static device array of
+ * GNG_PM_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends); the call
+ * makes no hipMalloc, hipFree or other allocating call.  Limits: a compiled
+ * grid_size of at most GNG_PM_NOALLOC_MAX_GRID and at most
+ * GNG_PM_NOALLOC_MAX_PARTICLES particles (larger inputs return
+ * GNG_ERR_INVALID).  Same semantics, inputs and results as
+ * gng_pm_solve_device on a plan of *cfg with default launch choices; the
+ * device is cfg->device (-1: current).  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other threads)
+ * race, so the caller serialises them (e.g. one stream). */
+#define GNG_PM_NOALLOC_MAX_GRID 64
+#define GNG_PM_NOALLOC_MAX_PARTICLES (1LL << 19)
+#define GNG_PM_NOALLOC_SCRATCH_BYTES (20u << 20)
+int gng_pm_solve_device_noalloc(const gng_pm_config *cfg, const float *d_x, const float *d_y, const float *d_z,
+                                const float *d_mass, int64_t n, const int64_t *d_idx, int64_t k, double g,
+                                float *d_ax, float *d_ay, float *d_az, void *stream);
+
 /* 3-D Hilbert key of (x, y, z), `bits` bits per axis: Skilling's
  * AxesToTranspose (J. Skilling, AIP Conf. Proc. 707, 381 (2004)) with the axes
  * visited in ascending order and the transposed bits packed from the most
