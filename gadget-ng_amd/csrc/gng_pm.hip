@@ -24,6 +24,7 @@
 #include <utility>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gng_ic.h"
@@ -449,6 +450,12 @@ struct gng_pm_plan {
   hipEvent_t done_ev = nullptr;
   hipStream_t done_stream = nullptr;
   bool done_valid = false;
+  // Host staging of the f64 host entry points: per worker thread a stream,
+  // two pinned slots and their events (allocated on first use).
+  int pipe_threads = 0;
+  hipStream_t pipe_stream[8] = {};
+  hipEvent_t pipe_ev[8][2] = {};
+  unsigned char* pipe_slot[8][2] = {};
   long long idx_cap = 0;
   long long* d_idx = nullptr;
   hipStream_t stream = nullptr;
@@ -520,6 +527,13 @@ void free_plan(gng_pm_plan* p) {
                   (void*)p->d_ic_sigma, (void*)p->d_ic_mask, (void*)p->d_part64})
     (void)hipFree(q);
   if (p->done_ev) (void)hipEventDestroy(p->done_ev);
+  for (int t = 0; t < 8; ++t) {
+    for (int k = 0; k < 2; ++k) {
+      if (p->pipe_ev[t][k]) (void)hipEventDestroy(p->pipe_ev[t][k]);
+      if (p->pipe_slot[t][k]) (void)hipHostFree(p->pipe_slot[t][k]);
+    }
+    if (p->pipe_stream[t]) (void)hipStreamDestroy(p->pipe_stream[t]);
+  }
   free_work(p);
   for (auto& slot : p->ev)
     for (auto& e : slot)
@@ -856,6 +870,149 @@ int ensure_particles(gng_pm_plan* p, long long n, long long k, bool need_idx) {
   return need_idx ? ensure_idx(p, k) : GNG_OK;
 }
 
+// ---- host staging: pageable caller memory <-> device through pinned slots ----
+//
+// The f64 host entry points move the caller's pageable arrays with
+// pipe_threads host threads.  Each owns a stream and two pinned slots of
+// kStageBytes: it fills a slot (copying, or converting f64 masses to f32),
+// starts the slot's DMA on its stream, and reuses the slot once that DMA's
+// event has completed.  Downloads run the other way (DMA into a slot, then
+// the thread converts f32 accelerations to f64 into the caller's array), two
+// slots in flight per thread.  The compute stream waits for every upload
+// stream, and every download stream waits for the compute stream.
+constexpr size_t kStageBytes = 4u << 20;
+
+int ensure_pipe(gng_pm_plan* p) {
+  if (p->pipe_threads) return GNG_OK;
+  unsigned hw = std::thread::hardware_concurrency();
+  const int T = (int)std::max(1u, std::min(8u, hw ? hw : 1u));
+  for (int t = 0; t < T; ++t) {
+    GNG_HIP(hipStreamCreateWithFlags(&p->pipe_stream[t], hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+      GNG_HIP(hipEventCreateWithFlags(&p->pipe_ev[t][k], hipEventDisableTiming));
+      GNG_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->pipe_slot[t][k]), kStageBytes, hipHostMallocDefault));
+    }
+  }
+  p->pipe_threads = T;
+  return GNG_OK;
+}
+
+// One transfer: kind 0 copies `count` bytes host -> device; kind 1 converts
+// `count` doubles host -> floats on the device; kind 2 converts `count`
+// floats on the device -> doubles host; kind 3 copies `count` bytes device
+// -> host.
+struct Xfer {
+  void* dev;
+  const void* src;
+  void* dst;
+  size_t count;
+  int kind;
+};
+
+// Splits the transfers into slot-sized chunks and runs them on the worker
+// threads; for downloads, `after` (recorded on the compute stream) gates
+// every DMA; for uploads, s waits for every worker stream when they are done.
+int run_xfers(gng_pm_plan* p, const std::vector<Xfer>& xs, hipStream_t s, hipEvent_t after) {
+  struct Chunk {
+    const Xfer* x;
+    size_t off, n;  // in elements of the transfer (bytes for kind 0)
+  };
+  std::vector<Chunk> chunks;
+  for (const Xfer& x : xs) {
+    const size_t per = (x.kind == 0 || x.kind == 3) ? kStageBytes : kStageBytes / 8;
+    for (size_t o = 0; o < x.count; o += per) chunks.push_back({&x, o, std::min(per, x.count - o)});
+  }
+  if (chunks.empty()) return GNG_OK;
+  const int T = std::min<int>(p->pipe_threads, (int)chunks.size());
+  std::vector<hipError_t> err((size_t)T, hipSuccess);
+  auto work = [&](int t) {
+    hipError_t e = hipSetDevice(p->device);
+    hipStream_t st = p->pipe_stream[t];
+    if (e == hipSuccess && after) e = hipStreamWaitEvent(st, after, 0);
+    bool used[2] = {false, false};
+    int k = 0;
+    std::vector<size_t> mine;
+    for (size_t c = (size_t)t; c < chunks.size(); c += (size_t)T) mine.push_back(c);
+    const bool down = !mine.empty() && chunks[mine[0]].x->kind >= 2;
+    auto issue_down = [&](size_t c, int slot) {
+      const Chunk& ch = chunks[c];
+      const bool raw = ch.x->kind == 3;
+      const void* d = raw ? (const void*)(static_cast<const unsigned char*>(ch.x->dev) + ch.off)
+                          : (const void*)(static_cast<const float*>(ch.x->dev) + ch.off);
+      hipError_t r = hipMemcpyAsync(p->pipe_slot[t][slot], d, raw ? ch.n : ch.n * sizeof(float),
+                                    hipMemcpyDeviceToHost, st);
+      if (r == hipSuccess) r = hipEventRecord(p->pipe_ev[t][slot], st);
+      return r;
+    };
+    if (down) {
+      // two DMAs in flight: chunk i + 1 goes out while chunk i is converted
+      if (e == hipSuccess) e = issue_down(mine[0], 0);
+      for (size_t i = 0; i < mine.size() && e == hipSuccess; ++i) {
+        const int slot = (int)(i & 1);
+        if (i + 1 < mine.size()) e = issue_down(mine[i + 1], slot ^ 1);
+        if (e == hipSuccess) e = hipEventSynchronize(p->pipe_ev[t][slot]);
+        if (e != hipSuccess) break;
+        const Chunk& ch = chunks[mine[i]];
+        if (ch.x->kind == 3) {
+          std::memcpy(static_cast<unsigned char*>(ch.x->dst) + ch.off, p->pipe_slot[t][slot], ch.n);
+        } else {
+          const float* src = reinterpret_cast<const float*>(p->pipe_slot[t][slot]);
+          double* dst = static_cast<double*>(ch.x->dst) + ch.off;
+          for (size_t j = 0; j < ch.n; ++j) dst[j] = (double)src[j];
+        }
+      }
+    } else {
+      for (size_t i = 0; i < mine.size() && e == hipSuccess; ++i) {
+        const int slot = k++ & 1;
+        if (used[slot]) e = hipEventSynchronize(p->pipe_ev[t][slot]);
+        if (e != hipSuccess) break;
+        const Chunk& ch = chunks[mine[i]];
+        unsigned char* buf = p->pipe_slot[t][slot];
+        size_t bytes;
+        void* dev;
+        if (ch.x->kind == 0) {
+          std::memcpy(buf, static_cast<const unsigned char*>(ch.x->src) + ch.off, ch.n);
+          bytes = ch.n;
+          dev = static_cast<unsigned char*>(ch.x->dev) + ch.off;
+        } else {
+          const double* src = static_cast<const double*>(ch.x->src) + ch.off;
+          float* f = reinterpret_cast<float*>(buf);
+          for (size_t j = 0; j < ch.n; ++j) f[j] = (float)src[j];
+          bytes = ch.n * sizeof(float);
+          dev = static_cast<float*>(ch.x->dev) + ch.off;
+        }
+        e = hipMemcpyAsync(dev, buf, bytes, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipEventRecord(p->pipe_ev[t][slot], st);
+        used[slot] = true;
+      }
+    }
+    err[(size_t)t] = e;
+  };
+  std::vector<std::thread> pool;
+  int started = 1;
+  try {
+    for (int t = 1; t < T; ++t, ++started) pool.emplace_back(work, t);
+  } catch (...) {
+  }
+  // workers that could not be started run here, after thread 0's share
+  work(0);
+  for (int t = started; t < T; ++t) work(t);
+  for (auto& th : pool) th.join();
+  for (int t = 0; t < T; ++t)
+    if (err[(size_t)t] != hipSuccess)
+      return fail(err[(size_t)t] == hipErrorOutOfMemory ? GNG_ERR_OOM : GNG_ERR_HIP,
+                  std::string("host staging: ") + hipGetErrorString(err[(size_t)t]));
+  if (!after) {
+    // uploads: the compute stream waits for every worker stream's last DMA
+    hipSetDevice(p->device);
+    for (int t = 0; t < T; ++t) {
+      GNG_HIP(hipEventRecord(p->pipe_ev[t][0], p->pipe_stream[t]));
+      GNG_HIP(hipStreamWaitEvent(s, p->pipe_ev[t][0], 0));
+    }
+  }
+  return GNG_OK;
+}
+
 // Device staging of the f64 entry points: positions 3 cap64, masses cap64,
 // accelerations 3 cap64 doubles.
 int ensure_part64(gng_pm_plan* p, long long n, long long k) {
@@ -910,26 +1067,28 @@ int solve_host_soa(gng_pm_plan* p, const float* x, const float* y, const float* 
   float *dax = d + 4 * cap, *day = d + 5 * cap, *daz = d + 6 * cap;
   hipStream_t s = p->stream;
   StreamOrder order(p, s);
-  const size_t nb = sizeof(float) * (size_t)n;
-  if (n > 0) {
-    GNG_HIP(hipMemcpyAsync(dx, x, nb, hipMemcpyHostToDevice, s));
-    GNG_HIP(hipMemcpyAsync(dy, y, nb, hipMemcpyHostToDevice, s));
-    GNG_HIP(hipMemcpyAsync(dz, z, nb, hipMemcpyHostToDevice, s));
-    GNG_HIP(hipMemcpyAsync(dm, m, nb, hipMemcpyHostToDevice, s));
-  }
-  if (idx && k > 0) {
+  if (idx)
     for (long long j = 0; j < k; ++j)
       if (idx[j] < 0 || idx[j] >= n) return fail(GNG_ERR_INVALID, "index out of range");
-    GNG_HIP(hipMemcpyAsync(p->d_idx, idx, sizeof(long long) * (size_t)k, hipMemcpyHostToDevice, s));
-  }
+  if ((rc = ensure_pipe(p))) return rc;
+  const size_t nb = sizeof(float) * (size_t)n;
+  std::vector<Xfer> up;
+  if (n > 0)
+    for (auto pr : {std::make_pair((void*)dx, (const void*)x), std::make_pair((void*)dy, (const void*)y),
+                    std::make_pair((void*)dz, (const void*)z), std::make_pair((void*)dm, (const void*)m)})
+      up.push_back({pr.first, pr.second, nullptr, nb, 0});
+  if (idx && k > 0) up.push_back({p->d_idx, idx, nullptr, sizeof(long long) * (size_t)k, 0});
+  if ((rc = run_xfers(p, up, s, nullptr))) return rc;
   rc = run_pipeline(p, soa32(dx, dy, dz, dm), n, idx ? p->d_idx : nullptr, k, g, acc32(dax, day, daz), s);
   if (rc) return rc;
+  GNG_HIP(hipEventRecord(p->done_ev, s));
   const size_t kb = sizeof(float) * (size_t)k;
-  if (k > 0) {
-    GNG_HIP(hipMemcpyAsync(ax, dax, kb, hipMemcpyDeviceToHost, s));
-    GNG_HIP(hipMemcpyAsync(ay, day, kb, hipMemcpyDeviceToHost, s));
-    GNG_HIP(hipMemcpyAsync(az, daz, kb, hipMemcpyDeviceToHost, s));
-  }
+  std::vector<Xfer> down;
+  if (k > 0)
+    for (auto pr : {std::make_pair((void*)dax, (void*)ax), std::make_pair((void*)day, (void*)ay),
+                    std::make_pair((void*)daz, (void*)az)})
+      down.push_back({pr.first, nullptr, pr.second, kb, 3});
+  if ((rc = run_xfers(p, down, s, p->done_ev))) return rc;
   GNG_HIP(hipStreamSynchronize(s));
   return GNG_OK;
 }
@@ -1313,17 +1472,35 @@ int gng_pm_accelerations_f64(gng_pm_plan* p, const double* pos, const double* ma
   StreamOrder order(p, s);
   if ((rc = ensure_part64(p, n, k))) return rc;
   if (idx && (rc = ensure_idx(p, k))) return rc;
+  if ((rc = ensure_pipe(p))) return rc;
+  // device layout: f64 positions (3 cap64), f32 masses (cap64 floats), f32
+  // accelerations (3 cap64 floats, interleaved like the output)
   double* dpos = p->d_part64;
-  double* dm = dpos + 3 * p->cap64;
-  double* dout = dm + p->cap64;
+  float* dm = reinterpret_cast<float*>(dpos + 3 * p->cap64);
+  float* dout = dm + p->cap64;
+  std::vector<Xfer> up;
   if (n > 0) {
-    GNG_HIP(hipMemcpyAsync(dpos, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
-    GNG_HIP(hipMemcpyAsync(dm, mass, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s));
+    up.push_back({dpos, pos, nullptr, sizeof(double) * 3 * (size_t)n, 0});
+    up.push_back({dm, mass, nullptr, (size_t)n, 1});  // masses enter the deposit as f32
   }
-  if (idx && k > 0) GNG_HIP(hipMemcpyAsync(p->d_idx, idx, sizeof(long long) * (size_t)k, hipMemcpyHostToDevice, s));
-  rc = run_pipeline(p, aos64(dpos, dm), n, idx ? p->d_idx : nullptr, k, g, aos64_out(dout), s);
+  if (idx && k > 0) up.push_back({p->d_idx, idx, nullptr, sizeof(long long) * (size_t)k, 0});
+  if ((rc = run_xfers(p, up, s, nullptr))) return rc;
+  PartIn in = aos64(dpos, nullptr);
+  in.mass64 = 0;
+  in.m = dm;
+  AccOut o;
+  o.x = dout;
+  o.y = dout + 1;
+  o.z = dout + 2;
+  o.stride = 3;
+  rc = run_pipeline(p, in, n, idx ? p->d_idx : nullptr, k, g, o, s);
   if (rc) return rc;
-  if (k > 0) GNG_HIP(hipMemcpyAsync(out, dout, sizeof(double) * 3 * (size_t)k, hipMemcpyDeviceToHost, s));
+  GNG_HIP(hipEventRecord(p->done_ev, s));
+  std::vector<Xfer> down;
+  if (k > 0) down.push_back({dout, nullptr, out, 3 * (size_t)k, 2});
+  if ((rc = run_xfers(p, down, s, p->done_ev))) return rc;
+  // the download streams are done (their last events were waited for); the
+  // next entry point orders itself after s
   GNG_HIP(hipStreamSynchronize(s));
   return GNG_OK;
 }
