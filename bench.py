@@ -154,6 +154,9 @@ def measured_traffic(group: str, nm: int, workload: str):
 
 
 def main() -> int:
+    import faulthandler
+
+    faulthandler.enable()  # a crash prints the Python stack of the call it happened in
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -511,14 +514,34 @@ def bench_ensemble(args) -> int:
     def graph_step():
         batch.solve_device(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, off, 1.0, out[0].ptr, out[1].ptr, out[2].ptr)
 
-    for _ in range(args.warmup):
-        graph_step()
-    pm.device_synchronize()
+    def time_steps(steps):
+        for _ in range(args.warmup):
+            graph_step()
+        pm.device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            graph_step()
+        pm.device_synchronize()
+        return time.perf_counter() - t0
+
+    # launch split: every stage one launch over up to k boxes of a mesh side
+    # (k = 2^30: one launch per side; 0: one pipeline per box), each timed,
+    # then the library's own tuner picks on the same particles
+    per_side = {}
+    for b in boxes:
+        per_side[b["grid_size"]] = per_side.get(b["grid_size"], 0) + 1
+    biggest = max(per_side.values())
+    split_ms = {}
+    for k in sorted({1 << 30, (biggest + 1) // 2, (biggest + 3) // 4, 0}, reverse=True):
+        batch.set_box_tables(k)
+        split_ms["one launch per side" if k == 1 << 30 else ("per box" if k == 0 else f"{k} boxes per launch")] = \
+            round(time_steps(max(args.steps // 2, 3)) * 1e3 / max(args.steps // 2, 3), 4)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        graph_step()
-    pm.device_synchronize()
-    wall = time.perf_counter() - t0
+    chosen = batch.tune_launches(d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, off, 1.0, out[0].ptr, out[1].ptr,
+                                 out[2].ptr, reps=3)
+    log(f"[bench] launch split ms per ensemble: {split_ms}; tuner chose {chosen} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    wall = time_steps(args.steps)
     acc = np.stack([o.numpy() for o in out], axis=1).astype(np.float64)
 
     # the same boxes, one plan each, launched one after another without a graph
@@ -540,19 +563,27 @@ def bench_ensemble(args) -> int:
     wall_serial = time.perf_counter() - t0
 
     worst = None
+    cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle  # checker only
+        import oracle  # checker and CPU baseline only
 
         oracle.build()
         worst = 0.0
+        t_cpu = 0.0
         for i, b in enumerate(boxes):
             p = b["particles"]
+            t1 = time.perf_counter()
             ref = oracle.pm_accelerations(p.pos64(), p.mass64(), b["box_size"], b["grid_size"], 1.0)
+            t_cpu += time.perf_counter() - t1
             got = acc[off[i]:off[i + 1]]
             worst = max(worst, float(np.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum())))
+        cpu = {"value": round(n / t_cpu / 1e6, 3), "unit": "M particle-forces/s", "cores": oracle.get_threads(),
+               "kind": "port", "sample": f"every box of the ensemble once ({n} particles), FP64 oracle lean R2C "
+                                         f"mode, {t_cpu:.2f} s in total"}
     ms = wall * 1e3 / args.steps
     b_alg = sum(workloads.algorithmic_bytes(int(off[i + 1] - off[i]), b["grid_size"]) for i, b in enumerate(boxes))
+    ach = b_alg / (ms * 1e-3) / 1e9
     sizes = {}
     for b in boxes:
         sizes[b["grid_size"]] = sizes.get(b["grid_size"], 0) + 1
@@ -564,8 +595,14 @@ def bench_ensemble(args) -> int:
         "config": {"workload": "64 boxes, N_side from {32,48,64,96,128}, mesh = 2 N_side, L = N_side Mpc/h, "
                                "Zel'dovich P(k)~k^n, n in [-2.5,-1.5], seeds 0x6E670004 + box",
                    "particles": n, "meshes": {str(k): v for k, v in sorted(sizes.items())}},
-        "hbm_gbps": round(b_alg / (ms * 1e-3) / 1e9, 1),
+        "roofline": {"bound": "hbm", "kernel": "whole ensemble (every stage of every box)", "achieved": round(ach, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "algorithmic_bytes": b_alg, "mean_ms": round(ms, 4)},
+        "cpu_baseline": cpu,
+        "hbm_gbps": round(ach, 1),
         "autotune": {"seconds": round(t_tune, 2), "choices": tuned},
+        "launch_split_ms": split_ms,
+        "launch_split_chosen": chosen,
         "serial_ms_per_step": round(wall_serial * 1e3 / args.steps, 4),
         "graph_speedup": round(wall_serial / wall, 3),
         "worst_box_rms_force_err": worst,

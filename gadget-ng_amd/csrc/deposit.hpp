@@ -54,10 +54,19 @@ struct TileMap {
 
 // Zero a buffer with vector stores (a fill kernel rather than a memset, so the
 // sequence stays safe to capture in a graph).
-__global__ __launch_bounds__(256) void k_zero(uint4* __restrict__ p, long long n16) {
+__device__ __forceinline__ void zero_body(uint4* __restrict__ p, long long n16) {
   const uint4 z = make_uint4(0u, 0u, 0u, 0u);
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
     p[i] = z;
+}
+
+__global__ __launch_bounds__(256) void k_zero(uint4* __restrict__ p, long long n16) {
+  zero_body(p, n16);
+}
+
+__global__ __launch_bounds__(256) void k_zero_box(const BoxArgs* __restrict__ bx, long long n16) {
+  const BoxArgs& b = bx[blockIdx.z];
+  zero_body(reinterpret_cast<uint4*>(b.count), n16);
 }
 
 __device__ __forceinline__ unsigned bucket_of_cell(int ix, int iy, int iz, const TileMap tm) {
@@ -81,7 +90,7 @@ constexpr int kBucketSlots = 2048;  // > 256 * kBucketPer keys: probing always e
 constexpr unsigned kEmptyKey = 0xffffffffu;
 
 template <typename T, typename M>
-__global__ __launch_bounds__(256) void k_bucket_count(const PosSrc<T> src, const M* __restrict__ m, long long n,
+__device__ __forceinline__ void bucket_count_body(const PosSrc<T> src, const M* __restrict__ m, long long n,
                                                       const CellMap cm, const TileMap tm,
                                                       unsigned* __restrict__ count, unsigned* __restrict__ bucket,
                                                       unsigned* __restrict__ rank, SolveScalars* sc) {
@@ -166,6 +175,20 @@ __global__ __launch_bounds__(256) void k_bucket_count(const PosSrc<T> src, const
   }
 }
 
+template <typename T, typename M>
+__global__ __launch_bounds__(256) void k_bucket_count(const PosSrc<T> src, const M* __restrict__ m, long long n,
+                                                      const CellMap cm, const TileMap tm,
+                                                      unsigned* __restrict__ count, unsigned* __restrict__ bucket,
+                                                      unsigned* __restrict__ rank, SolveScalars* sc) {
+  bucket_count_body<T, M>(src, m, n, cm, tm, count, bucket, rank, sc);
+}
+
+__global__ __launch_bounds__(256) void k_bucket_count_box(const BoxArgs* __restrict__ bx, const PosSrc<float> src, const float* __restrict__ m, const TileMap tm) {
+  const BoxArgs& b = bx[blockIdx.z];
+  bucket_count_body<float, float>(PosSrc<float>{src.x + b.p0, src.y + b.p0, src.z + b.p0, 1}, m + b.p0, b.n, b.cm, tm, b.count, b.bucket, b.rank,
+                                  b.sc);
+}
+
 // Exclusive scan of `count` (nb entries) into `offset` (nb + 1 entries).
 // Phase 1: each 256-thread block scans 4096 entries, keeps local offsets and
 // its total; phase 2: one block scans the totals; phase 3: adds them back.
@@ -193,7 +216,7 @@ __device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* l
   return wbase + s - v;
 }
 
-__global__ __launch_bounds__(256) void k_scan_local(const unsigned* __restrict__ count, unsigned* __restrict__ offset,
+__device__ __forceinline__ void scan_local_body(const unsigned* __restrict__ count, unsigned* __restrict__ offset,
                                                     unsigned* __restrict__ partial, long long nb) {
   __shared__ unsigned lds[4];
   const long long base = (long long)blockIdx.x * kScanBlock + (long long)threadIdx.x * kScanItems;
@@ -214,7 +237,17 @@ __global__ __launch_bounds__(256) void k_scan_local(const unsigned* __restrict__
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(256) void k_scan_partials(unsigned* __restrict__ partial, int nblocks,
+__global__ __launch_bounds__(256) void k_scan_local(const unsigned* __restrict__ count, unsigned* __restrict__ offset,
+                                                    unsigned* __restrict__ partial, long long nb) {
+  scan_local_body(count, offset, partial, nb);
+}
+
+__global__ __launch_bounds__(256) void k_scan_local_box(const BoxArgs* __restrict__ bx, long long nb) {
+  const BoxArgs& b = bx[blockIdx.z];
+  scan_local_body(b.count, b.offset, b.partial, nb);
+}
+
+__device__ __forceinline__ void scan_partials_body(unsigned* __restrict__ partial, int nblocks,
                                                        unsigned* __restrict__ offset, long long nb) {
   __shared__ unsigned lds[4];
   unsigned carry = 0;
@@ -229,7 +262,17 @@ __global__ __launch_bounds__(256) void k_scan_partials(unsigned* __restrict__ pa
   if (threadIdx.x == 0) offset[nb] = carry;
 }
 
-__global__ __launch_bounds__(256) void k_scan_add(unsigned* __restrict__ offset, const unsigned* __restrict__ partial,
+__global__ __launch_bounds__(256) void k_scan_partials(unsigned* __restrict__ partial, int nblocks,
+                                                       unsigned* __restrict__ offset, long long nb) {
+  scan_partials_body(partial, nblocks, offset, nb);
+}
+
+__global__ __launch_bounds__(256) void k_scan_partials_box(const BoxArgs* __restrict__ bx, int nblocks, long long nb) {
+  const BoxArgs& b = bx[blockIdx.z];
+  scan_partials_body(b.partial, nblocks, b.offset, nb);
+}
+
+__device__ __forceinline__ void scan_add_body(unsigned* __restrict__ offset, const unsigned* __restrict__ partial,
                                                   long long nb) {
   const long long base = (long long)blockIdx.x * kScanBlock;
   const unsigned add = partial[blockIdx.x];
@@ -237,12 +280,22 @@ __global__ __launch_bounds__(256) void k_scan_add(unsigned* __restrict__ offset,
     if (base + k < nb) offset[base + k] += add;
 }
 
+__global__ __launch_bounds__(256) void k_scan_add(unsigned* __restrict__ offset, const unsigned* __restrict__ partial,
+                                                  long long nb) {
+  scan_add_body(offset, partial, nb);
+}
+
+__global__ __launch_bounds__(256) void k_scan_add_box(const BoxArgs* __restrict__ bx, long long nb) {
+  const BoxArgs& b = bx[blockIdx.z];
+  scan_add_body(b.offset, b.partial, nb);
+}
+
 // Particle i to its bucket slot as a cell record, and the sum of the
 // particles' fixed-point masses round(m 2^s) (one integer atomic per block,
 // so the total does not depend on the order): the deposit subtracts the mean
 // per cell before it converts to FP32.
 template <typename T, typename M>
-__global__ __launch_bounds__(256) void k_scatter(const PosSrc<T> src, const M* __restrict__ m, long long n,
+__device__ __forceinline__ void scatter_body(const PosSrc<T> src, const M* __restrict__ m, long long n,
                                                  const CellMap cm, const unsigned* __restrict__ bucket,
                                                  const unsigned* __restrict__ rank,
                                                  const unsigned* __restrict__ offset, PartRec* __restrict__ sorted,
@@ -273,6 +326,21 @@ __global__ __launch_bounds__(256) void k_scatter(const PosSrc<T> src, const M* _
     const long long t = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
     if (t) atomicAdd(&sc->qsum[(blockIdx.x % kQSlots) * kQStride], (unsigned long long)t);
   }
+}
+
+template <typename T, typename M>
+__global__ __launch_bounds__(256) void k_scatter(const PosSrc<T> src, const M* __restrict__ m, long long n,
+                                                 const CellMap cm, const unsigned* __restrict__ bucket,
+                                                 const unsigned* __restrict__ rank,
+                                                 const unsigned* __restrict__ offset, PartRec* __restrict__ sorted,
+                                                 unsigned* __restrict__ perm, SolveScalars* sc) {
+  scatter_body<T, M>(src, m, n, cm, bucket, rank, offset, sorted, perm, sc);
+}
+
+__global__ __launch_bounds__(256) void k_scatter_box(const BoxArgs* __restrict__ bx, const PosSrc<float> src, const float* __restrict__ m) {
+  const BoxArgs& b = bx[blockIdx.z];
+  scatter_body<float, float>(PosSrc<float>{src.x + b.p0, src.y + b.p0, src.z + b.p0, 1}, m + b.p0, b.n, b.cm, b.bucket, b.rank, b.offset, b.sorted,
+                             nullptr, b.sc);
 }
 
 // LDS fixed-point add with wavefront aggregation: the lanes whose cell equals
@@ -323,7 +391,7 @@ __constant__ unsigned char kRangeHi[14] = {8, 2, 4, 6, 8, 4, 8, 4, 8, 8, 6, 8, 8
 constexpr int kDepositThreads = 512;
 
 template <int T, int NT = kDepositThreads>
-__global__ __launch_bounds__(NT) void k_deposit_tiles(const PartRec* __restrict__ sorted,
+__device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ sorted,
                                                        const unsigned* __restrict__ offset, const CellMap cm,
                                                        const TileMap tm, const SolveScalars* sc, long long ntot,
                                                        float* __restrict__ rho, long long rho_plane,
@@ -451,6 +519,22 @@ __global__ __launch_bounds__(NT) void k_deposit_tiles(const PartRec* __restrict_
     rho[(size_t)(uz * T + lz) * rho_plane + row] = (float)((double)(v - qmean) * unit);
     if (keep) keep[(size_t)(uz * T + lz) * cm.nm * cm.nm + row] = (float)((double)v * unit);
   }
+}
+
+template <int T, int NT = kDepositThreads>
+__global__ __launch_bounds__(NT) void k_deposit_tiles(const PartRec* __restrict__ sorted,
+                                                       const unsigned* __restrict__ offset, const CellMap cm,
+                                                       const TileMap tm, const SolveScalars* sc, long long ntot,
+                                                       float* __restrict__ rho, long long rho_plane,
+                                                       float* __restrict__ keep) {
+  deposit_tiles_body<T, NT>(sorted, offset, cm, tm, sc, ntot, rho, rho_plane, keep);
+}
+
+template <int T, int NT = kDepositThreads>
+__global__ __launch_bounds__(NT) void k_deposit_tiles_box(const BoxArgs* __restrict__ bx, const TileMap tm) {
+  const BoxArgs& b = bx[blockIdx.z];
+  deposit_tiles_body<T, NT>(b.sorted, b.offset, b.cm, tm, b.sc, b.n, reinterpret_cast<float*>(b.P),
+                            2 * mesh_plane(b.cm.nm), nullptr);
 }
 
 }  // namespace gng

@@ -302,6 +302,15 @@ struct MeshOps {
   void (*ic_z)(dim3, dim3, hipStream_t, IcParams, float2*, float2*, const float2*, int2);
   void (*y_inv_mul)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*, int3);
   void (*x_inv_mul)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*, int3);
+  // ensembles: every stage for all boxes of this mesh side in one launch
+  // (box b of the BoxArgs table on blockIdx.z); grids are per box, z = boxes
+  void (*deposit_box[2])(dim3, dim3, hipStream_t, const BoxArgs*, TileMap);
+  void (*x_fwd_box)(dim3, dim3, hipStream_t, const BoxArgs*, const float2*);
+  void (*y_fwd_box)(dim3, dim3, hipStream_t, const BoxArgs*, const float2*);
+  void (*z_green_box)(dim3, dim3, hipStream_t, const BoxArgs*, const float2*);
+  void (*z_green_col0_box)(dim3, dim3, hipStream_t, const BoxArgs*, const float2*);
+  void (*y_inv_box)(dim3, dim3, hipStream_t, const BoxArgs*, const float2*);
+  void (*x_inv_box)(dim3, dim3, hipStream_t, const BoxArgs*, const float2*);
 };
 
 template <int N>
@@ -381,6 +390,37 @@ MeshOps make_ops() {
   };
   o.x_inv_mul = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* P, float2* B, const float2* tw, int3 codes) {
     hipLaunchKernelGGL((k_fft_x_inv<N, R, true>), g, b, 0, s, A, P, B, nullptr, tw, 0, 1.0f, codes);
+  };
+  o.deposit_box[0] = [](dim3 g, dim3 b, hipStream_t s, const BoxArgs* bx, TileMap tm) {
+    hipLaunchKernelGGL((k_deposit_tiles_box<TD, 256>), g, b, 0, s, bx, tm);
+  };
+  o.deposit_box[1] = [](dim3 g, dim3 b, hipStream_t s, const BoxArgs* bx, TileMap tm) {
+    hipLaunchKernelGGL((k_deposit_tiles_box<TD, 512>), g, b, 0, s, bx, tm);
+  };
+  o.x_fwd_box = [](dim3 g, dim3 b, hipStream_t s, const BoxArgs* bx, const float2* tw) {
+    hipLaunchKernelGGL((k_fft_x_fwd_box<N, R>), g, b, 0, s, bx, tw);
+  };
+  o.y_fwd_box = [](dim3 g, dim3 b, hipStream_t s, const BoxArgs* bx, const float2* tw) {
+    hipLaunchKernelGGL((k_fft_y_fwd_box<N, C, TY>), g, b, 0, s, bx, tw);
+  };
+  o.z_green_box = [](dim3 g, dim3 b, hipStream_t s, const BoxArgs* bx, const float2* tw) {
+    hipLaunchKernelGGL((k_fft_z_green_box<N, YZCfg<N>::CZ, YZCfg<N>::TZ>), g, b, 0, s, bx, tw);
+  };
+  o.z_green_col0_box = [](dim3 g, dim3 b, hipStream_t s, const BoxArgs* bx, const float2* tw) {
+    hipLaunchKernelGGL((k_fft_z_green_col0_box<N>), g, b, 0, s, bx, tw);
+  };
+  o.y_inv_box = [](dim3 g, dim3 b, hipStream_t s, const BoxArgs* bx, const float2* tw) {
+    if constexpr (YZCfg<N>::YSPLIT) {
+      constexpr int CS = YZCfg<N>::CZ, TS = YZCfg<N>::TZ;
+      const dim3 gs((unsigned)(N / 2 / CS), g.y, g.z);
+      hipLaunchKernelGGL((k_fft_y_inv_box<N, CS, TS, 1>), gs, dim3(TS), 0, s, bx, tw);
+      hipLaunchKernelGGL((k_fft_y_inv_box<N, CS, TS, 2>), gs, dim3(TS), 0, s, bx, tw);
+    } else {
+      hipLaunchKernelGGL((k_fft_y_inv_box<N, C, TY>), g, b, 0, s, bx, tw);
+    }
+  };
+  o.x_inv_box = [](dim3 g, dim3 b, hipStream_t s, const BoxArgs* bx, const float2* tw) {
+    hipLaunchKernelGGL((k_fft_x_inv_box<N, R>), g, b, 0, s, bx, tw);
   };
   return o;
 }
@@ -1004,7 +1044,7 @@ int run_xfers(gng_pm_plan* p, const std::vector<Xfer>& xs, hipStream_t s, hipEve
                   std::string("host staging: ") + hipGetErrorString(err[(size_t)t]));
   if (!after) {
     // uploads: the compute stream waits for every worker stream's last DMA
-    hipSetDevice(p->device);
+    (void)hipSetDevice(p->device);
     for (int t = 0; t < T; ++t) {
       GNG_HIP(hipEventRecord(p->pipe_ev[t][0], p->pipe_stream[t]));
       GNG_HIP(hipStreamWaitEvent(s, p->pipe_ev[t][0], 0));
@@ -1946,6 +1986,11 @@ struct gng_pm_batch {
   hipGraphExec_t exec = nullptr;
   // key of the captured graph: pointers, offsets and g
   std::vector<uintptr_t> key;
+  // box tables: boxes grouped by mesh side, every stage one launch over up to
+  // box_split boxes of one side (0: one pipeline per box)
+  int box_split = 1 << 30;
+  BoxArgs* d_box = nullptr;  // one BoxArgs per box, grouped by mesh side
+  long long box_cap = 0;
 };
 
 namespace {
@@ -1961,12 +2006,124 @@ void free_batch(gng_pm_batch* b) {
   for (auto s : b->side) (void)hipStreamDestroy(s);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   for (auto p : b->plans) free_plan(p);
+  (void)hipFree(b->d_box);
   delete b;
+}
+
+// Boxes of one mesh side, in batch order.
+struct BoxGroup {
+  int nm;
+  std::vector<int> boxes;
+};
+
+std::vector<BoxGroup> box_groups(const gng_pm_batch* b) {
+  std::vector<BoxGroup> gs;
+  for (int i = 0; i < (int)b->plans.size(); ++i) {
+    const int nm = b->plans[i]->cfg.grid_size;
+    auto it = std::find_if(gs.begin(), gs.end(), [&](const BoxGroup& g) { return g.nm == nm; });
+    if (it == gs.end()) {
+      gs.push_back({nm, {}});
+      it = gs.end() - 1;
+    }
+    it->boxes.push_back(i);
+  }
+  return gs;
+}
+
+// Every box's table entry (synchronous upload; called before capture).
+int upload_box_tables(gng_pm_batch* b, const std::vector<BoxGroup>& gs, const int64_t* offsets, double g) {
+  std::vector<BoxArgs> h;
+  for (const BoxGroup& gr : gs)
+    for (int i : gr.boxes) {
+      gng_pm_plan* p = b->plans[i];
+      BoxArgs a{};
+      a.P = p->d_P;
+      a.A = p->d_A;
+      a.B = p->d_B;
+      a.gp = green_params(p, g);
+      a.cm = cell_map(p);
+      a.count = p->d_count;
+      a.offset = p->d_offset;
+      a.partial = p->d_partial;
+      a.sc = p->d_sc;
+      a.sorted = p->d_sorted;
+      a.bucket = p->d_bucket;
+      a.rank = p->d_rank;
+      a.p0 = offsets[i];
+      a.n = offsets[i + 1] - offsets[i];
+      h.push_back(a);
+    }
+  if ((long long)h.size() > b->box_cap) {
+    (void)hipFree(b->d_box);
+    b->d_box = nullptr;
+    b->box_cap = 0;
+    GNG_HIP(hipMalloc(&b->d_box, sizeof(BoxArgs) * h.size()));
+    b->box_cap = (long long)h.size();
+  }
+  GNG_HIP(hipMemcpy(b->d_box, h.data(), sizeof(BoxArgs) * h.size(), hipMemcpyHostToDevice));
+  return GNG_OK;
+}
+
+// The whole pipeline of one group of boxes (one mesh side), every stage one
+// launch over the group's boxes.  The launch choices are the first box's
+// (boxes of one side share them); persistent and plane-chunked variants are
+// not used here.
+void launch_box_group(gng_pm_batch* b, const BoxGroup& gr, const BoxArgs* bx, long long maxn, const float* x,
+                      const float* y, const float* z, const float* m, float* ax, float* ay, float* az,
+                      hipStream_t s) {
+  const gng_pm_plan* p0 = b->plans[gr.boxes[0]];
+  const MeshOps& o = *p0->ops;
+  const unsigned nbx = (unsigned)gr.boxes.size();
+  const int nm = gr.nm, H = nm / 2;
+  const PosSrc<float> src{x, y, z, 1};
+  hipLaunchKernelGGL(k_reset_scalars_box, dim3(1, 1, nbx), dim3(64), 0, s, bx);
+  const long long n16 = (p0->nb + 3) / 4;
+  hipLaunchKernelGGL(k_zero_box, dim3((unsigned)((n16 + 255) / 256), 1, nbx), dim3(256), 0, s, bx, n16);
+  if (maxn > 0)
+    hipLaunchKernelGGL(k_bucket_count_box,
+                       dim3((unsigned)((maxn + 256 * kBucketPer - 1) / (256 * kBucketPer)), 1, nbx), dim3(256), 0,
+                       s, bx, src, m, p0->tm);
+  hipLaunchKernelGGL(k_scan_local_box, dim3((unsigned)p0->scan_blocks, 1, nbx), dim3(256), 0, s, bx, p0->nb);
+  hipLaunchKernelGGL(k_scan_partials_box, dim3(1, 1, nbx), dim3(256), 0, s, bx, p0->scan_blocks, p0->nb);
+  hipLaunchKernelGGL(k_scan_add_box, dim3((unsigned)p0->scan_blocks, 1, nbx), dim3(256), 0, s, bx, p0->nb);
+  if (maxn > 0)
+    hipLaunchKernelGGL(k_scatter_box, dim3((unsigned)((maxn + 255) / 256), 1, nbx), dim3(256), 0, s, bx, src, m);
+  const unsigned ntiles = (unsigned)p0->tm.nt * p0->tm.nt * p0->tm.nt;
+  o.deposit_box[p0->tune.deposit_wide](dim3(ntiles, 1, nbx), dim3(p0->tune.deposit_wide ? 512 : 256), s, bx,
+                                       p0->tm);
+  const long long rows = (long long)nm * nm;
+  o.x_fwd_box(dim3((unsigned)(rows / o.x_rows), 1, nbx), dim3(256), s, bx, p0->d_tw);
+  o.y_fwd_box(dim3((unsigned)(H / o.yz_cols), (unsigned)nm, nbx), dim3((unsigned)o.yz_threads), s, bx, p0->d_tw);
+  o.z_green_box(dim3((unsigned)(H / o.z_cols), (unsigned)nm, nbx), dim3((unsigned)o.z_threads), s, bx, p0->d_tw);
+  o.z_green_col0_box(dim3((unsigned)(H + 1), 1, nbx), dim3(256), s, bx, p0->d_tw);
+  o.y_inv_box(dim3((unsigned)(H / o.yz_cols), (unsigned)nm, nbx), dim3((unsigned)o.yz_threads), s, bx, p0->d_tw);
+  o.x_inv_box(dim3((unsigned)(rows / o.x_rows), 3, nbx), dim3(256), s, bx, p0->d_tw);
+  if (maxn > 0)
+    hipLaunchKernelGGL(k_interp_box, dim3((unsigned)((maxn + 255) / 256), 1, nbx), dim3(256), 0, s, bx, src,
+                       AccDst<float>{ax, ay, az, 1});
 }
 
 }  // namespace
 
 extern "C" {
+
+int gng_pm_batch_set_box_tables(gng_pm_batch* b, int boxes_per_launch) {
+  if (!b) return fail(GNG_ERR_INVALID, "null batch");
+  if (boxes_per_launch < 0) return fail(GNG_ERR_INVALID, "boxes_per_launch must be >= 0");
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (b->box_split != boxes_per_launch) {
+    b->box_split = boxes_per_launch;
+    (void)hipSetDevice(b->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->key.clear();
+  }
+  return GNG_OK;
+}
+
+int gng_pm_batch_get_box_tables(const gng_pm_batch* b) { return b ? b->box_split : -1; }
 
 int gng_pm_batch_create(int n_boxes, const gng_pm_config* cfgs, gng_pm_batch** out) {
   if (!out) return fail(GNG_ERR_INVALID, "null output pointer");
@@ -2068,6 +2225,13 @@ int gng_pm_batch_solve_device(gng_pm_batch* b, const float* d_x, const float* d_
       if (rc) return rc;
       b->plans[i]->timing = false;
     }
+    const std::vector<BoxGroup> groups = box_groups(b);
+    if (b->box_split > 0) {
+      // a launch of the previous graph may still read the old tables
+      GNG_HIP(hipDeviceSynchronize());
+      const int rc = upload_box_tables(b, groups, offsets, g);
+      if (rc) return rc;
+    }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -2076,10 +2240,33 @@ int gng_pm_batch_solve_device(gng_pm_batch* b, const float* d_x, const float* d_
     int rc = GNG_OK;
     hipError_t e = hipEventRecord(b->root_ev, b->stream);
     for (int k = 0; k < b->branches && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->side[k], b->root_ev, 0);
-    for (int i = 0; i < nb && e == hipSuccess && rc == GNG_OK; ++i) {
-      const long long o = offsets[i], n = offsets[i + 1] - offsets[i];
-      rc = run_pipeline(b->plans[i], soa32(d_x + o, d_y + o, d_z + o, d_mass + o), n, nullptr, n, g,
-                        acc32(d_ax + o, d_ay + o, d_az + o), b->side[i % b->branches]);
+    if (b->box_split > 0) {
+      // each mesh side in runs of at most box_split boxes (contiguous in the
+      // table); every run is one launch per stage, runs go to the branches
+      // round robin
+      long long first = 0;
+      int run = 0;
+      for (size_t gi = 0; gi < groups.size() && e == hipSuccess; ++gi) {
+        const int ng = (int)groups[gi].boxes.size();
+        for (int r0 = 0; r0 < ng; r0 += b->box_split, ++run) {
+          BoxGroup part{groups[gi].nm, {}};
+          long long maxn = 0;
+          for (int r = r0; r < ng && r < r0 + b->box_split; ++r) {
+            const int i = groups[gi].boxes[r];
+            part.boxes.push_back(i);
+            maxn = std::max<long long>(maxn, offsets[i + 1] - offsets[i]);
+          }
+          launch_box_group(b, part, b->d_box + first + r0, maxn, d_x, d_y, d_z, d_mass, d_ax, d_ay, d_az,
+                           b->side[run % b->branches]);
+        }
+        first += ng;
+      }
+    } else {
+      for (int i = 0; i < nb && e == hipSuccess && rc == GNG_OK; ++i) {
+        const long long o = offsets[i], n = offsets[i + 1] - offsets[i];
+        rc = run_pipeline(b->plans[i], soa32(d_x + o, d_y + o, d_z + o, d_mass + o), n, nullptr, n, g,
+                          acc32(d_ax + o, d_ay + o, d_az + o), b->side[i % b->branches]);
+      }
     }
     for (int k = 0; k < b->branches && e == hipSuccess; ++k) {
       e = hipEventRecord(b->join_ev[k], b->side[k]);
@@ -2098,6 +2285,68 @@ int gng_pm_batch_solve_device(gng_pm_batch* b, const float* d_x, const float* d_
   }
   GNG_HIP(hipGraphLaunch(b->exec, reinterpret_cast<hipStream_t>(stream)));
   for (int i = 0; i < nb; ++i) b->plans[i]->last_n = offsets[i + 1] - offsets[i];
+  return GNG_OK;
+}
+
+// Times whole ensemble solves (best of `reps` after one warm-up, HIP events)
+// on the caller's particles for several launch splits: one table launch per
+// stage per mesh side, halves and quarters of each side's boxes, and one
+// pipeline per box; keeps the fastest.
+int gng_pm_batch_tune_launches(gng_pm_batch* b, const float* d_x, const float* d_y, const float* d_z,
+                               const float* d_mass, const int64_t* offsets, double g, float* d_ax, float* d_ay,
+                               float* d_az, int reps, int* chosen) {
+  if (!b || !offsets) return fail(GNG_ERR_INVALID, "null argument");
+  if (reps < 1 || reps > 100) return fail(GNG_ERR_INVALID, "reps must be in [1, 100]");
+  int biggest = 1;
+  for (const BoxGroup& gr : box_groups(b)) biggest = std::max(biggest, (int)gr.boxes.size());
+  std::vector<int> cand = {biggest, (biggest + 1) / 2, (biggest + 3) / 4, 0};
+  std::sort(cand.begin(), cand.end(), std::greater<int>());
+  cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+  GNG_HIP(hipSetDevice(b->device));
+  // timed on the batch's own stream, which lives as long as the graphs it
+  // launches (no stream of the tuner's own is left behind in a graph exec)
+  hipStream_t s = b->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  int rc = he == hipSuccess ? GNG_OK : fail(GNG_ERR_HIP, "event creation failed");
+  float best_ms = 1e30f;
+  int best = b->box_split;
+  for (int c : cand) {
+    if (rc) break;
+    if ((rc = gng_pm_batch_set_box_tables(b, c))) break;
+    // the first solve captures the graph; the second warms it
+    for (int w = 0; w < 2 && !rc; ++w)
+      rc = gng_pm_batch_solve_device(b, d_x, d_y, d_z, d_mass, offsets, g, d_ax, d_ay, d_az, s);
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(GNG_ERR_HIP, "warm-up failed");
+    float t_min = 1e30f;
+    for (int r = 0; r < reps && !rc; ++r) {
+      float ms = 0.0f;
+      if (hipEventRecord(e0, s) != hipSuccess) rc = fail(GNG_ERR_HIP, "event record failed");
+      if (!rc) rc = gng_pm_batch_solve_device(b, d_x, d_y, d_z, d_mass, offsets, g, d_ax, d_ay, d_az, s);
+      if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = fail(GNG_ERR_HIP, "event timing failed");
+      if (!rc) t_min = std::min(t_min, ms);
+    }
+    if (!rc && t_min < best_ms) {
+      best_ms = t_min;
+      best = c;
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (chosen) *chosen = best;
+  // the next solve captures a fresh graph for the chosen split
+  std::lock_guard<std::mutex> lk(b->mu);
+  b->box_split = best;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->key.clear();
   return GNG_OK;
 }
 

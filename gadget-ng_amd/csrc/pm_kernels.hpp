@@ -177,9 +177,46 @@ __device__ __forceinline__ int fixed_shift(const SolveScalars* sc, long long nto
   return 62 - e;
 }
 
-__global__ void k_reset_scalars(SolveScalars* sc) {
+struct GreenParams {
+  float c;        // 2 G / L^2
+  float a;        // (2 pi / L)^2 (r_s^2 / 2 + eps^2): W = exp(-a |n|^2)
+  double c64, a64;
+  const float* deconv;  // 1/sinc^4(pi n / N) per axis index, or null
+};
+
+// One box of an ensemble: the `_box` form of each stage kernel takes a device
+// array of these and works on box blockIdx.z, so one launch runs a stage for
+// every box of one mesh side.  Meshes, sort buffers and scalars are the box's
+// own plan's; the particles are [p0, p0 + n) of the ensemble's arrays.
+struct BoxArgs {
+  float2* P;
+  float2* A;
+  float2* B;
+  GreenParams gp;
+  CellMap cm;
+  unsigned* count;
+  unsigned* offset;
+  unsigned* partial;
+  SolveScalars* sc;
+  PartRec* sorted;
+  unsigned* bucket;
+  unsigned* rank;
+  long long n;
+  long long p0;
+};
+
+__device__ __forceinline__ void reset_scalars_body(SolveScalars* sc) {
   if (threadIdx.x == 0) sc->mmax_bits = 0u;
   if (threadIdx.x < kQSlots) sc->qsum[threadIdx.x * kQStride] = 0ull;
+}
+
+__global__ void k_reset_scalars(SolveScalars* sc) {
+  reset_scalars_body(sc);
+}
+
+__global__ void k_reset_scalars_box(const BoxArgs* __restrict__ bx) {
+  const BoxArgs& b = bx[blockIdx.z];
+  reset_scalars_body(b.sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -202,7 +239,7 @@ struct XCfg {
 // rho_plane: complex-sized (float-pair) stride between z planes of rho, which
 // is mesh_plane(N) when rho shares P's buffer and N*N/2 for a separate one.
 template <int N, int ROWS>
-__global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, const float2* __restrict__ tw_g,
+__device__ __forceinline__ void fft_x_fwd_body(const float* rho, float2* P, const float2* __restrict__ tw_g,
                                                    long long rho_plane) {
   constexpr int H = N / 2, T = 256, LS = XCfg<N>::LS;
   using F = RegFft<H, false, false, ROWS, T, LS, 1, false, N>;
@@ -248,6 +285,18 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, 
   }
 }
 
+template <int N, int ROWS>
+__global__ __launch_bounds__(256) void k_fft_x_fwd(const float* rho, float2* P, const float2* __restrict__ tw_g,
+                                                   long long rho_plane) {
+  fft_x_fwd_body<N, ROWS>(rho, P, tw_g, rho_plane);
+}
+
+template <int N, int ROWS>
+__global__ __launch_bounds__(256) void k_fft_x_fwd_box(const BoxArgs* __restrict__ bx, const float2* __restrict__ tw_g) {
+  const BoxArgs& b = bx[blockIdx.z];
+  fft_x_fwd_body<N, ROWS>(reinterpret_cast<const float*>(b.P), b.P, tw_g, mesh_plane(N));
+}
+
 // Mode multipliers of the initial-condition passes (codes): 0 -> 1,
 // 1 -> -i n, 2 -> n, 3 -> n^2.
 __device__ __forceinline__ float2 ic_mul(int code, float n, float2 v) {
@@ -265,7 +314,7 @@ __device__ __forceinline__ float2 ic_mul(int code, float n, float2 v) {
 // The pairwise pre-processing reads the row from LDS and feeds the first FFT
 // stage in registers; the last stage stores the real row directly.
 template <int N, int ROWS, bool MUL = false>
-__global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float2* __restrict__ P,
+__device__ __forceinline__ void fft_x_inv_body(float2* __restrict__ A, float2* __restrict__ P,
                                                    float2* __restrict__ B, float2* __restrict__ pot,
                                                    const float2* __restrict__ tw_g, int field0, float pot_scale,
                                                    int3 codes) {
@@ -324,6 +373,20 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float
   });
 }
 
+template <int N, int ROWS, bool MUL = false>
+__global__ __launch_bounds__(256) void k_fft_x_inv(float2* __restrict__ A, float2* __restrict__ P,
+                                                   float2* __restrict__ B, float2* __restrict__ pot,
+                                                   const float2* __restrict__ tw_g, int field0, float pot_scale,
+                                                   int3 codes) {
+  fft_x_inv_body<N, ROWS, MUL>(A, P, B, pot, tw_g, field0, pot_scale, codes);
+}
+
+template <int N, int ROWS>
+__global__ __launch_bounds__(256) void k_fft_x_inv_box(const BoxArgs* __restrict__ bx, const float2* __restrict__ tw_g) {
+  const BoxArgs& b = bx[blockIdx.z];
+  fft_x_inv_body<N, ROWS, false>(b.A, b.P, b.B, nullptr, tw_g, 0, 1.0f, make_int3(0, 0, 0));
+}
+
 // ---------------------------------------------------------------------------
 // y / z transforms: a workgroup owns C adjacent kx columns of one plane
 // ---------------------------------------------------------------------------
@@ -377,7 +440,7 @@ __device__ __forceinline__ int line_tile(int b) {
 // [0, C) (element (i, c) at buf[i*IS + c]); `stride` is the distance in
 // complex values between consecutive i.
 template <int N, int C, int T>
-__global__ __launch_bounds__(T) void k_fft_y_fwd(float2* __restrict__ P, const float2* __restrict__ tw_g) {
+__device__ __forceinline__ void fft_y_fwd_body(float2* __restrict__ P, const float2* __restrict__ tw_g) {
   constexpr int H = N / 2, IS = C + 1;
   using F = RegFft<N, false, false, C, T, 1, IS, true, N>;
   __shared__ float2 buf[N * IS];
@@ -398,12 +461,17 @@ __global__ __launch_bounds__(T) void k_fft_y_fwd(float2* __restrict__ P, const f
   });
 }
 
-struct GreenParams {
-  float c;        // 2 G / L^2
-  float a;        // (2 pi / L)^2 (r_s^2 / 2 + eps^2): W = exp(-a |n|^2)
-  double c64, a64;
-  const float* deconv;  // 1/sinc^4(pi n / N) per axis index, or null
-};
+template <int N, int C, int T>
+__global__ __launch_bounds__(T) void k_fft_y_fwd(float2* __restrict__ P, const float2* __restrict__ tw_g) {
+  fft_y_fwd_body<N, C, T>(P, tw_g);
+}
+
+template <int N, int C, int T>
+__global__ __launch_bounds__(T) void k_fft_y_fwd_box(const BoxArgs* __restrict__ bx, const float2* __restrict__ tw_g) {
+  const BoxArgs& b = bx[blockIdx.z];
+  fft_y_fwd_body<N, C, T>(b.P, tw_g);
+}
+
 
 __device__ __forceinline__ int freq_of(int j, int n) { return j <= n / 2 ? j : j - n; }
 
@@ -413,7 +481,7 @@ __device__ __forceinline__ int freq_of(int j, int n) { return j <= n / 2 ? j : j
 // the reversed radix plan so it starts from those registers.  The packed
 // kx = 0 column is done by k_fft_z_green_col0.
 template <int N, int C, int T>
-__global__ __launch_bounds__(T) void k_fft_z_green(const float2* __restrict__ P, float2* __restrict__ A,
+__device__ __forceinline__ void fft_z_green_body(const float2* __restrict__ P, float2* __restrict__ A,
                                                    float2* __restrict__ B, const float2* __restrict__ tw_g,
                                                    const GreenParams gp) {
   constexpr int H = N / 2, IS = C + 1;
@@ -470,6 +538,19 @@ __global__ __launch_bounds__(T) void k_fft_z_green(const float2* __restrict__ P,
       if (ok && (tile | l)) B[col0 + (unsigned)i * plane + l] = out[p][r];
     });
   }
+}
+
+template <int N, int C, int T>
+__global__ __launch_bounds__(T) void k_fft_z_green(const float2* __restrict__ P, float2* __restrict__ A,
+                                                   float2* __restrict__ B, const float2* __restrict__ tw_g,
+                                                   const GreenParams gp) {
+  fft_z_green_body<N, C, T>(P, A, B, tw_g, gp);
+}
+
+template <int N, int C, int T>
+__global__ __launch_bounds__(T) void k_fft_z_green_box(const BoxArgs* __restrict__ bx, const float2* __restrict__ tw_g) {
+  const BoxArgs& b = bx[blockIdx.z];
+  fft_z_green_body<N, C, T>(b.P, b.A, b.B, tw_g, b.gp);
 }
 
 // Persistent form of k_fft_z_green for large meshes: gridDim.x workgroups (one
@@ -564,7 +645,7 @@ __global__ __launch_bounds__(T) void k_fft_z_green_persist(const float2* __restr
 // One workgroup per pair {ky, N - ky} transforms both z lines, combines them
 // with FP64 Green's factors and writes A and B of both.
 template <int N>
-__global__ __launch_bounds__(256) void k_fft_z_green_col0(const float2* __restrict__ P, float2* __restrict__ A,
+__device__ __forceinline__ void fft_z_green_col0_body(const float2* __restrict__ P, float2* __restrict__ A,
                                                           float2* __restrict__ B, const float2* __restrict__ tw,
                                                           const GreenParams gp) {
   constexpr int H = N / 2, T = 256, LS = N + 1;
@@ -624,12 +705,25 @@ __global__ __launch_bounds__(256) void k_fft_z_green_col0(const float2* __restri
   }
 }
 
+template <int N>
+__global__ __launch_bounds__(256) void k_fft_z_green_col0(const float2* __restrict__ P, float2* __restrict__ A,
+                                                          float2* __restrict__ B, const float2* __restrict__ tw,
+                                                          const GreenParams gp) {
+  fft_z_green_col0_body<N>(P, A, B, tw, gp);
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_fft_z_green_col0_box(const BoxArgs* __restrict__ bx, const float2* __restrict__ tw_g) {
+  const BoxArgs& b = bx[blockIdx.z];
+  fft_z_green_col0_body<N>(b.P, b.A, b.B, tw_g, b.gp);
+}
+
 // Inverse y transforms: A -> A, -i n_y A -> P, B -> B, one after the other,
 // each starting from registers and storing from registers.  With MUL the
 // three take the multipliers codes.x (A), .y (P, from A) and .z (B) instead
 // (the initial-condition passes); the force solve instantiates MUL = false.
 template <int N, int C, int T, bool MUL = false, int PART = 0>
-__global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2* __restrict__ B,
+__device__ __forceinline__ void fft_y_inv_body(float2* __restrict__ A, float2* __restrict__ B,
                                                  float2* __restrict__ P, const float2* __restrict__ tw_g,
                                                  int3 codes) {
   // PART 1: only the two fields from A (A and P); PART 2: only B.  Two such
@@ -712,6 +806,19 @@ __global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2*
       if (ok) b[(unsigned)i * H + l] = out[p][r];
     });
   }
+}
+
+template <int N, int C, int T, bool MUL = false, int PART = 0>
+__global__ __launch_bounds__(T) void k_fft_y_inv(float2* __restrict__ A, float2* __restrict__ B,
+                                                 float2* __restrict__ P, const float2* __restrict__ tw_g,
+                                                 int3 codes) {
+  fft_y_inv_body<N, C, T, MUL, PART>(A, B, P, tw_g, codes);
+}
+
+template <int N, int C, int T, int PART = 0>
+__global__ __launch_bounds__(T) void k_fft_y_inv_box(const BoxArgs* __restrict__ bx, const float2* __restrict__ tw_g) {
+  const BoxArgs& b = bx[blockIdx.z];
+  fft_y_inv_body<N, C, T, false, PART>(b.A, b.B, b.P, tw_g, make_int3(0, 0, 0));
 }
 
 // Persistent forms of the y passes: gridDim.x workgroups walk the (column
@@ -877,7 +984,7 @@ __global__ __launch_bounds__(256) void k_interp_sorted(const PartRec* __restrict
 // CIC gather of (Fx, Fy, Fz) at particle idx[j] (idx null: j itself) in the
 // caller's order; the cell coordinate is formed in FP64 (cell_frac32).
 template <typename T, typename O>
-__global__ __launch_bounds__(256) void k_interp(const PosSrc<T> src, const long long* __restrict__ idx, long long k,
+__device__ __forceinline__ void interp_body(const PosSrc<T> src, const long long* __restrict__ idx, long long k,
                                                 const CellMap cm, long long fplane, const float* __restrict__ fx,
                                                 const float* __restrict__ fy, const float* __restrict__ fz,
                                                 const AccDst<O> out) {
@@ -895,6 +1002,22 @@ __global__ __launch_bounds__(256) void k_interp(const PosSrc<T> src, const long 
   out.x[o] = (O)a.x;
   out.y[o] = (O)a.y;
   out.z[o] = (O)a.z;
+}
+
+template <typename T, typename O>
+__global__ __launch_bounds__(256) void k_interp(const PosSrc<T> src, const long long* __restrict__ idx, long long k,
+                                                const CellMap cm, long long fplane, const float* __restrict__ fx,
+                                                const float* __restrict__ fy, const float* __restrict__ fz,
+                                                const AccDst<O> out) {
+  interp_body<T, O>(src, idx, k, cm, fplane, fx, fy, fz, out);
+}
+
+__global__ __launch_bounds__(256) void k_interp_box(const BoxArgs* __restrict__ bx, const PosSrc<float> src, const AccDst<float> out) {
+  const BoxArgs& b = bx[blockIdx.z];
+  interp_body<float, float>(PosSrc<float>{src.x + b.p0, src.y + b.p0, src.z + b.p0, 1}, nullptr, b.n, b.cm,
+                          2 * mesh_plane(b.cm.nm), reinterpret_cast<const float*>(b.A),
+                          reinterpret_cast<const float*>(b.P), reinterpret_cast<const float*>(b.B),
+                          AccDst<float>{out.x + b.p0, out.y + b.p0, out.z + b.p0, 1});
 }
 
 }  // namespace gng

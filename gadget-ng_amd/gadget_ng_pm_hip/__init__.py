@@ -163,6 +163,12 @@ def lib():
         L.gng_pm_batch_plan.restype = i32
         L.gng_pm_batch_solve_device.argtypes = [P, P, P, P, P, P, d, P, P, P, P]
         L.gng_pm_batch_solve_device.restype = i32
+        L.gng_pm_batch_set_box_tables.argtypes = [P, i32]
+        L.gng_pm_batch_set_box_tables.restype = i32
+        L.gng_pm_batch_get_box_tables.argtypes = [P]
+        L.gng_pm_batch_get_box_tables.restype = i32
+        L.gng_pm_batch_tune_launches.argtypes = [P, P, P, P, P, P, d, P, P, P, i32, ctypes.POINTER(i32)]
+        L.gng_pm_batch_tune_launches.restype = i32
         L.gng_hilbert_key3.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]
         L.gng_hilbert_key3.restype = ctypes.c_uint64
         L.gng_pm_autotune.argtypes = [P, i32]
@@ -640,6 +646,28 @@ class Batch:
         rc = lib().gng_pm_batch_autotune(self._h, int(reps))
         if rc != GNG_OK:
             raise HipExecutionError("gng_pm_batch_autotune", rc, last_error())
+
+    def set_box_tables(self, boxes_per_launch) -> None:
+        """k >= 1: every stage one launch over at most k boxes of one mesh
+        side (True: one launch per side); 0 or False: one pipeline per box."""
+        k = (1 << 30) if boxes_per_launch is True else int(boxes_per_launch)
+        rc = lib().gng_pm_batch_set_box_tables(self._h, k)
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_pm_batch_set_box_tables", rc, last_error())
+
+    def box_tables(self) -> int:
+        return int(lib().gng_pm_batch_get_box_tables(self._h))
+
+    def tune_launches(self, x: int, y: int, z: int, m: int, offsets, g: float, ax: int, ay: int, az: int,
+                      reps: int = 3) -> int:
+        """Time the launch splits on these particles and keep the fastest (returns k)."""
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        k = ctypes.c_int32()
+        rc = lib().gng_pm_batch_tune_launches(self._h, x, y, z, m, _ptr(off), float(g), ax, ay, az, int(reps),
+                                              ctypes.byref(k))
+        if rc != GNG_OK:
+            raise HipExecutionError("gng_pm_batch_tune_launches", rc, last_error())
+        return int(k.value)
 
     def tuning(self, i: int) -> dict:
         p = ctypes.c_void_p()

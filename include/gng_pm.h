@@ -263,9 +263,15 @@ int gng_pm_stage_times(gng_pm_plan *p, float *ms, int capacity);
  * (PmSolver::accelerations_for_indices); an ensemble is a set of such calls
  * run together.  Each box has its own configuration (mesh size, box size,
  * filters).  The first solve with a given set of pointers, offsets and g
- * captures the pipelines of all boxes into one HIP graph, with boxes spread
- * over parallel branches so small meshes run side by side; later solves with
- * the same arguments replay it. */
+ * captures the ensemble into one HIP graph; later solves with the same
+ * arguments replay it.  By default the boxes are grouped by mesh side and
+ * every stage (bucket count, scan, scatter, deposit, each FFT pass, the
+ * interpolation) is one launch over all boxes of a side, box b on
+ * blockIdx.z of the launch with its pointers and constants from a device
+ * table; the groups run on parallel graph branches.  With box tables off
+ * (gng_pm_batch_set_box_tables(b, 0)) each box runs its own pipeline and the
+ * boxes are spread over 8 branches.  Every split gives the same accelerations
+ * bit for bit. */
 typedef struct gng_pm_batch gng_pm_batch;
 
 int gng_pm_batch_create(int n_boxes, const gng_pm_config *cfgs, gng_pm_batch **out);
@@ -284,6 +290,18 @@ int gng_pm_batch_autotune(gng_pm_batch *b, int reps);
 int gng_pm_batch_solve_device(gng_pm_batch *b, const float *d_x, const float *d_y, const float *d_z,
                               const float *d_mass, const int64_t *offsets, double g, float *d_ax,
                               float *d_ay, float *d_az, void *stream);
+/* Launch split of the ensemble graph: with k >= 1 every stage is one launch
+ * over at most k boxes of one mesh side (the default, a very large k, is one
+ * launch per stage per mesh side); k = 0 runs one pipeline per box.  Runs go
+ * to 8 graph branches round robin.  The next solve captures a new graph. */
+int gng_pm_batch_set_box_tables(gng_pm_batch *b, int boxes_per_launch);
+int gng_pm_batch_get_box_tables(const gng_pm_batch *b);
+/* Times whole solves on the caller's particles (best of reps) for k = the
+ * largest mesh side's box count, its half and quarter, and 0, and keeps the
+ * fastest (written to *chosen when not NULL). */
+int gng_pm_batch_tune_launches(gng_pm_batch *b, const float *d_x, const float *d_y, const float *d_z,
+                               const float *d_mass, const int64_t *offsets, double g, float *d_ax,
+                               float *d_ay, float *d_az, int reps, int *chosen);
 
 /* ---- allocation-free solve ------------------------------------------------
  * No counterpart in the reference: for callers that cannot allocate device
