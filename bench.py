@@ -237,7 +237,9 @@ def main() -> int:
     d_out = pm.DeviceArray((n, 3), np.float64)
     plan = pm.Plan(nm, box, r_split=r_split, device=0)
     t1 = time.perf_counter()
-    tuning = plan.autotune(reps=3) if not args.no_autotune else plan.tuning()
+    # the autotuner times whole solves of the workload's own particles
+    tuning = plan.autotune_f64(d_pos.ptr, d_mass.ptr, n, d_out.ptr, reps=5) if not args.no_autotune \
+        else plan.tuning()
     log(f"[bench] launch choices ({time.perf_counter() - t1:.2f} s): {tuning}")
 
     if args.copy_calibration:

@@ -380,6 +380,75 @@ constexpr unsigned long long kRangeD = pack_range_d();
 __constant__ unsigned char kRangeLo[14] = {0, 1, 3, 5, 7, 2, 6, 3, 7, 4, 5, 7, 6, 7};
 __constant__ unsigned char kRangeHi[14] = {8, 2, 4, 6, 8, 4, 8, 4, 8, 8, 6, 8, 8, 8};
 
+// Heavy tiles over several workgroups.  Workgroup r of the deposit handles
+// tile rank r.  A tile whose 14 ranges hold more than kSliceItems particles
+// gets S = ceil(items / kSliceItems) slices (at most kMaxSlices): its own
+// workgroup takes slice 0, and S - 1 extra workgroups past the ntiles
+// primary ones take the others, each depositing an equal share of the
+// ranges into its own LDS tile.  The slices add their int64 tiles into a
+// scratch tile with global atomics (integer sums: the result does not depend
+// on the order or on how a tile is split) and the last slice to finish
+// converts the scratch tile and clears it for the next solve.  Clustered
+// inputs otherwise leave a few Plummer-core tiles running hundreds of rounds
+// on one workgroup each.
+constexpr unsigned kSliceItems = 16384;
+constexpr unsigned kMaxSlices = 1024;
+constexpr unsigned kNoSlice = 0xffffffffu;
+
+struct DepositSplit {
+  const unsigned* slices;        // slices of each tile rank (null: no splitting)
+  const unsigned* slot;          // scratch tile of each split tile rank
+  const unsigned* extra;         // extra workgroup e -> rank | slice << 20, or kNoSlice
+  const SolveScalars* sc;        // split_budget: extra workgroups reserved this solve
+  unsigned ntiles, slots;
+  unsigned long long* scratch;   // slots x T^3 partial tiles, zero between solves
+  unsigned* done;                // slots finished-slice counters, zero between solves
+};
+
+// Particles a tile's 14 ranges hold, from the bucket offsets.
+__device__ __forceinline__ unsigned tile_items(const unsigned* __restrict__ offset, const TileMap& tm, int tile) {
+  const int nt = tm.nt;
+  const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
+  unsigned w = 0;
+#pragma unroll
+  for (int r = 0; r < 14; ++r) {
+    const int d = (int)((kRangeD >> (3 * r)) & 7ull);
+    const int sx = (d & 1) ? (ux == 0 ? nt - 1 : ux - 1) : ux;
+    const int sy = (d & 2) ? (uy == 0 ? nt - 1 : uy - 1) : uy;
+    const int sz = (d & 4) ? (uz == 0 ? nt - 1 : uz - 1) : uz;
+    const unsigned b = tm.rank[(unsigned)(sz * nt + sy) * nt + sx] * 8u;
+    w += offset[b + kRangeHi[r]] - offset[b + kRangeLo[r]];
+  }
+  return w;
+}
+
+// Slices per tile rank.  A split tile reserves its S - 1 extra workgroups
+// from a budget of `slots` and writes their table entries; the first of them
+// also names its scratch tile.  A tile that finds the budget spent marks its
+// reservation unused and runs unsplit, so no more than ntiles + slots
+// workgroups are ever needed.
+__global__ __launch_bounds__(256) void k_deposit_plan(const unsigned* __restrict__ offset, const TileMap tm,
+                                                      unsigned* __restrict__ slices, unsigned* __restrict__ slot,
+                                                      unsigned* __restrict__ extra, SolveScalars* sc,
+                                                      unsigned slots) {
+  const unsigned ntiles = (unsigned)tm.nt * tm.nt * tm.nt;
+  const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ntiles) return;
+  const unsigned w = tile_items(offset, tm, (int)tm.order[r]);
+  unsigned S = w > kSliceItems ? (w + kSliceItems - 1) / kSliceItems : 1u;
+  if (S > kMaxSlices) S = kMaxSlices;
+  if (S > 1) {
+    const unsigned base = atomicAdd(&sc->split_budget, S - 1u);
+    if (base + S - 1u > slots) {
+      for (unsigned e = base; e < slots; ++e) extra[e] = kNoSlice;
+      S = 1;
+    } else {
+      slot[r] = base;
+      for (unsigned k = 1; k < S; ++k) extra[base + k - 1u] = r | (k << 20);
+    }
+  }
+  slices[r] = S;
+}
 // One workgroup per tile; T is the tile side (T^3 int64 accumulators in LDS).
 // Cell (x, y, z) of the shifted mass mesh is rho[z*rho_plane + y*nm + x];
 // keep (when not null) receives the unshifted mass per cell, unpadded.
@@ -390,20 +459,36 @@ __constant__ unsigned char kRangeHi[14] = {8, 2, 4, 6, 8, 4, 8, 4, 8, 8, 6, 8, 8
 // each keep twice as many particle loads and atomics in flight as four.
 constexpr int kDepositThreads = 512;
 
-template <int T, int NT = kDepositThreads>
+template <int T, int NT = kDepositThreads, bool SPLIT = false>
 __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ sorted,
                                                        const unsigned* __restrict__ offset, const CellMap cm,
                                                        const TileMap tm, const SolveScalars* sc, long long ntot,
                                                        float* __restrict__ rho, long long rho_plane,
-                                                       float* __restrict__ keep) {
+                                                       float* __restrict__ keep,
+                                                       const DepositSplit sp = DepositSplit{}) {
   constexpr int T3 = T * T * T;
   __shared__ unsigned long long acc[T3];
   __shared__ unsigned toff[8][9];
   __shared__ unsigned rpre[15], rdelta[14];
   const int tid = threadIdx.x;
-  for (int c = tid; c < T3; c += NT) acc[c] = 0ull;
   const int nt = tm.nt;
-  const int tile = (int)tm.order[blockIdx.x];
+  // this workgroup's tile rank and slice (k of S): the primary workgroups
+  // [0, ntiles) take slice 0 of their rank, extra workgroups the others
+  unsigned rank_r = blockIdx.x, k_slice = 0, n_slice = 1, my_slot = 0;
+  if constexpr (SPLIT) {
+    if (blockIdx.x >= sp.ntiles) {
+      const unsigned e = blockIdx.x - sp.ntiles;
+      if (e >= sp.slots || e >= sp.sc->split_budget) return;  // uniform: no slice for this workgroup
+      const unsigned v = sp.extra[e];
+      if (v == kNoSlice) return;
+      rank_r = v & 0xfffffu;
+      k_slice = v >> 20;
+    }
+    n_slice = sp.slices[rank_r];
+    if (n_slice > 1) my_slot = sp.slot[rank_r];
+  }
+  for (int c = tid; c < T3; c += NT) acc[c] = 0ull;
+  const int tile = (int)tm.order[rank_r];
   const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
   if (tid < 72) {
     const int d = tid / 9, j = tid % 9;
@@ -438,10 +523,17 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
 #pragma unroll
   for (int k = 0; k < 13; ++k) rend[k] = __builtin_amdgcn_readfirstlane(rpre[k + 1]);
   const int s = fixed_shift(sc, ntot);
-  const unsigned total = rpre[14];
+  // this slice's share [q0, total) of the tile's flattened ranges
+  const unsigned all = rpre[14];
+  unsigned q0 = 0, total = all;
+  if constexpr (SPLIT) {
+    const unsigned share = (all + n_slice - 1u) / n_slice;
+    q0 = min(all, k_slice * share);
+    total = min(all, q0 + share);
+  }
   // every lane of a wavefront runs the same number of iterations (lanes past
   // the end carry no particle), so the aggregation ballots see whole waves
-  const unsigned rounds = (total + NT - 1u) / NT;
+  const unsigned rounds = (total - q0 + NT - 1u) / NT;
   // range of q: the number of range ends at or below it (13 past the end)
   auto range_of = [&](unsigned q) {
     int r = 0;
@@ -460,7 +552,7 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
   };
   // m 2^s of a record: the FP32 mass scaled exactly
   const double scale = ldexp(1.0, s);
-  unsigned q = (unsigned)tid;
+  unsigned q = q0 + (unsigned)tid;
   int r = range_of(q);
   PartRec pnext = fetch(q, r);
   for (unsigned it = 0; it < rounds; ++it) {
@@ -509,25 +601,49 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
     }
   }
   __syncthreads();
+  unsigned long long* part = nullptr;
+  if (SPLIT && n_slice > 1) {
+    // add this slice's partial tile into the scratch tile; the last slice
+    // to finish converts the sum and clears the scratch tile
+    part = sp.scratch + (size_t)my_slot * T3;
+    for (int c = tid; c < T3; c += NT) {
+      const unsigned long long v = acc[c];
+      if (v) atomicAdd(part + c, v);
+    }
+    __threadfence();
+    __syncthreads();
+    __shared__ unsigned last;
+    if (tid == 0) last = atomicAdd(sp.done + my_slot, 1u) == n_slice - 1u;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+  }
   const double unit = ldexp(1.0, -s);
   // the integer part of the mean fixed-point mass per cell
   const long long qmean = (long long)(qsum_total(sc) / (unsigned long long)((long long)cm.nm * cm.nm * cm.nm));
   for (int c = tid; c < T3; c += NT) {
     const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
     const size_t row = (size_t)(uy * T + ly) * cm.nm + (ux * T + lx);
-    const long long v = (long long)acc[c];
+    long long v;
+    if (SPLIT && part) {
+      v = (long long)__hip_atomic_load(part + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      part[c] = 0ull;
+    } else {
+      v = (long long)acc[c];
+    }
     rho[(size_t)(uz * T + lz) * rho_plane + row] = (float)((double)(v - qmean) * unit);
     if (keep) keep[(size_t)(uz * T + lz) * cm.nm * cm.nm + row] = (float)((double)v * unit);
   }
+  if (SPLIT && part && tid == 0) sp.done[my_slot] = 0u;
 }
 
-template <int T, int NT = kDepositThreads>
+template <int T, int NT = kDepositThreads, bool SPLIT = false>
 __global__ __launch_bounds__(NT) void k_deposit_tiles(const PartRec* __restrict__ sorted,
                                                        const unsigned* __restrict__ offset, const CellMap cm,
                                                        const TileMap tm, const SolveScalars* sc, long long ntot,
                                                        float* __restrict__ rho, long long rho_plane,
-                                                       float* __restrict__ keep) {
-  deposit_tiles_body<T, NT>(sorted, offset, cm, tm, sc, ntot, rho, rho_plane, keep);
+                                                       float* __restrict__ keep, const DepositSplit sp) {
+  deposit_tiles_body<T, NT, SPLIT>(sorted, offset, cm, tm, sc, ntot, rho, rho_plane, keep, sp);
 }
 
 template <int T, int NT = kDepositThreads>

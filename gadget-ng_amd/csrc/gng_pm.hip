@@ -89,6 +89,7 @@ struct Tuning {
   int interp_sorted = 0;  // full-set gather in the deposit's sorted order
   int chunk_fwd = 0;      // z planes per chunk of the forward x/y pass pair (0: whole mesh)
   int chunk_inv = 0;      // same for the inverse y/x pass pair
+  int deposit_split = 1;  // heavy deposit tiles over several workgroups
 };
 
 // Planes per chunk when a pass pair runs chunk by chunk, so that the second
@@ -117,7 +118,7 @@ Tuning default_tuning(int nm) {
   return t;
 }
 
-constexpr int kKnobs = 6;
+constexpr int kKnobs = 7;
 int& knob(Tuning& t, int k) {
   switch (k) {
     case 0: return t.z_persistent;
@@ -125,7 +126,8 @@ int& knob(Tuning& t, int k) {
     case 2: return t.y_fwd_persistent;
     case 3: return t.y_inv_persistent;
     case 4: return t.chunk_fwd;
-    default: return t.chunk_inv;
+    case 5: return t.chunk_inv;
+    default: return t.deposit_split;
   }
 }
 
@@ -153,10 +155,11 @@ void load_tune_file_locked() {
   if (!fp) return;
   char arch[64];
   int nm, v[kKnobs];
-  while (std::fscanf(fp, "%63s %d %d %d %d %d %d %d", arch, &nm, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]) ==
-         2 + kKnobs) {
+  while (std::fscanf(fp, "%63s %d %d %d %d %d %d %d %d", arch, &nm, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
+                     &v[6]) == 2 + kKnobs) {
     Tuning t;
     for (int k = 0; k < 4; ++k) knob(t, k) = v[k] != 0;
+    t.deposit_split = v[6] != 0;
     t.chunk_fwd = v[4] == chunk_rule(nm, kChunkFwdMiB) ? v[4] : 0;
     t.chunk_inv = v[5] == chunk_rule(nm, kChunkInvMiB) ? v[5] : 0;
     t.tuned = 1;
@@ -181,9 +184,9 @@ void store_tuning(int dev, int nm, const Tuning& t) {
   FILE* fp = std::fopen(f, "w");
   if (!fp) return;
   for (const auto& kv : g_tune)
-    std::fprintf(fp, "%s %d %d %d %d %d %d %d\n", kv.first.first.c_str(), kv.first.second, kv.second.z_persistent,
-                 kv.second.deposit_wide, kv.second.y_fwd_persistent, kv.second.y_inv_persistent, kv.second.chunk_fwd,
-                 kv.second.chunk_inv);
+    std::fprintf(fp, "%s %d %d %d %d %d %d %d %d\n", kv.first.first.c_str(), kv.first.second,
+                 kv.second.z_persistent, kv.second.deposit_wide, kv.second.y_fwd_persistent,
+                 kv.second.y_inv_persistent, kv.second.chunk_fwd, kv.second.chunk_inv, kv.second.deposit_split);
   std::fclose(fp);
 }
 
@@ -280,9 +283,10 @@ struct MeshOps {
   int z_cols;  // the per-item z pass's tiles
   int z_threads;
   int tile;
-  // deposit[v]: workgroups of 256 (v = 0) or 512 (v = 1) threads
-  void (*deposit[2])(dim3, dim3, hipStream_t, const PartRec*, const unsigned*, CellMap, TileMap, const SolveScalars*,
-                     long long, float*, long long, float*);
+  // deposit[v][split]: workgroups of 256 (v = 0) or 512 (v = 1) threads;
+  // split: dense tiles shared by several workgroups
+  void (*deposit[2][2])(dim3, dim3, hipStream_t, const PartRec*, const unsigned*, CellMap, TileMap,
+                        const SolveScalars*, long long, float*, long long, float*, DepositSplit);
   void (*x_fwd)(dim3, dim3, hipStream_t, const float*, float2*, const float2*, long long);
   void (*y_fwd)(dim3, dim3, hipStream_t, float2*, const float2*);
   void (*z_green)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
@@ -328,14 +332,18 @@ MeshOps make_ops() {
   o.z_cols = YZCfg<N>::CZ;
   o.z_threads = YZCfg<N>::TZ;
   o.tile = TD;
-  o.deposit[0] = [](dim3 g, dim3 b, hipStream_t s, const PartRec* srt, const unsigned* off, CellMap cm, TileMap tm,
-                    const SolveScalars* sc, long long n, float* rho, long long rho_plane, float* keep) {
-    hipLaunchKernelGGL((k_deposit_tiles<TD, 256>), g, b, 0, s, srt, off, cm, tm, sc, n, rho, rho_plane, keep);
+#define GNG_DEPOSIT(V, NT, SPLIT)                                                                          \
+  o.deposit[V][SPLIT] = [](dim3 g, dim3 b, hipStream_t s, const PartRec* srt, const unsigned* off, CellMap cm,     \
+                           TileMap tm, const SolveScalars* sc, long long n, float* rho, long long rho_plane,        \
+                           float* keep, DepositSplit sp) {                                                          \
+    hipLaunchKernelGGL((k_deposit_tiles<TD, NT, SPLIT>), g, b, 0, s, srt, off, cm, tm, sc, n, rho, rho_plane, keep, \
+                       sp);                                                                                         \
   };
-  o.deposit[1] = [](dim3 g, dim3 b, hipStream_t s, const PartRec* srt, const unsigned* off, CellMap cm, TileMap tm,
-                    const SolveScalars* sc, long long n, float* rho, long long rho_plane, float* keep) {
-    hipLaunchKernelGGL((k_deposit_tiles<TD, 512>), g, b, 0, s, srt, off, cm, tm, sc, n, rho, rho_plane, keep);
-  };
+  GNG_DEPOSIT(0, 256, false)
+  GNG_DEPOSIT(1, 512, false)
+  GNG_DEPOSIT(0, 256, true)
+  GNG_DEPOSIT(1, 512, true)
+#undef GNG_DEPOSIT
 
   o.x_fwd = [](dim3 g, dim3 b, hipStream_t s, const float* rho, float2* P, const float2* tw, long long rho_plane) {
     hipLaunchKernelGGL((k_fft_x_fwd<N, R>), g, b, 0, s, rho, P, tw, rho_plane);
@@ -473,6 +481,16 @@ struct gng_pm_plan {
   unsigned* d_tile_rank = nullptr;   // tile -> Hilbert rank
   unsigned* d_tile_order = nullptr;  // Hilbert rank -> tile
 
+  // split deposit of heavy tiles: per tile rank its slice count, their
+  // exclusive scan, its scratch slot; per workgroup its tile rank; the
+  // scratch tiles and finished-slice counters (zero between solves)
+  unsigned* d_slices = nullptr;
+  unsigned* d_slot = nullptr;
+  unsigned* d_extra = nullptr;
+  unsigned long long* d_split_scratch = nullptr;
+  unsigned* d_split_done = nullptr;
+  long long split_slots = 0;
+
   // per-particle work buffers (grown on demand)
   long long work_cap = 0;
   PartRec* d_sorted = nullptr;
@@ -547,6 +565,12 @@ struct StreamOrder {
 };
 
 void free_work(gng_pm_plan* p) {
+  for (void* q : {(void*)p->d_slices, (void*)p->d_slot, (void*)p->d_extra, (void*)p->d_split_scratch,
+                  (void*)p->d_split_done})
+    (void)hipFree(q);
+  p->d_slices = p->d_slot = p->d_extra = p->d_split_done = nullptr;
+  p->d_split_scratch = nullptr;
+  p->split_slots = 0;
   (void)hipFree(p->d_sorted);
   (void)hipFree(p->d_perm);
   (void)hipFree(p->d_bucket);
@@ -723,6 +747,20 @@ int ensure_work(gng_pm_plan* p, long long n) {
   GNG_HIP(hipMalloc(&p->d_perm, sizeof(unsigned) * m));
   GNG_HIP(hipMalloc(&p->d_bucket, sizeof(unsigned) * m));
   GNG_HIP(hipMalloc(&p->d_rank, sizeof(unsigned) * m));
+  // split deposit: extra workgroups (and scratch tiles) for at most 2 n /
+  // kSliceItems slices beyond one per tile
+  const long long ntiles = p->nb / 8;
+  const long long slots = 2 * n / (long long)kSliceItems + 16;
+  const size_t t3 = (size_t)p->tm.T * p->tm.T * p->tm.T;
+  GNG_HIP(hipMalloc(&p->d_slices, sizeof(unsigned) * (size_t)ntiles));
+  GNG_HIP(hipMalloc(&p->d_slot, sizeof(unsigned) * (size_t)ntiles));
+  GNG_HIP(hipMalloc(&p->d_extra, sizeof(unsigned) * (size_t)slots));
+  GNG_HIP(hipMalloc(&p->d_split_scratch, sizeof(unsigned long long) * t3 * (size_t)slots));
+  GNG_HIP(hipMalloc(&p->d_split_done, sizeof(unsigned) * (size_t)slots));
+  // the scratch tiles and counters start at zero; every solve leaves them so
+  GNG_HIP(hipMemset(p->d_split_scratch, 0, sizeof(unsigned long long) * t3 * (size_t)slots));
+  GNG_HIP(hipMemset(p->d_split_done, 0, sizeof(unsigned) * (size_t)slots));
+  p->split_slots = slots;
   p->work_cap = n;
   return GNG_OK;
 }
@@ -760,8 +798,26 @@ int run_front(gng_pm_plan* p, const PartIn& in, long long n, hipStream_t s) {
                    p->tune.interp_sorted ? p->d_perm : nullptr, p->d_sc, s);
   mark(p, 2, s);
   const unsigned ntiles = (unsigned)p->tm.nt * p->tm.nt * p->tm.nt;
-  o.deposit[p->tune.deposit_wide](dim3(ntiles), dim3(p->tune.deposit_wide ? 512 : 256), s, p->d_sorted, p->d_offset,
-                                   cm, p->tm, p->d_sc, n, rho, rho_plane, p->keep_density ? p->d_rho : nullptr);
+  DepositSplit sp{};
+  unsigned deposit_groups = ntiles;
+  if (p->split_slots > 0 && p->tune.deposit_split) {
+    // heavy tiles over several workgroups: slices per tile, and the table of
+    // the extra workgroups past the ntiles primary ones
+    hipLaunchKernelGGL(k_deposit_plan, dim3((ntiles + 255) / 256), dim3(256), 0, s, p->d_offset, p->tm, p->d_slices,
+                       p->d_slot, p->d_extra, p->d_sc, (unsigned)p->split_slots);
+    sp.slices = p->d_slices;
+    sp.slot = p->d_slot;
+    sp.extra = p->d_extra;
+    sp.sc = p->d_sc;
+    sp.ntiles = ntiles;
+    sp.slots = (unsigned)p->split_slots;
+    sp.scratch = p->d_split_scratch;
+    sp.done = p->d_split_done;
+    deposit_groups = ntiles + (unsigned)p->split_slots;
+  }
+  o.deposit[p->tune.deposit_wide][sp.slices != nullptr](dim3(deposit_groups), dim3(p->tune.deposit_wide ? 512 : 256),
+                                                        s, p->d_sorted, p->d_offset, cm, p->tm, p->d_sc, n, rho,
+                                                        rho_plane, p->keep_density ? p->d_rho : nullptr, sp);
   mark(p, 3, s);
   const long long rows = (long long)nm * nm;
   const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
@@ -1781,23 +1837,18 @@ int gng_pm_density(gng_pm_plan* p, double* out) {
   return GNG_OK;
 }
 
-int gng_pm_autotune(gng_pm_plan* p, int reps) {
-  if (!p) return fail(GNG_ERR_INVALID, "null plan");
-  if (reps < 1 || reps > 100) return fail(GNG_ERR_INVALID, "reps must be in [1, 100]");
-  std::lock_guard<std::mutex> lk(p->mu);
-  int rc = set_device(p);
-  if (rc) return rc;
+}  // extern "C"
+
+namespace {
+
+// Coordinate descent over the launch choices from the defaults, timing whole
+// solves of `in` (best of `reps` after a warm-up, HIP events on the plan's
+// stream); each knob in turn keeps its faster value.  The caller holds the
+// plan's lock and has set the device.
+int autotune_impl(gng_pm_plan* p, const PartIn& in, long long n, const AccOut& out, int reps) {
   const int nm = p->cfg.grid_size;
-  long long n = (long long)nm * nm * nm / 8;
-  if (n < 4096) n = 4096;
-  if ((rc = ensure_particles(p, n, n, false))) return rc;
-  const size_t cap = (size_t)p->cap;
-  float* d = p->d_part;
   hipStream_t s = p->stream;
   StreamOrder order(p, s);
-  hipLaunchKernelGGL(k_synth_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, d + cap, d + 2 * cap,
-                     d + 3 * cap, n, (float)p->cfg.box_size);
-  GNG_HIP(hipGetLastError());
   const bool timing = p->timing;
   p->timing = false;
   hipEvent_t e0, e1;
@@ -1808,14 +1859,11 @@ int gng_pm_autotune(gng_pm_plan* p, int reps) {
   }
   // time one configuration: best of `reps` solves after a warm-up
   auto measure = [&](float& out_ms) -> int {
-    const PartIn in = soa32(d, d + cap, d + 2 * cap, d + 3 * cap);
-    const AccOut out = acc32(d + 4 * cap, d + 5 * cap, d + 6 * cap);
     int r2 = run_pipeline(p, in, n, nullptr, n, 1.0, out, s);
     float t_min = 1e30f;
     for (int r = 0; r < reps && r2 == GNG_OK; ++r) {
       if (hipEventRecord(e0, s) != hipSuccess) r2 = fail(GNG_ERR_HIP, "event record failed");
-      if (r2 == GNG_OK)
-        r2 = run_pipeline(p, in, n, nullptr, n, 1.0, out, s);
+      if (r2 == GNG_OK) r2 = run_pipeline(p, in, n, nullptr, n, 1.0, out, s);
       float ms = 0.0f;
       if (r2 == GNG_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
@@ -1825,14 +1873,13 @@ int gng_pm_autotune(gng_pm_plan* p, int reps) {
     out_ms = t_min;
     return r2;
   };
-  // coordinate descent from the defaults: each knob in turn keeps its faster value
   p->tune = default_tuning(nm);
   Tuning best = p->tune;
   float best_ms = 1e30f;
-  rc = measure(best_ms);
+  int rc = measure(best_ms);
   for (int k = 0; k < kKnobs && rc == GNG_OK; ++k) {
     Tuning trial = best;
-    if (k >= 4) {
+    if (k == 4 || k == 5) {
       // chunked or whole-mesh pass pairs
       const int rule = chunk_rule(nm, k == 4 ? kChunkFwdMiB : kChunkInvMiB);
       knob(trial, k) = knob(best, k) ? 0 : rule;
@@ -1863,6 +1910,44 @@ int gng_pm_autotune(gng_pm_plan* p, int reps) {
   return GNG_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int gng_pm_autotune(gng_pm_plan* p, int reps) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  if (reps < 1 || reps > 100) return fail(GNG_ERR_INVALID, "reps must be in [1, 100]");
+  std::lock_guard<std::mutex> lk(p->mu);
+  int rc = set_device(p);
+  if (rc) return rc;
+  const int nm = p->cfg.grid_size;
+  long long n = (long long)nm * nm * nm / 8;
+  if (n < 4096) n = 4096;
+  if ((rc = ensure_particles(p, n, n, false))) return rc;
+  const size_t cap = (size_t)p->cap;
+  float* d = p->d_part;
+  {
+    StreamOrder order(p, p->stream);
+    hipLaunchKernelGGL(k_synth_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, d, d + cap,
+                       d + 2 * cap, d + 3 * cap, n, (float)p->cfg.box_size);
+    GNG_HIP(hipGetLastError());
+  }
+  return autotune_impl(p, soa32(d, d + cap, d + 2 * cap, d + 3 * cap), n, acc32(d + 4 * cap, d + 5 * cap, d + 6 * cap),
+                       reps);
+}
+
+int gng_pm_autotune_f64(gng_pm_plan* p, const double* d_pos, const double* d_mass, int64_t n, double* d_out,
+                        int reps) {
+  if (!p) return fail(GNG_ERR_INVALID, "null plan");
+  if (reps < 1 || reps > 100) return fail(GNG_ERR_INVALID, "reps must be in [1, 100]");
+  int rc = validate(p, n, nullptr, n);
+  if (rc) return rc;
+  if (n > 0 && (!d_pos || !d_mass || !d_out)) return fail(GNG_ERR_INVALID, "null array");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if ((rc = set_device(p))) return rc;
+  return autotune_impl(p, aos64(d_pos, d_mass), n, aos64_out(d_out), reps);
+}
+
 int gng_pm_set_tuning(gng_pm_plan* p, const gng_pm_tuning_t* t) {
   if (!p || !t) return fail(GNG_ERR_INVALID, "null argument");
   const int nm = p->cfg.grid_size;
@@ -1878,6 +1963,7 @@ int gng_pm_set_tuning(gng_pm_plan* p, const gng_pm_tuning_t* t) {
   p->tune.interp_sorted = t->interp_sorted != 0;
   p->tune.chunk_fwd = t->plane_chunk_fwd;
   p->tune.chunk_inv = t->plane_chunk_inv;
+  p->tune.deposit_split = t->deposit_split != 0;
   p->tune.tuned = 0;
   return GNG_OK;
 }
@@ -1893,6 +1979,7 @@ int gng_pm_get_tuning(const gng_pm_plan* p, gng_pm_tuning_t* t) {
   t->interp_sorted = p->tune.interp_sorted;
   t->plane_chunk_fwd = p->tune.chunk_fwd;
   t->plane_chunk_inv = p->tune.chunk_inv;
+  t->deposit_split = p->tune.deposit_split;
   return GNG_OK;
 }
 

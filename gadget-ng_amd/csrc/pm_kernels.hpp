@@ -156,7 +156,8 @@ constexpr int kQSlots = 64;
 constexpr int kQStride = 16;  // 64-bit words between slots
 struct SolveScalars {
   unsigned int mmax_bits;  // max |m| as float bits (non-negative floats order as integers)
-  unsigned int pad[63];
+  unsigned int split_budget;  // scratch tiles reserved by split deposit tiles
+  unsigned int pad[62];
   unsigned long long qsum[kQSlots * kQStride];
 };
 
@@ -206,7 +207,10 @@ struct BoxArgs {
 };
 
 __device__ __forceinline__ void reset_scalars_body(SolveScalars* sc) {
-  if (threadIdx.x == 0) sc->mmax_bits = 0u;
+  if (threadIdx.x == 0) {
+    sc->mmax_bits = 0u;
+    sc->split_budget = 0u;
+  }
   if (threadIdx.x < kQSlots) sc->qsum[threadIdx.x * kQStride] = 0ull;
 }
 

@@ -94,13 +94,14 @@ class GngTuning(ctypes.Structure):
     _fields_ = [("z_persistent", ctypes.c_int32), ("deposit_wide", ctypes.c_int32),
                 ("y_fwd_persistent", ctypes.c_int32), ("y_inv_persistent", ctypes.c_int32),
                 ("tuned", ctypes.c_int32), ("interp_sorted", ctypes.c_int32),
-                ("plane_chunk_fwd", ctypes.c_int32), ("plane_chunk_inv", ctypes.c_int32)]
+                ("plane_chunk_fwd", ctypes.c_int32), ("plane_chunk_inv", ctypes.c_int32),
+                ("deposit_split", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {"z_persistent": bool(self.z_persistent), "deposit_threads": 512 if self.deposit_wide else 256,
                 "y_fwd_persistent": bool(self.y_fwd_persistent), "y_inv_persistent": bool(self.y_inv_persistent),
                 "interp_sorted": bool(self.interp_sorted), "chunk_fwd": int(self.plane_chunk_fwd),
-                "chunk_inv": int(self.plane_chunk_inv),
+                "chunk_inv": int(self.plane_chunk_inv), "deposit_split": bool(self.deposit_split),
                 "tuned": bool(self.tuned)}
 
 
@@ -172,6 +173,8 @@ def lib():
         L.gng_hilbert_key3.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]
         L.gng_hilbert_key3.restype = ctypes.c_uint64
         L.gng_pm_autotune.argtypes = [P, i32]
+        L.gng_pm_autotune_f64.argtypes = [P, P, P, i64, P, i32]
+        L.gng_pm_autotune_f64.restype = i32
         L.gng_pm_get_tuning.argtypes = [P, ctypes.POINTER(GngTuning)]
         L.gng_pm_batch_autotune.argtypes = [P, i32]
         L.gng_pm_set_tuning.argtypes = [P, ctypes.POINTER(GngTuning)]
@@ -424,9 +427,15 @@ class Plan:
         self._check(lib().gng_pm_autotune(self._h, int(reps)), "gng_pm_autotune")
         return self.tuning()
 
+    def autotune_f64(self, pos_ptr: int, m_ptr: int, n: int, out_ptr: int, reps: int = 5) -> dict:
+        """The autotuner timed on the caller's device Vec3 particles."""
+        self._check(lib().gng_pm_autotune_f64(self._h, pos_ptr, m_ptr, int(n), out_ptr, int(reps)),
+                    "gng_pm_autotune_f64")
+        return self.tuning()
+
     def set_tuning(self, z_persistent: bool = False, deposit_threads: int = 512, y_fwd_persistent: bool = False,
                    y_inv_persistent: bool = False, interp_sorted: bool = False, chunk_fwd: int = 0,
-                   chunk_inv: int = 0) -> None:
+                   chunk_inv: int = 0, deposit_split: bool = True) -> None:
         if deposit_threads not in (256, 512):
             raise ValueError("deposit_threads must be 256 or 512")
         t = GngTuning()
@@ -437,6 +446,7 @@ class Plan:
         t.interp_sorted = int(bool(interp_sorted))
         t.plane_chunk_fwd = int(chunk_fwd)
         t.plane_chunk_inv = int(chunk_inv)
+        t.deposit_split = int(bool(deposit_split))
         self._check(lib().gng_pm_set_tuning(self._h, ctypes.byref(t)), "gng_pm_set_tuning")
 
     def tuning(self) -> dict:

@@ -222,9 +222,20 @@ typedef struct gng_pm_tuning_t {
                                 64 at 1024^3; inv: 64 MiB, 64 at 512^3 and 16
                                 at 1024^3, where the default is 0); none at
                                 256^3 and below */
+  int32_t deposit_split;     /* deposit: tiles holding more than 16384 particles
+                                (dense clumps) are shared by several workgroups
+                                (1, default) or each run by one (0).  Same
+                                density bit for bit either way */
 } gng_pm_tuning_t;
 
 int gng_pm_autotune(gng_pm_plan *p, int reps);
+/* The same coordinate descent, timed on the caller's own particles (device
+ * memory in the layout of gng_pm_solve_device_f64; d_out receives the
+ * accelerations of the last trial), so choices that depend on how the
+ * particles are distributed (the deposit's workgroup size, the split of dense
+ * tiles) are made for that distribution. */
+int gng_pm_autotune_f64(gng_pm_plan *p, const double *d_pos, const double *d_mass, int64_t n, double *d_out,
+                        int reps);
 int gng_pm_get_tuning(const gng_pm_plan *p, gng_pm_tuning_t *out);
 /* Sets the choices explicitly (reproducible runs, testing every variant);
  * the plan then reports tuned = 0. */

@@ -27,7 +27,7 @@ def test_every_variant_matches_oracle(pm, oracle_mod, nm):
         assert rms_rel(got, ref) < 1e-5, (zp, dt, yf, yi)
         assert plan.tuning() == {"z_persistent": zp, "deposit_threads": dt, "y_fwd_persistent": yf,
                                  "y_inv_persistent": yi, "interp_sorted": False, "chunk_fwd": 0, "chunk_inv": 0,
-                                 "tuned": False}
+                                 "deposit_split": True, "tuned": False}
 
 
 @pytest.mark.parametrize("nm,chunks", [(32, (4, 16)), (64, (8, 32)), (96, (12, 48)), (128, (16, 64))])
@@ -114,3 +114,33 @@ def test_batch_autotune_recaptures(pm, oracle_mod):
         p = bx["particles"]
         ref = oracle_mod.pm_accelerations(p.pos64(), p.mass64(), bx["box_size"], bx["grid_size"], 1.0)
         assert rms_rel(a[off[b]:off[b + 1]], ref) <= 1e-5
+
+
+def test_split_deposit_of_dense_tiles_is_bit_identical(pm, oracle_mod):
+    """A Plummer clump of 150 000 particles inside one deposit tile: with the
+    split on, several workgroups share that tile (their exact int64 partial
+    tiles are summed in scratch); the density and the forces equal the
+    unsplit deposit's bit for bit, twice in a row (the scratch is left zero),
+    and match the oracle."""
+    from gadget_ng_pm_hip import workloads
+    nm, L = 64, 64.0
+    rng = np.random.default_rng(7)
+    sph, _ = workloads.plummer_spheres(150000, 1, L, 0.3, 11)
+    sph = np.asarray(sph).T
+    rel = (sph - sph[0] + 0.5 * L) % L - 0.5 * L  # unwrapped about one member
+    sph = (rel - rel.mean(axis=0) + 24.0) % L     # one clump, centred inside one 16-cell tile
+    back = rng.uniform(0.0, L, (60000, 3))
+    pos = np.concatenate([sph, back])
+    m = np.full(pos.shape[0], 1.0)
+    print(f"case split deposit N={pos.shape[0]} nm={nm}", flush=True)
+    plan = pm.Plan(nm, L, keep_density=True)
+    plan.set_tuning(deposit_split=False)
+    a0 = plan.accelerations_f64(pos, m, 1.0)
+    d0 = plan.density()
+    plan.set_tuning(deposit_split=True)
+    for _ in range(2):
+        a1 = plan.accelerations_f64(pos, m, 1.0)
+        assert np.array_equal(plan.density(), d0)
+        assert np.array_equal(a1, a0)
+    ref = oracle_mod.pm_accelerations(pos, m, L, nm, 1.0)
+    assert rms_rel(a1, ref) <= 1e-5
