@@ -1001,6 +1001,8 @@ def bench_learned(args) -> int:
                    "train_boxes": args.train_boxes, "eval_boxes": args.eval_boxes,
                    "treepm": "r_s = 1.25 cells, CIC-deconvolved long range + exact short range within 5 r_s, "
                              "eps = 0.05"},
+        "target": {"rms_rel": 0.01, "met_field": bool(mean("field_rms_rel", "pm_net") <= 0.01),
+                   "met_all": bool(mean("rms_rel", "pm_net") <= 0.01)},
         "without_net": {k: round(mean(k, "pm"), 5) for k in ("field_rms_rel", "field_median_rel", "field_mean_rel",
                                                                "rms_rel", "median_rel", "dense_median_rel")},
         "with_net": {k: round(mean(k, "pm_net"), 5) for k in ("field_rms_rel", "field_median_rel", "field_mean_rel",

@@ -601,13 +601,17 @@ __global__ __launch_bounds__(256) void k_loss(const float4* __restrict__ pos, co
   if (threadIdx.x == 0) atomicAdd(loss, red[0] + red[1] + red[2] + red[3]);
 }
 
+// rho (when not null) is the mass per cell of the box; particles whose base
+// cell holds more than rho_max keep their force unchanged.
 __global__ void k_apply(const float* __restrict__ x, const float* __restrict__ yy, const float* __restrict__ z,
                         long long n, const float4* __restrict__ ymesh, int nm, float cs, float out_scale,
-                        float* __restrict__ ax, float* __restrict__ ay, float* __restrict__ az) {
+                        const float* __restrict__ rho, float rho_max, float* __restrict__ ax,
+                        float* __restrict__ ay, float* __restrict__ az) {
   const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   Cic c;
   cic_nodes(x[p], yy[p], z[p], cs, nm, 0, c);
+  if (rho && rho[c.node[0]] > rho_max) return;
   float px = 0.0f, py = 0.0f, pz = 0.0f;
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
@@ -1007,20 +1011,28 @@ int gng_net_loss(gng_net* net, double* loss_out) {
   return GNG_OK;
 }
 
-int gng_net_apply(gng_net* net, int32_t box, const float* d_x, const float* d_y, const float* d_z, int64_t n,
-                  double box_size, float out_scale, float* d_ax, float* d_ay, float* d_az, void* stream) {
+int gng_net_apply_field(gng_net* net, int32_t box, const float* d_x, const float* d_y, const float* d_z,
+                        int64_t n, double box_size, float out_scale, const float* d_rho, double rho_max,
+                        float* d_ax, float* d_ay, float* d_az, void* stream) {
   if (!net || !d_x || !d_y || !d_z || !d_ax || !d_ay || !d_az) return fail(GNG_ERR_INVALID, "null argument");
   if (box < 0 || box >= net->max_boxes) return fail(GNG_ERR_INVALID, "box slot out of range");
   if (n < 0) return fail(GNG_ERR_INVALID, "negative particle count");
   if (!(box_size > 0.0)) return fail(GNG_ERR_INVALID, "box_size must be positive");
+  if (d_rho && !(rho_max >= 0.0)) return fail(GNG_ERR_INVALID, "rho_max must be non-negative");
   if (n == 0) return GNG_OK;
   GNG_HIP(hipSetDevice(net->device));
   const float cs = (float)((double)net->nm / box_size);
   hipLaunchKernelGGL(k_apply, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), d_x, d_y, d_z,
                      (long long)n, reinterpret_cast<const float4*>(net->d_y) + (long long)box * net->M, net->nm, cs,
-                     out_scale, d_ax, d_ay, d_az);
+                     out_scale, d_rho, (float)rho_max, d_ax, d_ay, d_az);
   GNG_HIP(hipGetLastError());
   return GNG_OK;
+}
+
+int gng_net_apply(gng_net* net, int32_t box, const float* d_x, const float* d_y, const float* d_z, int64_t n,
+                  double box_size, float out_scale, float* d_ax, float* d_ay, float* d_az, void* stream) {
+  return gng_net_apply_field(net, box, d_x, d_y, d_z, n, box_size, out_scale, nullptr, 0.0, d_ax, d_ay, d_az,
+                             stream);
 }
 
 }  // extern "C"

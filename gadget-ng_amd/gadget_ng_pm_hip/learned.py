@@ -286,6 +286,14 @@ class Net:
         _check(lib().gng_net_apply(self._h, int(box), x, y, z, int(n), float(box_size), float(out_scale), ax, ay,
                                    az, stream or None), "gng_net_apply")
 
+    def apply_field(self, box: int, x: int, y: int, z: int, n: int, box_size: float, out_scale: float, rho: int,
+                    rho_max: float, ax: int, ay: int, az: int, stream: int = 0) -> None:
+        """apply() for particles whose base cell holds at most rho_max (rho =
+        device mass-per-cell mesh of the box); the others keep their force."""
+        _check(lib().gng_net_apply_field(self._h, int(box), x, y, z, int(n), float(box_size), float(out_scale),
+                                         rho or None, float(rho_max), ax, ay, az, stream or None),
+               "gng_net_apply_field")
+
     def device_buffer(self, name: str) -> int:
         p = ctypes.c_void_p()
         _check(lib().gng_net_device_buffer(self._h, BUFFERS[name], ctypes.byref(p)), "gng_net_device_buffer")
@@ -518,8 +526,13 @@ class LearnedCorrection:
         x, y, z, m = s.dev
         device_synchronize()
         t0 = time.perf_counter()
+        rho_mean = float(np.sum(s.dev[3].numpy(), dtype=np.float64)) / cells ** 3
+        dm, _ = other.pm.device_meshes()
         net.infer(1)
-        net.apply(0, x.ptr, y.ptr, z.ptr, n, other.L, self.f0, corr[0].ptr, corr[1].ptr, corr[2].ptr)
+        # the correction goes to the field only: dense-cell particles carry no
+        # weight in the loss and keep the PM force
+        net.apply_field(0, x.ptr, y.ptr, z.ptr, n, other.L, self.f0, dm, (1.0 + self.dense_contrast) * rho_mean,
+                        corr[0].ptr, corr[1].ptr, corr[2].ptr)
         device_synchronize()
         t_net = time.perf_counter() - t0
         f_corr = np.stack([c.numpy() for c in corr], axis=1)
@@ -529,8 +542,7 @@ class LearnedCorrection:
         t0 = time.perf_counter()
         other.pm.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, a[0].ptr, a[1].ptr, a[2].ptr, self.g)
         dm, dp = other.pm.device_meshes()
-        net.input_from_meshes(0, dm, dp, float(np.sum(s.dev[3].numpy(), dtype=np.float64)) / cells ** 3,
-                              other.phi_scale)
+        net.input_from_meshes(0, dm, dp, rho_mean, other.phi_scale)
         device_synchronize()
         t_pm = time.perf_counter() - t0
         t0 = time.perf_counter()

@@ -127,6 +127,13 @@ int gng_net_loss(gng_net* net, double* loss_out);
  * (float32 SoA device arrays). */
 int gng_net_apply(gng_net* net, int32_t box, const float* d_x, const float* d_y, const float* d_z, int64_t n,
                   double box_size, float out_scale, float* d_ax, float* d_ay, float* d_az, void* stream);
+/* gng_net_apply for the field only: particles whose base cell holds more
+ * than rho_max (d_rho = the box's mass per cell, nm^3 float32, the mesh kept
+ * by a plan created with keep_density) keep their force, as they carry no
+ * weight in the training loss.  d_rho = NULL applies to every particle. */
+int gng_net_apply_field(gng_net* net, int32_t box, const float* d_x, const float* d_y, const float* d_z,
+                        int64_t n, double box_size, float out_scale, const float* d_rho, double rho_max,
+                        float* d_ax, float* d_ay, float* d_az, void* stream);
 
 #ifdef __cplusplus
 }
