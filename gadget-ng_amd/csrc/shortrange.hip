@@ -8,7 +8,7 @@
 //
 // Method:
 //   1. particles are counted into a chaining mesh of nc^3 cells of side
-//      L/nc >= r_cut and scanned; a stable radix sort by cell (rocPRIM) puts
+//      L/nc >= r_cut and scanned; a stable radix sort by cell (sort.hpp) puts
 //      them in cell order with ascending particle index inside each cell, and
 //      they are gathered as float4 x,y,z,m plus the original index.  The
 //      order of every sum is then fixed, so the result is bit-identical from
@@ -28,10 +28,8 @@
 #include <new>
 #include <string>
 
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/iterator/counting_iterator.hpp>
-
 #include "../../include/gng_learned.h"
+#include "sort.hpp"
 #include "status.hpp"
 
 namespace gng {
@@ -210,7 +208,7 @@ struct gng_sr {
   unsigned* d_cell_of = nullptr;
   unsigned* d_cell_sorted = nullptr;  // radix-sort outputs
   unsigned* d_order = nullptr;
-  void* d_temp = nullptr;  // radix-sort work space
+  void* d_temp = nullptr;  // radix-sort work space (sized for cap_n)
   size_t temp_bytes = 0;
   long long cap_cells = 0;
   unsigned* d_cnt = nullptr;
@@ -234,6 +232,71 @@ void free_sr(gng_sr* s) {
   (void)hipFree(s->d_off);
   (void)hipFree(s->d_choff);
   (void)hipFree(s->d_pairs);
+}
+
+}  // namespace
+
+namespace {
+
+// Chaining-mesh bookkeeping shared by the entry points: cell of every
+// particle, per-cell counts and offsets, chunk offsets, and the stable
+// cell-sorted permutation d_order.  Returns nc through *nc_out.
+int cell_sort(gng_sr* sr, const float* d_x, const float* d_y, const float* d_z, int64_t n, double box_size,
+              double r_cut, hipStream_t s, int* nc_out) {
+  int nc = (int)std::floor(box_size / r_cut);
+  if (nc > 256) nc = 256;
+  const long long ncell = (long long)nc * nc * nc;
+  if (n > sr->cap_n) {
+    (void)hipFree(sr->d_sorted);
+    (void)hipFree(sr->d_sidx);
+    (void)hipFree(sr->d_cell_of);
+    (void)hipFree(sr->d_cell_sorted);
+    (void)hipFree(sr->d_order);
+    (void)hipFree(sr->d_temp);
+    sr->d_sorted = nullptr;
+    sr->d_sidx = sr->d_cell_of = sr->d_cell_sorted = sr->d_order = nullptr;
+    sr->d_temp = nullptr;
+    sr->cap_n = 0;
+    GNG_HIP(hipMalloc(&sr->d_sorted, sizeof(float4) * (size_t)n));
+    GNG_HIP(hipMalloc(&sr->d_sidx, sizeof(unsigned) * (size_t)n));
+    GNG_HIP(hipMalloc(&sr->d_cell_of, sizeof(unsigned) * (size_t)n));
+    GNG_HIP(hipMalloc(&sr->d_cell_sorted, sizeof(unsigned) * (size_t)n));
+    GNG_HIP(hipMalloc(&sr->d_order, sizeof(unsigned) * (size_t)n));
+    GNG_HIP(hipMalloc(&sr->d_temp, gng::radix::sort_workspace(n)));
+    sr->cap_n = n;
+  }
+  if (ncell > sr->cap_cells) {
+    (void)hipFree(sr->d_cnt);
+    (void)hipFree(sr->d_off);
+    (void)hipFree(sr->d_choff);
+    sr->d_cnt = sr->d_off = sr->d_choff = nullptr;
+    sr->cap_cells = 0;
+    GNG_HIP(hipMalloc(&sr->d_cnt, sizeof(unsigned) * (size_t)ncell));
+    GNG_HIP(hipMalloc(&sr->d_off, sizeof(unsigned) * (size_t)(ncell + 1)));
+    GNG_HIP(hipMalloc(&sr->d_choff, sizeof(unsigned) * (size_t)(ncell + 1)));
+    sr->cap_cells = ncell;
+  }
+  const unsigned cb = (unsigned)((ncell + 255) / 256), pb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_fill_u32, dim3(cb), dim3(256), 0, s, sr->d_cnt, ncell, 0u);
+  const float cs = (float)((double)nc / box_size);
+  hipLaunchKernelGGL(k_count, dim3(pb), dim3(256), 0, s, d_x, d_y, d_z, (long long)n, nc, cs, sr->d_cnt,
+                     sr->d_cell_of);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, sr->d_cnt, ncell, sr->d_off, sr->d_choff);
+  GNG_HIP(hipGetLastError());
+  // stable LSD radix sort over the bits a cell index needs
+  int end_bit = 1;
+  while ((1LL << end_bit) < ncell) ++end_bit;
+  GNG_HIP(gng::radix::sort_pairs(sr->d_cell_of, n, end_bit, sr->d_cell_sorted, sr->d_order, sr->d_temp, s));
+  *nc_out = nc;
+  return GNG_OK;
+}
+
+int check_geometry(int64_t n, double box_size, double r_cut) {
+  if (n < 0) return fail(GNG_ERR_INVALID, "negative particle count");
+  if (n >= (1LL << 31)) return fail(GNG_ERR_INVALID, "n of 2^31 or more is not supported");
+  if (!(box_size > 0.0) || !(r_cut > 0.0)) return fail(GNG_ERR_INVALID, "box_size and r_cut must be positive");
+  if (box_size < 3.0 * r_cut) return fail(GNG_ERR_INVALID, "box_size must be at least 3 r_cut");
+  return GNG_OK;
 }
 
 }  // namespace
@@ -266,76 +329,37 @@ void gng_sr_destroy(gng_sr* sr) {
   delete sr;
 }
 
+
+int gng_sr_cell_order(gng_sr* sr, const float* d_x, const float* d_y, const float* d_z, int64_t n,
+                      double box_size, double r_cut, uint32_t* d_order, void* stream) {
+  if (!sr) return fail(GNG_ERR_INVALID, "null short-range context");
+  if (int e = check_geometry(n, box_size, r_cut)) return e;
+  if (n == 0) return GNG_OK;
+  if (!d_x || !d_y || !d_z || !d_order) return fail(GNG_ERR_INVALID, "null array");
+  GNG_HIP(hipSetDevice(sr->device));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  int nc = 0;
+  if (int e = cell_sort(sr, d_x, d_y, d_z, n, box_size, r_cut, s, &nc)) return e;
+  GNG_HIP(hipMemcpyAsync(d_order, sr->d_order, sizeof(unsigned) * (size_t)n, hipMemcpyDeviceToDevice, s));
+  return GNG_OK;
+}
+
 int gng_sr_accelerations(gng_sr* sr, const float* d_x, const float* d_y, const float* d_z, const float* d_m,
                          int64_t n, double box_size, double r_split, double r_cut, double eps2, double g,
                          int32_t kernel, float* d_ax, float* d_ay, float* d_az, void* stream) {
   if (!sr) return fail(GNG_ERR_INVALID, "null short-range context");
-  if (n < 0) return fail(GNG_ERR_INVALID, "negative particle count");
-  if (n >= (1LL << 31)) return fail(GNG_ERR_INVALID, "n of 2^31 or more is not supported");
-  if (!(box_size > 0.0) || !(r_split > 0.0) || !(r_cut > 0.0) || !(eps2 >= 0.0))
-    return fail(GNG_ERR_INVALID, "box_size, r_split, r_cut must be positive and eps2 non-negative");
+  if (int e = check_geometry(n, box_size, r_cut)) return e;
+  if (!(r_split > 0.0) || !(eps2 >= 0.0)) return fail(GNG_ERR_INVALID, "r_split must be positive and eps2 non-negative");
   if (kernel != GNG_SR_EXACT && kernel != GNG_SR_REFERENCE) return fail(GNG_ERR_INVALID, "unknown split kernel");
-  if (box_size < 3.0 * r_cut) return fail(GNG_ERR_INVALID, "box_size must be at least 3 r_cut");
   if (n == 0) return GNG_OK;
   if (!d_x || !d_y || !d_z || !d_m || !d_ax || !d_ay || !d_az) return fail(GNG_ERR_INVALID, "null array");
   GNG_HIP(hipSetDevice(sr->device));
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-
-  int nc = (int)std::floor(box_size / r_cut);
-  if (nc > 256) nc = 256;
+  int nc = 0;
+  if (int e = cell_sort(sr, d_x, d_y, d_z, n, box_size, r_cut, s, &nc)) return e;
   const long long ncell = (long long)nc * nc * nc;
-  if (n > sr->cap_n) {
-    (void)hipFree(sr->d_sorted);
-    (void)hipFree(sr->d_sidx);
-    (void)hipFree(sr->d_cell_of);
-    (void)hipFree(sr->d_cell_sorted);
-    (void)hipFree(sr->d_order);
-    sr->d_sorted = nullptr;
-    sr->d_sidx = sr->d_cell_of = sr->d_cell_sorted = sr->d_order = nullptr;
-    sr->cap_n = 0;
-    GNG_HIP(hipMalloc(&sr->d_sorted, sizeof(float4) * (size_t)n));
-    GNG_HIP(hipMalloc(&sr->d_sidx, sizeof(unsigned) * (size_t)n));
-    GNG_HIP(hipMalloc(&sr->d_cell_of, sizeof(unsigned) * (size_t)n));
-    GNG_HIP(hipMalloc(&sr->d_cell_sorted, sizeof(unsigned) * (size_t)n));
-    GNG_HIP(hipMalloc(&sr->d_order, sizeof(unsigned) * (size_t)n));
-    sr->cap_n = n;
-  }
-  // stable LSD radix sort over the bits a cell index needs
-  unsigned end_bit = 1;
-  while ((1LL << end_bit) < ncell) ++end_bit;
-  const rocprim::counting_iterator<unsigned> iota(0u);
-  size_t need = 0;
-  GNG_HIP(rocprim::radix_sort_pairs(nullptr, need, sr->d_cell_of, sr->d_cell_sorted, iota, sr->d_order,
-                                    (unsigned)n, 0u, end_bit, s));
-  if (need > sr->temp_bytes) {
-    (void)hipFree(sr->d_temp);
-    sr->d_temp = nullptr;
-    sr->temp_bytes = 0;
-    GNG_HIP(hipMalloc(&sr->d_temp, need));
-    sr->temp_bytes = need;
-  }
-  if (ncell > sr->cap_cells) {
-    (void)hipFree(sr->d_cnt);
-    (void)hipFree(sr->d_off);
-    (void)hipFree(sr->d_choff);
-    sr->d_cnt = sr->d_off = sr->d_choff = nullptr;
-    sr->cap_cells = 0;
-    GNG_HIP(hipMalloc(&sr->d_cnt, sizeof(unsigned) * (size_t)ncell));
-    GNG_HIP(hipMalloc(&sr->d_off, sizeof(unsigned) * (size_t)(ncell + 1)));
-    GNG_HIP(hipMalloc(&sr->d_choff, sizeof(unsigned) * (size_t)(ncell + 1)));
-    sr->cap_cells = ncell;
-  }
-  const unsigned cb = (unsigned)((ncell + 255) / 256), pb = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_fill_u32, dim3(cb), dim3(256), 0, s, sr->d_cnt, ncell, 0u);
+  const unsigned pb = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s, sr->d_pairs);
-  const float cs = (float)((double)nc / box_size);
-  hipLaunchKernelGGL(k_count, dim3(pb), dim3(256), 0, s, d_x, d_y, d_z, (long long)n, nc, cs, sr->d_cnt,
-                     sr->d_cell_of);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, sr->d_cnt, ncell, sr->d_off, sr->d_choff);
-  GNG_HIP(hipGetLastError());
-  size_t tb = sr->temp_bytes;
-  GNG_HIP(rocprim::radix_sort_pairs(sr->d_temp, tb, sr->d_cell_of, sr->d_cell_sorted, iota, sr->d_order, (unsigned)n,
-                                    0u, end_bit, s));
   hipLaunchKernelGGL(k_gather, dim3(pb), dim3(256), 0, s, d_x, d_y, d_z, d_m, (long long)n, sr->d_order,
                      sr->d_sorted, sr->d_sidx);
   GNG_HIP(hipGetLastError());

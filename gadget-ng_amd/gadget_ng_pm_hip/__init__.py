@@ -194,6 +194,8 @@ def lib():
         L.gng_sr_accelerations.restype = i32
         L.gng_sr_last_pair_count.argtypes = [P]
         L.gng_sr_last_pair_count.restype = i64
+        L.gng_sr_cell_order.argtypes = [P, P, P, P, i64, d, d, P, P]
+        L.gng_sr_cell_order.restype = i32
         L.gng_net_param_count.restype = i32
         L.gng_net_create.argtypes = [i32, i32, i32, ctypes.POINTER(P)]
         L.gng_net_destroy.argtypes = [P]

@@ -131,6 +131,17 @@ class ShortRange:
     def last_pair_count(self) -> int:
         return int(lib().gng_sr_last_pair_count(self._h))
 
+    def cell_order(self, pos, box_size: float, r_cut: float) -> np.ndarray:
+        """The stable cell-sorted permutation the pair sums visit (uint32)."""
+        pos = np.asarray(pos).reshape(-1, 3)
+        n = pos.shape[0]
+        d = [DeviceArray.from_host(np.ascontiguousarray(pos[:, a], dtype=np.float32)) for a in range(3)]
+        out = DeviceArray((n,), np.uint32)
+        _check(lib().gng_sr_cell_order(self._h, d[0].ptr, d[1].ptr, d[2].ptr, int(n), float(box_size),
+                                       float(r_cut), out.ptr, None), "gng_sr_cell_order")
+        device_synchronize()
+        return out.numpy()
+
 
 class TreePmSolver:
     """TreePmSolver (crates/gadget-ng-treepm/src/solver.rs) on one GPU.

@@ -65,6 +65,16 @@ int gng_sr_accelerations(gng_sr* sr, const float* d_x, const float* d_y, const f
 /* Pair interactions evaluated by the last call (inside and outside r_cut). */
 int64_t gng_sr_last_pair_count(const gng_sr* sr);
 
+/* The order in which gng_sr_accelerations() visits the particles: d_order[j]
+ * (uint32, n entries) is the index of the j-th particle sorted by chaining-
+ * mesh cell (nc = min(256, floor(box_size / r_cut)) cells per side, cell
+ * (cz * nc + cy) * nc + cx of floor(x nc / L) mod nc), equal cells in
+ * ascending index.  Stable and identical from run to run.  The reference has
+ * no counterpart; the order fixes every pair sum's order, and callers can
+ * use it to lay particles out in cell order. */
+int gng_sr_cell_order(gng_sr* sr, const float* d_x, const float* d_y, const float* d_z, int64_t n,
+                      double box_size, double r_cut, uint32_t* d_order, void* stream);
+
 /* ---- learned correction -------------------------------------------------- */
 
 enum gng_net_buffer {

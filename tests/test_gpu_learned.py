@@ -106,6 +106,25 @@ def test_short_range_is_bit_identical_from_call_to_call(learned):
     assert np.array_equal(a, b) and np.array_equal(b, c)
 
 
+@pytest.mark.parametrize("n,L,rc", [(5000, 20.0, 2.5), (300_000, 100.0, 6.25), (70_000, 256.0, 1.0),
+                                    (4096, 7.5, 2.5)])
+def test_cell_order_is_the_stable_sort_by_cell(learned, n, L, rc):
+    """The project's radix sort against numpy's stable argsort, for 1 (3^3
+    cells), 2 (16^3) and 3 (256^3) passes of 8 bits, partial and exact
+    4096-pair tiles, many particles per cell and wrapped positions."""
+    print(f"case short range cell order n={n} L={L} r_cut={rc}", flush=True)
+    g = np.random.default_rng(n)
+    pos, _ = _clustered(n // 2, n - n // 2, L, n % 97)
+    pos = pos.astype(np.float32)
+    pos[: n // 10] += np.float32(L) * g.integers(-2, 3, (n // 10, 3)).astype(np.float32)
+    nc = min(256, int(np.floor(L / rc)))
+    cs = np.float32(nc / L)
+    cell = np.floor(pos * cs).astype(np.int64) % nc
+    key = (cell[:, 2] * nc + cell[:, 1]) * nc + cell[:, 0]
+    got = learned.ShortRange(device=0).cell_order(pos, L, rc)
+    assert np.array_equal(got, np.argsort(key, kind="stable").astype(np.uint32))
+
+
 def test_treepm_long_plus_short_is_newton(pm, learned):
     g = np.random.default_rng(5)
     L, n, nm = 1.0, 12, 64
