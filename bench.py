@@ -936,11 +936,11 @@ def bench_ic(args) -> int:
 
 
 BF16_DENSE_PEAK_TFLOPS = 2500.0
-NET_FLOP_PER_CELL = 2 * 27 * (2 * 16 + 16 * 16 + 16 * 3)  # 18 144
-# compulsory HBM bytes per cell of one inference: x0 in (2 bf16), x1 out and
+NET_FLOP_PER_CELL = 2 * 27 * (8 * 16 + 16 * 16 + 16 * 3)  # 23 328
+# compulsory HBM bytes per cell of one inference: x0 in (8 bf16), x1 out and
 # in (16 bf16 each way), x2 out and in, y out (4 f32)
-NET_BYTES_PER_CELL = 4 + 32 + 32 + 32 + 32 + 16  # 148
-# 18 144 / 148 = 123 flop/byte, below the bf16 ridge of 2500 TFLOP/s / 8 TB/s = 312 flop/byte:
+NET_BYTES_PER_CELL = 16 + 32 + 32 + 32 + 32 + 16  # 160
+# 23 328 / 160 = 146 flop/byte, below the bf16 ridge of 2500 TFLOP/s / 8 TB/s = 312 flop/byte:
 # the roofline that bounds the network is HBM bandwidth
 
 
@@ -1008,6 +1008,7 @@ def bench_learned(args) -> int:
         "with_net": {k: round(mean(k, "pm_net"), 5) for k in ("field_rms_rel", "field_median_rel", "field_mean_rel",
                                                                 "rms_rel", "median_rel", "dense_median_rel")},
         "cost_s": {"network_forward_and_apply": round(t_net, 5), "short_range_sum": round(t_sr, 5),
+                   "network_input": round(float(np.mean([e["time_input_s"] for e in evals])), 5),
                    "pm": round(float(np.mean([e["time_pm_s"] for e in evals])), 5),
                    "treepm_long_range": round(float(np.mean([e["time_treepm_long_s"] for e in evals])), 5)},
         "roofline": {"bound": "hbm", "kernel": "network forward (3 convolutions)",

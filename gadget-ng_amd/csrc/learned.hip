@@ -1,12 +1,18 @@
 // learned.hip -- learned sub-cell force correction: a 3-layer network of
 // periodic 3^3 convolutions on the PM mesh, trained and run on the MFMA units.
 //
-// Layers: conv(2->16) SiLU, conv(16->16) SiLU, conv(16->3).  Activations and
+// Layers: conv(8->16) SiLU, conv(16->16) SiLU, conv(16->3).  Activations and
 // weights are bfloat16, products accumulate in FP32 with
 // v_mfma_f32_16x16x32_bf16.  The convolution is written as an implicit GEMM
 // per 16-cell x row:  D[cell][o] = sum_k A[cell][k] B[k][o], k = tap*CIN + c,
-// so one MFMA takes two taps of 16 channels (CIN = 16), eight taps of 4
-// channels or sixteen taps of 2 channels.  A workgroup stages its 16x8x8
+// so one MFMA takes two taps of 16 channels (CIN = 16), four taps of 8
+// channels or eight taps of 4.
+//
+// Input channels per cell (kIn = 8): the density contrast, the potential
+// minus its 7^3 box mean, and, from the particles, the mass-weighted offsets
+// of the particles from the cell centre (3 channels) and their mass-weighted
+// spread about their own centre of mass, all per mean cell mass; the last
+// two channels are zero.  A workgroup stages its 16x8x8
 // output tile plus a one-cell periodic halo in LDS once and keeps every
 // weight fragment of the layer in registers.  Inference skips the FP32
 // pre-activations that only the backward pass reads.
@@ -49,7 +55,8 @@ constexpr int WHX = WX + 2, WHY = WY + 2, WHZ = WZ + 2, WHC = WHX * WHY * WHZ;
 constexpr int kWgradBlocks = 512;
 
 // parameter layout (floats)
-constexpr int P_W1 = 0, P_B1 = P_W1 + 27 * 2 * 16, P_W2 = P_B1 + 16, P_B2 = P_W2 + 27 * 16 * 16,
+constexpr int kIn = 8;  // input channels (6 used)
+constexpr int P_W1 = 0, P_B1 = P_W1 + 27 * kIn * 16, P_W2 = P_B1 + 16, P_B2 = P_W2 + 27 * 16 * 16,
               P_W3 = P_B2 + 16, P_B3 = P_W3 + 27 * 16 * 3, P_COUNT = P_B3 + 3;
 
 // k-steps of 32 (forward and data-gradient convolutions)
@@ -89,8 +96,8 @@ __host__ __device__ constexpr int tap_off(int t, int hx, int hy) {
 // read one channel half of sixteen consecutive cells then read 256
 // consecutive bytes, which no two of them share a bank for.
 template <int CIN>
-struct HaloWord {  // 16-byte words for 16 channels, one word per cell otherwise
-  using T = typename std::conditional<CIN == 16, u32x4, typename std::conditional<CIN == 4, u32x2, unsigned>::type>::type;
+struct HaloWord {  // 16-byte words for 8 and 16 channels, one word per cell for fewer
+  using T = typename std::conditional<CIN >= 8, u32x4, typename std::conditional<CIN == 4, u32x2, unsigned>::type>::type;
   static constexpr int per_cell = CIN == 16 ? 2 : 1;
 };
 
@@ -177,11 +184,10 @@ __device__ __forceinline__ bf16x8 a_frag(const unsigned short* tile, int base, i
       v.w = b.y;
     }
   } else {
-    static_assert(CIN == 2, "CIN must be 2, 4 or 16");
-    const int t0 = 16 * s + 4 * g;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (t0 + q < 27) v[q] = *reinterpret_cast<const unsigned*>(tile + (base + tap_off(t0 + q, HX, HY)) * 2);
+    static_assert(CIN == 8, "CIN must be 4, 8 or 16");
+    // tap 4s + g, all 8 channels in one 16-byte word
+    const int t = 4 * s + g;
+    if (t < 27) v = reinterpret_cast<const u32x4*>(tile)[base + tap_off(t, HX, HY)];
   }
   return __builtin_bit_cast(bf16x8, v);
 }
@@ -482,17 +488,98 @@ __global__ void k_boxmean(const float* __restrict__ in, float* __restrict__ out,
   out[i] = s * (1.0f / (2 * kHalfWidth + 1));
 }
 
-// x0 = (mass / rho_mean - 1, (phi - box7(phi)) / phi_scale): the potential
-// minus its 7^3 box mean keeps the part set by mass within a few cells and
-// drops the box-scale modes, which carry no sub-cell information.
+// Particle moments per cell, as 64-bit fixed point (value * 2^shift) summed
+// with integer atomics, so the sums do not depend on the order of the adds:
+// mom[5 c + a] = sum m (f_a - 1/2) (a = x, y, z), mom[5 c + 3] =
+// sum m |f - 1/2|^2, mom[5 c + 4] = sum m, over the particles whose base cell
+// (the cell of cic_nodes) is c; f = in-cell fraction.  A workgroup first
+// merges its particles by cell in an LDS hash table (open addressing, twice
+// as many slots as particles, so every insert finds a slot), then adds one
+// set of sums per distinct cell to the mesh: particles crowded into a few
+// cells cost a few global atomics per workgroup instead of one set each.
+constexpr int kMoments = 5;
+constexpr int kMomThreads = 256, kMomSlots = 2 * kMomThreads;
+__global__ __launch_bounds__(kMomThreads) void k_moments(const float* __restrict__ x, const float* __restrict__ y,
+                                                         const float* __restrict__ z, const float* __restrict__ m,
+                                                         long long n, float cs, int nm, double scale,
+                                                         unsigned long long* __restrict__ mom) {
+  constexpr unsigned kEmpty = 0xFFFFFFFFu;
+  __shared__ unsigned key[kMomSlots];
+  __shared__ unsigned long long acc[kMomSlots][kMoments];
+  for (int i = threadIdx.x; i < kMomSlots; i += kMomThreads) {
+    key[i] = kEmpty;
+#pragma unroll
+    for (int a = 0; a < kMoments; ++a) acc[i][a] = 0ull;
+  }
+  __syncthreads();
+  const long long p = (long long)blockIdx.x * kMomThreads + threadIdx.x;
+  if (p < n) {
+    const float fx = x[p] * cs, fy = y[p] * cs, fz = z[p] * cs;
+    const float ix = floorf(fx), iy = floorf(fy), iz = floorf(fz);
+    const double dx = (double)(fx - ix) - 0.5, dy = (double)(fy - iy) - 0.5, dz = (double)(fz - iz) - 0.5;
+    int i0 = (int)ix % nm, j0 = (int)iy % nm, k0 = (int)iz % nm;
+    i0 += i0 < 0 ? nm : 0;
+    j0 += j0 < 0 ? nm : 0;
+    k0 += k0 < 0 ? nm : 0;
+    const unsigned c = (unsigned)(((long long)k0 * nm + j0) * nm + i0);  // nm^3 <= 2^30
+    unsigned h = (c * 2654435761u) >> 23;  // top 9 bits: a slot in [0, 512)
+    for (;;) {
+      const unsigned prev = atomicCAS(&key[h], kEmpty, c);
+      if (prev == kEmpty || prev == c) break;
+      h = (h + 1) & (kMomSlots - 1);
+    }
+    const double ms = (double)m[p] * scale;
+    atomicAdd(&acc[h][0], (unsigned long long)__double2ll_rn(ms * dx));
+    atomicAdd(&acc[h][1], (unsigned long long)__double2ll_rn(ms * dy));
+    atomicAdd(&acc[h][2], (unsigned long long)__double2ll_rn(ms * dz));
+    atomicAdd(&acc[h][3], (unsigned long long)__double2ll_rn(ms * (dx * dx + dy * dy + dz * dz)));
+    atomicAdd(&acc[h][4], (unsigned long long)__double2ll_rn(ms));
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kMomSlots; i += kMomThreads) {
+    const unsigned c = key[i];
+    if (c == kEmpty) continue;
+    unsigned long long* dst = mom + (long long)c * kMoments;
+#pragma unroll
+    for (int a = 0; a < kMoments; ++a) atomicAdd(dst + a, acc[i][a]);
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// x0 channels: mass / rho_mean - 1, (phi - box7(phi)) / phi_scale, and from
+// the moments (when mom is not null, else zeros) D_a / rho_mean for
+// D = sum m (f - 1/2), and (sum m |f - 1/2|^2 - |D|^2 / sum m) / rho_mean,
+// the spread about the particles' centre of mass; channels 6 and 7 are zero.
+// The potential minus its 7^3 box mean keeps the part set by mass within a
+// few cells and drops the box-scale modes, which carry no sub-cell
+// information.
 __global__ void k_input(const float* __restrict__ rho, const float* __restrict__ phi,
-                        const float* __restrict__ phi_mean, long long M, float inv_rho, float inv_phi,
-                        unsigned* __restrict__ x0) {
+                        const float* __restrict__ phi_mean, const unsigned long long* __restrict__ mom,
+                        double mom_unit, long long M, float inv_rho, float inv_phi, u32x4* __restrict__ x0) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
-  const unsigned lo = f2bf(rho[i] * inv_rho - 1.0f);
-  const unsigned hi = f2bf((phi[i] - phi_mean[i]) * inv_phi);
-  x0[i] = lo | (hi << 16);
+  float c2 = 0.0f, c3 = 0.0f, c4 = 0.0f, c5 = 0.0f;
+  if (mom) {
+    // mom_unit = 2^-shift / rho_mean
+    const long long* q = reinterpret_cast<const long long*>(mom) + i * kMoments;
+    const double dx = (double)q[0] * mom_unit, dy = (double)q[1] * mom_unit, dz = (double)q[2] * mom_unit;
+    const double m2 = (double)q[3] * mom_unit, m0 = (double)q[4] * mom_unit;
+    const double spread = m0 > 0.0 ? m2 - (dx * dx + dy * dy + dz * dz) / m0 : 0.0;
+    c2 = (float)dx;
+    c3 = (float)dy;
+    c4 = (float)dz;
+    c5 = (float)(spread > 0.0 ? spread : 0.0);
+  }
+  u32x4 v;
+  v.x = pack_bf16(rho[i] * inv_rho - 1.0f, (phi[i] - phi_mean[i]) * inv_phi);
+  v.y = pack_bf16(c2, c3);
+  v.z = pack_bf16(c4, c5);
+  v.w = 0u;
+  x0[i] = v;
 }
 
 struct Cic {
@@ -663,6 +750,7 @@ struct gng_net {
   double* d_loss = nullptr;
   long long loss_n = 0;
   float* d_tmp = nullptr;  // 2 x M floats for the input's box means
+  unsigned long long* d_mom = nullptr;  // kMoments x M fixed-point particle moments (allocated on first use)
 };
 
 namespace {
@@ -672,7 +760,7 @@ hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 void free_net(gng_net* t) {
   void* ptrs[] = {t->d_params, t->d_grad, t->d_m1, t->d_m2, t->d_f1, t->d_f2, t->d_f3, t->d_g3, t->d_g2,
                   t->d_b16, t->d_x0, t->d_z1, t->d_x1, t->d_z2, t->d_x2, t->d_y, t->d_gy, t->d_dz3,
-                  t->d_dz2, t->d_dz1, t->d_partial, t->d_loss, t->d_tmp};
+                  t->d_dz2, t->d_dz1, t->d_partial, t->d_loss, t->d_tmp, t->d_mom};
   for (void* p : ptrs) (void)hipFree(p);
 }
 
@@ -684,7 +772,7 @@ struct BufInfo {
 BufInfo buffer(const gng_net* t, int which) {
   const long long cells = t->M * t->max_boxes;
   switch (which) {
-    case GNG_NET_X0: return {t->d_x0, cells * 2 * 2};
+    case GNG_NET_X0: return {t->d_x0, cells * kIn * 2};
     case GNG_NET_Z1: return {t->d_z1, cells * 16 * 4};
     case GNG_NET_X1: return {t->d_x1, cells * 16 * 2};
     case GNG_NET_Z2: return {t->d_z2, cells * 16 * 4};
@@ -702,8 +790,8 @@ BufInfo buffer(const gng_net* t, int which) {
 
 int repack(gng_net* t, hipStream_t s) {
   const float* p = t->d_params;
-  constexpr int K2 = KSteps<2>::v, K4 = KSteps<4>::v, K16 = KSteps<16>::v;
-  hipLaunchKernelGGL(k_pack, dim3(K2), dim3(64), 0, s, p + P_W1, 2, 16, 2, K2, 0, t->d_f1);
+  constexpr int K1 = KSteps<kIn>::v, K4 = KSteps<4>::v, K16 = KSteps<16>::v;
+  hipLaunchKernelGGL(k_pack, dim3(K1), dim3(64), 0, s, p + P_W1, kIn, 16, kIn, K1, 0, t->d_f1);
   hipLaunchKernelGGL(k_pack, dim3(K16), dim3(64), 0, s, p + P_W2, 16, 16, 16, K16, 0, t->d_f2);
   hipLaunchKernelGGL(k_pack, dim3(K16), dim3(64), 0, s, p + P_W3, 16, 3, 16, K16, 0, t->d_f3);
   hipLaunchKernelGGL(k_pack, dim3(K4), dim3(64), 0, s, p + P_W3, 16, 3, 4, K4, 1, t->d_g3);
@@ -755,13 +843,13 @@ int gng_net_create(int32_t grid_size, int32_t max_boxes, int32_t device, gng_net
   A(&t->d_grad, P_COUNT * 4);
   A(&t->d_m1, P_COUNT * 4);
   A(&t->d_m2, P_COUNT * 4);
-  A(&t->d_f1, KSteps<2>::v * 64 * 16);
+  A(&t->d_f1, KSteps<kIn>::v * 64 * 16);
   A(&t->d_f2, KSteps<16>::v * 64 * 16);
   A(&t->d_f3, KSteps<16>::v * 64 * 16);
   A(&t->d_g3, KSteps<4>::v * 64 * 16);
   A(&t->d_g2, KSteps<16>::v * 64 * 16);
   A(&t->d_b16, 48 * 4);
-  A(&t->d_x0, cells * 4);
+  A(&t->d_x0, cells * kIn * 2);
   A(&t->d_z1, cells * 64);
   A(&t->d_x1, cells * 32);
   A(&t->d_z2, cells * 64);
@@ -865,24 +953,50 @@ int gng_net_write(gng_net* net, int32_t which, const void* host, int64_t bytes) 
   return GNG_OK;
 }
 
-int gng_net_input(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
-                  float phi_scale, void* stream) {
+int gng_net_input_particles(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
+                            float phi_scale, const float* d_x, const float* d_y, const float* d_z, const float* d_m,
+                            int64_t n, double box_size, void* stream) {
   if (!net || !d_rho || !d_phi) return fail(GNG_ERR_INVALID, "null argument");
   if (box < 0 || box >= net->max_boxes) return fail(GNG_ERR_INVALID, "box slot out of range");
   if (!(rho_mean > 0.0f) || !(phi_scale > 0.0f)) return fail(GNG_ERR_INVALID, "scales must be positive");
+  const bool particles = n > 0;
+  if (n < 0) return fail(GNG_ERR_INVALID, "negative particle count");
+  if (particles && (!d_x || !d_y || !d_z || !d_m)) return fail(GNG_ERR_INVALID, "null particle array");
+  if (particles && !(box_size > 0.0)) return fail(GNG_ERR_INVALID, "box_size must be positive");
   GNG_HIP(hipSetDevice(net->device));
   hipStream_t s = as_stream(stream);
   const dim3 g((unsigned)((net->M + 255) / 256));
   const int nm = net->nm;
+  double mom_unit = 0.0;
+  if (particles) {
+    if (!net->d_mom) GNG_HIP(hipMalloc(&net->d_mom, sizeof(unsigned long long) * kMoments * (size_t)net->M));
+    // every per-cell sum is at most the total mass rho_mean * M (non-negative
+    // masses), kept below 2^61
+    const double total = (double)rho_mean * (double)net->M;
+    const int shift = 61 - (int)std::ceil(std::log2(total));
+    const long long words = kMoments * net->M;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, net->d_mom, words);
+    hipLaunchKernelGGL(k_moments, dim3((unsigned)((n + kMomThreads - 1) / kMomThreads)), dim3(kMomThreads), 0, s,
+                       d_x, d_y, d_z, d_m,
+                       (long long)n, (float)((double)nm / box_size), nm, std::ldexp(1.0, shift), net->d_mom);
+    mom_unit = std::ldexp(1.0, -shift) / (double)rho_mean;
+  }
   float* t0 = net->d_tmp;
   float* t1 = net->d_tmp + net->M;
   hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, d_phi, t0, nm, 1LL);
   hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, t0, t1, nm, (long long)nm);
   hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, t1, t0, nm, (long long)nm * nm);
-  hipLaunchKernelGGL(k_input, g, dim3(256), 0, s, d_rho, d_phi, t0, net->M, 1.0f / rho_mean, 1.0f / phi_scale,
-                     reinterpret_cast<unsigned*>(net->d_x0) + (long long)box * net->M);
+  hipLaunchKernelGGL(k_input, g, dim3(256), 0, s, d_rho, d_phi, t0,
+                     particles ? (const unsigned long long*)net->d_mom : nullptr, mom_unit, net->M, 1.0f / rho_mean,
+                     1.0f / phi_scale, reinterpret_cast<u32x4*>(net->d_x0) + (long long)box * net->M);
   GNG_HIP(hipGetLastError());
   return GNG_OK;
+}
+
+int gng_net_input(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
+                  float phi_scale, void* stream) {
+  return gng_net_input_particles(net, box, d_rho, d_phi, rho_mean, phi_scale, nullptr, nullptr, nullptr, nullptr, 0,
+                                 0.0, stream);
 }
 
 int gng_net_forward(gng_net* net, int32_t boxes, void* stream) {
@@ -892,7 +1006,7 @@ int gng_net_forward(gng_net* net, int32_t boxes, void* stream) {
   GNG_HIP(hipSetDevice(net->device));
   hipStream_t s = as_stream(stream);
   const dim3 g = conv_grid(net, boxes);
-  hipLaunchKernelGGL((k_conv<2, MODE_HIDDEN>), g, dim3(kConvThreads), 0, s, net->d_x0, net->d_f1, net->d_b16, net->d_z1,
+  hipLaunchKernelGGL((k_conv<kIn, MODE_HIDDEN>), g, dim3(kConvThreads), 0, s, net->d_x0, net->d_f1, net->d_b16, net->d_z1,
                      net->d_x1, (const float*)nullptr, net->nm);
   hipLaunchKernelGGL((k_conv<16, MODE_HIDDEN>), g, dim3(kConvThreads), 0, s, net->d_x1, net->d_f2, net->d_b16 + 16,
                      net->d_z2, net->d_x2, (const float*)nullptr, net->nm);
@@ -909,7 +1023,7 @@ int gng_net_infer(gng_net* net, int32_t boxes, void* stream) {
   GNG_HIP(hipSetDevice(net->device));
   hipStream_t s = as_stream(stream);
   const dim3 g = conv_grid(net, boxes);
-  hipLaunchKernelGGL((k_conv<2, MODE_INFER>), g, dim3(kConvThreads), 0, s, net->d_x0, net->d_f1, net->d_b16, (float*)nullptr,
+  hipLaunchKernelGGL((k_conv<kIn, MODE_INFER>), g, dim3(kConvThreads), 0, s, net->d_x0, net->d_f1, net->d_b16, (float*)nullptr,
                      net->d_x1, (const float*)nullptr, net->nm);
   hipLaunchKernelGGL((k_conv<16, MODE_INFER>), g, dim3(kConvThreads), 0, s, net->d_x1, net->d_f2, net->d_b16 + 16,
                      (float*)nullptr, net->d_x2, (const float*)nullptr, net->nm);
@@ -968,10 +1082,10 @@ int gng_net_backward(gng_net* net, int32_t boxes, void* stream) {
   hipLaunchKernelGGL((k_conv<16, MODE_BWD>), g, dim3(kConvThreads), 0, s, net->d_dz2, net->d_g2, (const float*)nullptr,
                      (float*)nullptr, net->d_dz1, net->d_z1, nm);
   // layer 1
-  hipLaunchKernelGGL((k_conv_wgrad<2, 16>), dim3(wb), dim3(256), 0, s, net->d_x0, net->d_dz1, net->d_partial, nm,
+  hipLaunchKernelGGL((k_conv_wgrad<kIn, 16>), dim3(wb), dim3(256), 0, s, net->d_x0, net->d_dz1, net->d_partial, nm,
                      boxes);
-  hipLaunchKernelGGL((k_wgrad_reduce<2, 16>), dim3((55 * 16 + 255) / 256), dim3(256), 0, s, net->d_partial, wb,
-                     gr + P_W1, gr + P_B1);
+  hipLaunchKernelGGL((k_wgrad_reduce<kIn, 16>), dim3(((27 * kIn + 1) * 16 + 255) / 256), dim3(256), 0, s,
+                     net->d_partial, wb, gr + P_W1, gr + P_B1);
   GNG_HIP(hipGetLastError());
   return GNG_OK;
 }

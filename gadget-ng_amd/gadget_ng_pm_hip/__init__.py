@@ -209,6 +209,7 @@ def lib():
         L.gng_net_device_buffer.argtypes = [P, i32, ctypes.POINTER(P)]
         L.gng_net_device_buffer.restype = i32
         L.gng_net_input.argtypes = [P, i32, P, P, f32, f32, P]
+        L.gng_net_input_particles.argtypes = [P, i32, P, P, f32, f32, P, P, P, P, i64, d, P]
         L.gng_net_forward.argtypes = [P, i32, P]
         L.gng_net_infer.argtypes = [P, i32, P]
         L.gng_net_infer.restype = i32
@@ -220,8 +221,9 @@ def lib():
         L.gng_net_apply.argtypes = [P, i32, P, P, P, i64, d, f32, P, P, P, P]
         L.gng_net_apply_field.argtypes = [P, i32, P, P, P, i64, d, f32, P, d, P, P, P, P]
         for f in (L.gng_net_create, L.gng_net_set_params, L.gng_net_get_params, L.gng_net_read, L.gng_net_write,
-                  L.gng_net_input, L.gng_net_forward, L.gng_net_loss_grad, L.gng_net_backward, L.gng_net_adam,
-                  L.gng_net_train_step, L.gng_net_loss, L.gng_net_apply, L.gng_net_apply_field):
+                  L.gng_net_input, L.gng_net_input_particles, L.gng_net_forward, L.gng_net_loss_grad,
+                  L.gng_net_backward, L.gng_net_adam, L.gng_net_train_step, L.gng_net_loss, L.gng_net_apply,
+                  L.gng_net_apply_field):
             f.restype = i32
         L.gng_pm64_create.argtypes = [ctypes.POINTER(GngConfig), ctypes.POINTER(P)]
         L.gng_pm64_destroy.argtypes = [P]

@@ -32,9 +32,10 @@ SR_REFERENCE = 1
 BUFFERS = {"x0": 0, "z1": 1, "x1": 2, "z2": 3, "x2": 4, "y": 5, "gy": 6, "dz3": 7, "dz2": 8, "dz1": 9,
            "grad": 10, "params": 11}
 # (dtype, channels) of the mesh buffers
-_MESH = {"x0": ("bf16", 2), "z1": ("f32", 16), "x1": ("bf16", 16), "z2": ("f32", 16), "x2": ("bf16", 16),
+_MESH = {"x0": ("bf16", 8), "z1": ("f32", 16), "x1": ("bf16", 16), "z2": ("f32", 16), "x2": ("bf16", 16),
          "y": ("f32", 4), "gy": ("f32", 4), "dz3": ("bf16", 4), "dz2": ("bf16", 16), "dz1": ("bf16", 16)}
-LAYERS = ((2, 16), (16, 16), (16, 3))
+LAYERS = ((8, 16), (16, 16), (16, 3))
+X0_CHANNELS = LAYERS[0][0]
 
 
 def _check(rc: int, what: str) -> None:
@@ -264,6 +265,13 @@ class Net:
         _check(lib().gng_net_input(self._h, int(box), d_mass, d_phi, float(rho_mean), float(phi_scale),
                                    stream or None), "gng_net_input")
 
+    def input_from_particles(self, box: int, d_mass: int, d_phi: int, rho_mean: float, phi_scale: float,
+                             x: int, y: int, z: int, m: int, n: int, box_size: float, stream: int = 0) -> None:
+        """Input with the particle-moment channels (gng_net_input_particles)."""
+        _check(lib().gng_net_input_particles(self._h, int(box), d_mass, d_phi, float(rho_mean), float(phi_scale),
+                                             x, y, z, m, int(n), float(box_size), stream or None),
+               "gng_net_input_particles")
+
     def forward(self, boxes: int = 1, stream: int = 0) -> None:
         _check(lib().gng_net_forward(self._h, int(boxes), stream or None), "gng_net_forward")
 
@@ -412,7 +420,7 @@ class LearnedCorrection:
         t0 = time.perf_counter()
         self.pm.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, a_pm[0].ptr, a_pm[1].ptr, a_pm[2].ptr, self.g)
         dm, dp = self.pm.device_meshes()
-        net.input_from_meshes(slot, dm, dp, rho_mean, self.phi_scale)
+        net.input_from_particles(slot, dm, dp, rho_mean, self.phi_scale, x.ptr, y.ptr, z.ptr, m.ptr, n, self.L)
         device_synchronize()
         t1 = time.perf_counter()
         self.lr.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, a_tp[0].ptr, a_tp[1].ptr, a_tp[2].ptr, self.g)
@@ -476,8 +484,8 @@ class LearnedCorrection:
                 p4, t4 = self._records(s, j)
                 pos_l.append(p4)
                 tgt_l.append(t4)
-            x0 = DeviceArray((len(grp) * M * 2,), np.uint16)
-            _dcopy(x0.ptr, self.net.device_buffer("x0"), len(grp) * M * 4)
+            x0 = DeviceArray((len(grp) * M * X0_CHANNELS,), np.uint16)
+            _dcopy(x0.ptr, self.net.device_buffer("x0"), len(grp) * M * X0_CHANNELS * 2)
             pos4 = DeviceArray.from_host(np.concatenate(pos_l))
             tgt4 = DeviceArray.from_host(np.concatenate(tgt_l))
             self._groups.append((len(grp), x0, pos4, tgt4, pos4.shape[0]))
@@ -496,7 +504,7 @@ class LearnedCorrection:
         hist = []
         for it in range(steps):
             nb, x0, pos4, tgt4, n = self._groups[it % len(self._groups)]
-            _dcopy(self.net.device_buffer("x0"), x0.ptr, nb * M * 4)
+            _dcopy(self.net.device_buffer("x0"), x0.ptr, nb * M * X0_CHANNELS * 2)
             cur_lr = lr * (0.1 if it >= int(0.8 * steps) else 1.0)
             self.net.train_step(nb, pos4.ptr, tgt4.ptr, n, self.L, self.f0, cur_lr)
             if log_every and (it % log_every == 0 or it == steps - 1):
@@ -511,7 +519,7 @@ class LearnedCorrection:
         M = self.nm ** 3
         tot = cnt = 0.0
         for nb, x0, pos4, tgt4, n in self._groups:
-            _dcopy(self.net.device_buffer("x0"), x0.ptr, nb * M * 4)
+            _dcopy(self.net.device_buffer("x0"), x0.ptr, nb * M * X0_CHANNELS * 2)
             self.net.forward(nb)
             self.net.loss_grad(pos4.ptr, tgt4.ptr, n, self.L, self.f0)
             tot += self.net.loss() * n
@@ -535,27 +543,33 @@ class LearnedCorrection:
         n = s.n
         corr = [DeviceArray.from_host(np.ascontiguousarray(s.f_pm[:, a])) for a in range(3)]
         x, y, z, m = s.dev
-        device_synchronize()
-        t0 = time.perf_counter()
-        rho_mean = float(np.sum(s.dev[3].numpy(), dtype=np.float64)) / cells ** 3
+        rho_mean = float(np.sum(m.numpy(), dtype=np.float64)) / cells ** 3
+        rho_max = (1.0 + self.dense_contrast) * rho_mean
         dm, _ = other.pm.device_meshes()
         net.infer(1)
         # the correction goes to the field only: dense-cell particles carry no
         # weight in the loss and keep the PM force
-        net.apply_field(0, x.ptr, y.ptr, z.ptr, n, other.L, self.f0, dm, (1.0 + self.dense_contrast) * rho_mean,
-                        corr[0].ptr, corr[1].ptr, corr[2].ptr)
+        net.apply_field(0, x.ptr, y.ptr, z.ptr, n, other.L, self.f0, dm, rho_max, corr[0].ptr, corr[1].ptr,
+                        corr[2].ptr)
         device_synchronize()
-        t_net = time.perf_counter() - t0
         f_corr = np.stack([c.numpy() for c in corr], axis=1)
-        # time the pieces once more, warm
+        # time the pieces once more, warm: PM, network input, network + apply
         a = [DeviceArray((n,), np.float32) for _ in range(3)]
         device_synchronize()
         t0 = time.perf_counter()
         other.pm.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, a[0].ptr, a[1].ptr, a[2].ptr, self.g)
-        dm, dp = other.pm.device_meshes()
-        net.input_from_meshes(0, dm, dp, rho_mean, other.phi_scale)
         device_synchronize()
         t_pm = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        dm, dp = other.pm.device_meshes()
+        net.input_from_particles(0, dm, dp, rho_mean, other.phi_scale, x.ptr, y.ptr, z.ptr, m.ptr, n, other.L)
+        device_synchronize()
+        t_in = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        net.infer(1)
+        net.apply_field(0, x.ptr, y.ptr, z.ptr, n, other.L, self.f0, dm, rho_max, a[0].ptr, a[1].ptr, a[2].ptr)
+        device_synchronize()
+        t_net = time.perf_counter() - t0
         t0 = time.perf_counter()
         other.lr.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, a[0].ptr, a[1].ptr, a[2].ptr, self.g)
         device_synchronize()
@@ -568,5 +582,6 @@ class LearnedCorrection:
         return {"seed": seed, "cells": cells, "n": n,
                 "pm": force_errors(s.f_pm, s.f_true, s.n_lattice, s.in_field),
                 "pm_net": force_errors(f_corr, s.f_true, s.n_lattice, s.in_field),
-                "time_pm_s": t_pm, "time_net_s": t_net, "time_treepm_long_s": t_lr, "time_short_range_s": t_sr,
+                "time_pm_s": t_pm, "time_input_s": t_in, "time_net_s": t_net, "time_treepm_long_s": t_lr,
+                "time_short_range_s": t_sr,
                 "sr_pairs": other.sr.last_pair_count()}

@@ -18,8 +18,9 @@
  *                     103-110), which misses the second term.
  *
  * Learned correction (no reference counterpart).  A 3-layer network of 3^3
- * periodic convolutions, conv(2->16) SiLU, conv(16->16) SiLU, conv(16->3),
- * on the CIC density/potential mesh of a PM solve; its 3-channel output mesh
+ * periodic convolutions, conv(8->16) SiLU, conv(16->16) SiLU, conv(16->3),
+ * on the CIC density/potential mesh of a PM solve and the particles' moments
+ * per cell (gng_net_input_particles); its 3-channel output mesh
  * is CIC-interpolated to the particles and added to the PM acceleration.
  * Weights and activations are bfloat16, products accumulate in FP32 on the
  * MFMA units (v_mfma_f32_16x16x32_bf16); the master weights, gradients and Adam moments are FP32.  Mesh
@@ -28,7 +29,7 @@
  * (grid_size a multiple of 16).
  *
  * Parameters (gng_net_param_count() floats, in this order):
- *   w1[27][2][16] b1[16] w2[27][16][16] b2[16] w3[27][16][3] b3[3]
+ *   w1[27][8][16] b1[16] w2[27][16][16] b2[16] w3[27][16][3] b3[3]
  * with tap t = (dz+1)*9 + (dy+1)*3 + (dx+1) reading input cell c + (dz,dy,dx).
  *
  * Training particles are two float4 arrays: pos4 = (x, y, z, box index) and
@@ -78,7 +79,7 @@ int gng_sr_cell_order(gng_sr* sr, const float* d_x, const float* d_y, const floa
 /* ---- learned correction -------------------------------------------------- */
 
 enum gng_net_buffer {
-  GNG_NET_X0 = 0,    /* bf16 [box][M][2]  network input                      */
+  GNG_NET_X0 = 0,    /* bf16 [box][M][8]  network input                      */
   GNG_NET_Z1 = 1,    /* f32  [box][M][16] layer-1 pre-activation             */
   GNG_NET_X1 = 2,    /* bf16 [box][M][16] layer-1 activation                 */
   GNG_NET_Z2 = 3,    /* f32  [box][M][16]                                     */
@@ -109,10 +110,23 @@ int gng_net_write(gng_net* net, int32_t which, const void* host, int64_t bytes);
 int gng_net_device_buffer(gng_net* net, int32_t which, void** d_ptr);
 
 /* Input slot `box` from a PM solve's device meshes (FP32 mass per cell and
- * potential, grid_size^3 each, gng_pm_device_meshes):
- * x0 = (mass/rho_mean - 1, (phi - box7(phi))/phi_scale), rho_mean = mean mass
- * per cell, box7 = the periodic 7^3 box mean.  Both channels are linear in
- * the mass, as the PM force error is, and independent of the box size. */
+ * potential, grid_size^3 each, gng_pm_device_meshes) and its n particles
+ * (float32 SoA device arrays, non-negative masses, box of side box_size):
+ *   x0[0] = mass/rho_mean - 1,
+ *   x0[1] = (phi - box7(phi))/phi_scale,
+ *   x0[2..4] = D/rho_mean, D = sum m (f - 1/2),
+ *   x0[5] = (sum m |f - 1/2|^2 - |D|^2 / sum m)/rho_mean,
+ *   x0[6..7] = 0,
+ * with the sums over the particles whose base cell floor(x nm / L) mod nm is
+ * the cell and f their in-cell fraction; rho_mean = mean mass per cell,
+ * box7 = the periodic 7^3 box mean.  Every channel is linear in the mass, as
+ * the PM force error is, and independent of the box size.  The moments are
+ * 64-bit fixed-point sums, so the input does not depend on the order of the
+ * particles.  n = 0 leaves channels 2-5 at zero. */
+int gng_net_input_particles(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
+                            float phi_scale, const float* d_x, const float* d_y, const float* d_z, const float* d_m,
+                            int64_t n, double box_size, void* stream);
+/* gng_net_input_particles without particles (channels 2-7 zero). */
 int gng_net_input(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
                   float phi_scale, void* stream);
 

@@ -10,7 +10,10 @@ Conventions (shared with the HIP kernels):
   iz*nm^2 + iy*nm + ix, channels last), periodic on every axis;
 * a 3^3 convolution tap t = (dz+1)*9 + (dy+1)*3 + (dx+1) reads the input cell
   c + (dz, dy, dx); weights are ``w[t, c_in, c_out]`` (correlation form);
-* the network is conv(2->16) SiLU, conv(16->16) SiLU, conv(16->3) linear;
+* the network is conv(8->16) SiLU, conv(16->16) SiLU, conv(16->3) linear;
+  input channels: density contrast, potential minus its 7^3 box mean, the
+  particles' mass-weighted offsets from the cell centre (3) and spread about
+  their centre of mass, per mean cell mass, and two zero channels;
 * CIC weights at particle positions follow crates/gadget-ng-pm/src/cic.rs:100-170
   (node-centred trilinear weights, periodic wrap of all 8 nodes).
 
@@ -39,7 +42,7 @@ from scipy.special import erfc as _erfc
 SQRT_PI = math.sqrt(math.pi)
 
 TAPS = [(dz, dy, dx) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
-LAYERS = ((2, 16), (16, 16), (16, 3))
+LAYERS = ((8, 16), (16, 16), (16, 3))
 
 
 # ---------------------------------------------------------------------------
@@ -269,13 +272,39 @@ def box_mean(a, half: int = 3):
     return out
 
 
-def net_input(mass, potential, rho_mean: float, phi_scale: float):
+def particle_moments(pos, m, box: float, nm: int):
+    """Per cell [iz, iy, ix]: D = sum m (f - 1/2) (3 components), Q = sum m
+    |f - 1/2|^2 and S = sum m over the particles whose base cell floor(x nm /
+    L) mod nm it is, f = in-cell fraction; returns (D, Q, S)."""
+    c = np.asarray(pos, np.float64) * (nm / box)
+    i0 = np.floor(c)
+    f = c - i0 - 0.5
+    i0 = i0.astype(np.int64) % nm
+    flat = (i0[:, 2] * nm + i0[:, 1]) * nm + i0[:, 0]
+    m = np.asarray(m, np.float64)
+    M = nm ** 3
+    D = np.stack([np.bincount(flat, m * f[:, a], M) for a in range(3)], axis=1)
+    Q = np.bincount(flat, m * np.sum(f * f, axis=1), M)
+    S = np.bincount(flat, m, M)
+    shape = (nm, nm, nm)
+    return D.reshape(shape + (3,)), Q.reshape(shape), S.reshape(shape)
+
+
+def net_input(mass, potential, rho_mean: float, phi_scale: float, moments=None):
     """Channel 0 = mass/rho_mean - 1 (density contrast), channel 1 =
     (Phi - box7(Phi)) / phi_scale (the potential of the mass within a few
-    cells); both linear in the mass, as the PM force error is."""
-    x = np.empty(mass.shape + (2,))
+    cells), channels 2-4 = D / rho_mean and channel 5 = (Q - |D|^2/S) /
+    rho_mean from particle_moments (zero without moments), channels 6-7 = 0;
+    every channel is linear in the mass, as the PM force error is."""
+    x = np.zeros(mass.shape + (LAYERS[0][0],))
     x[..., 0] = np.asarray(mass, np.float64) / rho_mean - 1.0
     x[..., 1] = (np.asarray(potential, np.float64) - box_mean(potential)) / phi_scale
+    if moments is not None:
+        D, Q, S = moments
+        x[..., 2:5] = D / rho_mean
+        with np.errstate(divide="ignore", invalid="ignore"):
+            spread = np.where(S > 0, Q - np.sum(D * D, axis=-1) / np.where(S > 0, S, 1.0), 0.0)
+        x[..., 5] = np.maximum(spread, 0.0) / rho_mean
     return x
 
 

@@ -121,7 +121,7 @@ def test_learned_symbols_declared_and_parameter_count():
                 "gng_pm_keep_potential", "gng_dev_axpy"):
         assert sym in names
     # w1[27][2][16] b1[16] w2[27][16][16] b2[16] w3[27][16][3] b3[3]
-    assert pm.lib().gng_net_param_count() == 27 * 2 * 16 + 16 + 27 * 16 * 16 + 16 + 27 * 16 * 3 + 3
+    assert pm.lib().gng_net_param_count() == 27 * 8 * 16 + 16 + 27 * 16 * 16 + 16 + 27 * 16 * 3 + 3
 
 
 @pytest.mark.skipif("has_device()")

@@ -152,7 +152,7 @@ def test_short_range_empty_and_errors(learned, pm):
 
 def _forward(learned, nm, boxes, seed):
     g = np.random.default_rng(seed)
-    x0 = lo.bf16_round(g.normal(size=(boxes, nm, nm, nm, 2)).astype(np.float32))
+    x0 = lo.bf16_round(g.normal(size=(boxes, nm, nm, nm, lo.LAYERS[0][0])).astype(np.float32))
     p = _params(seed)
     net = learned.Net(nm, boxes, device=0)
     net.set_params(p)
@@ -290,7 +290,7 @@ def test_training_fits_a_teacher_network(learned, pm):
     nm, boxes, L, osc = 16, 2, 16.0, 1.0
     print("case teacher training", flush=True)
     g = np.random.default_rng(21)
-    x0 = lo.bf16_round(g.normal(size=(boxes, nm, nm, nm, 2)).astype(np.float32))
+    x0 = lo.bf16_round(g.normal(size=(boxes, nm, nm, nm, lo.LAYERS[0][0])).astype(np.float32))
     teacher = _params(100, big_out=True)
     net = learned.Net(nm, boxes, device=0)
     net.set_params(teacher)
@@ -330,6 +330,42 @@ def test_net_input_matches_oracle(learned, pm):
     assert bf16_ulp_ok(got[..., 0], ref[..., 0], frac=1.0)
     # the high-passed potential differs from a float64 evaluation by FP32 rounding of phi ~ 1e3
     assert np.abs(got[..., 1] - ref[..., 1]).max() <= 2e-3 * np.abs(ref[..., 1]).max() + 1e-4
+    assert not got[..., 2:].any()
+
+
+def test_net_input_particle_moments_match_oracle(learned, pm):
+    """Channels 2-5 from the particles (L = nm, so x nm / L = x exactly in
+    FP32 and FP64), with crowded cells, wrapped positions and particle order
+    reversed (the fixed-point sums give the same bits)."""
+    nm, L = 32, 32.0
+    g = np.random.default_rng(41)
+    n = 40000
+    pos = g.uniform(0.0, L, (n, 3))
+    pos[: n // 4] = g.normal(L / 2, 0.4, (n // 4, 3))          # a clump over a few cells
+    pos[n // 4: n // 4 + 500] += L * g.integers(-1, 2, (500, 3))  # outside the box
+    pos = pos.astype(np.float32)
+    m = g.uniform(0.5, 1.5, n).astype(np.float32)
+    rho_mean = float(np.sum(m, dtype=np.float64)) / nm ** 3
+    mass = g.gamma(2.0, 0.5, (nm, nm, nm)).astype(np.float32)
+    phi = g.normal(size=(nm, nm, nm)).astype(np.float32)
+    print(f"case net input particle moments n={n} nm={nm}", flush=True)
+    net = learned.Net(nm, 2, device=0)
+    dm, dp = pm.DeviceArray.from_host(mass), pm.DeviceArray.from_host(phi)
+    outs = []
+    for order in (np.arange(n), np.arange(n)[::-1]):
+        d = [pm.DeviceArray.from_host(np.ascontiguousarray(pos[order, a])) for a in range(3)]
+        dmass = pm.DeviceArray.from_host(np.ascontiguousarray(m[order]))
+        net.input_from_particles(0, dm.ptr, dp.ptr, rho_mean, 3.0, d[0].ptr, d[1].ptr, d[2].ptr, dmass.ptr, n, L)
+        outs.append(net.read("x0")[0])
+    assert np.array_equal(outs[0], outs[1])
+    got = outs[0]
+    mom = lo.particle_moments(pos.astype(np.float64), m.astype(np.float64), L, nm)
+    ref = lo.net_input(mass, phi, rho_mean, 3.0, moments=mom).astype(np.float32)
+    assert bf16_ulp_ok(got[..., 2:5], lo.bf16_round(ref[..., 2:5]), frac=1.0)
+    # the spread is a difference of two sums; compare against its scale
+    scale = np.sum(mom[1]) / rho_mean / np.count_nonzero(mom[2])
+    assert np.abs(got[..., 5] - ref[..., 5]).max() <= 1e-2 * np.abs(ref[..., 5]).max() + 1e-3 * scale
+    assert not got[..., 6:].any()
 
 
 def test_inference_pass_equals_training_forward(learned):

@@ -110,7 +110,7 @@ def _loss_of(params, x0, pos, box, target, fref, f0, osc):
 def test_network_gradients_against_finite_differences():
     g = np.random.default_rng(12)
     n, box = 4, 4.0
-    x0 = g.normal(size=(n, n, n, 2))
+    x0 = g.normal(size=(n, n, n, lo.LAYERS[0][0]))
     params = lo.init_params(seed=3)
     params["w3"] *= 10.0
     for k in ("b1", "b2", "b3"):
@@ -256,3 +256,37 @@ def test_potential_gradient_matches_the_oracle_force_meshes():
     ox, oy, oz = O.solve_forces(mass, 1.0, nm, box, mode=O.MODE_FAITHFUL)
     assert np.allclose(fx, ox, rtol=1e-10, atol=1e-10 * np.abs(ox).max())
     assert np.allclose(fz, oz, rtol=1e-10, atol=1e-10 * np.abs(oz).max())
+
+
+def test_particle_moments_against_per_particle_sums():
+    """particle_moments against a loop over particles, and the spread channel
+    against the direct sum m |f - f_com|^2 (parallel-axis theorem)."""
+    g = np.random.default_rng(8)
+    nm, box = 4, 2.0
+    pos = np.concatenate([g.uniform(-box, 2 * box, (60, 3)), g.normal(1.1, 0.05, (30, 3))])
+    m = g.uniform(0.2, 2.0, len(pos))
+    D, Q, S = lo.particle_moments(pos, m, box, nm)
+    D2, Q2, S2 = np.zeros((nm, nm, nm, 3)), np.zeros((nm, nm, nm)), np.zeros((nm, nm, nm))
+    groups = {}
+    for p, mp in zip(pos, m):
+        c = [p[a] * nm / box for a in range(3)]
+        i = [int(math.floor(v)) % nm for v in c]
+        f = np.array([v - math.floor(v) for v in c])
+        key = (i[2], i[1], i[0])
+        D2[key] += mp * (f - 0.5)
+        Q2[key] += mp * np.sum((f - 0.5) ** 2)
+        S2[key] += mp
+        groups.setdefault(key, []).append((mp, f))
+    assert np.allclose(D, D2, atol=1e-12) and np.allclose(Q, Q2, atol=1e-12) and np.allclose(S, S2, atol=1e-12)
+    x = lo.net_input(np.ones((nm, nm, nm)), np.zeros((nm, nm, nm)), 0.5, 1.0, moments=(D, Q, S))
+    for key, items in groups.items():
+        ms = np.array([a for a, _ in items])
+        fs = np.array([b for _, b in items])
+        com = (ms[:, None] * fs).sum(0) / ms.sum()
+        spread = np.sum(ms * np.sum((fs - com) ** 2, axis=1))
+        assert x[key][5] == pytest.approx(spread / 0.5, rel=1e-9, abs=1e-12)
+        assert np.allclose(x[key][2:5], (ms[:, None] * (fs - 0.5)).sum(0) / 0.5)
+    # a lone particle has no spread; particles at cell centres give no moments
+    centre = (np.array([[0.5, 1.5, 2.5]]) * box / nm)
+    D, Q, S = lo.particle_moments(centre, [3.0], box, nm)
+    assert np.abs(D).max() < 1e-12 and np.abs(Q).max() < 1e-12 and S.sum() == 3.0
