@@ -944,6 +944,13 @@ NET_BYTES_PER_CELL = 16 + 32 + 32 + 32 + 32 + 16  # 160
 # the roofline that bounds the network is HBM bandwidth
 
 
+# force-error figures of the learned bench: field = particles in cells at most
+# (1 + dense_contrast) times the mean mass; field_near = field particles in
+# cells next to a denser one, field_far = the rest of the field
+ERROR_KEYS = ("field_rms_rel", "field_median_rel", "field_mean_rel", "field_near_rms_rel", "field_far_rms_rel",
+              "field_far_median_rel", "rms_rel", "median_rel", "dense_median_rel")
+
+
 def bench_learned(args) -> int:
     """Learned sub-cell correction (clustered boxes at one particle per cell): train on seeded 64^3
     boxes made by the method itself, evaluate on held-out 256^3 boxes against
@@ -1003,10 +1010,10 @@ def bench_learned(args) -> int:
                              "eps = 0.05"},
         "target": {"rms_rel": 0.01, "met_field": bool(mean("field_rms_rel", "pm_net") <= 0.01),
                    "met_all": bool(mean("rms_rel", "pm_net") <= 0.01)},
-        "without_net": {k: round(mean(k, "pm"), 5) for k in ("field_rms_rel", "field_median_rel", "field_mean_rel",
-                                                               "rms_rel", "median_rel", "dense_median_rel")},
-        "with_net": {k: round(mean(k, "pm_net"), 5) for k in ("field_rms_rel", "field_median_rel", "field_mean_rel",
-                                                                "rms_rel", "median_rel", "dense_median_rel")},
+        "without_net": {k: round(mean(k, "pm"), 5) for k in ERROR_KEYS},
+        "with_net": {k: round(mean(k, "pm_net"), 5) for k in ERROR_KEYS},
+        "populations": {k: int(round(mean(k, "pm"))) for k in ("field_count", "field_near_count", "field_far_count",
+                                                                "dense_count")},
         "cost_s": {"network_forward_and_apply": round(t_net, 5), "short_range_sum": round(t_sr, 5),
                    "network_input": round(float(np.mean([e["time_input_s"] for e in evals])), 5),
                    "pm": round(float(np.mean([e["time_pm_s"] for e in evals])), 5),

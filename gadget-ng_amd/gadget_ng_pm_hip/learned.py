@@ -338,14 +338,17 @@ class Sample:
     f_true: np.ndarray              # (n, 3) float32, TreePM (long range + exact short range)
     pos: np.ndarray                 # (n, 3) float32
     in_field: np.ndarray            # (n,) bool, base cell below the dense threshold
+    near_dense: np.ndarray          # (n,) bool, base cell is a dense cell or one of its 26 neighbours
     times: Dict[str, float] = field(default_factory=dict)
 
 
-def force_errors(f: np.ndarray, f_true: np.ndarray, n_lattice: int, field=None) -> Dict[str, float]:
+def force_errors(f: np.ndarray, f_true: np.ndarray, n_lattice: int, field=None, near=None) -> Dict[str, float]:
     """rms_rel = sqrt(sum|f - t|^2 / sum|t|^2) and the per-particle relative
     error |f - t| / |t| (median, mean) over all particles, the lattice and
     sphere members, and the field (particles in cells below the dense
-    threshold) and dense populations when `field` (bool mask) is given."""
+    threshold) and dense populations when `field` (bool mask) is given; with
+    `near` (base cell next to a dense cell) the field splits further into
+    field_near_ and field_far_."""
     f = np.asarray(f, np.float64)
     t = np.asarray(f_true, np.float64)
     d = np.linalg.norm(f - t, axis=1)
@@ -368,6 +371,9 @@ def force_errors(f: np.ndarray, f_true: np.ndarray, n_lattice: int, field=None) 
     if field is not None:
         out.update(pop(field, "field_"))
         out.update(pop(~field, "dense_"))
+        if near is not None:
+            out.update(pop(field & near, "field_near_"))
+            out.update(pop(field & ~near, "field_far_"))
     return out
 
 
@@ -445,8 +451,13 @@ class LearnedCorrection:
         pos = np.stack([p.x, p.y, p.z], axis=1)
         mass = self.pm.density()
         c = np.floor(pos.astype(np.float64) * (self.nm / self.L)).astype(np.int64) % self.nm
-        fld = mass[c[:, 2], c[:, 1], c[:, 0]] <= (1.0 + self.dense_contrast) * rho_mean
-        return Sample(p.n, n_lat, self.L, dev, f_pm, f_tp, pos, fld, times)
+        dense = mass > (1.0 + self.dense_contrast) * rho_mean
+        fld = ~dense[c[:, 2], c[:, 1], c[:, 0]]
+        halo = dense.copy()
+        for ax in range(3):
+            halo = halo | np.roll(halo, 1, axis=ax) | np.roll(halo, -1, axis=ax)
+        near = halo[c[:, 2], c[:, 1], c[:, 0]]
+        return Sample(p.n, n_lat, self.L, dev, f_pm, f_tp, pos, fld, near, times)
 
     def _records(self, s: Sample, slot: int):
         """Training records sorted by base cell: pos4 (x, y, z, slot), tgt4
@@ -580,8 +591,8 @@ class LearnedCorrection:
         device_synchronize()
         t_sr = time.perf_counter() - t0
         return {"seed": seed, "cells": cells, "n": n,
-                "pm": force_errors(s.f_pm, s.f_true, s.n_lattice, s.in_field),
-                "pm_net": force_errors(f_corr, s.f_true, s.n_lattice, s.in_field),
+                "pm": force_errors(s.f_pm, s.f_true, s.n_lattice, s.in_field, s.near_dense),
+                "pm_net": force_errors(f_corr, s.f_true, s.n_lattice, s.in_field, s.near_dense),
                 "time_pm_s": t_pm, "time_input_s": t_in, "time_net_s": t_net, "time_treepm_long_s": t_lr,
                 "time_short_range_s": t_sr,
                 "sr_pairs": other.sr.last_pair_count()}
