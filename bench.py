@@ -177,7 +177,7 @@ def main() -> int:
                     help="skip timing the FP64 GPU form of the reference algorithm")
     ap.add_argument("--train-boxes", type=int, default=200, help="learned: training seeds 1..N (64^3 boxes)")
     ap.add_argument("--train-steps", type=int, default=2000, help="learned: Adam steps")
-    ap.add_argument("--eval-boxes", type=int, default=2, help="learned: held-out 256^3 seeds from 1001")
+    ap.add_argument("--eval-boxes", type=int, default=20, help="learned: held-out 256^3 seeds from 1001")
     ap.add_argument("--only-headline", action="store_true",
                     help="time the f64 entry point only (no f32 entry, host entry points or copy rate): for "
                          "profiler runs, whose per-solve figures need one kind of solve")
@@ -449,13 +449,19 @@ def main() -> int:
         if not args.no_cpu_faithful and args.workload == "zeldovich":
             # the reference-Rayon-faithful mode: per-thread deposit grids and
             # single-threaded Z2Z passes (cic.rs:355-401, fft_poisson.rs:356-373)
-            t1 = time.perf_counter()
-            oracle.pm_accelerations(pos64, mass64, box, nm, 1.0, r_split=r_split, mode=oracle.MODE_FAITHFUL,
-                                    deposit_mode=oracle.DEPOSIT_PRIVATE_GRIDS)
-            tf = time.perf_counter() - t1
+            ft = []
+            for _ in range(args.cpu_runs):
+                t1 = time.perf_counter()
+                oracle.pm_accelerations(pos64, mass64, box, nm, 1.0, r_split=r_split, mode=oracle.MODE_FAITHFUL,
+                                        deposit_mode=oracle.DEPOSIT_PRIVATE_GRIDS)
+                ft.append(time.perf_counter() - t1)
+            tf = statistics.median(ft)
             result["cpu_faithful"] = {"value": round(n / tf / 1e6, 4), "unit": "M particle-forces/s",
-                                      "cores": cores, "seconds": round(tf, 2), "runs": 1}
-            log(f"[bench] CPU oracle faithful mode: {tf:.1f} s")
+                                      "cores": cores, "seconds": round(tf, 2), "runs": len(ft),
+                                      "sample": "the full workload, FP64 oracle in the reference-Rayon-faithful "
+                                                "mode (per-thread deposit grids, single-threaded Z2Z passes), "
+                                                "median of the timed runs"}
+            log(f"[bench] CPU oracle faithful mode: {tf:.1f} s (median of {len(ft)})")
         # calibration against the reference's only published PM timings (AP-08 input,
         # serial CPU PM: 1.83 ms at grid 32, 22.4 ms at grid 64), with this GPU beside it
         ap = {}
