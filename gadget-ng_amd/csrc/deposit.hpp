@@ -429,6 +429,15 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
     const int sz = (d & 4) ? (uz == 0 ? nt - 1 : uz - 1) : uz;
     toff[d][j] = offset[tm.rank[(unsigned)(sz * nt + sy) * nt + sx] * 8u + j];
   }
+  // the fixed-point mass total, reduced by the third wavefront while the
+  // offsets load (one slot per lane; NT >= 256)
+  __shared__ unsigned long long qtot;
+  static_assert(NT >= 192 && kQSlots == 64, "one qsum slot per lane of the third wavefront");
+  if (tid >= 128 && tid < 192) {
+    unsigned long long v = sc->qsum[(tid - 128) * kQStride];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (tid == 128) qtot = v;
+  }
   __syncthreads();
   if (tid < 64) {
     unsigned len = 0, beg = 0;
@@ -484,18 +493,9 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
   };
   // m 2^s of a record: the FP32 mass scaled exactly
   const double scale = ldexp(1.0, s);
-  unsigned q = q0 + (unsigned)tid;
-  int r = range_of(q);
-  PartRec pnext = fetch(q, r);
-  for (unsigned it = 0; it < rounds; ++it) {
-    const bool live = q < total;
+  auto deposit_one = [&](const PartRec& p, int r, bool live) {
     const int d = (int)((kRangeD >> (3 * r)) & 7ull);
     const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-    const PartRec p = pnext;
-    // the next round's particle is loaded while this one is deposited
-    q += NT;
-    r = range_of(q);
-    if (it + 1 < rounds) pnext = fetch(q, r);
     // FP64 weights: the 43-bit fractions convert exactly, 1 - f is exact, and
     // the products round far below the fixed-point unit
     const double fx = rec_frac(p, 0), fy = rec_frac(p, 1), fz = rec_frac(p, 2);
@@ -531,6 +531,36 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
         }
       }
     }
+  };
+  // kPrefetch rounds of records are in flight at once: a tile of the uniform
+  // headline holds about 2.5 rounds at 256 threads, so all of its loads
+  // share one memory latency.  Slot j holds the record of round it + j; the
+  // round loop is unrolled by kPrefetch so every slot index is static.
+  constexpr int kPrefetch = 3;
+  const unsigned qbase = q0 + (unsigned)tid;
+  PartRec buf[kPrefetch];
+  int rbuf[kPrefetch];
+#pragma unroll
+  for (int j = 0; j < kPrefetch; ++j) {
+    const unsigned qj = qbase + (unsigned)j * NT;
+    rbuf[j] = range_of(qj);
+    buf[j] = fetch(qj, rbuf[j]);
+  }
+  for (unsigned it = 0; it < rounds; it += kPrefetch) {
+#pragma unroll
+    for (int j = 0; j < kPrefetch; ++j) {
+      if (it + j < rounds) {  // uniform over the workgroup
+        const unsigned qc = qbase + (it + j) * NT;
+        const PartRec p = buf[j];
+        const int r = rbuf[j];
+        if (it + j + kPrefetch < rounds) {
+          const unsigned qn = qc + (unsigned)kPrefetch * NT;
+          rbuf[j] = range_of(qn);
+          buf[j] = fetch(qn, rbuf[j]);
+        }
+        deposit_one(p, r, qc < total);
+      }
+    }
   }
   __syncthreads();
   unsigned long long* part = nullptr;
@@ -552,7 +582,7 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
   }
   const double unit = ldexp(1.0, -s);
   // the integer part of the mean fixed-point mass per cell
-  const long long qmean = (long long)(qsum_total(sc) / (unsigned long long)((long long)cm.nm * cm.nm * cm.nm));
+  const long long qmean = (long long)(qtot / (unsigned long long)((long long)cm.nm * cm.nm * cm.nm));
   for (int c = tid; c < T3; c += NT) {
     const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
     const size_t row = (size_t)(uy * T + ly) * cm.nm + (ux * T + lx);

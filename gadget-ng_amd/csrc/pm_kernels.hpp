@@ -161,14 +161,6 @@ struct SolveScalars {
   unsigned long long qsum[kQSlots * kQStride];
 };
 
-// The integer sum of the partial mass sums (the order does not matter).
-__device__ __forceinline__ unsigned long long qsum_total(const SolveScalars* sc) {
-  unsigned long long t = 0ull;
-#pragma unroll 8
-  for (int k = 0; k < kQSlots; ++k) t += sc->qsum[k * kQStride];
-  return t;
-}
-
 // Fixed-point exponent: contributions are w*m*2^s with N*max|m|*2^s < 2^62,
 // so no cell sum can overflow int64 whatever the clustering.
 __device__ __forceinline__ int fixed_shift(const SolveScalars* sc, long long ntot) {
