@@ -225,30 +225,55 @@ __global__ __launch_bounds__(256) void k_scan_add_box(const BoxArgs* __restrict_
 // Particle i to its bucket slot as a cell record, and the sum of the
 // particles' fixed-point masses round(m 2^s) (one integer atomic per block,
 // so the total does not depend on the order): the deposit subtracts the mean
-// per cell before it converts to FP32.
+// per cell before it converts to FP32.  Each thread moves kScatterPer
+// particles (block stride 256, so a wavefront still reads 64 consecutive
+// particles); all of their loads are issued before the dependent offset
+// loads, and those before any store, so a thread waits for two memory
+// latencies however many particles it moves.
+constexpr int kScatterPer = 4;
+
 template <typename T, typename M>
 __device__ __forceinline__ void scatter_body(const PosSrc<T> src, const M* __restrict__ m, long long n,
                                                  const CellMap cm, const unsigned* __restrict__ bucket,
                                                  const unsigned* __restrict__ rank,
                                                  const unsigned* __restrict__ offset, PartRec* __restrict__ sorted,
                                                  unsigned* __restrict__ perm, SolveScalars* sc) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long i0 = (long long)blockIdx.x * (256 * kScatterPer) + threadIdx.x;
+  double px[kScatterPer], py[kScatterPer], pz[kScatterPer];
+  float mf[kScatterPer];
+  unsigned slot[kScatterPer];
+#pragma unroll
+  for (int k = 0; k < kScatterPer; ++k) {
+    const long long i = i0 + 256LL * k;
+    if (i < n) {
+      const long long o = i * src.stride;
+      px[k] = (double)src.x[o];
+      py[k] = (double)src.y[o];
+      pz[k] = (double)src.z[o];
+      mf[k] = (float)m[i];
+      slot[k] = bucket[i];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kScatterPer; ++k) {
+    const long long i = i0 + 256LL * k;
+    if (i < n) slot[k] = offset[slot[k]] + rank[i];
+  }
+  const int shift = fixed_shift(sc, n);
   long long q = 0;
-  if (i < n) {
-    // every load first (the slot's offset depends on the bucket), so the
-    // memory round trips overlap instead of following each coordinate's math
-    const long long o = i * src.stride;
-    const double px = (double)src.x[o], py = (double)src.y[o], pz = (double)src.z[o];
-    const float mf = (float)m[i];
-    const unsigned slot = offset[bucket[i]] + rank[i];
-    int c[3];
-    unsigned long long f[3];
-    cell_fix(px, cm, c[0], f[0]);
-    cell_fix(py, cm, c[1], f[1]);
-    cell_fix(pz, cm, c[2], f[2]);
-    sorted[slot] = encode_rec(c, f, mf);
-    if (perm) perm[slot] = (unsigned)i;  // only for the sorted-order interpolation
-    q = __double2ll_rn(ldexp((double)mf, fixed_shift(sc, n)));
+#pragma unroll
+  for (int k = 0; k < kScatterPer; ++k) {
+    const long long i = i0 + 256LL * k;
+    if (i < n) {
+      int c[3];
+      unsigned long long f[3];
+      cell_fix(px[k], cm, c[0], f[0]);
+      cell_fix(py[k], cm, c[1], f[1]);
+      cell_fix(pz[k], cm, c[2], f[2]);
+      sorted[slot[k]] = encode_rec(c, f, mf[k]);
+      if (perm) perm[slot[k]] = (unsigned)i;  // only for the sorted-order interpolation
+      q += __double2ll_rn(ldexp((double)mf[k], shift));
+    }
   }
   for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off);
   __shared__ long long wsum[4];

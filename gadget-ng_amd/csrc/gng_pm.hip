@@ -692,7 +692,8 @@ template <typename T, typename M>
 void launch_scatter_t(const PartIn& in, long long n, const CellMap& cm, const unsigned* bucket,
                       const unsigned* rank, const unsigned* offset, PartRec* sorted, unsigned* perm,
                       SolveScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL((k_scatter<T, M>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pos_src<T>(in),
+  hipLaunchKernelGGL((k_scatter<T, M>), dim3((unsigned)((n + 256 * kScatterPer - 1) / (256 * kScatterPer))), dim3(256),
+                     0, s, pos_src<T>(in),
                      (const M*)in.m, n, cm, bucket, rank, offset, sorted, perm, sc);
 }
 void launch_scatter(const PartIn& in, long long n, const CellMap& cm, const unsigned* bucket, const unsigned* rank,
@@ -2174,7 +2175,8 @@ void launch_box_group(gng_pm_batch* b, const BoxGroup& gr, const BoxArgs* bx, lo
   hipLaunchKernelGGL(k_scan_partials_box, dim3(1, 1, nbx), dim3(256), 0, s, bx, p0->scan_blocks, p0->nb);
   hipLaunchKernelGGL(k_scan_add_box, dim3((unsigned)p0->scan_blocks, 1, nbx), dim3(256), 0, s, bx, p0->nb);
   if (maxn > 0)
-    hipLaunchKernelGGL(k_scatter_box, dim3((unsigned)((maxn + 255) / 256), 1, nbx), dim3(256), 0, s, bx, src, m);
+    hipLaunchKernelGGL(k_scatter_box, dim3((unsigned)((maxn + 256 * kScatterPer - 1) / (256 * kScatterPer)), 1, nbx),
+                       dim3(256), 0, s, bx, src, m);
   const unsigned ntiles = (unsigned)p0->tm.nt * p0->tm.nt * p0->tm.nt;
   o.deposit_box[p0->tune.deposit_wide](dim3(ntiles, 1, nbx), dim3(p0->tune.deposit_wide ? 512 : 256), s, bx,
                                        p0->tm);
