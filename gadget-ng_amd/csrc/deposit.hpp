@@ -348,7 +348,7 @@ __constant__ unsigned char kRangeHi[14] = {8, 2, 4, 6, 8, 4, 8, 4, 8, 8, 6, 8, 8
 // converts the scratch tile and clears it for the next solve.  Clustered
 // inputs otherwise leave a few Plummer-core tiles running hundreds of rounds
 // on one workgroup each.
-constexpr unsigned kSliceItems = 16384;
+constexpr unsigned kSliceItems = 32768;
 constexpr unsigned kMaxSlices = 1024;
 constexpr unsigned kNoSlice = 0xffffffffu;
 

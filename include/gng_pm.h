@@ -222,7 +222,7 @@ typedef struct gng_pm_tuning_t {
                                 64 at 1024^3; inv: 64 MiB, 64 at 512^3 and 16
                                 at 1024^3, where the default is 0); none at
                                 256^3 and below */
-  int32_t deposit_split;     /* deposit: tiles holding more than 16384 particles
+  int32_t deposit_split;     /* deposit: tiles holding more than 32768 particles
                                 (dense clumps) are shared by several workgroups
                                 (1, default) or each run by one (0).  Same
                                 density bit for bit either way */
