@@ -324,6 +324,8 @@ MeshOps make_ops() {
   constexpr int TY = YZCfg<N>::T;
   constexpr int TD = deposit_tile(N);
   static_assert(N % TD == 0, "deposit tiles must cover the mesh");
+  static_assert((long long)(N / TD) * (N / TD) * (N / TD) <= (1LL << 20),
+                "the split deposit's extra-workgroup table keeps a tile rank in 20 bits");
   MeshOps o;
   o.nm = N;
   o.x_rows = R;
