@@ -153,6 +153,84 @@ def measured_traffic(group: str, nm: int, workload: str):
     return (int(total) if found else None), src
 
 
+def profile_summary(args) -> int:
+    """Per-kernel table of a rocprofv3 run: calls, mean/min/max duration from
+    the kernel statistics, HBM bytes per dispatch from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for
+    the largest device-to-device copy in the same run (the 1 GiB copy of
+    --copy-calibration), and per-solve bytes for chunked passes."""
+    import collections
+    import csv
+    import glob
+
+    kt_dir, fetch_dir, write_dir = args.profile_summary
+
+    def one(pattern, d):
+        found = sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True))
+        if not found:
+            raise SystemExit(f"no {pattern} under {d}")
+        return found[0]
+
+    def short(name):
+        name = name.replace("void ", "")
+        i = name.find("(")
+        return name[:i] if i > 0 else name
+
+    stats = {r["Name"]: r for r in csv.DictReader(open(one("*kernel_stats.csv", kt_dir)))}
+
+    def counter(d, which):
+        rows = [r for r in csv.DictReader(open(one("*counter_collection.csv", d))) if r["Counter_Name"] == which]
+        per = collections.defaultdict(list)
+        for r in rows:
+            per[short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        copy = max(float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"])
+        scale = float(1 << 30) / (copy * 1024.0)
+        return {k: sum(v) / len(v) * 1024.0 * scale for k, v in per.items() if "copyBuffer" not in k}, scale
+
+    reads, ff = counter(fetch_dir, "FETCH_SIZE")
+    writes, wf = counter(write_dir, "WRITE_SIZE")
+    solves = sum(int(r["Calls"]) for n, r in stats.items()
+                 if short(n).split("::")[-1].startswith(args.summary_solve_kernel) and "col0" not in n)
+    rows, kernels = [], {}
+    for n, r in stats.items():
+        sn = short(n)
+        calls = int(r["Calls"])
+        mean = float(r["AverageNs"]) / 1e3
+        per_solve = calls / solves if solves else 1.0
+        rb, wb = reads.get(sn), writes.get(sn)
+        rows.append((float(r["TotalDurationNs"]), sn, calls, per_solve, mean, float(r["MinNs"]) / 1e3,
+                     float(r["MaxNs"]) / 1e3, rb, wb))
+        if rb is not None and wb is not None:
+            kernels[sn] = {"mean_us": mean, "read_bytes": rb, "write_bytes": wb, "calls": calls,
+                           "per_solve_bytes": (rb + wb) * per_solve, "per_solve_us": mean * per_solve}
+    rows.sort(reverse=True)
+    lines = [f"# {args.summary_title}", "", "Command (from the repository root, on the GPU box):", "", "```",
+             args.summary_command, "```", ""]
+    if args.summary_commit:
+        lines += [f"Measured at commit {args.summary_commit}.", ""]
+    lines += [f"HBM traffic scaling from a 1 GiB device-to-device copy in the same run: FETCH_SIZE reads "
+              f"{1 / ff:.3f} of the true bytes (factor {ff:.3f} applied), WRITE_SIZE reads {1 / wf:.3f} "
+              f"(factor {wf:.3f}).", "",
+              "FETCH_SIZE and WRITE_SIZE count traffic between L2 and the memory fabric, so bytes served by the "
+              "256 MiB Infinity Cache count as HBM bytes here.  A chunked pass runs several launches per solve "
+              "(column 'per solve'; solves counted by the once-per-solve z pass).", "",
+              "| kernel | calls | per solve | mean us | min us | max us | HBM read MiB | HBM write MiB | TB/s |",
+              "|---|---|---|---|---|---|---|---|---|"]
+    for _, sn, calls, ps, mean, mn, mx, rb, wb in rows:
+        tbs = ((rb or 0.0) + (wb or 0.0)) / (mean * 1e-6) / 1e12 if rb is not None else float("nan")
+        lines.append(f"| `{sn}` | {calls} | {ps:.2f} | {mean:.1f} | {mn:.1f} | {mx:.1f} | "
+                     f"{(rb or 0.0) / 2 ** 20:.1f} | {(wb or 0.0) / 2 ** 20:.1f} | {tbs:.2f} |")
+    if args.summary_md:
+        with open(args.summary_md, "w") as f:
+            f.write("\n".join(lines) + "\n")
+    if args.summary_json:
+        with open(args.summary_json, "w") as f:
+            json.dump({"command": args.summary_command, "commit": args.summary_commit, "fetch_factor": ff,
+                       "write_factor": wf, "solves": solves, "kernels": kernels}, f, indent=1)
+    print("\n".join(lines))
+    return 0
+
+
 def main() -> int:
     import faulthandler
 
@@ -184,7 +262,20 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a kernel of known traffic for scaling "
                          "profiler byte counters)")
+    ap.add_argument("--profile-summary", nargs=3, metavar=("KT_DIR", "FETCH_DIR", "WRITE_DIR"),
+                    help="no GPU work: summarise three rocprofv3 output folders (--kernel-trace --stats, "
+                         "--pmc FETCH_SIZE, --pmc WRITE_SIZE) into --summary-md and --summary-json")
+    ap.add_argument("--summary-md", help="profile summary: markdown table to write")
+    ap.add_argument("--summary-json", help="profile summary: per-kernel traffic (read by roofline.traffic)")
+    ap.add_argument("--summary-title", default="rocprofv3 summary")
+    ap.add_argument("--summary-command", default="", help="the profiled command, quoted in the summary")
+    ap.add_argument("--summary-commit", default="", help="the commit the profile was measured at")
+    ap.add_argument("--summary-solve-kernel", default="k_fft_z_green",
+                    help="kernel launched once per solve (counts the solves)")
     args = ap.parse_args()
+
+    if args.profile_summary:
+        return profile_summary(args)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this PM solver has no multi-GPU path "
