@@ -770,6 +770,27 @@ int ensure_work(gng_pm_plan* p, long long n) {
 
 // Deposit and the forward x and y transforms (shared by the force solve and
 // the power spectrum); the mass mesh is in P as the half spectrum after y.
+// The counting sort of the deposit: bucket counts, their scan, and the
+// scatter of the particles into bucket order as cell records (perm, when not
+// null, receives the original index of every sorted slot).  stage_marks
+// records the bucket stage's timing event (a solve's pipeline only).
+void run_sort(gng_pm_plan* p, const PartIn& in, long long n, unsigned* perm, bool stage_marks, hipStream_t s) {
+  const CellMap cm = cell_map(p);
+  hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, p->d_sc);
+  {
+    const long long n16 = (p->nb + 3) / 4;  // counts padded to 16-byte words
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<uint4*>(p->d_count), n16);
+  }
+  if (n > 0) launch_bucket(in, n, cm, p->tm, p->d_count, p->d_bucket, p->d_rank, p->d_sc, s);
+  if (stage_marks) mark(p, 1, s);
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_count, p->d_offset,
+                     p->d_partial, p->nb);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(256), 0, s, p->d_partial, p->scan_blocks, p->d_offset, p->nb);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_offset, p->d_partial, p->nb);
+  if (n > 0) launch_scatter(in, n, cm, p->d_bucket, p->d_rank, p->d_offset, p->d_sorted, perm, p->d_sc, s);
+}
+
 int run_front(gng_pm_plan* p, const PartIn& in, long long n, hipStream_t s) {
   int rc = ensure_work(p, n > 0 ? n : 1);
   if (rc) return rc;
@@ -784,21 +805,7 @@ int run_front(gng_pm_plan* p, const PartIn& in, long long n, hipStream_t s) {
   const long long rho_plane = fplane;
 
   mark(p, 0, s);
-  hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, p->d_sc);
-  {
-    const long long n16 = (p->nb + 3) / 4;  // counts padded to 16-byte words
-    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, s,
-                       reinterpret_cast<uint4*>(p->d_count), n16);
-  }
-  if (n > 0) launch_bucket(in, n, cm, p->tm, p->d_count, p->d_bucket, p->d_rank, p->d_sc, s);
-  mark(p, 1, s);
-  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_count, p->d_offset,
-                     p->d_partial, p->nb);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(256), 0, s, p->d_partial, p->scan_blocks, p->d_offset, p->nb);
-  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_offset, p->d_partial, p->nb);
-  if (n > 0)
-    launch_scatter(in, n, cm, p->d_bucket, p->d_rank, p->d_offset, p->d_sorted,
-                   p->tune.interp_sorted ? p->d_perm : nullptr, p->d_sc, s);
+  run_sort(p, in, n, p->tune.interp_sorted ? p->d_perm : nullptr, true, s);
   mark(p, 2, s);
   const unsigned ntiles = (unsigned)p->tm.nt * p->tm.nt * p->tm.nt;
   DepositSplit sp{};
@@ -1616,6 +1623,22 @@ int gng_pm_solve_device_f64(gng_pm_plan* p, const double* d_pos, const double* d
   StreamOrder order(p, s);
   return run_pipeline(p, aos64(d_pos, d_mass), n, reinterpret_cast<const long long*>(d_idx), k, g,
                       aos64_out(d_out), s);
+}
+
+int gng_pm_deposit_order(gng_pm_plan* p, const double* d_pos, const double* d_mass, int64_t n, uint32_t* d_order,
+                         void* stream) {
+  int rc = validate(p, n, nullptr, n);
+  if (rc) return rc;
+  if (n > 0 && (!d_pos || !d_mass || !d_order)) return fail(GNG_ERR_INVALID, "null array");
+  if (n == 0) return GNG_OK;
+  std::lock_guard<std::mutex> lk(p->mu);
+  if ((rc = set_device(p))) return rc;
+  if ((rc = ensure_work(p, n))) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  StreamOrder order(p, s);
+  run_sort(p, aos64(d_pos, d_mass), n, d_order, false, s);
+  GNG_HIP(hipGetLastError());
+  return GNG_OK;
 }
 
 int gng_pm_solve_device_soa64(gng_pm_plan* p, const double* d_x, const double* d_y, const double* d_z,

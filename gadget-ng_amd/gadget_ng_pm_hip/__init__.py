@@ -135,6 +135,8 @@ def lib():
         L.gng_pm_solve_device.argtypes = [P, P, P, P, P, i64, P, i64, d, P, P, P, P]
         L.gng_pm_solve_device_f64.argtypes = [P, P, P, i64, P, i64, d, P, P]
         L.gng_pm_solve_device_f64.restype = i32
+        L.gng_pm_deposit_order.argtypes = [P, P, P, i64, P, P]
+        L.gng_pm_deposit_order.restype = i32
         L.gng_pm_solve_device_soa64.argtypes = [P, P, P, P, P, i64, P, i64, d, P, P, P, P]
         L.gng_pm_solve_device_soa64.restype = i32
         L.gng_pm_solve_device_noalloc.argtypes = [ctypes.POINTER(GngConfig), P, P, P, P, i64, P, i64, d, P, P, P, P]
@@ -410,6 +412,19 @@ class Plan:
         k = n if k is None else k
         self._check(lib().gng_pm_solve_device_f64(self._h, pos_ptr, m_ptr, n, idx_ptr, k, float(g), out_ptr,
                                                   stream or None), "gng_pm_solve_device_f64")
+
+    def deposit_order(self, pos, mass) -> np.ndarray:
+        """The deposit's sorted particle order for (n, 3) f64 positions and n
+        masses (gng_pm_deposit_order): uint32 index of the particle in each
+        sorted slot."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        dpos = DeviceArray.from_host(pos)
+        dm = DeviceArray.from_host(np.ascontiguousarray(mass, dtype=np.float64))
+        out = DeviceArray((n,), np.uint32)
+        self._check(lib().gng_pm_deposit_order(self._h, dpos.ptr, dm.ptr, n, out.ptr, None), "gng_pm_deposit_order")
+        device_synchronize()
+        return out.numpy()
 
     def solve_device_soa64(self, x_ptr: int, y_ptr: int, z_ptr: int, m_ptr: int, n: int, ax_ptr: int, ay_ptr: int,
                            az_ptr: int, g: float, stream: int = 0, idx_ptr: Optional[int] = None,

@@ -139,6 +139,19 @@ int gng_pm_solve_device(gng_pm_plan *p, const float *d_x, const float *d_y, cons
 int gng_pm_solve_device_f64(gng_pm_plan *p, const double *d_pos, const double *d_mass, int64_t n,
                             const int64_t *d_idx, int64_t k, double g, double *d_out, void *stream);
 
+/* The order in which the deposit visits n device Vec3 particles (f64 positions
+ * and masses, as gng_pm_solve_device_f64): d_order[j] (uint32) is the
+ * particle in sorted slot j.  Slots are grouped by bucket = Hilbert rank of
+ * the particle's 16^3 (or the mesh's) deposit tile times 8 plus its class
+ * (bit a set when the base cell is on the tile's last layer along axis a),
+ * buckets ascending; the order inside a bucket varies from run to run (no
+ * result depends on it).  Runs no deposit or FFT; the plan's meshes are left
+ * as they were.  No counterpart in the reference; it exposes the spatial
+ * order of the deposit (the role sfc.rs:118-183 plays for the reference's
+ * domain decomposition) to callers that lay particles out for locality. */
+int gng_pm_deposit_order(gng_pm_plan *p, const double *d_pos, const double *d_mass, int64_t n, uint32_t *d_order,
+                         void *stream);
+
 /* Device-resident solve on SoA f64 positions (the integrator's particle state,
  * crates/gadget-ng-integrators/src/leapfrog.rs:5-90 keeps positions in f64),
  * f32 masses and f32 accelerations; otherwise as gng_pm_solve_device. */
