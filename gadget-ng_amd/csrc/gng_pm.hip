@@ -376,6 +376,11 @@ MeshOps make_ops() {
       const dim3 gs((unsigned)(N / 2 / CS), g.y);
       hipLaunchKernelGGL((k_fft_y_inv<N, CS, TS, false, 1>), gs, dim3(TS), 0, s, A, B, P, tw, make_int3(0, 0, 0));
       hipLaunchKernelGGL((k_fft_y_inv<N, CS, TS, false, 2>), gs, dim3(TS), 0, s, A, B, P, tw, make_int3(0, 0, 0));
+    } else if constexpr (YZCfg<N>::CI != C) {
+      // narrower tiles than the caller's grid counts
+      constexpr int CI = YZCfg<N>::CI, TI = YZCfg<N>::TI;
+      hipLaunchKernelGGL((k_fft_y_inv<N, CI, TI>), dim3((unsigned)(N / 2 / CI), g.y), dim3(TI), 0, s, A, B, P, tw,
+                         make_int3(0, 0, 0));
     } else {
       hipLaunchKernelGGL((k_fft_y_inv<N, C, TY>), g, b, 0, s, A, B, P, tw, make_int3(0, 0, 0));
     }

@@ -415,6 +415,13 @@ struct YZCfg {
   // the inverse y pass at N = 1024 as two launches (fields from A, then B) on
   // full 128-byte tiles, 16 values per thread
   static constexpr bool YSPLIT = N >= 1024;
+  // The combined inverse y pass keeps three fields live (about 112 VGPRs at 8
+  // values per thread), so a 1024-thread workgroup has a CU to itself and
+  // nothing loads while it transforms.  It runs instead on tiles of 8 columns
+  // and 512 threads, two workgroups to a CU (their line halves meet in one L2
+  // through line_tile): 1.119 -> 1.094 ms for the pair at N = 512.
+  static constexpr int CI = (!YSPLIT && T == 1024 && C > 8) ? 8 : C;
+  static constexpr int TI = CI == C ? T : CI * N / 8;
 };
 
 // Column tiles narrower than a 128-byte line (C = 8 at nm = 1024) share each
