@@ -425,8 +425,16 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
                                                        const DepositSplit sp = DepositSplit{}) {
   constexpr int T3 = T * T * T;
   __shared__ unsigned long long acc[T3];
-  __shared__ unsigned toff[8][9];
-  __shared__ unsigned rpre[15], rdelta[14];
+  // Prologue scratch inside the accumulator tile, which is cleared only after
+  // it has been read: the bucket offsets of the 8 source tiles (72 words),
+  // the range prefix sums (15) and slot deltas (14), and the mass total.
+  // With no other LDS the 16^3 tile takes exactly 32 KB, so five workgroups
+  // share a CU's 160 KB instead of four.
+  static_assert(T3 * 2 >= 106, "the prologue scratch must fit in the tile");
+  unsigned* const toff = reinterpret_cast<unsigned*>(acc);  // [8][9]
+  unsigned* const rpre = toff + 72;                         // [15]
+  unsigned* const rdelta = toff + 88;                       // [14]
+  unsigned long long* const qtotp = acc + 52;               // words 104-105
   const int tid = threadIdx.x;
   const int nt = tm.nt;
   // this workgroup's tile rank and slice (k of S): the primary workgroups
@@ -444,7 +452,6 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
     n_slice = sp.slices[rank_r];
     if (n_slice > 1) my_slot = sp.slot[rank_r];
   }
-  for (int c = tid; c < T3; c += NT) acc[c] = 0ull;
   const int tile = (int)tm.order[rank_r];
   const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
   if (tid < 72) {
@@ -452,24 +459,23 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
     const int sx = (d & 1) ? (ux == 0 ? nt - 1 : ux - 1) : ux;
     const int sy = (d & 2) ? (uy == 0 ? nt - 1 : uy - 1) : uy;
     const int sz = (d & 4) ? (uz == 0 ? nt - 1 : uz - 1) : uz;
-    toff[d][j] = offset[tm.rank[(unsigned)(sz * nt + sy) * nt + sx] * 8u + j];
+    toff[d * 9 + j] = offset[tm.rank[(unsigned)(sz * nt + sy) * nt + sx] * 8u + j];
   }
   // the fixed-point mass total, reduced by the third wavefront while the
   // offsets load (one slot per lane; NT >= 256)
-  __shared__ unsigned long long qtot;
   static_assert(NT >= 192 && kQSlots == 64, "one qsum slot per lane of the third wavefront");
   if (tid >= 128 && tid < 192) {
     unsigned long long v = sc->qsum[(tid - 128) * kQStride];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-    if (tid == 128) qtot = v;
+    if (tid == 128) *qtotp = v;
   }
   __syncthreads();
   if (tid < 64) {
     unsigned len = 0, beg = 0;
     if (tid < 14) {
       const int d = (int)((kRangeD >> (3 * tid)) & 7ull);
-      beg = toff[d][kRangeLo[tid]];
-      len = toff[d][kRangeHi[tid]] - beg;
+      beg = toff[d * 9 + kRangeLo[tid]];
+      len = toff[d * 9 + kRangeHi[tid]] - beg;
     }
     unsigned incl = len;
     for (int off = 1; off < 16; off <<= 1) {
@@ -484,13 +490,19 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
     if (tid == 0) rpre[0] = 0;
   }
   __syncthreads();
-  // range ends, uniform over the workgroup, in scalar registers
-  unsigned rend[13];
+  // range ends and slot deltas, uniform over the workgroup, in scalar
+  // registers; then the scratch is free for the accumulators
+  unsigned rend[13], dstep[13];
 #pragma unroll
   for (int k = 0; k < 13; ++k) rend[k] = __builtin_amdgcn_readfirstlane(rpre[k + 1]);
+  const unsigned dbase = __builtin_amdgcn_readfirstlane(rdelta[0]);
+#pragma unroll
+  for (int k = 0; k < 13; ++k) dstep[k] = __builtin_amdgcn_readfirstlane(rdelta[k + 1] - rdelta[k]);
+  const unsigned all = __builtin_amdgcn_readfirstlane(rpre[14]);
+  const unsigned long long qtot = *qtotp;
+  __syncthreads();
   const int s = fixed_shift(sc, ntot);
   // this slice's share [q0, total) of the tile's flattened ranges
-  const unsigned all = rpre[14];
   unsigned q0 = 0, total = all;
   if constexpr (SPLIT) {
     const unsigned share = (all + n_slice - 1u) / n_slice;
@@ -507,10 +519,16 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
     for (int k = 0; k < 13; ++k) r += q >= rend[k] ? 1 : 0;
     return r;
   };
-  auto fetch = [&](unsigned q, int r) {
+  // slot of flattened item q: q + the delta of its range, summed as masked
+  // steps over the range starts at or below q (range ends never decrease;
+  // written as arithmetic so that it stays in registers)
+  auto fetch = [&](unsigned q, int) {
     PartRec v;
     if (q < total) {
-      v = sorted[q + rdelta[r]];
+      unsigned dl = dbase;
+#pragma unroll
+      for (int k = 0; k < 13; ++k) dl += dstep[k] & (0u - (unsigned)(q >= rend[k]));
+      v = sorted[q + dl];
     } else {
       v.a[0] = v.a[1] = v.a[2] = 0ull;
     }
@@ -571,6 +589,9 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
     rbuf[j] = range_of(qj);
     buf[j] = fetch(qj, rbuf[j]);
   }
+  // the accumulators are cleared while the first records load
+  for (int c = tid; c < T3; c += NT) acc[c] = 0ull;
+  __syncthreads();
   for (unsigned it = 0; it < rounds; it += kPrefetch) {
 #pragma unroll
     for (int j = 0; j < kPrefetch; ++j) {
@@ -599,10 +620,11 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
     }
     __threadfence();
     __syncthreads();
-    __shared__ unsigned last;
-    if (tid == 0) last = atomicAdd(sp.done + my_slot, 1u) == n_slice - 1u;
+    // the accumulators have been read: their first word holds the flag
+    unsigned* const last = reinterpret_cast<unsigned*>(acc);
+    if (tid == 0) *last = atomicAdd(sp.done + my_slot, 1u) == n_slice - 1u;
     __syncthreads();
-    if (!last) return;
+    if (!*last) return;
     __threadfence();
   }
   const double unit = ldexp(1.0, -s);
@@ -624,8 +646,10 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
   if (SPLIT && part && tid == 0) sp.done[my_slot] = 0u;
 }
 
+// At most 96 VGPRs, so five 256-thread workgroups (the LDS limit of a 16^3
+// tile) fit a CU.
 template <int T, int NT = kDepositThreads, bool SPLIT = false>
-__global__ __launch_bounds__(NT) void k_deposit_tiles(const PartRec* __restrict__ sorted,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_deposit_tiles(const PartRec* __restrict__ sorted,
                                                        const unsigned* __restrict__ offset, const CellMap cm,
                                                        const TileMap tm, const SolveScalars* sc, long long ntot,
                                                        float* __restrict__ rho, long long rho_plane,
@@ -634,7 +658,7 @@ __global__ __launch_bounds__(NT) void k_deposit_tiles(const PartRec* __restrict_
 }
 
 template <int T, int NT = kDepositThreads>
-__global__ __launch_bounds__(NT) void k_deposit_tiles_box(const BoxArgs* __restrict__ bx, const TileMap tm) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_deposit_tiles_box(const BoxArgs* __restrict__ bx, const TileMap tm) {
   const BoxArgs& b = bx[blockIdx.z];
   deposit_tiles_body<T, NT>(b.sorted, b.offset, b.cm, tm, b.sc, b.n, reinterpret_cast<float*>(b.P),
                             2 * mesh_plane(b.cm.nm), nullptr);
