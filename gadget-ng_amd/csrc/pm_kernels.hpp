@@ -82,6 +82,11 @@ __device__ __forceinline__ void cell_fix(double x, const CellMap cm, int& i, uns
   const double nmf = (double)cm.nm;
   double c = x * cm.inv_cell;
   if (__builtin_expect(!(c >= 0.0 && c < nmf), 0)) {
+    if (!isfinite(c)) {
+      i = 0;
+      frac = 0ull;
+      return;
+    }
     c = fmod(c, nmf);
     if (c < 0.0) c += nmf;
   }

@@ -33,7 +33,10 @@
  *     the device (cic.rs:50-52) for FP32 and FP64 positions alike; the
  *     CIC weights are FP64 and accumulate in 64-bit fixed point, so the
  *     density is bit-identical from run to run; the FP32 mesh holds the mass
- *     per cell minus the mean (FFT, Green's function and interpolation FP32).
+ *     per cell minus the mean (FFT, Green's function and interpolation FP32);
+ *   - a non-finite coordinate puts its particle on node 0 of that axis (the
+ *     reference's weights would turn the whole mesh into NaN); no index
+ *     leaves the mesh whatever the input.
  * A handle is not reentrant: calls on one handle must not overlap.  The GPU
  * work of successive calls on one handle is ordered even across streams: an
  * entry point that runs on another stream than the previous call's first

@@ -138,3 +138,29 @@ def test_uniform_lattice_gives_zero_force_in_f64(pm):
     plan = pm.Plan(16, box)
     a = plan.accelerations_f64(pos, m, 1.0)
     assert np.abs(a).max() <= 1e-10
+
+
+def test_non_finite_coordinates_stay_on_the_mesh(pm, oracle_mod):
+    """NaN and infinite coordinates put their particle on node 0 of that
+    axis: the solve stays in bounds and equals the oracle's on the same
+    particles with those coordinates replaced by 0 (the reference would
+    return NaN everywhere)."""
+    rng = np.random.default_rng(17)
+    n, box, nm = 5000, 20.0, 32
+    pos = rng.uniform(0.0, box, (n, 3))
+    m = rng.uniform(0.5, 1.5, n)
+    bad = np.array([0, 1, 2, 3])
+    pos[0, 0] = np.nan
+    pos[1, 1] = np.inf
+    pos[2, 2] = -np.inf
+    pos[3] = (np.nan, np.nan, np.nan)
+    print("case f64 non-finite coordinates N=5000 nm=32", flush=True)
+    plan = pm.Plan(nm, box)
+    got = plan.accelerations_f64(pos, m, 1.0)
+    fixed = np.where(np.isfinite(pos), pos, 0.0)
+    ref = oracle_mod.pm_accelerations(fixed, m, box, nm, 1.0)
+    assert np.all(np.isfinite(got))
+    keep = np.setdiff1d(np.arange(n), bad)
+    assert rms_rel(got[keep], ref[keep]) <= 1e-5
+    a32 = plan.accelerations_f32(*(pos[:, c].astype(np.float32) for c in range(3)), m.astype(np.float32), 1.0)
+    assert np.all(np.isfinite(a32))
