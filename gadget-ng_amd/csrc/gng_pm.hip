@@ -90,6 +90,7 @@ struct Tuning {
   int chunk_fwd = 0;      // z planes per chunk of the forward x/y pass pair (0: whole mesh)
   int chunk_inv = 0;      // same for the inverse y/x pass pair
   int deposit_split = 1;  // heavy deposit tiles over several workgroups
+  int y_inv_narrow = 1;   // inverse y pass on the narrower tile where one is compiled
 };
 
 // Planes per chunk when a pass pair runs chunk by chunk, so that the second
@@ -118,7 +119,7 @@ Tuning default_tuning(int nm) {
   return t;
 }
 
-constexpr int kKnobs = 7;
+constexpr int kKnobs = 8;
 int& knob(Tuning& t, int k) {
   switch (k) {
     case 0: return t.z_persistent;
@@ -127,7 +128,8 @@ int& knob(Tuning& t, int k) {
     case 3: return t.y_inv_persistent;
     case 4: return t.chunk_fwd;
     case 5: return t.chunk_inv;
-    default: return t.deposit_split;
+    case 6: return t.deposit_split;
+    default: return t.y_inv_narrow;
   }
 }
 
@@ -155,11 +157,12 @@ void load_tune_file_locked() {
   if (!fp) return;
   char arch[64];
   int nm, v[kKnobs];
-  while (std::fscanf(fp, "%63s %d %d %d %d %d %d %d %d", arch, &nm, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
-                     &v[6]) == 2 + kKnobs) {
+  while (std::fscanf(fp, "%63s %d %d %d %d %d %d %d %d %d", arch, &nm, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
+                     &v[6], &v[7]) == 2 + kKnobs) {
     Tuning t;
     for (int k = 0; k < 4; ++k) knob(t, k) = v[k] != 0;
     t.deposit_split = v[6] != 0;
+    t.y_inv_narrow = v[7] != 0;
     t.chunk_fwd = v[4] == chunk_rule(nm, kChunkFwdMiB) ? v[4] : 0;
     t.chunk_inv = v[5] == chunk_rule(nm, kChunkInvMiB) ? v[5] : 0;
     t.tuned = 1;
@@ -184,9 +187,10 @@ void store_tuning(int dev, int nm, const Tuning& t) {
   FILE* fp = std::fopen(f, "w");
   if (!fp) return;
   for (const auto& kv : g_tune)
-    std::fprintf(fp, "%s %d %d %d %d %d %d %d %d\n", kv.first.first.c_str(), kv.first.second,
+    std::fprintf(fp, "%s %d %d %d %d %d %d %d %d %d\n", kv.first.first.c_str(), kv.first.second,
                  kv.second.z_persistent, kv.second.deposit_wide, kv.second.y_fwd_persistent,
-                 kv.second.y_inv_persistent, kv.second.chunk_fwd, kv.second.chunk_inv, kv.second.deposit_split);
+                 kv.second.y_inv_persistent, kv.second.chunk_fwd, kv.second.chunk_inv, kv.second.deposit_split,
+                 kv.second.y_inv_narrow);
   std::fclose(fp);
 }
 
@@ -295,6 +299,10 @@ struct MeshOps {
   void (*y_fwd_persist)(dim3, dim3, hipStream_t, float2*, const float2*);
   void (*y_inv_persist)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
   void (*y_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
+  // the inverse y pass on narrower tiles (same grid argument); equal to y_inv
+  // where the mesh has no narrower variant (has_narrow_y_inv = false)
+  void (*y_inv_narrow)(dim3, dim3, hipStream_t, float2*, float2*, float2*, const float2*);
+  bool has_narrow_y_inv;
   void (*x_inv)(dim3, dim3, hipStream_t, float2*, float2*, float2*, float2*, const float2*, int, float);
   // power spectrum back end
   void (*z_pk)(dim3, dim3, hipStream_t, const float2*, const float2*, const double*, double*, long long,
@@ -376,15 +384,22 @@ MeshOps make_ops() {
       const dim3 gs((unsigned)(N / 2 / CS), g.y);
       hipLaunchKernelGGL((k_fft_y_inv<N, CS, TS, false, 1>), gs, dim3(TS), 0, s, A, B, P, tw, make_int3(0, 0, 0));
       hipLaunchKernelGGL((k_fft_y_inv<N, CS, TS, false, 2>), gs, dim3(TS), 0, s, A, B, P, tw, make_int3(0, 0, 0));
-    } else if constexpr (YZCfg<N>::CI != C) {
-      // narrower tiles than the caller's grid counts
-      constexpr int CI = YZCfg<N>::CI, TI = YZCfg<N>::TI;
-      hipLaunchKernelGGL((k_fft_y_inv<N, CI, TI>), dim3((unsigned)(N / 2 / CI), g.y), dim3(TI), 0, s, A, B, P, tw,
-                         make_int3(0, 0, 0));
     } else {
       hipLaunchKernelGGL((k_fft_y_inv<N, C, TY>), g, b, 0, s, A, B, P, tw, make_int3(0, 0, 0));
     }
   };
+  o.has_narrow_y_inv = YZCfg<N>::CI != C;
+  if constexpr (YZCfg<N>::CI != C) {
+    o.y_inv_narrow = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* B, float2* P, const float2* tw) {
+      // narrower tiles than the caller's grid counts
+      constexpr int CI = YZCfg<N>::CI, TI = YZCfg<N>::TI;
+      (void)b;
+      hipLaunchKernelGGL((k_fft_y_inv<N, CI, TI>), dim3((unsigned)(N / 2 / CI), g.y), dim3(TI), 0, s, A, B, P, tw,
+                         make_int3(0, 0, 0));
+    };
+  } else {
+    o.y_inv_narrow = o.y_inv;
+  }
   o.x_inv = [](dim3 g, dim3 b, hipStream_t s, float2* A, float2* P, float2* B, float2* pot, const float2* tw,
                 int field0, float pot_scale) {
     hipLaunchKernelGGL((k_fft_x_inv<N, R>), g, b, 0, s, A, P, B, pot, tw, field0, pot_scale, make_int3(0, 0, 0));
@@ -886,13 +901,14 @@ int run_pipeline(gng_pm_plan* p, const PartIn& in, long long n, const long long*
   }
   o.z_green_col0(dim3((unsigned)(H + 1)), dim3(256), s, p->d_P, p->d_A, p->d_B, p->d_tw, gp);
   mark(p, 6, s);
+  const auto y_inv = p->tune.y_inv_narrow ? o.y_inv_narrow : o.y_inv;
   const int cz = p->tune.chunk_inv;
   if (cz > 0 && !p->tune.y_inv_persistent && !p->keep_potential) {
     // chunks of cz planes: each x pass reads the three fields its y pass just wrote
     const long long pl = mesh_plane(nm);
     for (int z0 = 0; z0 < nm; z0 += cz) {
-      o.y_inv(dim3((unsigned)(H / o.yz_cols), (unsigned)cz), yz_block, s, p->d_A + pl * z0, p->d_B + pl * z0,
-              p->d_P + pl * z0, p->d_tw);
+      y_inv(dim3((unsigned)(H / o.yz_cols), (unsigned)cz), yz_block, s, p->d_A + pl * z0, p->d_B + pl * z0,
+            p->d_P + pl * z0, p->d_tw);
       o.x_inv(dim3((unsigned)((long long)cz * nm / o.x_rows), 3), dim3(256), s, p->d_A + pl * z0, p->d_P + pl * z0,
               p->d_B + pl * z0, nullptr, p->d_tw, 0, 1.0f);
     }
@@ -903,7 +919,7 @@ int run_pipeline(gng_pm_plan* p, const PartIn& in, long long n, const long long*
   if (p->tune.y_inv_persistent && p->cus > 0)
     o.y_inv_persist(dim3((unsigned)p->cus), yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
   else
-    o.y_inv(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
+    y_inv(yz_grid, yz_block, s, p->d_A, p->d_B, p->d_P, p->d_tw);
   mark(p, 7, s);
   if (p->keep_potential)
     // the potential reads A before the force pass below overwrites it
@@ -1928,6 +1944,7 @@ int autotune_impl(gng_pm_plan* p, const PartIn& in, long long n, const AccOut& o
   // run-to-run spread of best-of-reps timings on a warm GPU (a few 0.1 %)
   constexpr float kMargin = 0.995f;
   for (int k = 0; k < kKnobs && rc == GNG_OK; ++k) {
+    if (k == 7 && !p->ops->has_narrow_y_inv) continue;  // one tile width compiled
     Tuning trial = best;
     if (k == 4 || k == 5) {
       // chunked or whole-mesh pass pairs
@@ -2014,6 +2031,7 @@ int gng_pm_set_tuning(gng_pm_plan* p, const gng_pm_tuning_t* t) {
   p->tune.chunk_fwd = t->plane_chunk_fwd;
   p->tune.chunk_inv = t->plane_chunk_inv;
   p->tune.deposit_split = t->deposit_split != 0;
+  p->tune.y_inv_narrow = t->y_inv_narrow != 0;
   p->tune.tuned = 0;
   return GNG_OK;
 }
@@ -2030,6 +2048,7 @@ int gng_pm_get_tuning(const gng_pm_plan* p, gng_pm_tuning_t* t) {
   t->plane_chunk_fwd = p->tune.chunk_fwd;
   t->plane_chunk_inv = p->tune.chunk_inv;
   t->deposit_split = p->tune.deposit_split;
+  t->y_inv_narrow = p->tune.y_inv_narrow;
   return GNG_OK;
 }
 

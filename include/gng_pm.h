@@ -242,6 +242,12 @@ typedef struct gng_pm_tuning_t {
                                 (dense clumps) are shared by several workgroups
                                 (1, default) or each run by one (0).  Same
                                 density bit for bit either way */
+  int32_t y_inv_narrow;      /* inverse y pass on 8-column tiles of 512 threads
+                                (1, default) instead of 16 columns of 1024, where
+                                both are compiled (nm = 512; elsewhere ignored):
+                                the three live fields leave a 1024-thread
+                                workgroup alone on its CU.  Same values either
+                                way */
 } gng_pm_tuning_t;
 
 int gng_pm_autotune(gng_pm_plan *p, int reps);

@@ -27,7 +27,7 @@ def test_every_variant_matches_oracle(pm, oracle_mod, nm):
         assert rms_rel(got, ref) < 1e-5, (zp, dt, yf, yi)
         assert plan.tuning() == {"z_persistent": zp, "deposit_threads": dt, "y_fwd_persistent": yf,
                                  "y_inv_persistent": yi, "interp_sorted": False, "chunk_fwd": 0, "chunk_inv": 0,
-                                 "deposit_split": True, "tuned": False}
+                                 "deposit_split": True, "y_inv_narrow": True, "tuned": False}
 
 
 @pytest.mark.parametrize("nm,chunks", [(32, (4, 16)), (64, (8, 32)), (96, (12, 48)), (128, (16, 64))])
@@ -144,3 +144,27 @@ def test_split_deposit_of_dense_tiles_is_bit_identical(pm, oracle_mod):
         assert np.array_equal(a1, a0)
     ref = oracle_mod.pm_accelerations(pos, m, L, nm, 1.0)
     assert rms_rel(a1, ref) <= 1e-5
+
+
+def test_inverse_y_tile_widths_are_bit_identical(pm, oracle_mod):
+    """At nm = 512 the inverse y pass is compiled for 16-column tiles of 1024
+    threads and 8-column tiles of 512: every column's transform is the same,
+    so the forces agree bit for bit, chunked or not, and match the oracle."""
+    nm, L = 512, 100.0
+    g = np.random.default_rng(512)
+    n = 200000
+    pos = g.uniform(0, L, (n, 3))
+    m = g.uniform(0.5, 1.5, n)
+    print(f"case inverse y tile widths nm={nm} n={n}", flush=True)
+    plan = pm.Plan(nm, L, device=0)
+    out = {}
+    for narrow in (False, True):
+        for ci in (0, 64):
+            plan.set_tuning(y_inv_narrow=narrow, chunk_inv=ci)
+            assert plan.tuning()["y_inv_narrow"] == narrow
+            out[(narrow, ci)] = plan.accelerations_f64(pos, m, 1.0)
+    ref = out[(False, 0)]
+    for k, v in out.items():
+        assert np.array_equal(v, ref), k
+    idx = g.choice(n, 20000, replace=False)
+    assert rms_rel(ref[idx], oracle_mod.pm_accelerations(pos, m, L, nm, 1.0, idx=idx)) <= 1e-5
