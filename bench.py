@@ -1058,6 +1058,12 @@ def bench_learned(args) -> int:
     import gadget_ng_pm_hip as pm
     from gadget_ng_pm_hip import learned
 
+    if args.copy_calibration:
+        src = pm.DeviceArray((1 << 28,), np.float32)
+        dst = pm.DeviceArray((1 << 28,), np.float32)
+        pm.device_copy(dst, src)
+        pm.device_synchronize()
+        del src, dst
     lc = learned.LearnedCorrection(cells=64, batch=8)
     t0 = time.perf_counter()
     info = lc.build_training_set(range(1, args.train_boxes + 1), workers=-1)
