@@ -646,6 +646,70 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
   if (SPLIT && part && tid == 0) sp.done[my_slot] = 0u;
 }
 
+// Particle moments per cell from the sorted cell records of the last sort,
+// kCellMomParts workgroups per tile, one for each part of its z layers
+// (5 int64 LDS accumulators per cell of the part), 1024 threads each, so that
+// a dense tile (a Plummer core holds 10^5 particles) is read by several
+// workgroups at once and in few rounds: for the particles whose base cell c is in the tile,
+// mom[5c + a] = sum m (f_a - 1/2) (a = x, y, z), mom[5c + 3] =
+// sum m |f - 1/2|^2, mom[5c + 4] = sum m, with f the 43-bit in-cell fraction
+// of the record.  The sums are 64-bit fixed point (the deposit's scale), so
+// they do not depend on the particle order; every cell of the tile is
+// written once (zero for an empty cell), so no clearing pass and no global
+// atomics are needed.
+constexpr int kCellMoments = 5;
+constexpr int kCellMomThreads = 1024;
+constexpr int kCellMomParts = 2;
+template <int T>
+__global__ __launch_bounds__(kCellMomThreads) void k_cell_moments(const PartRec* __restrict__ sorted,
+                                                                  const unsigned* __restrict__ offset,
+                                                                  const CellMap cm, const TileMap tm,
+                                                                  const SolveScalars* sc, long long ntot,
+                                                                  double* __restrict__ mom) {
+  constexpr int HL = T / kCellMomParts, HC = T * T * HL, NT = kCellMomThreads;
+  static_assert(T % kCellMomParts == 0, "tiles split into parts of whole layers");
+  __shared__ unsigned long long acc[kCellMoments][HC];
+  const int tid = threadIdx.x;
+  const int nt = tm.nt;
+  const int tile = blockIdx.x, half = blockIdx.y;
+  const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
+  const unsigned b0 = tm.rank[tile] * 8u;
+  const unsigned beg = offset[b0], end = offset[b0 + 8];
+  const int s = fixed_shift(sc, ntot);
+  const double scale = ldexp(1.0, s), unit = ldexp(1.0, -s);
+  const unsigned rounds = (end - beg + NT - 1u) / NT;
+  {
+    for (int c = tid; c < HC; c += NT)
+#pragma unroll
+      for (int k = 0; k < kCellMoments; ++k) acc[k][c] = 0ull;
+    __syncthreads();
+    // every lane runs the same rounds, so the aggregation ballots see whole waves
+    for (unsigned it = 0; it < rounds; ++it) {
+      const unsigned q = beg + it * NT + (unsigned)tid;
+      bool live = q < end;
+      PartRec p;
+      p.a[0] = p.a[1] = p.a[2] = 0ull;
+      if (live) p = sorted[q];
+      const int lx = rec_cell(p, 0) % T, ly = rec_cell(p, 1) % T, lz = rec_cell(p, 2) % T;
+      live = live && lz / HL == half;
+      const int c = ((lz - half * HL) * T + ly) * T + lx;
+      const double fx = rec_frac(p, 0) - 0.5, fy = rec_frac(p, 1) - 0.5, fz = rec_frac(p, 2) - 0.5;
+      const double ms = (double)rec_mass(p) * scale;
+      const long long v[kCellMoments] = {__double2ll_rn(ms * fx), __double2ll_rn(ms * fy), __double2ll_rn(ms * fz),
+                                          __double2ll_rn(ms * (fx * fx + fy * fy + fz * fz)), __double2ll_rn(ms)};
+#pragma unroll
+      for (int k = 0; k < kCellMoments; ++k) lds_add_aggregated(acc[k], live ? c : 0, live ? v[k] : 0ll, live);
+    }
+    __syncthreads();
+    for (int c = tid; c < HC; c += NT) {
+      const int lx = c % T, ly = (c / T) % T, lz = c / (T * T) + half * HL;
+      const size_t cell = ((size_t)(uz * T + lz) * cm.nm + (uy * T + ly)) * cm.nm + (ux * T + lx);
+#pragma unroll
+      for (int k = 0; k < kCellMoments; ++k) mom[cell * kCellMoments + k] = (double)(long long)acc[k][c] * unit;
+    }
+  }
+}
+
 // At most 96 VGPRs, so five 256-thread workgroups (the LDS limit of a 16^3
 // tile) fit a CU.
 template <int T, int NT = kDepositThreads, bool SPLIT = false>

@@ -291,6 +291,9 @@ struct MeshOps {
   // split: dense tiles shared by several workgroups
   void (*deposit[2][2])(dim3, dim3, hipStream_t, const PartRec*, const unsigned*, CellMap, TileMap,
                         const SolveScalars*, long long, float*, long long, float*, DepositSplit);
+  // per-cell particle moments of the sorted records, one workgroup per tile
+  void (*cell_moments)(dim3, dim3, hipStream_t, const PartRec*, const unsigned*, CellMap, TileMap,
+                       const SolveScalars*, long long, double*);
   void (*x_fwd)(dim3, dim3, hipStream_t, const float*, float2*, const float2*, long long);
   void (*y_fwd)(dim3, dim3, hipStream_t, float2*, const float2*);
   void (*z_green)(dim3, dim3, hipStream_t, const float2*, float2*, float2*, const float2*, GreenParams);
@@ -354,6 +357,10 @@ MeshOps make_ops() {
   GNG_DEPOSIT(0, 256, true)
   GNG_DEPOSIT(1, 512, true)
 #undef GNG_DEPOSIT
+  o.cell_moments = [](dim3 g, dim3 b, hipStream_t s, const PartRec* srt, const unsigned* off, CellMap cm,
+                      TileMap tm, const SolveScalars* sc, long long n, double* mom) {
+    hipLaunchKernelGGL((k_cell_moments<TD>), g, b, 0, s, srt, off, cm, tm, sc, n, mom);
+  };
 
   o.x_fwd = [](dim3 g, dim3 b, hipStream_t s, const float* rho, float2* P, const float2* tw, long long rho_plane) {
     hipLaunchKernelGGL((k_fft_x_fwd<N, R>), g, b, 0, s, rho, P, tw, rho_plane);
@@ -542,6 +549,7 @@ struct gng_pm_plan {
 
   long long last_n = 0;
   bool density_valid = false;
+  long long sorted_n = -1;  // particles of the last counting sort (its records are in d_sorted)
 
   // power spectrum work space (allocated on first use)
   double* d_pk_w = nullptr;     // 1 / sinc(freq(i) / nm)^2, nm values
@@ -809,6 +817,7 @@ void run_sort(gng_pm_plan* p, const PartIn& in, long long n, unsigned* perm, boo
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(256), 0, s, p->d_partial, p->scan_blocks, p->d_offset, p->nb);
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_offset, p->d_partial, p->nb);
   if (n > 0) launch_scatter(in, n, cm, p->d_bucket, p->d_rank, p->d_offset, p->d_sorted, perm, p->d_sc, s);
+  p->sorted_n = n;
 }
 
 int run_front(gng_pm_plan* p, const PartIn& in, long long n, hipStream_t s) {
@@ -2088,6 +2097,21 @@ int gng_pm_device_meshes(gng_pm_plan* p, const float** d_mass, const float** d_p
       return fail(GNG_ERR_INVALID, "no kept potential: call gng_pm_keep_potential(p, 1) first");
     *d_potential = p->d_pot;
   }
+  return GNG_OK;
+}
+
+int gng_pm_cell_moments(gng_pm_plan* p, double* d_mom, void* stream) {
+  if (!p || !d_mom) return fail(GNG_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (p->sorted_n < 0) return fail(GNG_ERR_INVALID, "no particles sorted yet: solve first");
+  int rc = set_device(p);
+  if (rc) return rc;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  StreamOrder order(p, s);
+  const unsigned ntiles = (unsigned)p->tm.nt * p->tm.nt * p->tm.nt;
+  p->ops->cell_moments(dim3(ntiles, kCellMomParts), dim3(kCellMomThreads), s, p->d_sorted, p->d_offset, cell_map(p), p->tm,
+                       p->d_sc, p->sorted_n, d_mom);
+  GNG_HIP(hipGetLastError());
   return GNG_OK;
 }
 

@@ -557,17 +557,27 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, long long n) {
 // The potential minus its 7^3 box mean keeps the part set by mass within a
 // few cells and drops the box-scale modes, which carry no sub-cell
 // information.
+// The moments come as 64-bit fixed-point sums (mom, times mom_unit =
+// 2^-shift / rho_mean) or as doubles (momd, times mom_unit = 1 / rho_mean),
+// five per cell in the order of k_moments.
 __global__ void k_input(const float* __restrict__ rho, const float* __restrict__ phi,
                         const float* __restrict__ phi_mean, const unsigned long long* __restrict__ mom,
-                        double mom_unit, long long M, float inv_rho, float inv_phi, u32x4* __restrict__ x0) {
+                        const double* __restrict__ momd, double mom_unit, long long M, float inv_rho,
+                        float inv_phi, u32x4* __restrict__ x0) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
   float c2 = 0.0f, c3 = 0.0f, c4 = 0.0f, c5 = 0.0f;
-  if (mom) {
-    // mom_unit = 2^-shift / rho_mean
-    const long long* q = reinterpret_cast<const long long*>(mom) + i * kMoments;
-    const double dx = (double)q[0] * mom_unit, dy = (double)q[1] * mom_unit, dz = (double)q[2] * mom_unit;
-    const double m2 = (double)q[3] * mom_unit, m0 = (double)q[4] * mom_unit;
+  if (mom || momd) {
+    double v[kMoments];
+    if (mom) {
+      const long long* q = reinterpret_cast<const long long*>(mom) + i * kMoments;
+#pragma unroll
+      for (int k = 0; k < kMoments; ++k) v[k] = (double)q[k] * mom_unit;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kMoments; ++k) v[k] = momd[i * kMoments + k] * mom_unit;
+    }
+    const double dx = v[0], dy = v[1], dz = v[2], m2 = v[3], m0 = v[4];
     const double spread = m0 > 0.0 ? m2 - (dx * dx + dy * dy + dz * dz) / m0 : 0.0;
     c2 = (float)dx;
     c3 = (float)dy;
@@ -814,6 +824,27 @@ dim3 conv_grid(const gng_net* t, int boxes) {
 
 }  // namespace
 
+namespace {
+
+// The box means of the potential and the input kernel, moments from one of
+// the two sources (or none).
+int net_input(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean, float phi_scale,
+              const unsigned long long* mom, const double* momd, double mom_unit, hipStream_t s) {
+  const dim3 g((unsigned)((net->M + 255) / 256));
+  const int nm = net->nm;
+  float* t0 = net->d_tmp;
+  float* t1 = net->d_tmp + net->M;
+  hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, d_phi, t0, nm, 1LL);
+  hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, t0, t1, nm, (long long)nm);
+  hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, t1, t0, nm, (long long)nm * nm);
+  hipLaunchKernelGGL(k_input, g, dim3(256), 0, s, d_rho, d_phi, t0, mom, momd, mom_unit, net->M, 1.0f / rho_mean,
+                     1.0f / phi_scale, reinterpret_cast<u32x4*>(net->d_x0) + (long long)box * net->M);
+  GNG_HIP(hipGetLastError());
+  return GNG_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int32_t gng_net_param_count(void) { return P_COUNT; }
@@ -953,6 +984,17 @@ int gng_net_write(gng_net* net, int32_t which, const void* host, int64_t bytes) 
   return GNG_OK;
 }
 
+
+int gng_net_input_moments(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
+                          float phi_scale, const double* d_mom, void* stream) {
+  if (!net || !d_rho || !d_phi || !d_mom) return fail(GNG_ERR_INVALID, "null argument");
+  if (box < 0 || box >= net->max_boxes) return fail(GNG_ERR_INVALID, "box slot out of range");
+  if (!(rho_mean > 0.0f) || !(phi_scale > 0.0f)) return fail(GNG_ERR_INVALID, "scales must be positive");
+  GNG_HIP(hipSetDevice(net->device));
+  return net_input(net, box, d_rho, d_phi, rho_mean, phi_scale, nullptr, d_mom, 1.0 / (double)rho_mean,
+                   as_stream(stream));
+}
+
 int gng_net_input_particles(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
                             float phi_scale, const float* d_x, const float* d_y, const float* d_z, const float* d_m,
                             int64_t n, double box_size, void* stream) {
@@ -965,7 +1007,6 @@ int gng_net_input_particles(gng_net* net, int32_t box, const float* d_rho, const
   if (particles && !(box_size > 0.0)) return fail(GNG_ERR_INVALID, "box_size must be positive");
   GNG_HIP(hipSetDevice(net->device));
   hipStream_t s = as_stream(stream);
-  const dim3 g((unsigned)((net->M + 255) / 256));
   const int nm = net->nm;
   double mom_unit = 0.0;
   if (particles) {
@@ -981,16 +1022,8 @@ int gng_net_input_particles(gng_net* net, int32_t box, const float* d_rho, const
                        (long long)n, (float)((double)nm / box_size), nm, std::ldexp(1.0, shift), net->d_mom);
     mom_unit = std::ldexp(1.0, -shift) / (double)rho_mean;
   }
-  float* t0 = net->d_tmp;
-  float* t1 = net->d_tmp + net->M;
-  hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, d_phi, t0, nm, 1LL);
-  hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, t0, t1, nm, (long long)nm);
-  hipLaunchKernelGGL(k_boxmean, g, dim3(256), 0, s, t1, t0, nm, (long long)nm * nm);
-  hipLaunchKernelGGL(k_input, g, dim3(256), 0, s, d_rho, d_phi, t0,
-                     particles ? (const unsigned long long*)net->d_mom : nullptr, mom_unit, net->M, 1.0f / rho_mean,
-                     1.0f / phi_scale, reinterpret_cast<u32x4*>(net->d_x0) + (long long)box * net->M);
-  GNG_HIP(hipGetLastError());
-  return GNG_OK;
+  return net_input(net, box, d_rho, d_phi, rho_mean, phi_scale,
+                   particles ? (const unsigned long long*)net->d_mom : nullptr, nullptr, mom_unit, s);
 }
 
 int gng_net_input(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,

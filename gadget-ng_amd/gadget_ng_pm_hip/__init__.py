@@ -137,6 +137,8 @@ def lib():
         L.gng_pm_solve_device_f64.restype = i32
         L.gng_pm_deposit_order.argtypes = [P, P, P, i64, P, P]
         L.gng_pm_deposit_order.restype = i32
+        L.gng_pm_cell_moments.argtypes = [P, P, P]
+        L.gng_pm_cell_moments.restype = i32
         L.gng_pm_solve_device_soa64.argtypes = [P, P, P, P, P, i64, P, i64, d, P, P, P, P]
         L.gng_pm_solve_device_soa64.restype = i32
         L.gng_pm_solve_device_noalloc.argtypes = [ctypes.POINTER(GngConfig), P, P, P, P, i64, P, i64, d, P, P, P, P]
@@ -212,6 +214,7 @@ def lib():
         L.gng_net_device_buffer.restype = i32
         L.gng_net_input.argtypes = [P, i32, P, P, f32, f32, P]
         L.gng_net_input_particles.argtypes = [P, i32, P, P, f32, f32, P, P, P, P, i64, d, P]
+        L.gng_net_input_moments.argtypes = [P, i32, P, P, f32, f32, P, P]
         L.gng_net_forward.argtypes = [P, i32, P]
         L.gng_net_infer.argtypes = [P, i32, P]
         L.gng_net_infer.restype = i32
@@ -223,7 +226,8 @@ def lib():
         L.gng_net_apply.argtypes = [P, i32, P, P, P, i64, d, f32, P, P, P, P]
         L.gng_net_apply_field.argtypes = [P, i32, P, P, P, i64, d, f32, P, d, P, P, P, P]
         for f in (L.gng_net_create, L.gng_net_set_params, L.gng_net_get_params, L.gng_net_read, L.gng_net_write,
-                  L.gng_net_input, L.gng_net_input_particles, L.gng_net_forward, L.gng_net_loss_grad,
+                  L.gng_net_input, L.gng_net_input_particles, L.gng_net_input_moments, L.gng_net_forward,
+                  L.gng_net_loss_grad,
                   L.gng_net_backward, L.gng_net_adam, L.gng_net_train_step, L.gng_net_loss, L.gng_net_apply,
                   L.gng_net_apply_field):
             f.restype = i32
@@ -481,6 +485,19 @@ class Plan:
         out = np.empty((nm, nm, nm), dtype=np.float64)
         self._check(lib().gng_pm_potential(self._h, _ptr(out)), "gng_pm_potential")
         return out
+
+    def cell_moments_device(self, mom_ptr: int, stream: int = 0) -> None:
+        """Enqueue the per-cell particle moments of the last solve into a device
+        buffer of 5 doubles per cell (gng_pm_cell_moments)."""
+        self._check(lib().gng_pm_cell_moments(self._h, mom_ptr, stream or None), "gng_pm_cell_moments")
+
+    def cell_moments(self) -> np.ndarray:
+        """(nm, nm, nm, 5) f64 moments of the last solve's particles."""
+        nm = self.grid_size
+        out = DeviceArray((nm, nm, nm, 5), np.float64)
+        self.cell_moments_device(out.ptr)
+        device_synchronize()
+        return out.numpy()
 
     def device_meshes(self):
         """(mass-per-cell, potential) device pointers of the last solve."""

@@ -265,6 +265,12 @@ class Net:
         _check(lib().gng_net_input(self._h, int(box), d_mass, d_phi, float(rho_mean), float(phi_scale),
                                    stream or None), "gng_net_input")
 
+    def input_from_moments(self, box: int, d_mass: int, d_phi: int, rho_mean: float, phi_scale: float,
+                           d_mom: int, stream: int = 0) -> None:
+        """Input with the moment channels from per-cell moments (Plan.cell_moments_device)."""
+        _check(lib().gng_net_input_moments(self._h, int(box), d_mass, d_phi, float(rho_mean), float(phi_scale),
+                                           d_mom, stream or None), "gng_net_input_moments")
+
     def input_from_particles(self, box: int, d_mass: int, d_phi: int, rho_mean: float, phi_scale: float,
                              x: int, y: int, z: int, m: int, n: int, box_size: float, stream: int = 0) -> None:
         """Input with the particle-moment channels (gng_net_input_particles)."""
@@ -408,6 +414,8 @@ class LearnedCorrection:
         self.lr = Plan(self.nm, self.L, r_split=self.r_s, deconvolve=True, device=device)
         self.sr = ShortRange(device=device)
         self.net = Net(self.nm, self.batch, device=device)
+        # per-cell particle moments of the last PM solve (5 doubles per cell)
+        self._mom = DeviceArray((self.nm ** 3 * 5,), np.float64)
         self.f0: Optional[float] = None
         # potential unit: phi_scale * G * (mean mass per cell) / (cell size)
         self.phi_scale = float(phi_scale) * self.g
@@ -426,7 +434,8 @@ class LearnedCorrection:
         t0 = time.perf_counter()
         self.pm.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, a_pm[0].ptr, a_pm[1].ptr, a_pm[2].ptr, self.g)
         dm, dp = self.pm.device_meshes()
-        net.input_from_particles(slot, dm, dp, rho_mean, self.phi_scale, x.ptr, y.ptr, z.ptr, m.ptr, n, self.L)
+        self.pm.cell_moments_device(self._mom.ptr)
+        net.input_from_moments(slot, dm, dp, rho_mean, self.phi_scale, self._mom.ptr)
         device_synchronize()
         t1 = time.perf_counter()
         self.lr.solve_device(x.ptr, y.ptr, z.ptr, m.ptr, n, a_tp[0].ptr, a_tp[1].ptr, a_tp[2].ptr, self.g)
@@ -573,7 +582,8 @@ class LearnedCorrection:
         t_pm = time.perf_counter() - t0
         t0 = time.perf_counter()
         dm, dp = other.pm.device_meshes()
-        net.input_from_particles(0, dm, dp, rho_mean, other.phi_scale, x.ptr, y.ptr, z.ptr, m.ptr, n, other.L)
+        other.pm.cell_moments_device(other._mom.ptr)
+        net.input_from_moments(0, dm, dp, rho_mean, other.phi_scale, other._mom.ptr)
         device_synchronize()
         t_in = time.perf_counter() - t0
         t0 = time.perf_counter()

@@ -126,6 +126,12 @@ int gng_net_device_buffer(gng_net* net, int32_t which, void** d_ptr);
 int gng_net_input_particles(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
                             float phi_scale, const float* d_x, const float* d_y, const float* d_z, const float* d_m,
                             int64_t n, double box_size, void* stream);
+/* gng_net_input_particles with the moments given per cell (d_mom: 5 doubles
+ * per cell in that order, sum m (f - 1/2) per axis, sum m |f - 1/2|^2,
+ * sum m), as gng_pm_cell_moments writes them for the particles of a PM
+ * solve: no atomics, the FP64 cell coordinate of the deposit. */
+int gng_net_input_moments(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
+                          float phi_scale, const double* d_mom, void* stream);
 /* gng_net_input_particles without particles (channels 2-7 zero). */
 int gng_net_input(gng_net* net, int32_t box, const float* d_rho, const float* d_phi, float rho_mean,
                   float phi_scale, void* stream);

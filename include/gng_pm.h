@@ -279,6 +279,17 @@ int gng_pm_potential(gng_pm_plan *p, double *out);
  * is destroyed and are overwritten by the next solve. */
 int gng_pm_device_meshes(gng_pm_plan *p, const float **d_mass, const float **d_potential);
 
+/* Particle moments per cell of the particles of the last solve (or
+ * gng_pm_deposit_order query), on `stream`: d_mom (device, 5 doubles per
+ * cell, cell = (iz nm + iy) nm + ix) receives sum m (f_a - 1/2) for the
+ * three axes, sum m |f - 1/2|^2 and sum m over the particles whose base cell
+ * it is, f being the in-cell fraction of the FP64 cell coordinate the
+ * deposit uses.  The sums are 64-bit fixed point inside each deposit tile
+ * (no atomics leave the tile), so they do not depend on the particle order.
+ * No counterpart in the reference; the learned correction's network input
+ * (gng_net_input_moments) reads them. */
+int gng_pm_cell_moments(gng_pm_plan *p, double *d_mom, void *stream);
+
 /* Per-stage GPU times in milliseconds, from HIP events recorded on the solve's
  * stream between the stages of every solve issued while timing is enabled
  * (gng_pm_set_timing resets the record), averaged over the most recent 64

@@ -378,3 +378,38 @@ def test_inference_pass_equals_training_forward(learned):
     assert np.array_equal(net.read("y"), y_train)
     assert np.array_equal(net.read("x1"), x1_train)
     assert np.array_equal(net.read("z1"), z1)  # inference leaves the pre-activations alone
+
+
+@pytest.mark.parametrize("nm,box", [(32, 32.0), (48, 20.0)])
+def test_cell_moments_of_a_solve_match_oracle(learned, pm, nm, box):
+    """gng_pm_cell_moments (moments of the last solve's particles from the
+    deposit's sorted records, two workgroups per tile) against the oracle's
+    per-particle sums, with a dense clump and wrapped positions; then the
+    network input built from them (gng_net_input_moments) against the oracle
+    input."""
+    g = np.random.default_rng(nm)
+    n = 60000
+    pos = g.uniform(0.0, box, (n, 3))
+    pos[: n // 3] = g.normal(0.4 * box, 0.03 * box, (n // 3, 3))
+    pos[n // 3: n // 3 + 500] += box * g.integers(-1, 2, (500, 3))
+    m = g.uniform(0.5, 1.5, n)
+    print(f"case cell moments nm={nm} n={n}", flush=True)
+    plan = pm.Plan(nm, box, keep_density=True, keep_potential=True)
+    plan.accelerations_f64(pos, m, 1.0)
+    got = plan.cell_moments()
+    D, Q, S = lo.particle_moments(pos, np.asarray(m, np.float32).astype(np.float64), box, nm)
+    scale = S.max()
+    assert np.abs(got[..., 4] - S).max() <= 1e-12 * scale
+    assert np.abs(got[..., :3] - D).max() <= 1e-12 * scale
+    assert np.abs(got[..., 3] - Q).max() <= 1e-12 * scale
+    # the network input from these moments
+    net = learned.Net(nm, 1, device=0)
+    dmom = pm.DeviceArray.from_host(got.reshape(-1))
+    dm, dp = plan.device_meshes()
+    rho_mean = float(np.sum(np.asarray(m, np.float32), dtype=np.float64)) / nm ** 3
+    net.input_from_moments(0, dm, dp, rho_mean, 3.0, dmom.ptr)
+    x0 = net.read("x0")[0]
+    ref = lo.net_input(plan.density(), plan.potential(), rho_mean, 3.0, moments=(D, Q, S)).astype(np.float32)
+    assert bf16_ulp_ok(x0[..., 2:5], lo.bf16_round(ref[..., 2:5]), frac=1.0)
+    spread = np.sum(Q) / rho_mean / max(np.count_nonzero(S), 1)
+    assert np.abs(x0[..., 5] - ref[..., 5]).max() <= 1e-2 * np.abs(ref[..., 5]).max() + 1e-3 * spread
