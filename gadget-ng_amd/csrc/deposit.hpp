@@ -51,7 +51,25 @@ struct TileMap {
   int nt;  // tiles per axis (nm / T)
   const unsigned* rank;
   const unsigned* order;
+  // src[8r + d]: first bucket (rank * 8) of source tile d of the tile of rank
+  // r, the tile u - d (bit 0 x, bit 1 y, bit 2 z, periodic), so a deposit
+  // workgroup finds its 72 bucket offsets after one table load
+  const unsigned* src;
 };
+
+// Fill tm.src from rank[] for the tiles t = t0, t0 + step, ...
+__device__ __host__ inline void fill_tile_src(int nt, const unsigned* rank, unsigned* src, int t0, int step) {
+  const int nt3 = nt * nt * nt;
+  for (int t = t0; t < nt3; t += step) {
+    const int ux = t % nt, uy = (t / nt) % nt, uz = t / (nt * nt);
+    for (int d = 0; d < 8; ++d) {
+      const int sx = (d & 1) ? (ux == 0 ? nt - 1 : ux - 1) : ux;
+      const int sy = (d & 2) ? (uy == 0 ? nt - 1 : uy - 1) : uy;
+      const int sz = (d & 4) ? (uz == 0 ? nt - 1 : uz - 1) : uz;
+      src[(size_t)rank[t] * 8 + d] = rank[(sz * nt + sy) * nt + sx] * 8u;
+    }
+  }
+}
 
 // Zero a buffer with vector stores (a fill kernel rather than a memset, so the
 // sequence stays safe to capture in a graph).
@@ -362,19 +380,14 @@ struct DepositSplit {
   unsigned* done;                // slots finished-slice counters, zero between solves
 };
 
-// Particles a tile's 14 ranges hold, from the bucket offsets.
-__device__ __forceinline__ unsigned tile_items(const unsigned* __restrict__ offset, const TileMap& tm, int tile) {
-  const int nt = tm.nt;
-  const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
+// Particles the 14 ranges of the tile of rank r hold, from the bucket offsets.
+__device__ __forceinline__ unsigned tile_items(const unsigned* __restrict__ offset, const TileMap& tm, unsigned r) {
   unsigned w = 0;
 #pragma unroll
-  for (int r = 0; r < 14; ++r) {
-    const int d = (int)((kRangeD >> (3 * r)) & 7ull);
-    const int sx = (d & 1) ? (ux == 0 ? nt - 1 : ux - 1) : ux;
-    const int sy = (d & 2) ? (uy == 0 ? nt - 1 : uy - 1) : uy;
-    const int sz = (d & 4) ? (uz == 0 ? nt - 1 : uz - 1) : uz;
-    const unsigned b = tm.rank[(unsigned)(sz * nt + sy) * nt + sx] * 8u;
-    w += offset[b + kRangeHi[r]] - offset[b + kRangeLo[r]];
+  for (int k = 0; k < 14; ++k) {
+    const int d = (int)((kRangeD >> (3 * k)) & 7ull);
+    const unsigned b = tm.src[(size_t)r * 8 + d];
+    w += offset[b + kRangeHi[k]] - offset[b + kRangeLo[k]];
   }
   return w;
 }
@@ -391,7 +404,7 @@ __global__ __launch_bounds__(256) void k_deposit_plan(const unsigned* __restrict
   const unsigned ntiles = (unsigned)tm.nt * tm.nt * tm.nt;
   const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= ntiles) return;
-  const unsigned w = tile_items(offset, tm, (int)tm.order[r]);
+  const unsigned w = tile_items(offset, tm, r);
   unsigned S = w > kSliceItems ? (w + kSliceItems - 1) / kSliceItems : 1u;
   if (S > kMaxSlices) S = kMaxSlices;
   if (S > 1) {
@@ -452,14 +465,13 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
     n_slice = sp.slices[rank_r];
     if (n_slice > 1) my_slot = sp.slot[rank_r];
   }
+  // the tile itself is needed only for the output rows; its load and the
+  // source-bucket load below depend on the rank alone
   const int tile = (int)tm.order[rank_r];
   const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
   if (tid < 72) {
     const int d = tid / 9, j = tid % 9;
-    const int sx = (d & 1) ? (ux == 0 ? nt - 1 : ux - 1) : ux;
-    const int sy = (d & 2) ? (uy == 0 ? nt - 1 : uy - 1) : uy;
-    const int sz = (d & 4) ? (uz == 0 ? nt - 1 : uz - 1) : uz;
-    toff[d * 9 + j] = offset[tm.rank[(unsigned)(sz * nt + sy) * nt + sx] * 8u + j];
+    toff[d * 9 + j] = offset[tm.src[(size_t)rank_r * 8 + d] + j];
   }
   // the fixed-point mass total, reduced by the third wavefront while the
   // offsets load (one slot per lane; NT >= 256)

@@ -257,7 +257,7 @@ __host__ __device__ uint64_t hilbert_key3(uint32_t x, uint32_t y, uint32_t z, in
 // counting the smaller keys: for small tile counts only).  One workgroup.
 __global__ __launch_bounds__(256) void k_plan_tables(int nm, int nt, int bits, float2* __restrict__ tw,
                                                      float* __restrict__ dc, unsigned* __restrict__ rank,
-                                                     unsigned* __restrict__ order) {
+                                                     unsigned* __restrict__ order, unsigned* __restrict__ src) {
   for (int m = threadIdx.x; m < nm; m += blockDim.x) {
     double sn, cs;
     sincospi(2.0 * (double)m / (double)nm, &sn, &cs);
@@ -277,6 +277,8 @@ __global__ __launch_bounds__(256) void k_plan_tables(int nm, int nt, int bits, f
     rank[t] = r;
     order[r] = (unsigned)t;
   }
+  __syncthreads();
+  fill_tile_src(nt, rank, src, threadIdx.x, blockDim.x);
 }
 
 struct MeshOps {
@@ -509,6 +511,7 @@ struct gng_pm_plan {
   unsigned* d_partial = nullptr;
   unsigned* d_tile_rank = nullptr;   // tile -> Hilbert rank
   unsigned* d_tile_order = nullptr;  // Hilbert rank -> tile
+  unsigned* d_tile_src = nullptr;    // Hilbert rank -> its 8 source tiles' first buckets
 
   // split deposit of heavy tiles: per tile rank its slice count, their
   // exclusive scan, its scratch slot; per workgroup its tile rank; the
@@ -616,7 +619,7 @@ void free_plan(gng_pm_plan* p) {
   (void)hipSetDevice(p->device);
   for (void* q : {(void*)p->d_P, (void*)p->d_A, (void*)p->d_B, (void*)p->d_rho, (void*)p->d_pot, (void*)p->d_tw, (void*)p->d_deconv,
                   (void*)p->d_sc, (void*)p->d_count, (void*)p->d_offset, (void*)p->d_partial, (void*)p->d_part,
-                  (void*)p->d_idx, (void*)p->d_tile_rank, (void*)p->d_tile_order, (void*)p->d_pk_w,
+                  (void*)p->d_idx, (void*)p->d_tile_rank, (void*)p->d_tile_order, (void*)p->d_tile_src, (void*)p->d_pk_w,
                   (void*)p->d_pk_part, (void*)p->d_pk_modes, (void*)p->d_pk_msum, (void*)p->d_pk_out,
                   (void*)p->d_ic_sigma, (void*)p->d_ic_mask, (void*)p->d_part64})
     (void)hipFree(q);
@@ -1539,6 +1542,7 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
   GNG_ALLOC(p->d_partial, sizeof(unsigned) * (size_t)p->scan_blocks);
   GNG_ALLOC(p->d_tile_rank, sizeof(unsigned) * (size_t)(p->nb / 8));
   GNG_ALLOC(p->d_tile_order, sizeof(unsigned) * (size_t)(p->nb / 8));
+  GNG_ALLOC(p->d_tile_src, sizeof(unsigned) * (size_t)p->nb);
 #undef GNG_ALLOC
   {
     // Hilbert ranks of the tiles (keys on the enclosing power-of-two grid)
@@ -1552,22 +1556,26 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
         for (int tx = 0; tx < nt; ++tx)
           keys.emplace_back(hilbert_key3(tx, ty, tz, bits), (unsigned)((tz * nt + ty) * nt + tx));
     std::sort(keys.begin(), keys.end());
-    std::vector<unsigned> rank(keys.size()), order(keys.size());
+    std::vector<unsigned> rank(keys.size()), order(keys.size()), src(keys.size() * 8);
     for (size_t r = 0; r < keys.size(); ++r) {
       order[r] = keys[r].second;
       rank[keys[r].second] = (unsigned)r;
     }
+    fill_tile_src(nt, rank.data(), src.data(), 0, 1);
     hipError_t e2 = hipMemcpy(p->d_tile_rank, rank.data(), sizeof(unsigned) * rank.size(), hipMemcpyHostToDevice);
     if (e2 == hipSuccess)
       e2 = hipMemcpy(p->d_tile_order, order.data(), sizeof(unsigned) * order.size(), hipMemcpyHostToDevice);
+    if (e2 == hipSuccess)
+      e2 = hipMemcpy(p->d_tile_src, src.data(), sizeof(unsigned) * src.size(), hipMemcpyHostToDevice);
     if (e2 != hipSuccess) return bail(fail(GNG_ERR_HIP, std::string("tile order upload: ") + hipGetErrorString(e2)));
     p->tm.rank = p->d_tile_rank;
     p->tm.order = p->d_tile_order;
+    p->tm.src = p->d_tile_src;
   }
   hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_plan_tables, dim3(1), dim3(256), 0, p->stream, nm, 0, 0, p->d_tw, p->d_deconv, nullptr,
-                       nullptr);
+                       nullptr, nullptr);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
@@ -2678,6 +2686,7 @@ int gng_pm_solve_device_noalloc(const gng_pm_config* cfg, const float* d_x, cons
   p.d_partial = c.take<unsigned>((size_t)p.scan_blocks);
   p.d_tile_rank = c.take<unsigned>((size_t)(p.nb / 8));
   p.d_tile_order = c.take<unsigned>((size_t)(p.nb / 8));
+  p.d_tile_src = c.take<unsigned>((size_t)p.nb);
   p.d_sorted = c.take<PartRec>((size_t)kNoAllocMaxParticles);
   p.d_bucket = c.take<unsigned>((size_t)kNoAllocMaxParticles);
   p.d_rank = c.take<unsigned>((size_t)kNoAllocMaxParticles);
@@ -2685,13 +2694,14 @@ int gng_pm_solve_device_noalloc(const gng_pm_config* cfg, const float* d_x, cons
   p.work_cap = kNoAllocMaxParticles;
   p.tm.rank = p.d_tile_rank;
   p.tm.order = p.d_tile_order;
+  p.tm.src = p.d_tile_src;
   int bits = 0;
   while ((1 << bits) < p.tm.nt) ++bits;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   // the tables are rebuilt on every call: the scratch keeps whatever the
   // last call on this GPU (any mesh size) left in it
   hipLaunchKernelGGL(k_plan_tables, dim3(1), dim3(256), 0, s, nm, p.tm.nt, bits, p.d_tw, p.d_deconv, p.d_tile_rank,
-                     p.d_tile_order);
+                     p.d_tile_order, p.d_tile_src);
   GNG_HIP(hipGetLastError());
   return run_pipeline(&p, soa32(d_x, d_y, d_z, d_mass), n, reinterpret_cast<const long long*>(d_idx), k, g,
                       acc32(d_ax, d_ay, d_az), s);

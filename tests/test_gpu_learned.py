@@ -286,7 +286,9 @@ def test_adam_step_matches_formula(learned, pm):
 
 def test_training_fits_a_teacher_network(learned, pm):
     """Targets made by a fixed random network: Adam on the HIP kernels drives
-    the loss down by an order of magnitude."""
+    the loss down by an order of magnitude.  dL/dy reaches the mesh through
+    FP32 atomics, so the trajectory is not bit-reproducible; after 600 steps
+    repeated runs end at 0.043-0.045 of the first loss (0.08-0.16 after 300)."""
     nm, boxes, L, osc = 16, 2, 16.0, 1.0
     print("case teacher training", flush=True)
     g = np.random.default_rng(21)
@@ -308,7 +310,7 @@ def test_training_fits_a_teacher_network(learned, pm):
     net.forward(boxes)
     net.loss_grad(dp.ptr, dt.ptr, len(pos4), L, osc)
     first = net.loss()
-    for _ in range(300):
+    for _ in range(600):
         net.train_step(boxes, dp.ptr, dt.ptr, len(pos4), L, osc, 3e-3)
     net.forward(boxes)
     net.loss_grad(dp.ptr, dt.ptr, len(pos4), L, osc)
