@@ -55,6 +55,10 @@ struct TileMap {
   // r, the tile u - d (bit 0 x, bit 1 y, bit 2 z, periodic), so a deposit
   // workgroup finds its 72 bucket offsets after one table load
   const unsigned* src;
+  // per tile rank, kRangeWords words of the flattened layout of its 14
+  // bucket ranges, filled by k_tile_ranges for each solve (null: the deposit
+  // works it out from the offsets itself)
+  unsigned* ranges;
 };
 
 // Fill tm.src from rank[] for the tiles t = t0, t0 + step, ...
@@ -380,6 +384,47 @@ struct DepositSplit {
   unsigned* done;                // slots finished-slice counters, zero between solves
 };
 
+// The flattened range layout of every tile rank r, written after the scatter
+// so that a deposit workgroup reads its whole prologue with scalar loads:
+// ranges[kRangeWords r + k] is, for k < 13, the end of range k when the 14
+// ranges are laid end to end; word 13 is their total, word 14 the slot delta
+// of range 0 (sorted slot = flattened index + delta) and word 15 + k the
+// change of that delta from range k to k + 1.  Thread 0 also sums the
+// scatter's fixed-point mass slots into sc->qtot.
+constexpr int kRangeWords = 32;
+__global__ __launch_bounds__(256) void k_tile_ranges(const unsigned* __restrict__ offset, const TileMap tm,
+                                                     SolveScalars* sc) {
+  const unsigned ntiles = (unsigned)tm.nt * tm.nt * tm.nt;
+  const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r == 0) {
+    unsigned long long q = 0;
+    for (int k = 0; k < kQSlots; ++k) q += sc->qsum[k * kQStride];
+    sc->qtot = q;
+  }
+  if (r >= ntiles) return;
+  unsigned w[kRangeWords];
+  unsigned pre = 0, prev = 0;
+#pragma unroll
+  for (int k = 0; k < 14; ++k) {
+    const int d = (int)((kRangeD >> (3 * k)) & 7ull);
+    const unsigned b = tm.src[(size_t)r * 8 + d];
+    const unsigned beg = offset[b + kRangeLo[k]];
+    const unsigned len = offset[b + kRangeHi[k]] - beg;
+    const unsigned delta = beg - pre;
+    pre += len;
+    if (k < 13) w[k] = pre;
+    if (k == 0) w[14] = delta;
+    else w[14 + k] = delta - prev;
+    prev = delta;
+  }
+  w[13] = pre;
+#pragma unroll
+  for (int k = 28; k < kRangeWords; ++k) w[k] = 0u;
+  uint4* dst = reinterpret_cast<uint4*>(tm.ranges + (size_t)r * kRangeWords);
+#pragma unroll
+  for (int k = 0; k < kRangeWords / 4; ++k) dst[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+}
+
 // Particles the 14 ranges of the tile of rank r hold, from the bucket offsets.
 __device__ __forceinline__ unsigned tile_items(const unsigned* __restrict__ offset, const TileMap& tm, unsigned r) {
   unsigned w = 0;
@@ -429,7 +474,10 @@ __global__ __launch_bounds__(256) void k_deposit_plan(const unsigned* __restrict
 // each keep twice as many particle loads and atomics in flight as four.
 constexpr int kDepositThreads = 512;
 
-template <int T, int NT = kDepositThreads, bool SPLIT = false>
+//
+// TABLE: the range layout and the mass total come from k_tile_ranges (scalar
+// loads, no LDS prologue); otherwise the workgroup computes them.
+template <int T, int NT = kDepositThreads, bool SPLIT = false, bool TABLE = false>
 __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ sorted,
                                                        const unsigned* __restrict__ offset, const CellMap cm,
                                                        const TileMap tm, const SolveScalars* sc, long long ntot,
@@ -469,6 +517,21 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
   // source-bucket load below depend on the rank alone
   const int tile = (int)tm.order[rank_r];
   const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
+  // range ends and slot deltas, uniform over the workgroup, in scalar
+  // registers
+  unsigned rend[13], dstep[13], dbase, all;
+  unsigned long long qtot;
+  if constexpr (TABLE) {
+    const unsigned* __restrict__ w = tm.ranges + (size_t)rank_r * kRangeWords;
+#pragma unroll
+    for (int k = 0; k < 13; ++k) {
+      rend[k] = w[k];
+      dstep[k] = w[15 + k];
+    }
+    all = w[13];
+    dbase = w[14];
+    qtot = sc->qtot;
+  } else {
   if (tid < 72) {
     const int d = tid / 9, j = tid % 9;
     toff[d * 9 + j] = offset[tm.src[(size_t)rank_r * 8 + d] + j];
@@ -502,17 +565,16 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
     if (tid == 0) rpre[0] = 0;
   }
   __syncthreads();
-  // range ends and slot deltas, uniform over the workgroup, in scalar
-  // registers; then the scratch is free for the accumulators
-  unsigned rend[13], dstep[13];
+  // then the scratch is free for the accumulators
 #pragma unroll
   for (int k = 0; k < 13; ++k) rend[k] = __builtin_amdgcn_readfirstlane(rpre[k + 1]);
-  const unsigned dbase = __builtin_amdgcn_readfirstlane(rdelta[0]);
+  dbase = __builtin_amdgcn_readfirstlane(rdelta[0]);
 #pragma unroll
   for (int k = 0; k < 13; ++k) dstep[k] = __builtin_amdgcn_readfirstlane(rdelta[k + 1] - rdelta[k]);
-  const unsigned all = __builtin_amdgcn_readfirstlane(rpre[14]);
-  const unsigned long long qtot = *qtotp;
+  all = __builtin_amdgcn_readfirstlane(rpre[14]);
+  qtot = *qtotp;
   __syncthreads();
+  }
   const int s = fixed_shift(sc, ntot);
   // this slice's share [q0, total) of the tile's flattened ranges
   unsigned q0 = 0, total = all;
@@ -730,7 +792,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
                                                        const TileMap tm, const SolveScalars* sc, long long ntot,
                                                        float* __restrict__ rho, long long rho_plane,
                                                        float* __restrict__ keep, const DepositSplit sp) {
-  deposit_tiles_body<T, NT, SPLIT>(sorted, offset, cm, tm, sc, ntot, rho, rho_plane, keep, sp);
+  deposit_tiles_body<T, NT, SPLIT, true>(sorted, offset, cm, tm, sc, ntot, rho, rho_plane, keep, sp);
 }
 
 template <int T, int NT = kDepositThreads>

@@ -512,6 +512,7 @@ struct gng_pm_plan {
   unsigned* d_tile_rank = nullptr;   // tile -> Hilbert rank
   unsigned* d_tile_order = nullptr;  // Hilbert rank -> tile
   unsigned* d_tile_src = nullptr;    // Hilbert rank -> its 8 source tiles' first buckets
+  unsigned* d_tile_ranges = nullptr; // Hilbert rank -> its flattened bucket ranges (per solve)
 
   // split deposit of heavy tiles: per tile rank its slice count, their
   // exclusive scan, its scratch slot; per workgroup its tile rank; the
@@ -619,7 +620,8 @@ void free_plan(gng_pm_plan* p) {
   (void)hipSetDevice(p->device);
   for (void* q : {(void*)p->d_P, (void*)p->d_A, (void*)p->d_B, (void*)p->d_rho, (void*)p->d_pot, (void*)p->d_tw, (void*)p->d_deconv,
                   (void*)p->d_sc, (void*)p->d_count, (void*)p->d_offset, (void*)p->d_partial, (void*)p->d_part,
-                  (void*)p->d_idx, (void*)p->d_tile_rank, (void*)p->d_tile_order, (void*)p->d_tile_src, (void*)p->d_pk_w,
+                  (void*)p->d_idx, (void*)p->d_tile_rank, (void*)p->d_tile_order, (void*)p->d_tile_src, (void*)p->d_tile_ranges,
+                  (void*)p->d_pk_w,
                   (void*)p->d_pk_part, (void*)p->d_pk_modes, (void*)p->d_pk_msum, (void*)p->d_pk_out,
                   (void*)p->d_ic_sigma, (void*)p->d_ic_mask, (void*)p->d_part64})
     (void)hipFree(q);
@@ -840,6 +842,7 @@ int run_front(gng_pm_plan* p, const PartIn& in, long long n, hipStream_t s) {
   run_sort(p, in, n, p->tune.interp_sorted ? p->d_perm : nullptr, true, s);
   mark(p, 2, s);
   const unsigned ntiles = (unsigned)p->tm.nt * p->tm.nt * p->tm.nt;
+  hipLaunchKernelGGL(k_tile_ranges, dim3((ntiles + 255) / 256), dim3(256), 0, s, p->d_offset, p->tm, p->d_sc);
   DepositSplit sp{};
   unsigned deposit_groups = ntiles;
   if (p->split_slots > 0 && p->tune.deposit_split) {
@@ -1543,6 +1546,7 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
   GNG_ALLOC(p->d_tile_rank, sizeof(unsigned) * (size_t)(p->nb / 8));
   GNG_ALLOC(p->d_tile_order, sizeof(unsigned) * (size_t)(p->nb / 8));
   GNG_ALLOC(p->d_tile_src, sizeof(unsigned) * (size_t)p->nb);
+  GNG_ALLOC(p->d_tile_ranges, sizeof(unsigned) * (size_t)(p->nb / 8) * kRangeWords);
 #undef GNG_ALLOC
   {
     // Hilbert ranks of the tiles (keys on the enclosing power-of-two grid)
@@ -1571,6 +1575,7 @@ int gng_pm_create(const gng_pm_config* cfg, gng_pm_plan** out) {
     p->tm.rank = p->d_tile_rank;
     p->tm.order = p->d_tile_order;
     p->tm.src = p->d_tile_src;
+    p->tm.ranges = p->d_tile_ranges;
   }
   hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
   if (e == hipSuccess) {
@@ -2687,6 +2692,7 @@ int gng_pm_solve_device_noalloc(const gng_pm_config* cfg, const float* d_x, cons
   p.d_tile_rank = c.take<unsigned>((size_t)(p.nb / 8));
   p.d_tile_order = c.take<unsigned>((size_t)(p.nb / 8));
   p.d_tile_src = c.take<unsigned>((size_t)p.nb);
+  p.d_tile_ranges = c.take<unsigned>((size_t)(p.nb / 8) * kRangeWords);
   p.d_sorted = c.take<PartRec>((size_t)kNoAllocMaxParticles);
   p.d_bucket = c.take<unsigned>((size_t)kNoAllocMaxParticles);
   p.d_rank = c.take<unsigned>((size_t)kNoAllocMaxParticles);
@@ -2695,6 +2701,7 @@ int gng_pm_solve_device_noalloc(const gng_pm_config* cfg, const float* d_x, cons
   p.tm.rank = p.d_tile_rank;
   p.tm.order = p.d_tile_order;
   p.tm.src = p.d_tile_src;
+  p.tm.ranges = p.d_tile_ranges;
   int bits = 0;
   while ((1 << bits) < p.tm.nt) ++bits;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);

@@ -162,7 +162,8 @@ constexpr int kQStride = 16;  // 64-bit words between slots
 struct SolveScalars {
   unsigned int mmax_bits;  // max |m| as float bits (non-negative floats order as integers)
   unsigned int split_budget;  // scratch tiles reserved by split deposit tiles
-  unsigned int pad[62];
+  unsigned long long qtot;    // sum of the qsum slots, once the scatter is done (k_tile_ranges)
+  unsigned int pad[60];
   unsigned long long qsum[kQSlots * kQStride];
 };
 
