@@ -294,7 +294,7 @@ __device__ __forceinline__ void scatter_body(const PosSrc<T> src, const M* __res
       cell_fix(pz[k], cm, c[2], f[2]);
       sorted[slot[k]] = encode_rec(c, f, mf[k]);
       if (perm) perm[slot[k]] = (unsigned)i;  // only for the sorted-order interpolation
-      q += __double2ll_rn(ldexp((double)mf[k], shift));
+      q += rint_small(ldexp((double)mf[k], shift));
     }
   }
   for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off);
@@ -640,7 +640,7 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
         for (int c = 0; c < 2; ++c) {
           const int lx = dx ? 0 : lx0 + c;
           const bool valid = live && (a & cz) == dz && (b & cy) == dy && (c & cx) == dx;
-          const long long qv = valid ? __double2ll_rn(wx[c] * (wy[b] * wz[a])) : 0ll;
+          const long long qv = valid ? rint_small(wx[c] * (wy[b] * wz[a])) : 0ll;
           if (crowded)
             lds_add_aggregated(acc, valid ? (lz * T + ly) * T + lx : 0, qv, valid);
           else if (valid)
@@ -769,8 +769,8 @@ __global__ __launch_bounds__(kCellMomThreads) void k_cell_moments(const PartRec*
       const int c = ((lz - half * HL) * T + ly) * T + lx;
       const double fx = rec_frac(p, 0) - 0.5, fy = rec_frac(p, 1) - 0.5, fz = rec_frac(p, 2) - 0.5;
       const double ms = (double)rec_mass(p) * scale;
-      const long long v[kCellMoments] = {__double2ll_rn(ms * fx), __double2ll_rn(ms * fy), __double2ll_rn(ms * fz),
-                                          __double2ll_rn(ms * (fx * fx + fy * fy + fz * fz)), __double2ll_rn(ms)};
+      const long long v[kCellMoments] = {rint_small(ms * fx), rint_small(ms * fy), rint_small(ms * fz),
+                                          rint_small(ms * (fx * fx + fy * fy + fz * fz)), rint_small(ms)};
 #pragma unroll
       for (int k = 0; k < kCellMoments; ++k) lds_add_aggregated(acc[k], live ? c : 0, live ? v[k] : 0ll, live);
     }

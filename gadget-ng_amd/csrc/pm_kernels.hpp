@@ -72,6 +72,14 @@ struct AccDst {
 constexpr int kFracBits = 43;
 constexpr unsigned long long kFracOne = 1ull << kFracBits;
 
+// x rounded to the nearest integer, ties to even (the value __double2ll_rn
+// gives), for |x| < 2^51: adding 1.5 * 2^52 leaves that integer in the low
+// mantissa bits.  One FP64 add and one integer subtraction instead of the
+// emulated FP64 -> int64 conversion.
+__device__ __forceinline__ long long rint_small(double x) {
+  return __double_as_longlong(x + 6755399441055744.0) - 0x4338000000000000LL;
+}
+
 // Cell index and in-cell fraction of one coordinate, in FP64:
 // c = (x * inv_cell) rem_euclid nm, i = floor(c) (cic.rs:50-56), and the
 // fraction c - i (exact in FP64) rounded to kFracBits bits.  A fraction that
@@ -92,7 +100,7 @@ __device__ __forceinline__ void cell_fix(double x, const CellMap cm, int& i, uns
   }
   const double f = floor(c);
   int ii = (int)f;
-  unsigned long long q = (unsigned long long)__double2ll_rn((c - f) * (double)kFracOne);
+  unsigned long long q = (unsigned long long)rint_small((c - f) * (double)kFracOne);
   if (q >= kFracOne) {
     q = 0ull;
     ++ii;
@@ -168,12 +176,15 @@ struct SolveScalars {
 };
 
 // Fixed-point exponent: contributions are w*m*2^s with N*max|m|*2^s < 2^62,
-// so no cell sum can overflow int64 whatever the clustering.
+// so no cell sum can overflow int64 whatever the clustering, and with
+// max|m|*2^s < 2^51 (binding only below about 2^11 particles), so every
+// single contribution (w <= 1) converts with rint_small.
 __device__ __forceinline__ int fixed_shift(const SolveScalars* sc, long long ntot) {
   const float mmax = __uint_as_float(sc->mmax_bits);
-  int e = 0;
+  int e = 0, em = 0;
   (void)frexp((double)ntot * (double)mmax, &e);
-  return 62 - e;
+  (void)frexp((double)mmax, &em);
+  return min(62 - e, 51 - em);
 }
 
 struct GreenParams {
