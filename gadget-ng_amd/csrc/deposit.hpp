@@ -586,22 +586,27 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
   // every lane of a wavefront runs the same number of iterations (lanes past
   // the end carry no particle), so the aggregation ballots see whole waves
   const unsigned rounds = (total - q0 + NT - 1u) / NT;
-  // range of q: the number of range ends at or below it (13 past the end)
-  auto range_of = [&](unsigned q) {
-    int r = 0;
+  // Range r of flattened item q (the number of range ends at or below it, 13
+  // past the end) and its record: the sorted slot is q + the delta of its
+  // range, summed as masked steps over the range starts at or below q (range
+  // ends never decrease; written as arithmetic so that it stays in
+  // registers).  Range 0, the tile's own bucket, holds most of its particles:
+  // a wavefront whose 64 items all lie in it (the items of a wavefront are
+  // consecutive, lane 0's first) skips the per-lane search.
+  auto fetch = [&](unsigned q, int& r) {
+    const unsigned qw = __builtin_amdgcn_readfirstlane(q);
+    unsigned dl = dbase;
+    r = 0;
+    if (qw + 63u >= rend[0]) {
 #pragma unroll
-    for (int k = 0; k < 13; ++k) r += q >= rend[k] ? 1 : 0;
-    return r;
-  };
-  // slot of flattened item q: q + the delta of its range, summed as masked
-  // steps over the range starts at or below q (range ends never decrease;
-  // written as arithmetic so that it stays in registers)
-  auto fetch = [&](unsigned q, int) {
+      for (int k = 0; k < 13; ++k) {
+        const bool ge = q >= rend[k];
+        r += ge ? 1 : 0;
+        dl += dstep[k] & (0u - (unsigned)ge);
+      }
+    }
     PartRec v;
     if (q < total) {
-      unsigned dl = dbase;
-#pragma unroll
-      for (int k = 0; k < 13; ++k) dl += dstep[k] & (0u - (unsigned)(q >= rend[k]));
       v = sorted[q + dl];
     } else {
       v.a[0] = v.a[1] = v.a[2] = 0ull;
@@ -660,7 +665,6 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
 #pragma unroll
   for (int j = 0; j < kPrefetch; ++j) {
     const unsigned qj = qbase + (unsigned)j * NT;
-    rbuf[j] = range_of(qj);
     buf[j] = fetch(qj, rbuf[j]);
   }
   // the accumulators are cleared while the first records load
@@ -675,7 +679,6 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
         const int r = rbuf[j];
         if (it + j + kPrefetch < rounds) {
           const unsigned qn = qc + (unsigned)kPrefetch * NT;
-          rbuf[j] = range_of(qn);
           buf[j] = fetch(qn, rbuf[j]);
         }
         deposit_one(p, r, qc < total);
