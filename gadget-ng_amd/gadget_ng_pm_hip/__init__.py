@@ -371,7 +371,12 @@ class Plan:
             raise HipExecutionError(what, rc, last_error())
 
     def accelerations_f64(self, pos: np.ndarray, mass: np.ndarray, g: float, idx=None,
-                          eps2: float = 0.0) -> np.ndarray:
+                          eps2: float = 0.0, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """Accelerations (k x 3 float64) of pos[idx] (all n without idx).  out,
+        when given, is a C-contiguous float64 (k, 3) array the result is
+        written into and returned, as the reference's solvers fill the
+        caller's `out: &mut [Vec3]` (a caller that keeps it across steps
+        saves the page faults of a fresh array on every call)."""
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         mass = np.ascontiguousarray(mass, dtype=np.float64)
         n = pos.shape[0]
@@ -382,12 +387,19 @@ class Plan:
         else:
             idx = np.ascontiguousarray(idx, dtype=np.int64)
             k, ip = idx.shape[0], _ptr(idx)
-        out = np.empty((k, 3), dtype=np.float64)
+        if out is None:
+            out = np.empty((k, 3), dtype=np.float64)
+        elif (not isinstance(out, np.ndarray) or out.dtype != np.float64 or out.shape != (k, 3)
+              or not out.flags.c_contiguous or not out.flags.writeable):
+            raise ValueError(f"out must be a writeable C-contiguous float64 array of shape ({k}, 3)")
         self._check(lib().gng_pm_accelerations_f64(self._h, _ptr(pos), _ptr(mass), n, ip, k, float(eps2), float(g),
                                                    _ptr(out)), "gng_pm_accelerations_f64")
         return out
 
-    def accelerations_f32(self, x, y, z, m, g: float, idx=None):
+    def accelerations_f32(self, x, y, z, m, g: float, idx=None, out: Optional[np.ndarray] = None):
+        """Accelerations (3 x k float32, one row per axis) of particles idx
+        (all n without idx); out, when given, is a writeable C-contiguous
+        float32 (3, k) array the result is written into and returned."""
         arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, z, m)]
         n = arrs[0].shape[0]
         if idx is None:
@@ -395,7 +407,11 @@ class Plan:
         else:
             idx = np.ascontiguousarray(idx, dtype=np.int64)
             k, ip = idx.shape[0], _ptr(idx)
-        out = np.empty((3, k), dtype=np.float32)
+        if out is None:
+            out = np.empty((3, k), dtype=np.float32)
+        elif (not isinstance(out, np.ndarray) or out.dtype != np.float32 or out.shape != (3, k)
+              or not out.flags.c_contiguous or not out.flags.writeable):
+            raise ValueError(f"out must be a writeable C-contiguous float32 array of shape (3, {k})")
         self._check(lib().gng_pm_accelerations_f32(self._h, *(_ptr(a) for a in arrs), n, ip, k, float(g),
                                                    _ptr(out[0]), _ptr(out[1]), _ptr(out[2])),
                     "gng_pm_accelerations_f32")
@@ -752,6 +768,12 @@ def _as_positions(global_positions) -> np.ndarray:
     return np.ascontiguousarray(global_positions, dtype=np.float64).reshape(-1, 3)
 
 
+def _fills_directly(out, k: int) -> bool:
+    """out can receive the accelerations in place (no temporary and copy)."""
+    return (isinstance(out, np.ndarray) and out.dtype == np.float64 and out.shape == (k, 3)
+            and out.flags.c_contiguous and out.flags.writeable)
+
+
 @dataclass
 class PmSolver:
     """Drop-in for gadget_ng_pm::PmSolver on one GPU (solver.rs:23-58).
@@ -783,7 +805,10 @@ class PmSolver:
         idx = np.asarray(global_indices, dtype=np.int64)
         if out is not None and out.shape[0] != idx.shape[0]:
             raise AssertionError("out and global_indices differ in length")  # solver.rs:83
-        res = self._plan().accelerations_f64(_as_positions(global_positions), global_masses, g, idx, eps2)
+        pos = _as_positions(global_positions)
+        if _fills_directly(out, idx.shape[0]):
+            return self._plan().accelerations_f64(pos, global_masses, g, idx, eps2, out=out)
+        res = self._plan().accelerations_f64(pos, global_masses, g, idx, eps2)
         if out is not None:
             out[...] = res
             return out
@@ -880,7 +905,11 @@ class HipPmSolver:
         assert idx.shape[0] == out.shape[0]
         if idx.shape[0] == 0:
             return
-        out[...] = self._plan.accelerations_f64(_as_positions(global_positions), global_masses, g, idx, eps2)
+        pos = _as_positions(global_positions)
+        if _fills_directly(out, idx.shape[0]):
+            self._plan.accelerations_f64(pos, global_masses, g, idx, eps2, out=out)
+        else:
+            out[...] = self._plan.accelerations_f64(pos, global_masses, g, idx, eps2)
 
     def accelerations_for_indices(self, global_positions, global_masses, eps2: float, g: float, global_indices,
                                   out: np.ndarray) -> None:

@@ -146,6 +146,41 @@ def test_f32_f64_and_device_entry_points_agree(pm):
     assert np.array_equal(got, a64[sel])
 
 
+def test_host_entries_fill_a_caller_output_array(pm):
+    """out= writes into the caller's array (the reference fills `out: &mut
+    [Vec3]`): same values as a fresh result, the array itself is returned,
+    a wrong shape, dtype or layout is refused, and the drop-in solvers fill
+    a suitable out in place."""
+    rng = np.random.default_rng(14)
+    n, nm, box = 20000, 32, 25.0
+    pos, m = uniform_particles(rng, n, box)
+    plan = pm.Plan(nm, box)
+    ref = plan.accelerations_f64(pos, m, 1.0)
+    out = np.full((n, 3), np.nan)
+    assert plan.accelerations_f64(pos, m, 1.0, out=out) is out
+    assert np.array_equal(out, ref)
+    sel = rng.choice(n, 777, replace=False).astype(np.int64)
+    outk = np.empty((777, 3))
+    plan.accelerations_f64(pos, m, 1.0, idx=sel, out=outk)
+    assert np.array_equal(outk, ref[sel])
+    for bad in (np.empty((n, 3), np.float32), np.empty((n, 4)), np.empty((3, n)).T, np.empty((n - 1, 3))):
+        with pytest.raises(ValueError):
+            plan.accelerations_f64(pos, m, 1.0, out=bad)
+    o32 = np.empty((3, n), np.float32)
+    r32 = plan.accelerations_f32(pos[:, 0], pos[:, 1], pos[:, 2], m, 1.0)
+    assert plan.accelerations_f32(pos[:, 0], pos[:, 1], pos[:, 2], m, 1.0, out=o32) is o32
+    assert np.array_equal(o32, r32)
+    with pytest.raises(ValueError):
+        plan.accelerations_f32(pos[:, 0], pos[:, 1], pos[:, 2], m, 1.0, out=np.empty((n, 3), np.float32))
+    solver = pm.PmSolver(nm, box)
+    o = np.empty((777, 3))
+    assert solver.accelerations_for_indices(pos, m, 0.0, 1.0, sel, out=o) is o
+    assert np.array_equal(o, ref[sel])
+    ol = np.empty((777, 3), order="F")  # not C-contiguous: filled through a copy
+    solver.accelerations_for_indices(pos, m, 0.0, 1.0, sel, out=ol)
+    assert np.array_equal(ol, ref[sel])
+
+
 def test_hip_pm_solver_mirror(pm, oracle_mod):
     assert pm.HipPmSolver.is_available()
     s = pm.HipPmSolver.try_new_with_r_split(32, 16.0, 1.2)
