@@ -425,6 +425,53 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const unsigned* __restrict_
   for (int k = 0; k < kRangeWords / 4; ++k) dst[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
 }
 
+// The 8 CIC targets of one cell record of source range r (d = the range's
+// neighbour offset) into the LDS tile acc of T^3 int64 accumulators: FP64
+// weights (the 43-bit fractions convert exactly, 1 - f is exact, and the
+// products round far below the fixed-point unit) times m 2^s (scale), as
+// 64-bit fixed point.  Every lane of the wavefront calls it (live marks the
+// lanes with a record), so the ballots see whole waves.
+template <int T>
+__device__ __forceinline__ void deposit_record(unsigned long long* acc, const PartRec& p, int r, bool live,
+                                               double scale) {
+  const int lane = threadIdx.x & 63;
+  const int d = (int)((kRangeD >> (3 * r)) & 7ull);
+  const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
+  const double fx = rec_frac(p, 0), fy = rec_frac(p, 1), fz = rec_frac(p, 2);
+  const double ms = (double)rec_mass(p) * scale;
+  const double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy};
+  const double wz[2] = {(1.0 - fz) * ms, fz * ms};
+  // local base index in the particle's own tile; its class follows from it
+  const int lx0 = rec_cell(p, 0) % T, ly0 = rec_cell(p, 1) % T, lz0 = rec_cell(p, 2) % T;
+  const int cx = lx0 == T - 1, cy = ly0 == T - 1, cz = lz0 == T - 1;
+  // Crowded wavefront: some lane shares its base cell (and source range)
+  // with the lane before it, as in dense clumps, where the sorted order puts
+  // a cell's particles next to each other.  Only then do the targets go
+  // through the wave-level aggregation; otherwise each lane adds directly.
+  // Integer sums are exact, so both paths give the same totals.
+  const int key = (((lz0 * T + ly0) * T + lx0) << 3) | d;
+  const int prev_key = __shfl_up(key, 1);
+  const bool crowded = __ballot(live && lane != 0 && prev_key == key) != 0ull;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int lz = dz ? 0 : lz0 + a;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int ly = dy ? 0 : ly0 + b;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int lx = dx ? 0 : lx0 + c;
+        const bool valid = live && (a & cz) == dz && (b & cy) == dy && (c & cx) == dx;
+        const long long qv = valid ? rint_small(wx[c] * (wy[b] * wz[a])) : 0ll;
+        if (crowded)
+          lds_add_aggregated(acc, valid ? (lz * T + ly) * T + lx : 0, qv, valid);
+        else if (valid)
+          atomicAdd(acc + (lz * T + ly) * T + lx, (unsigned long long)qv);
+      }
+    }
+  }
+}
+
 // Particles the 14 ranges of the tile of rank r hold, from the bucket offsets.
 __device__ __forceinline__ unsigned tile_items(const unsigned* __restrict__ offset, const TileMap& tm, unsigned r) {
   unsigned w = 0;
@@ -615,45 +662,7 @@ __device__ __forceinline__ void deposit_tiles_body(const PartRec* __restrict__ s
   };
   // m 2^s of a record: the FP32 mass scaled exactly
   const double scale = ldexp(1.0, s);
-  auto deposit_one = [&](const PartRec& p, int r, bool live) {
-    const int d = (int)((kRangeD >> (3 * r)) & 7ull);
-    const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-    // FP64 weights: the 43-bit fractions convert exactly, 1 - f is exact, and
-    // the products round far below the fixed-point unit
-    const double fx = rec_frac(p, 0), fy = rec_frac(p, 1), fz = rec_frac(p, 2);
-    const double ms = (double)rec_mass(p) * scale;
-    const double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy};
-    const double wz[2] = {(1.0 - fz) * ms, fz * ms};
-    // local base index in the particle's own tile; its class follows from it
-    const int lx0 = rec_cell(p, 0) % T, ly0 = rec_cell(p, 1) % T, lz0 = rec_cell(p, 2) % T;
-    const int cx = lx0 == T - 1, cy = ly0 == T - 1, cz = lz0 == T - 1;
-    // Crowded wavefront: some lane shares its base cell (and source range)
-    // with the lane before it, as in dense clumps, where the sorted order puts
-    // a cell's particles next to each other.  Only then do the targets go
-    // through the wave-level aggregation; otherwise each lane adds directly.
-    // Integer sums are exact, so both paths give the same totals.
-    const int key = (((lz0 * T + ly0) * T + lx0) << 3) | d;
-    const int prev_key = __shfl_up(key, 1);
-    const bool crowded = __ballot(live && (tid & 63) != 0 && prev_key == key) != 0ull;
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int lz = dz ? 0 : lz0 + a;
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int ly = dy ? 0 : ly0 + b;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const int lx = dx ? 0 : lx0 + c;
-          const bool valid = live && (a & cz) == dz && (b & cy) == dy && (c & cx) == dx;
-          const long long qv = valid ? rint_small(wx[c] * (wy[b] * wz[a])) : 0ll;
-          if (crowded)
-            lds_add_aggregated(acc, valid ? (lz * T + ly) * T + lx : 0, qv, valid);
-          else if (valid)
-            atomicAdd(acc + (lz * T + ly) * T + lx, (unsigned long long)qv);
-        }
-      }
-    }
-  };
+  auto deposit_one = [&](const PartRec& p, int r, bool live) { deposit_record<T>(acc, p, r, live, scale); };
   // kPrefetch rounds of records are in flight at once: a tile of the uniform
   // headline holds about 2.5 rounds at 256 threads, so all of its loads
   // share one memory latency.  Slot j holds the record of round it + j; the
@@ -796,6 +805,127 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
                                                        float* __restrict__ rho, long long rho_plane,
                                                        float* __restrict__ keep, const DepositSplit sp) {
   deposit_tiles_body<T, NT, SPLIT, true>(sorted, offset, cm, tm, sc, ntot, rho, rho_plane, keep, sp);
+}
+
+// Two tiles per workgroup: ranks 2w and 2w + 1, neighbours along the
+// Hilbert curve, from the range table.  Their rounds of records form one
+// sequence, so the prefetch ring runs across the tile boundary: the second
+// tile's records load while the first tile is converted and written.  The
+// accumulator tile, the per-record work and the result are those of
+// k_deposit_tiles (integer sums: bit for bit the same density).
+template <int T, int NT>
+__device__ __forceinline__ void deposit_pair_body(const PartRec* __restrict__ sorted, const CellMap cm,
+                                                  const TileMap tm, const SolveScalars* sc, long long ntot,
+                                                  float* __restrict__ rho, long long rho_plane,
+                                                  float* __restrict__ keep) {
+  constexpr int T3 = T * T * T;
+  __shared__ unsigned long long acc[T3];
+  const int tid = threadIdx.x;
+  const int nt = tm.nt;
+  const unsigned ntiles = (unsigned)(nt * nt * nt);
+  const unsigned ra = 2u * blockIdx.x, rb = ra + 1u;
+  const bool has_b = rb < ntiles;  // uniform: the last workgroup may hold one tile
+  const unsigned* __restrict__ wa = tm.ranges + (size_t)ra * kRangeWords;
+  const unsigned* __restrict__ wb = tm.ranges + (size_t)(has_b ? rb : ra) * kRangeWords;
+  const unsigned tot_a = wa[13], dbase_a = wa[14];
+  const unsigned tot_b = has_b ? wb[13] : 0u, dbase_b = wb[14];
+  const unsigned rounds_a = (tot_a + NT - 1u) / NT, rounds_b = (tot_b + NT - 1u) / NT;
+  const unsigned rounds = rounds_a + rounds_b;
+  const int tile_a = (int)tm.order[ra], tile_b = has_b ? (int)tm.order[rb] : 0;
+  const int s = fixed_shift(sc, ntot);
+  const double scale = ldexp(1.0, s), unit = ldexp(1.0, -s);
+  const long long qmean = (long long)(sc->qtot / (unsigned long long)((long long)cm.nm * cm.nm * cm.nm));
+  // record of concatenated round g (tile a below rounds_a, then tile b): its
+  // range r and whether the lane has one; a wavefront whose 64 items all lie
+  // in range 0 skips the per-lane range search, as in k_deposit_tiles
+  // (the range words are read through the uniform table row of the tile, so
+  // they come from the scalar cache rather than living in registers)
+  auto fetch = [&](unsigned g, int& r, bool& live) {
+    const bool in_b = g >= rounds_a;  // uniform
+    const unsigned* __restrict__ w = in_b ? wb : wa;
+    const unsigned q = (in_b ? g - rounds_a : g) * NT + (unsigned)tid;
+    const unsigned qw = __builtin_amdgcn_readfirstlane(q);
+    unsigned dl = in_b ? dbase_b : dbase_a;
+    r = 0;
+    if (qw + 63u >= w[0]) {
+#pragma unroll
+      for (int k = 0; k < 13; ++k) {
+        const bool ge = q >= w[k];
+        r += ge ? 1 : 0;
+        dl += w[15 + k] & (0u - (unsigned)ge);
+      }
+    }
+    live = q < (in_b ? tot_b : tot_a);
+    PartRec v;
+    if (live) {
+      v = sorted[q + dl];
+    } else {
+      v.a[0] = v.a[1] = v.a[2] = 0ull;
+    }
+    return v;
+  };
+  // the accumulated tile as FP32 mass per cell minus the mean (and, with
+  // keep, unshifted), then the accumulators cleared for the next tile
+  auto write_tile = [&](int tile) {
+    const int ux = tile % nt, uy = (tile / nt) % nt, uz = tile / (nt * nt);
+    for (int c = tid; c < T3; c += NT) {
+      const int lx = c % T, ly = (c / T) % T, lz = c / (T * T);
+      const size_t row = (size_t)(uy * T + ly) * cm.nm + (ux * T + lx);
+      const long long v = (long long)acc[c];
+      rho[(size_t)(uz * T + lz) * rho_plane + row] = (float)((double)(v - qmean) * unit);
+      if (keep) keep[(size_t)(uz * T + lz) * cm.nm * cm.nm + row] = (float)((double)v * unit);
+    }
+  };
+  constexpr int kPrefetch = 3;
+  PartRec buf[kPrefetch];
+  int rbuf[kPrefetch];
+  bool lbuf[kPrefetch];
+#pragma unroll
+  for (int j = 0; j < kPrefetch; ++j) {
+    rbuf[j] = 0;
+    lbuf[j] = false;
+    if ((unsigned)j < rounds) buf[j] = fetch((unsigned)j, rbuf[j], lbuf[j]);
+  }
+  for (int c = tid; c < T3; c += NT) acc[c] = 0ull;
+  __syncthreads();
+  for (unsigned it = 0; it < rounds; it += kPrefetch) {
+#pragma unroll
+    for (int j = 0; j < kPrefetch; ++j) {
+      const unsigned g = it + (unsigned)j;
+      if (g < rounds) {  // uniform over the workgroup
+        if (g == rounds_a) {
+          // tile a is complete: write it and start tile b from zero
+          __syncthreads();
+          write_tile(tile_a);
+          __syncthreads();
+          for (int c = tid; c < T3; c += NT) acc[c] = 0ull;
+          __syncthreads();
+        }
+        const PartRec p = buf[j];
+        const int r = rbuf[j];
+        const bool live = lbuf[j];
+        if (g + kPrefetch < rounds) buf[j] = fetch(g + kPrefetch, rbuf[j], lbuf[j]);
+        deposit_record<T>(acc, p, r, live, scale);
+      }
+    }
+  }
+  __syncthreads();
+  if (rounds_b == 0u) {
+    // tile b had no records (or there is no tile b): tile a is still in acc
+    write_tile(tile_a);
+    if (!has_b) return;  // uniform
+    __syncthreads();
+    for (int c = tid; c < T3; c += NT) acc[c] = 0ull;
+    __syncthreads();
+  }
+  write_tile(tile_b);
+}
+
+template <int T, int NT = kDepositThreads>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_deposit_pairs(
+    const PartRec* __restrict__ sorted, const CellMap cm, const TileMap tm, const SolveScalars* sc, long long ntot,
+    float* __restrict__ rho, long long rho_plane, float* __restrict__ keep) {
+  deposit_pair_body<T, NT>(sorted, cm, tm, sc, ntot, rho, rho_plane, keep);
 }
 
 template <int T, int NT = kDepositThreads>

@@ -91,6 +91,7 @@ struct Tuning {
   int chunk_inv = 0;      // same for the inverse y/x pass pair
   int deposit_split = 1;  // heavy deposit tiles over several workgroups
   int y_inv_narrow = 1;   // inverse y pass on the narrower tile where one is compiled
+  int deposit_pair = 0;   // unsplit deposit: two tiles per workgroup
 };
 
 // Planes per chunk when a pass pair runs chunk by chunk, so that the second
@@ -119,7 +120,9 @@ Tuning default_tuning(int nm) {
   return t;
 }
 
-constexpr int kKnobs = 8;
+// knob 8 (two tiles per deposit workgroup) only acts with the split off, so
+// it comes after knob 6 in the coordinate descent
+constexpr int kKnobs = 9;
 int& knob(Tuning& t, int k) {
   switch (k) {
     case 0: return t.z_persistent;
@@ -129,14 +132,15 @@ int& knob(Tuning& t, int k) {
     case 4: return t.chunk_fwd;
     case 5: return t.chunk_inv;
     case 6: return t.deposit_split;
-    default: return t.y_inv_narrow;
+    case 7: return t.y_inv_narrow;
+    default: return t.deposit_pair;
   }
 }
 
 // Process-wide cache of measured choices, keyed by (device architecture, nm).
-// When GNG_PM_TUNE_FILE names a file, it is read once ("arch nm z_persistent
-// deposit_wide y_fwd_persistent y_inv_persistent" per line) and rewritten after
-// every new measurement, so later processes reuse the decisions.
+// When GNG_PM_TUNE_FILE names a file, it is read once (per line: arch, nm and
+// the kKnobs knobs in knob() order) and rewritten after every new measurement,
+// so later processes reuse the decisions.
 std::mutex g_tune_mu;
 std::map<std::pair<std::string, int>, Tuning> g_tune;
 bool g_tune_loaded = false;
@@ -157,12 +161,13 @@ void load_tune_file_locked() {
   if (!fp) return;
   char arch[64];
   int nm, v[kKnobs];
-  while (std::fscanf(fp, "%63s %d %d %d %d %d %d %d %d %d", arch, &nm, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
-                     &v[6], &v[7]) == 2 + kKnobs) {
+  while (std::fscanf(fp, "%63s %d %d %d %d %d %d %d %d %d %d", arch, &nm, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
+                     &v[6], &v[7], &v[8]) == 2 + kKnobs) {
     Tuning t;
     for (int k = 0; k < 4; ++k) knob(t, k) = v[k] != 0;
     t.deposit_split = v[6] != 0;
     t.y_inv_narrow = v[7] != 0;
+    t.deposit_pair = v[8] != 0;
     t.chunk_fwd = v[4] == chunk_rule(nm, kChunkFwdMiB) ? v[4] : 0;
     t.chunk_inv = v[5] == chunk_rule(nm, kChunkInvMiB) ? v[5] : 0;
     t.tuned = 1;
@@ -187,10 +192,10 @@ void store_tuning(int dev, int nm, const Tuning& t) {
   FILE* fp = std::fopen(f, "w");
   if (!fp) return;
   for (const auto& kv : g_tune)
-    std::fprintf(fp, "%s %d %d %d %d %d %d %d %d %d\n", kv.first.first.c_str(), kv.first.second,
+    std::fprintf(fp, "%s %d %d %d %d %d %d %d %d %d %d\n", kv.first.first.c_str(), kv.first.second,
                  kv.second.z_persistent, kv.second.deposit_wide, kv.second.y_fwd_persistent,
                  kv.second.y_inv_persistent, kv.second.chunk_fwd, kv.second.chunk_inv, kv.second.deposit_split,
-                 kv.second.y_inv_narrow);
+                 kv.second.y_inv_narrow, kv.second.deposit_pair);
   std::fclose(fp);
 }
 
@@ -293,6 +298,9 @@ struct MeshOps {
   // split: dense tiles shared by several workgroups
   void (*deposit[2][2])(dim3, dim3, hipStream_t, const PartRec*, const unsigned*, CellMap, TileMap,
                         const SolveScalars*, long long, float*, long long, float*, DepositSplit);
+  // deposit_pairs[v]: two tiles per workgroup (unsplit deposit only)
+  void (*deposit_pairs[2])(dim3, dim3, hipStream_t, const PartRec*, CellMap, TileMap, const SolveScalars*, long long,
+                           float*, long long, float*);
   // per-cell particle moments of the sorted records, one workgroup per tile
   void (*cell_moments)(dim3, dim3, hipStream_t, const PartRec*, const unsigned*, CellMap, TileMap,
                        const SolveScalars*, long long, double*);
@@ -359,6 +367,14 @@ MeshOps make_ops() {
   GNG_DEPOSIT(0, 256, true)
   GNG_DEPOSIT(1, 512, true)
 #undef GNG_DEPOSIT
+  o.deposit_pairs[0] = [](dim3 g, dim3 b, hipStream_t s, const PartRec* srt, CellMap cm, TileMap tm,
+                          const SolveScalars* sc, long long n, float* rho, long long rho_plane, float* keep) {
+    hipLaunchKernelGGL((k_deposit_pairs<TD, 256>), g, b, 0, s, srt, cm, tm, sc, n, rho, rho_plane, keep);
+  };
+  o.deposit_pairs[1] = [](dim3 g, dim3 b, hipStream_t s, const PartRec* srt, CellMap cm, TileMap tm,
+                          const SolveScalars* sc, long long n, float* rho, long long rho_plane, float* keep) {
+    hipLaunchKernelGGL((k_deposit_pairs<TD, 512>), g, b, 0, s, srt, cm, tm, sc, n, rho, rho_plane, keep);
+  };
   o.cell_moments = [](dim3 g, dim3 b, hipStream_t s, const PartRec* srt, const unsigned* off, CellMap cm,
                       TileMap tm, const SolveScalars* sc, long long n, double* mom) {
     hipLaunchKernelGGL((k_cell_moments<TD>), g, b, 0, s, srt, off, cm, tm, sc, n, mom);
@@ -860,9 +876,16 @@ int run_front(gng_pm_plan* p, const PartIn& in, long long n, hipStream_t s) {
     sp.done = p->d_split_done;
     deposit_groups = ntiles + (unsigned)p->split_slots;
   }
-  o.deposit[p->tune.deposit_wide][sp.slices != nullptr](dim3(deposit_groups), dim3(p->tune.deposit_wide ? 512 : 256),
-                                                        s, p->d_sorted, p->d_offset, cm, p->tm, p->d_sc, n, rho,
-                                                        rho_plane, p->keep_density ? p->d_rho : nullptr, sp);
+  if (!sp.slices && p->tune.deposit_pair) {
+    o.deposit_pairs[p->tune.deposit_wide](dim3((ntiles + 1u) / 2u), dim3(p->tune.deposit_wide ? 512 : 256), s,
+                                          p->d_sorted, cm, p->tm, p->d_sc, n, rho, rho_plane,
+                                          p->keep_density ? p->d_rho : nullptr);
+  } else {
+    o.deposit[p->tune.deposit_wide][sp.slices != nullptr](dim3(deposit_groups),
+                                                          dim3(p->tune.deposit_wide ? 512 : 256), s, p->d_sorted,
+                                                          p->d_offset, cm, p->tm, p->d_sc, n, rho, rho_plane,
+                                                          p->keep_density ? p->d_rho : nullptr, sp);
+  }
   mark(p, 3, s);
   const long long rows = (long long)nm * nm;
   const dim3 yz_grid((unsigned)(H / o.yz_cols), (unsigned)nm);
@@ -2054,6 +2077,7 @@ int gng_pm_set_tuning(gng_pm_plan* p, const gng_pm_tuning_t* t) {
   p->tune.chunk_inv = t->plane_chunk_inv;
   p->tune.deposit_split = t->deposit_split != 0;
   p->tune.y_inv_narrow = t->y_inv_narrow != 0;
+  p->tune.deposit_pair = t->deposit_pair != 0;
   p->tune.tuned = 0;
   return GNG_OK;
 }
@@ -2071,6 +2095,7 @@ int gng_pm_get_tuning(const gng_pm_plan* p, gng_pm_tuning_t* t) {
   t->plane_chunk_inv = p->tune.chunk_inv;
   t->deposit_split = p->tune.deposit_split;
   t->y_inv_narrow = p->tune.y_inv_narrow;
+  t->deposit_pair = p->tune.deposit_pair;
   return GNG_OK;
 }
 

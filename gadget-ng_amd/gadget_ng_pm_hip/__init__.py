@@ -95,14 +95,16 @@ class GngTuning(ctypes.Structure):
                 ("y_fwd_persistent", ctypes.c_int32), ("y_inv_persistent", ctypes.c_int32),
                 ("tuned", ctypes.c_int32), ("interp_sorted", ctypes.c_int32),
                 ("plane_chunk_fwd", ctypes.c_int32), ("plane_chunk_inv", ctypes.c_int32),
-                ("deposit_split", ctypes.c_int32), ("y_inv_narrow", ctypes.c_int32)]
+                ("deposit_split", ctypes.c_int32), ("y_inv_narrow", ctypes.c_int32),
+                ("deposit_pair", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {"z_persistent": bool(self.z_persistent), "deposit_threads": 512 if self.deposit_wide else 256,
                 "y_fwd_persistent": bool(self.y_fwd_persistent), "y_inv_persistent": bool(self.y_inv_persistent),
                 "interp_sorted": bool(self.interp_sorted), "chunk_fwd": int(self.plane_chunk_fwd),
                 "chunk_inv": int(self.plane_chunk_inv), "deposit_split": bool(self.deposit_split),
-                "y_inv_narrow": bool(self.y_inv_narrow), "tuned": bool(self.tuned)}
+                "y_inv_narrow": bool(self.y_inv_narrow), "deposit_pair": bool(self.deposit_pair),
+                "tuned": bool(self.tuned)}
 
 
 _lib = None
@@ -475,7 +477,8 @@ class Plan:
 
     def set_tuning(self, z_persistent: bool = False, deposit_threads: int = 512, y_fwd_persistent: bool = False,
                    y_inv_persistent: bool = False, interp_sorted: bool = False, chunk_fwd: int = 0,
-                   chunk_inv: int = 0, deposit_split: bool = True, y_inv_narrow: bool = True) -> None:
+                   chunk_inv: int = 0, deposit_split: bool = True, y_inv_narrow: bool = True,
+                   deposit_pair: bool = False) -> None:
         if deposit_threads not in (256, 512):
             raise ValueError("deposit_threads must be 256 or 512")
         t = GngTuning()
@@ -488,6 +491,7 @@ class Plan:
         t.plane_chunk_inv = int(chunk_inv)
         t.deposit_split = int(bool(deposit_split))
         t.y_inv_narrow = int(bool(y_inv_narrow))
+        t.deposit_pair = int(bool(deposit_pair))
         self._check(lib().gng_pm_set_tuning(self._h, ctypes.byref(t)), "gng_pm_set_tuning")
 
     def tuning(self) -> dict:

@@ -248,6 +248,10 @@ typedef struct gng_pm_tuning_t {
                                 the three live fields leave a 1024-thread
                                 workgroup alone on its CU.  Same values either
                                 way */
+  int32_t deposit_pair;      /* deposit without split tiles: two tiles per
+                                workgroup (1), whose records load while the
+                                other tile is written, or one (0, default).
+                                Same density bit for bit either way */
 } gng_pm_tuning_t;
 
 int gng_pm_autotune(gng_pm_plan *p, int reps);

@@ -27,7 +27,8 @@ def test_every_variant_matches_oracle(pm, oracle_mod, nm):
         assert rms_rel(got, ref) < 1e-5, (zp, dt, yf, yi)
         assert plan.tuning() == {"z_persistent": zp, "deposit_threads": dt, "y_fwd_persistent": yf,
                                  "y_inv_persistent": yi, "interp_sorted": False, "chunk_fwd": 0, "chunk_inv": 0,
-                                 "deposit_split": True, "y_inv_narrow": True, "tuned": False}
+                                 "deposit_split": True, "y_inv_narrow": True, "deposit_pair": False,
+                                 "tuned": False}
 
 
 @pytest.mark.parametrize("nm,chunks", [(32, (4, 16)), (64, (8, 32)), (96, (12, 48)), (128, (16, 64))])
@@ -168,3 +169,34 @@ def test_inverse_y_tile_widths_are_bit_identical(pm, oracle_mod):
         assert np.array_equal(v, ref), k
     idx = g.choice(n, 20000, replace=False)
     assert rms_rel(ref[idx], oracle_mod.pm_accelerations(pos, m, L, nm, 1.0, idx=idx)) <= 1e-5
+
+
+@pytest.mark.parametrize("nm", [8, 12, 24, 48, 128, 512])
+def test_paired_tile_deposit_is_bit_identical(pm, oracle_mod, nm):
+    """Two tiles per deposit workgroup (the second tile's records load while
+    the first is written) give the one-tile deposit's density and forces bit
+    for bit: uniform particles plus a dense clump, with odd tile counts (1 at
+    nm = 8, 12, 24 and 27 at 48: the last workgroup holds one tile) and even
+    ones, with and without the split of dense tiles (which takes precedence)."""
+    g = np.random.default_rng(nm + 5)
+    L = 10.0
+    n = min(nm ** 3 // 2, 400000)
+    pos = g.uniform(0, L, (n, 3))
+    pos[: n // 4] = (g.normal(0.61 * L, 0.01 * L, (n // 4, 3))) % L
+    pos[n // 4: n // 4 + 50] = 0.5 * L  # one crowded cell
+    m = g.uniform(0.5, 1.5, n)
+    print(f"case paired-tile deposit nm={nm} n={n}", flush=True)
+    plan = pm.Plan(nm, L, device=0, keep_density=True)
+    out = {}
+    for threads in (256, 512):
+        for split in (False, True):
+            for pair in (False, True):
+                plan.set_tuning(deposit_threads=threads, deposit_split=split, deposit_pair=pair)
+                assert plan.tuning()["deposit_pair"] == pair
+                out[(threads, split, pair)] = (plan.accelerations_f64(pos, m, 1.0), plan.density())
+    a0, d0 = out[(256, False, False)]
+    for k, (a, d) in out.items():
+        assert np.array_equal(d, d0), k
+        assert np.array_equal(a, a0), k
+    idx = g.choice(n, min(n, 20000), replace=False)
+    assert rms_rel(a0[idx], oracle_mod.pm_accelerations(pos, m, L, nm, 1.0, idx=idx)) <= 1e-5
