@@ -11,7 +11,10 @@ three range tables from the kernel source and replays the scheme in numpy on
 the kernel's own cell records (the FP64 cell index and 43-bit fraction of
 cell_fix), for 1, 2 and 3 tiles per axis and inputs crowded onto tile
 boundaries: every CIC target must be added exactly once, so the mesh equals
-the oracle's deposit (cic.rs:37-93)."""
+the oracle's deposit (cic.rs:37-93).  It also restates k_tile_ranges (the
+per-tile layout of the 14 ranges that the deposit prologue reads) and the
+record lookup built on it, and checks that they visit exactly the sorted
+slots of each tile's ranges."""
 import os
 import re
 
@@ -129,3 +132,63 @@ def test_cell_fix_matches_the_oracle_cell_coordinate():
     up = (i != ref_i)
     assert np.all(((ref_i[up] + 1) % nm == i[up]) & (q[up] == 0) & (ref_f[up] > 1.0 - 2.0 ** -FRAC_BITS))
     assert np.abs(q[~up] * 2.0 ** -FRAC_BITS - ref_f[~up]).max() <= 2.0 ** -(FRAC_BITS + 1)
+
+
+def tile_ranges(offset, src, r):
+    """k_tile_ranges for tile rank r in numpy: the 14 ranges laid end to end;
+    words 0-12 the running ends, 13 the total, 14 the slot delta of range 0,
+    15 + k the change of the delta from range k to k + 1."""
+    w = np.zeros(32, dtype=np.int64)
+    pre, prev = 0, 0
+    for k in range(14):
+        b = src[8 * r + D_LIST[k]]
+        beg = offset[b + LO[k]]
+        length = offset[b + HI[k]] - beg
+        delta = beg - pre
+        pre += length
+        if k < 13:
+            w[k] = pre
+        if k == 0:
+            w[14] = delta
+        else:
+            w[14 + k] = delta - prev
+        prev = delta
+    w[13] = pre
+    return w
+
+
+@pytest.mark.parametrize("nt,seed", [(1, 0), (2, 1), (3, 2), (4, 3)])
+def test_range_table_addresses_every_record_of_the_14_ranges(nt, seed):
+    """The deposit prologue's range table and the record lookup built on it
+    (rank r of flattened item q = the range ends at or below q; sorted slot =
+    q + delta, summed from the steps; a wavefront of 64 items all below the
+    first end skips the search) visit, for every tile, exactly the sorted
+    slots of its 14 bucket ranges, each with its own range index.  Bucket
+    counts are random (many empty) and tile ranks a random permutation."""
+    rng = np.random.default_rng(seed)
+    ntiles = nt ** 3
+    rank = rng.permutation(ntiles)
+    counts = rng.integers(0, 40, ntiles * 8) * (rng.random(ntiles * 8) < 0.7)
+    offset = np.concatenate([[0], np.cumsum(counts)])
+    src = np.zeros(8 * ntiles, dtype=np.int64)
+    for t in range(ntiles):
+        ux, uy, uz = t % nt, (t // nt) % nt, t // (nt * nt)
+        for d in range(8):
+            s = ((((uz - (d >> 2 & 1)) % nt) * nt + (uy - (d >> 1 & 1)) % nt) * nt + (ux - (d & 1)) % nt)
+            src[8 * rank[t] + d] = rank[s] * 8
+    for r in range(ntiles):
+        w = tile_ranges(offset, src, r)
+        want = sorted((int(q), k) for k in range(14)
+                      for q in range(offset[src[8 * r + D_LIST[k]] + LO[k]], offset[src[8 * r + D_LIST[k]] + HI[k]]))
+        got = []
+        total = int(w[13])
+        for q0 in range(0, total, 64):  # one wavefront: items q0 .. q0 + 63
+            for q in range(q0, min(q0 + 64, total)):
+                if q0 + 63 < w[0]:
+                    rr, dl = 0, int(w[14])
+                else:
+                    ge = [q >= w[k] for k in range(13)]
+                    rr = sum(ge)
+                    dl = int(w[14]) + sum(int(w[15 + k]) for k in range(13) if ge[k])
+                got.append((q + dl, rr))
+        assert sorted(got) == want, r
