@@ -1043,6 +1043,11 @@ def bench_ic(args) -> int:
 
 
 BF16_DENSE_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
+# FP32 operations of every evaluated short-range pair before the cut: 3
+# differences, 3 x (multiply, round, fused subtract) for the minimum image, 3
+# for the squared distance
+SR_FLOP_PER_PAIR_MIN = 15
 NET_FLOP_PER_CELL = 2 * 27 * (8 * 16 + 16 * 16 + 16 * 3)  # 23 328
 # compulsory HBM bytes per cell of one inference: x0 in (8 bf16), x1 out and
 # in (16 bf16 each way), x2 out and in, y out (4 f32)
@@ -1107,6 +1112,7 @@ def bench_learned(args) -> int:
         return float(np.mean([e[which][key] for e in evals]))
 
     t_sr = float(np.mean([e["time_short_range_s"] for e in evals]))
+    sr_pairs = float(np.mean([e["sr_pairs"] for e in evals]))
     t_net = float(np.mean([e["time_net_s"] for e in evals]))
     result = {
         "metric": "learned sub-cell correction, 256^3 clustered particles on a 256^3 mesh: held-out field-particle "
@@ -1137,6 +1143,17 @@ def bench_learned(args) -> int:
                      "traffic": None, "bytes_per_cell": NET_BYTES_PER_CELL, "mean_ms": round(t_fwd * 1e3, 4),
                      "mfma_tflops": round(achieved, 2), "mfma_frac": round(achieved / BF16_DENSE_PEAK_TFLOPS, 5),
                      "flop_per_cell": NET_FLOP_PER_CELL},
+        # the short-range direct sum (k_force): every particle against every
+        # particle of its 27 chaining-mesh cells; each evaluated pair costs at
+        # least SR_FLOP_PER_PAIR_MIN FP32 operations (difference, minimum image,
+        # squared distance) before the cut, so this rate bounds the kernel's
+        # FP32 throughput from below
+        "short_range": {"ms": round(t_sr * 1e3, 2), "pairs_evaluated": int(sr_pairs),
+                        "gpairs_per_s": round(sr_pairs / t_sr / 1e9, 1),
+                        "flop_per_pair_min": SR_FLOP_PER_PAIR_MIN,
+                        "fp32_tflops_min": round(sr_pairs * SR_FLOP_PER_PAIR_MIN / t_sr / 1e12, 2),
+                        "fp32_peak_tflops": FP32_PEAK_TFLOPS,
+                        "frac_min": round(sr_pairs * SR_FLOP_PER_PAIR_MIN / t_sr / 1e12 / FP32_PEAK_TFLOPS, 4)},
         "training": {"data_s": round(t_data, 2), "train_s": round(t_train, 2), "loss_first": hist[0],
                      "loss_last": hist[-1], "f0": lc.f0},
     }
