@@ -8,9 +8,10 @@
 //
 // Method:
 //   1. particles are counted into a chaining mesh of nc^3 cells of side
-//      L/nc >= r_cut and scanned; a stable radix sort by cell (sort.hpp) puts
-//      them in cell order with ascending particle index inside each cell, and
-//      they are gathered as float4 x,y,z,m plus the original index.  The
+//      L/nc >= r_cut (nc >= 3) and scanned; a stable radix sort by cell
+//      (sort.hpp) puts them in cell order with ascending particle index inside
+//      each cell, and they are gathered as float4 (position relative to the
+//      corner of their unwrapped cell, mass) plus the original index.  The
 //      order of every sum is then fixed, so the result is bit-identical from
 //      call to call;
 //   2. the work list is (cell, chunk of 256 particles); its length is read
@@ -18,7 +19,11 @@
 //      hardware dispatcher balances cells of very different occupancy;
 //   3. a workgroup streams the 27 neighbour cells through LDS 256 particles at
 //      a time and each lane sums its particle's pairs: FP32 inside a staged
-//      chunk, FP64 across chunks.
+//      chunk, FP64 across chunks.  A neighbour cell at offset d (cells, each
+//      component -1, 0 or 1) holds its particles at rel + d * L/nc from the
+//      lane's cell corner, so the displacement needs no per-pair minimum
+//      image: every pair within r_cut <= L/nc <= L/3 is its own minimum image
+//      in that form, and no pair outside the cut comes inside it.
 //
 // The split factor uses erfc from Abramowitz & Stegun 7.1.26 (|error| <=
 // 1.5e-7), sharing exp(-x^2) with the exact-split term.
@@ -44,17 +49,24 @@ __global__ void k_fill_u32(unsigned* __restrict__ p, long long n, unsigned v) {
 
 __global__ void k_fill_u64(unsigned long long* p) { *p = 0ull; }
 
-__device__ __forceinline__ int wrap_cell(float v, float cs, int nc) {
-  int c = (int)floorf(v * cs);
-  c %= nc;
-  return c < 0 ? c + nc : c;
+// The unwrapped chaining-mesh cell floor(v nc / L) of one coordinate, in FP64
+// (cs = nc / L), and its index wrapped into [0, nc); a non-finite coordinate
+// lands in cell 0.  k_count and k_gather both start from cell_floor, so a
+// particle's cell and its in-cell offset always agree.
+__device__ __forceinline__ double cell_floor(float v, double cs) { return floor((double)v * cs); }
+
+__device__ __forceinline__ int wrap_cell(double fl, int nc) {
+  double c = fmod(fl, (double)nc);
+  if (c < 0.0) c += (double)nc;
+  return c >= 0.0 && c < (double)nc ? (int)c : 0;
 }
 
 __global__ void k_count(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-                        long long n, int nc, float cs, unsigned* __restrict__ cnt, unsigned* __restrict__ cell_of) {
+                        long long n, int nc, double cs, unsigned* __restrict__ cnt, unsigned* __restrict__ cell_of) {
   const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const int cx = wrap_cell(x[p], cs, nc), cy = wrap_cell(y[p], cs, nc), cz = wrap_cell(z[p], cs, nc);
+  const int cx = wrap_cell(cell_floor(x[p], cs), nc), cy = wrap_cell(cell_floor(y[p], cs), nc),
+            cz = wrap_cell(cell_floor(z[p], cs), nc);
   const unsigned c = (unsigned)(((long long)cz * nc + cy) * nc + cx);
   cell_of[p] = c;
   atomicAdd(&cnt[c], 1u);
@@ -96,19 +108,26 @@ __global__ __launch_bounds__(1024) void k_scan(const unsigned* __restrict__ cnt,
   }
 }
 
-// sorted[i] = particle order[i] (the cell-sorted permutation)
+// sorted[i] = particle order[i] (the cell-sorted permutation): its position
+// relative to the corner of its unwrapped cell, v - floor(v nc / L) L / nc,
+// formed in FP64 from the same floor as k_count's (so it is also the offset
+// from the wrapped cell's corner, whatever image of the box v lies in), and
+// its mass
 __global__ void k_gather(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-                         const float* __restrict__ m, long long n, const unsigned* __restrict__ order,
-                         float4* __restrict__ sorted, unsigned* __restrict__ sidx) {
+                         const float* __restrict__ m, long long n, const unsigned* __restrict__ order, double cs,
+                         double cside, float4* __restrict__ sorted, unsigned* __restrict__ sidx) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const unsigned p = order[i];
-  sorted[i] = make_float4(x[p], y[p], z[p], m[p]);
+  const float vx = x[p], vy = y[p], vz = z[p];
+  sorted[i] = make_float4((float)((double)vx - cell_floor(vx, cs) * cside),
+                          (float)((double)vy - cell_floor(vy, cs) * cside),
+                          (float)((double)vz - cell_floor(vz, cs) * cside), m[p]);
   sidx[i] = p;
 }
 
 struct Params {
-  float L, invL, rc2, eps2, inv_s2rs, g;
+  float csize, rc2, eps2, inv_s2rs, g;  // csize: chaining cell side L/nc
   int nc;
   long long ncell;
 };
@@ -137,7 +156,10 @@ __global__ __launch_bounds__(256) void k_force(const float4* __restrict__ sorted
   double sx = 0.0, sy = 0.0, sz = 0.0;
   unsigned long long seen = 0;
   for (int nb = 0; nb < 27; ++nb) {
-    int ux = cx + nb % 3 - 1, uy = cy + (nb / 3) % 3 - 1, uz = cz + nb / 9 - 1;
+    const int ox = nb % 3 - 1, oy = (nb / 3) % 3 - 1, oz = nb / 9 - 1;
+    // the lane's particle relative to this neighbour cell's corner
+    const float px = pi.x - (float)ox * pr.csize, py = pi.y - (float)oy * pr.csize, pz = pi.z - (float)oz * pr.csize;
+    int ux = cx + ox, uy = cy + oy, uz = cz + oz;
     ux += ux < 0 ? nc : 0;
     ux -= ux >= nc ? nc : 0;
     uy += uy < 0 ? nc : 0;
@@ -155,17 +177,15 @@ __global__ __launch_bounds__(256) void k_force(const float4* __restrict__ sorted
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
       for (int k = 0; k < cnt; ++k) {
         const float4 pj = js[k];
-        float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-        dx -= pr.L * rintf(dx * pr.invL);
-        dy -= pr.L * rintf(dy * pr.invL);
-        dz -= pr.L * rintf(dz * pr.invL);
+        const float dx = pj.x - px, dy = pj.y - py, dz = pj.z - pz;
         const float r2b = dx * dx + dy * dy + dz * dz;
         if (r2b <= pr.rc2 && r2b > 0.0f && jb + (unsigned)k != i) {
           const float r2 = r2b + pr.eps2;
           const float rinv = rsqrtf(r2);
           const float x = r2 * rinv * pr.inv_s2rs;
           const float e = __expf(-x * x);
-          const float t = 1.0f / (1.0f + 0.3275911f * x);
+          // hardware reciprocal (1 ulp), far below the polynomial's 1.5e-7
+          const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * x);
           const float poly =
               t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
           const float S = e * (KERNEL == GNG_SR_EXACT ? poly + 1.1283791670955126f * x : poly);
@@ -278,7 +298,7 @@ int cell_sort(gng_sr* sr, const float* d_x, const float* d_y, const float* d_z, 
   }
   const unsigned cb = (unsigned)((ncell + 255) / 256), pb = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_fill_u32, dim3(cb), dim3(256), 0, s, sr->d_cnt, ncell, 0u);
-  const float cs = (float)((double)nc / box_size);
+  const double cs = (double)nc / box_size;
   hipLaunchKernelGGL(k_count, dim3(pb), dim3(256), 0, s, d_x, d_y, d_z, (long long)n, nc, cs, sr->d_cnt,
                      sr->d_cell_of);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, sr->d_cnt, ncell, sr->d_off, sr->d_choff);
@@ -360,7 +380,10 @@ int gng_sr_accelerations(gng_sr* sr, const float* d_x, const float* d_y, const f
   const long long ncell = (long long)nc * nc * nc;
   const unsigned pb = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s, sr->d_pairs);
-  hipLaunchKernelGGL(k_gather, dim3(pb), dim3(256), 0, s, d_x, d_y, d_z, d_m, (long long)n, sr->d_order,
+  // the same cs as the cell count (cell_sort), and the cell side
+  const double cs = (double)nc / box_size, cside = box_size / (double)nc;
+  const float csize = (float)cside;
+  hipLaunchKernelGGL(k_gather, dim3(pb), dim3(256), 0, s, d_x, d_y, d_z, d_m, (long long)n, sr->d_order, cs, cside,
                      sr->d_sorted, sr->d_sidx);
   GNG_HIP(hipGetLastError());
   unsigned chunks = 0;
@@ -368,8 +391,7 @@ int gng_sr_accelerations(gng_sr* sr, const float* d_x, const float* d_y, const f
   GNG_HIP(hipStreamSynchronize(s));
   if (chunks == 0) return fail(GNG_ERR_HIP, "short-range work list is empty");
   Params pr;
-  pr.L = (float)box_size;
-  pr.invL = (float)(1.0 / box_size);
+  pr.csize = csize;
   pr.rc2 = (float)(r_cut * r_cut);
   pr.eps2 = (float)eps2;
   pr.inv_s2rs = (float)(1.0 / (std::sqrt(2.0) * r_split));

@@ -118,8 +118,7 @@ def test_cell_order_is_the_stable_sort_by_cell(learned, n, L, rc):
     pos = pos.astype(np.float32)
     pos[: n // 10] += np.float32(L) * g.integers(-2, 3, (n // 10, 3)).astype(np.float32)
     nc = min(256, int(np.floor(L / rc)))
-    cs = np.float32(nc / L)
-    cell = np.floor(pos * cs).astype(np.int64) % nc
+    cell = np.floor(pos.astype(np.float64) * (nc / L)).astype(np.int64) % nc  # FP64, as the kernel
     key = (cell[:, 2] * nc + cell[:, 1]) * nc + cell[:, 0]
     got = learned.ShortRange(device=0).cell_order(pos, L, rc)
     assert np.array_equal(got, np.argsort(key, kind="stable").astype(np.uint32))
