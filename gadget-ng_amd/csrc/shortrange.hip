@@ -128,6 +128,7 @@ __global__ void k_gather(const float* __restrict__ x, const float* __restrict__ 
 
 struct Params {
   float csize, rc2, eps2, inv_s2rs, g;  // csize: chaining cell side L/nc
+  float log2e_s2rs2;                    // log2(e) / (2 r_s^2)
   int nc;
   long long ncell;
 };
@@ -171,7 +172,11 @@ __global__ __launch_bounds__(256) void k_force(const float4* __restrict__ sorted
     seen += j1 - j0;
     for (unsigned jb = j0; jb < j1; jb += kChunk) {
       __syncthreads();
-      if (jb + threadIdx.x < j1) js[threadIdx.x] = sorted[jb + threadIdx.x];
+      if (jb + threadIdx.x < j1) {
+        float4 v = sorted[jb + threadIdx.x];
+        v.w *= pr.g;  // G m, staged once per particle rather than per pair
+        js[threadIdx.x] = v;
+      }
       __syncthreads();
       const int cnt = (int)(j1 - jb < (unsigned)kChunk ? j1 - jb : (unsigned)kChunk);
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
@@ -183,13 +188,14 @@ __global__ __launch_bounds__(256) void k_force(const float4* __restrict__ sorted
           const float r2 = r2b + pr.eps2;
           const float rinv = rsqrtf(r2);
           const float x = r2 * rinv * pr.inv_s2rs;
-          const float e = __expf(-x * x);
+          // exp(-x^2) with x^2 = r2 / (2 r_s^2), on the hardware exp2
+          const float e = __builtin_amdgcn_exp2f(-r2 * pr.log2e_s2rs2);
           // hardware reciprocal (1 ulp), far below the polynomial's 1.5e-7
           const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * x);
           const float poly =
               t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
           const float S = e * (KERNEL == GNG_SR_EXACT ? poly + 1.1283791670955126f * x : poly);
-          const float f = pr.g * pj.w * S * rinv * rinv * rinv;
+          const float f = pj.w * S * (rinv * rinv * rinv);
           fx += f * dx;
           fy += f * dy;
           fz += f * dz;
@@ -396,6 +402,7 @@ int gng_sr_accelerations(gng_sr* sr, const float* d_x, const float* d_y, const f
   pr.eps2 = (float)eps2;
   pr.inv_s2rs = (float)(1.0 / (std::sqrt(2.0) * r_split));
   pr.g = (float)g;
+  pr.log2e_s2rs2 = (float)(1.4426950408889634 / (2.0 * r_split * r_split));
   pr.nc = nc;
   pr.ncell = ncell;
   if (kernel == GNG_SR_EXACT)
