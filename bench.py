@@ -1045,9 +1045,9 @@ def bench_ic(args) -> int:
 BF16_DENSE_PEAK_TFLOPS = 2500.0
 FP32_PEAK_TFLOPS = 157.3
 # FP32 operations of every evaluated short-range pair before the cut: 3
-# differences, 3 x (multiply, round, fused subtract) for the minimum image, 3
-# for the squared distance
-SR_FLOP_PER_PAIR_MIN = 15
+# differences and the squared distance (a multiply and two fused adds, 5
+# flops); the periodic offset is applied once per neighbour cell, not per pair
+SR_FLOP_PER_PAIR_MIN = 8
 NET_FLOP_PER_CELL = 2 * 27 * (8 * 16 + 16 * 16 + 16 * 3)  # 23 328
 # compulsory HBM bytes per cell of one inference: x0 in (8 bf16), x1 out and
 # in (16 bf16 each way), x2 out and in, y out (4 f32)
@@ -1145,8 +1145,8 @@ def bench_learned(args) -> int:
                      "flop_per_cell": NET_FLOP_PER_CELL},
         # the short-range direct sum (k_force): every particle against every
         # particle of its 27 chaining-mesh cells; each evaluated pair costs at
-        # least SR_FLOP_PER_PAIR_MIN FP32 operations (difference, minimum image,
-        # squared distance) before the cut, so this rate bounds the kernel's
+        # least SR_FLOP_PER_PAIR_MIN FP32 operations (differences, squared
+        # distance) before the cut, so this rate bounds the kernel's
         # FP32 throughput from below
         "short_range": {"ms": round(t_sr * 1e3, 2), "pairs_evaluated": int(sr_pairs),
                         "gpairs_per_s": round(sr_pairs / t_sr / 1e9, 1),
