@@ -184,7 +184,10 @@ __global__ __launch_bounds__(256) void k_force(const float4* __restrict__ sorted
         const float4 pj = js[k];
         const float dx = pj.x - px, dy = pj.y - py, dz = pj.z - pz;
         const float r2b = dx * dx + dy * dy + dz * dz;
-        if (r2b <= pr.rc2 && r2b > 0.0f && jb + (unsigned)k != i) {
+        // r2b > 0 skips coincident pairs and the particle itself (for the
+        // lane's own cell the offset is 0 * csize = 0, so its displacement
+        // is exactly zero)
+        if (r2b <= pr.rc2 && r2b > 0.0f) {
           const float r2 = r2b + pr.eps2;
           const float rinv = rsqrtf(r2);
           const float x = r2 * rinv * pr.inv_s2rs;
