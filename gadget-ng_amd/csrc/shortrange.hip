@@ -180,6 +180,7 @@ __global__ __launch_bounds__(256) void k_force(const float4* __restrict__ sorted
       __syncthreads();
       const int cnt = (int)(j1 - jb < (unsigned)kChunk ? j1 - jb : (unsigned)kChunk);
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll 4
       for (int k = 0; k < cnt; ++k) {
         const float4 pj = js[k];
         const float dx = pj.x - px, dy = pj.y - py, dz = pj.z - pz;
