@@ -38,6 +38,10 @@ CSRC = os.path.join(_ROOT, "csrc")
 REPO_ROOT = os.path.dirname(_ROOT)
 HEADER = os.path.join(REPO_ROOT, "include", "gng_pm.h")
 
+# the GNG_PM_ABI_VERSION whose struct layouts (GngConfig, GngTuning) these
+# bindings mirror
+ABI_VERSION = 2
+
 GNG_OK = 0
 GNG_ERR_INVALID = 1
 GNG_ERR_HIP = 2
@@ -124,6 +128,9 @@ def lib():
         d = ctypes.c_double
         i32 = ctypes.c_int
         L.gng_pm_abi_version.restype = i32
+        if L.gng_pm_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH} has ABI version {L.gng_pm_abi_version()}, this binding expects "
+                               f"{ABI_VERSION}: rebuild the library")
         L.gng_last_error_string.restype = ctypes.c_char_p
         L.gng_pm_mesh_supported.argtypes = [i32]
         L.gng_pm_mesh_supported.restype = i32

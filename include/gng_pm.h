@@ -51,7 +51,10 @@
 extern "C" {
 #endif
 
-#define GNG_PM_ABI_VERSION 1
+/* Raised whenever a struct's layout or an entry point's signature changes,
+ * so a binding can refuse a library it does not match (2: gng_pm_tuning_t
+ * has the fields through deposit_pair). */
+#define GNG_PM_ABI_VERSION 2
 
 /* Error codes (0 = success). */
 enum gng_status {

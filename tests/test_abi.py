@@ -45,7 +45,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_mesh_table():
     L = pm.lib()
-    assert L.gng_pm_abi_version() == 1
+    assert L.gng_pm_abi_version() == pm.ABI_VERSION == 2
     for nm in (4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 1024):
         assert L.gng_pm_mesh_supported(nm) == 1, nm
     for nm in (0, 1, 2, 3, 5, 7, 10, 100, 2048, -8):
