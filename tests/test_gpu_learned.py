@@ -93,6 +93,25 @@ def test_short_range_matches_direct_sum(learned, kernel, L):
     assert sr.last_pair_count() > len(pos)
 
 
+@pytest.mark.parametrize("L", [7.5, 20.0])
+def test_short_range_of_positions_in_other_box_images(learned, L):
+    """Particles given in other images of the box (shifted by whole box
+    lengths, up to two either way, and some exactly on cell and box faces):
+    the kernel forms each particle's cell and its offset inside it from one
+    FP64 floor, so the sums match the oracle's minimum-image brute force on
+    the same shifted positions."""
+    print(f"case short range box images L={L}", flush=True)
+    g = np.random.default_rng(int(L * 10))
+    pos, m = _clustered(1500, 1500, L, 3)
+    pos = pos.astype(np.float32)
+    pos[:200] = np.float32(L / 3.0) * g.integers(0, 3, (200, 3)).astype(np.float32)  # cell and box faces
+    pos[::3] += np.float32(L) * g.integers(-2, 3, (pos[::3].shape[0], 3)).astype(np.float32)
+    rs, rc, eps2 = 0.5, 2.5, 0.01
+    got = learned.ShortRange(device=0).accelerations(pos, m, L, rs, rc, eps2=eps2)
+    ref = lo.short_range_direct(pos.astype(np.float64), m.astype(np.float64), L, rs, rc, eps2=eps2)
+    assert rms_rel(got, ref) < 1e-5
+
+
 def test_short_range_is_bit_identical_from_call_to_call(learned):
     """The cell lists come from a stable sort, so every sum runs in one fixed
     order; a clustered set (many particles per cell) and a second object
