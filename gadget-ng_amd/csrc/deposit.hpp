@@ -87,6 +87,13 @@ __global__ __launch_bounds__(256) void k_zero(uint4* __restrict__ p, long long n
   zero_body(p, n16);
 }
 
+// The bucket counts cleared and, by the first workgroup, the solve's scalars
+// reset: the two clears before a sort in one launch.
+__global__ __launch_bounds__(256) void k_sort_reset(uint4* __restrict__ p, long long n16, SolveScalars* sc) {
+  if (blockIdx.x == 0) reset_scalars_body(sc);
+  zero_body(p, n16);
+}
+
 __global__ __launch_bounds__(256) void k_zero_box(const BoxArgs* __restrict__ bx, long long n16) {
   const BoxArgs& b = bx[blockIdx.z];
   zero_body(reinterpret_cast<uint4*>(b.count), n16);
@@ -237,6 +244,12 @@ __global__ __launch_bounds__(256) void k_scan_partials_box(const BoxArgs* __rest
 __global__ __launch_bounds__(256) void k_scan_add(unsigned* __restrict__ offset, const unsigned* __restrict__ partial,
                                                   long long nb) {
   scan_add_body(offset, partial, nb);
+}
+
+__global__ __launch_bounds__(256) void k_scan_add_prefix(unsigned* __restrict__ offset,
+                                                         const unsigned* __restrict__ partial, int nblocks,
+                                                         long long nb) {
+  scan_add_prefix_body(offset, partial, nblocks, nb);
 }
 
 __global__ __launch_bounds__(256) void k_scan_add_box(const BoxArgs* __restrict__ bx, long long nb) {

@@ -825,18 +825,17 @@ int ensure_work(gng_pm_plan* p, long long n) {
 // records the bucket stage's timing event (a solve's pipeline only).
 void run_sort(gng_pm_plan* p, const PartIn& in, long long n, unsigned* perm, bool stage_marks, hipStream_t s) {
   const CellMap cm = cell_map(p);
-  hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, p->d_sc);
   {
     const long long n16 = (p->nb + 3) / 4;  // counts padded to 16-byte words
-    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, s,
-                       reinterpret_cast<uint4*>(p->d_count), n16);
+    hipLaunchKernelGGL(k_sort_reset, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<uint4*>(p->d_count), n16, p->d_sc);
   }
   if (n > 0) launch_bucket(in, n, cm, p->tm, p->d_count, p->d_bucket, p->d_rank, p->d_sc, s);
   if (stage_marks) mark(p, 1, s);
   hipLaunchKernelGGL(k_scan_local, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_count, p->d_offset,
                      p->d_partial, p->nb);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(256), 0, s, p->d_partial, p->scan_blocks, p->d_offset, p->nb);
-  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_offset, p->d_partial, p->nb);
+  hipLaunchKernelGGL(k_scan_add_prefix, dim3((unsigned)p->scan_blocks), dim3(256), 0, s, p->d_offset, p->d_partial,
+                     p->scan_blocks, p->nb);
   if (n > 0) launch_scatter(in, n, cm, p->d_bucket, p->d_rank, p->d_offset, p->d_sorted, perm, p->d_sc, s);
   p->sorted_n = n;
 }

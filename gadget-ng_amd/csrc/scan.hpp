@@ -69,6 +69,26 @@ __device__ __forceinline__ void scan_partials_body(unsigned* __restrict__ partia
   if (threadIdx.x == 0) offset[nb] = carry;
 }
 
+// scan_add_body without the partials pass: workgroup b sums the totals of
+// the nblocks workgroups before it itself (a few hundred at most), and the
+// last one also writes the grand total at offset[nb].  partial keeps the
+// workgroup totals of scan_local_body.
+__device__ __forceinline__ void scan_add_prefix_body(unsigned* __restrict__ offset,
+                                                     const unsigned* __restrict__ partial, int nblocks,
+                                                     long long nb) {
+  __shared__ unsigned lds[4];
+  unsigned s = 0;
+  for (int j = threadIdx.x; j < (int)blockIdx.x; j += 256) s += partial[j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+  __syncthreads();
+  const unsigned add = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  const long long base = (long long)blockIdx.x * kScanBlock;
+  for (int k = threadIdx.x; k < kScanBlock; k += 256)
+    if (base + k < nb) offset[base + k] += add;
+  if ((int)blockIdx.x == nblocks - 1 && threadIdx.x == 0) offset[nb] = add + partial[blockIdx.x];
+}
+
 __device__ __forceinline__ void scan_add_body(unsigned* __restrict__ offset, const unsigned* __restrict__ partial,
                                                   long long nb) {
   const long long base = (long long)blockIdx.x * kScanBlock;
